@@ -1,0 +1,226 @@
+/*
+ * upbp_capi.h -- C ABI of the MI355X-native UPBP renderer (libupbp.so).
+ *
+ * Plain pointers, sizes and POD structs only; no C++ or torch types cross this
+ * boundary. Every entry point returns 0 on success and a negative UPBP_E_* code
+ * on failure (it never calls exit()); upbp_last_error() then holds a message.
+ *
+ * Each declaration names the reference interface it replaces
+ * (nolmoonen/gpuupbp, paths relative to the reference root).
+ */
+#ifndef UPBP_CAPI_H
+#define UPBP_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define UPBP_OK 0
+#define UPBP_E_ARG (-1)      /* invalid argument / command line */
+#define UPBP_E_IO (-2)       /* file could not be read or written */
+#define UPBP_E_PARSE (-3)    /* malformed OBJ / MTL / AUX content */
+#define UPBP_E_HIP (-4)      /* HIP runtime failure */
+#define UPBP_E_NODEVICE (-5) /* no GPU visible */
+#define UPBP_E_CAPACITY (-6) /* a fixed-size buffer would overflow */
+#define UPBP_HELP 1          /* parse_commandline printed help; nothing to render */
+
+/* ---- technique / mode bits: same values as gpuupbp/shared/shared_enums.h:30-56 */
+#define UPBP_BEAM_SHORT (1u << 0)
+#define UPBP_BEAM_LONG (1u << 1)
+#define UPBP_BPT (1u << 2)
+#define UPBP_SURF (1u << 3)
+#define UPBP_PP3D (1u << 4)
+#define UPBP_PB2D (1u << 5)
+#define UPBP_BB1D (1u << 6)
+#define UPBP_BP2D (1u << 7)
+#define UPBP_PREVIOUS (1u << 9)
+#define UPBP_COMPATIBLE (1u << 10)
+#define UPBP_SPECULAR_ONLY (1u << 12)
+
+/* light types: gpuupbp/shared/light.h:38 */
+#define UPBP_LIGHT_AREA 0
+#define UPBP_LIGHT_BACKGROUND 1
+#define UPBP_LIGHT_DIRECTIONAL 2
+#define UPBP_LIGHT_POINT 3
+
+/* ---- configuration: mirrors struct Config (gpuupbp/misc/config.hpp:44-120) */
+typedef struct upbp_config {
+    uint32_t algorithm_flags;
+    int32_t iteration_count;
+    float max_time;
+    uint32_t res_x, res_y;
+    uint32_t max_path_length;
+    uint32_t min_path_length;
+    float path_count_per_iter; /* resolved to an absolute count after parsing */
+    uint32_t query_beam_type;  /* UPBP_BEAM_SHORT or UPBP_BEAM_LONG */
+    uint32_t photon_beam_type;
+    float bb1d_radius_initial, bb1d_radius_alpha;
+    float bb1d_used_light_subpath_count; /* negative = relative to light paths */
+    float bp2d_radius_initial, bp2d_radius_alpha;
+    float pb2d_radius_initial, pb2d_radius_alpha;
+    float pp3d_radius_initial, pp3d_radius_alpha;
+    float surf_radius_initial, surf_radius_alpha; /* negative radius = relative to scene radius */
+    int32_t continuous_output;
+    int32_t ignore_fully_spec_paths;
+    int32_t gpu_assert;
+    uint64_t iteration_offset;
+    int32_t do_log;
+    int32_t use_shading_normal;
+    int32_t scene_id;
+    char scene_obj_file[1024];
+    char output_name[1024];
+    char additional_args[512];
+} upbp_config;
+
+/* ---- flattened scene: replaces struct Scene + Record (gpuupbp/shared/scene.h:41-127) */
+typedef struct upbp_material {
+    float diffuse[3];
+    float phong[3];
+    float phong_exponent;
+    float mirror[3];
+    float ior;        /* < 0: no Fresnel, no transmission */
+    int32_t priority; /* -1 thin wall, -2 global medium */
+    int32_t real;     /* 1 real geometry, 0 imaginary (medium boundary only) */
+    int32_t med_idx;  /* -1 if no medium is associated */
+} upbp_material;
+
+typedef struct upbp_medium {
+    float absorption[3];
+    float emission[3];
+    float scattering[3];
+    float continuation_prob;
+    float mean_cosine;
+} upbp_medium;
+
+typedef struct upbp_light {
+    int32_t type;   /* UPBP_LIGHT_* */
+    int32_t mat_id; /* enclosing material, -1 = global medium */
+    int32_t med_id;
+    float p0[3], p1[3], p2[3]; /* area light triangle */
+    float position[3];         /* point light position / directional light direction */
+    float intensity[3];        /* area/point/directional emission, background colour */
+    float scale;               /* background scale */
+} upbp_light;
+
+typedef struct upbp_camera {
+    float origin[3], target[3], roll[3];
+    float horizontal_fov_deg;
+    float focal_dist;
+    int32_t mat_id, med_id;
+} upbp_camera;
+
+typedef struct upbp_triangles {
+    uint32_t count;
+    const float* verts;    /* 9 floats per triangle: p0, p1, p2 */
+    const float* normals;  /* 12 floats per triangle: n0, n1, n2, face normal */
+    const int32_t* mat_id; /* per triangle */
+    const int32_t* light_id; /* per triangle, -1 if not emissive */
+} upbp_triangles;
+
+typedef struct upbp_scene {
+    upbp_triangles real;
+    upbp_triangles imag;
+    uint32_t mat_count;
+    const upbp_material* materials;
+    uint32_t med_count;
+    const upbp_medium* media;
+    uint32_t light_count;
+    const upbp_light* lights;
+    uint32_t global_medium_idx;
+    int32_t background_light_idx; /* -1 if none */
+    upbp_camera camera;
+    float scene_center[3];
+    float scene_radius;
+} upbp_scene;
+
+/* Per-iteration statistics (device counters plus HIP-event times). */
+typedef struct upbp_iter_stats {
+    double ms_light, ms_build, ms_camera, ms_total;
+    double ms_dominant;            /* summed duration of the dominant kernel in this iteration */
+    uint64_t dominant_launches;    /* its launch count */
+    uint64_t light_vertices, surface_vertices, medium_vertices, beams;
+    uint64_t light_extend_steps, camera_extend_steps;
+    uint64_t bpt_connections, point_queries, segment_queries;
+    uint64_t merge_accepts, vertex_drops, beam_drops, path_errors;
+    double algorithmic_bytes;      /* compulsory HBM bytes of this iteration */
+    double dominant_bytes;         /* compulsory HBM bytes of the dominant kernel */
+} upbp_iter_stats;
+
+typedef struct upbp_scene_handle upbp_scene_handle;
+typedef struct upbp_renderer upbp_renderer;
+
+/* Replaces parse_commandline (gpuupbp/misc/config.cpp:735-1370): same options,
+ * defaults, messages and output-name rules. Returns 0, UPBP_HELP, or UPBP_E_ARG. */
+int upbp_parse_commandline(int argc, const char** argv, upbp_config* cfg);
+
+/* Fills cfg with the parser defaults (config.cpp:738-785). */
+void upbp_config_defaults(upbp_config* cfg);
+
+/* Replaces SceneLoader::load_from_obj + load_scene (gpuupbp/misc/scene_loader_obj.cpp:926-1104,
+ * gpuupbp/misc/scene_loader.hpp:44-76): reads file.obj, its mtllib and file.obj.aux. */
+int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_t res_y, upbp_scene_handle** out);
+const upbp_scene* upbp_scene_get(const upbp_scene_handle* h);
+void upbp_scene_free(upbp_scene_handle* h);
+
+/* Replaces tinyexr_wrapper::save_exr (gpuupbp/misc/tinyexr_wrapper.cpp:34-95): scanline,
+ * uncompressed, HALF, channels B,G,R. rgb is W*H*3 floats, row-major from the top row. */
+int upbp_save_exr(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
+/* Replaces Renderer::init (gpuupbp/renderer/renderer_init.cpp:675-929). Copies the scene to the
+ * GPU, builds the triangle LBVHs and sizes the SoA vertex/beam caches. The caller keeps
+ * ownership of scene and cfg. */
+int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp_renderer** out);
+
+/* Replaces Renderer::run_iteration (gpuupbp/renderer/renderer.cpp:37-418): light pass, photon
+ * LBVH builds, camera pass, FP64 accumulation on the device. stats may be NULL. */
+int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_stats* stats);
+
+/* Replaces the D2H copy + add_framebuffer + scale (renderer.cpp:381-392, gpuupbp.cpp:132):
+ * writes the FP64 mean image (W*H*3) over iters_done iterations into a host buffer. */
+int upbp_read_accum(upbp_renderer* r, double* rgb, int iters_done);
+
+/* Replaces Renderer::cleanup (renderer.cpp:420-508). Idempotent; NULL is accepted. */
+void upbp_destroy(upbp_renderer* r);
+
+/* Replaces the stderr + exit(1) error path (gpuupbp/host/exception.hpp:44-123). */
+const char* upbp_last_error(const upbp_renderer* r);
+const char* upbp_global_error(void);
+
+/* Writes a seeded procedural scene as an OBJ + MTL + AUX triple (benchmark and test inputs; no
+ * reference counterpart). kind: "box", "blobs" or "furnace". */
+int upbp_generate_scene(const char* kind, uint32_t seed, const char* obj_path);
+
+/* Blocks until all work queued by r has finished. */
+int upbp_sync(upbp_renderer* r);
+
+/* ---- test hooks (parity tests and autotuning only) ----------------------
+ * Range queries over a device LBVH, returning the accepted index set of every
+ * query in CSR form (offsets[n_queries+1], indices). They replace the OptiX
+ * photon-map traversals of gpuupbp/kernel/pde/functs_{surf,pp3d,pb2d,bb1d,bp2d}.cuh
+ * with the acceptance predicates of gpuupbp/kernel/intersection.cuh:33-173.
+ *   kind 0: point query,   predicate point_point(q, p, r^2)        (SURF / P-P3D)
+ *   kind 1: segment query, predicate test_intersection_bre on (t0,t1) (P-B2D)
+ *   kind 2: point query over beams,   point_beam(beam, q, r^2)      (B-P2D)
+ *   kind 3: segment query over beams, test_intersection_beam_beam   (B-B1D)
+ * points: n_prims*3 floats; beams: n_prims*7 floats (origin, unit dir, length);
+ * queries: kind 0/2: 3 floats, kind 1/3: 8 floats (origin, unit dir, t0, t1).
+ * indices has room for max_indices entries; *n_indices receives the total. */
+int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries,
+                          uint32_t n_queries, float radius, uint32_t* offsets, uint32_t* indices,
+                          uint64_t max_indices, uint64_t* n_indices);
+
+/* Closest-hit / all-hits ray queries against a triangle LBVH (replaces the optixTrace calls of
+ * gpuupbp/kernel/intersector.cuh:120,214). rays: 8 floats each (origin, dir, tmin, tmax).
+ * Closest: hit_t[i] (INFINITY on miss), hit_prim[i] (-1 on miss). */
+int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays,
+                          float* hit_t, int32_t* hit_prim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UPBP_CAPI_H */

@@ -1,0 +1,71 @@
+/*
+ * oracle.h -- CPU oracle for the UPBP renderer hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load liboracle.so, and only as the checker
+ * or the CPU baseline; the product (libupbp.so) never links or calls it.
+ *
+ * A scalar FP32 restatement of the reference's per-path light and camera
+ * tracing (nolmoonen/gpuupbp gpuupbp/kernel/trace_light.cu, trace_camera.cu and
+ * the headers they include), with its own CPU BVH instead of OptiX, threaded
+ * with std::thread. Each function cites the reference lines it follows.
+ *
+ * Parity pins (see tests/): range-query predicates against hand cases and
+ * brute force; BSDF / phase / light sampling against closed forms; camera
+ * mapping against its inverse; the renderer against an analytic direct-view
+ * case and against the FP64 MIS oracle's sum-to-one invariant.
+ */
+#ifndef UPBP_ORACLE_H
+#define UPBP_ORACLE_H
+
+#include "../include/upbp_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_stats {
+    uint64_t light_vertices, surface_vertices, medium_vertices, beams;
+    uint64_t path_errors;
+    double seconds;
+} oracle_stats;
+
+/* Renders iterations [first_iteration, first_iteration + n_iterations) and writes the
+ * mean image (W*H*3, FP64) into out_rgb. fb_first (optional, W*H*3 floats) receives
+ * the FP32 framebuffer of the first iteration alone. n_threads <= 0: all cores. */
+int oracle_render(const upbp_scene* scene, const upbp_config* cfg, int first_iteration, int n_iterations,
+                  int n_threads, double* out_rgb, float* fb_first, oracle_stats* stats);
+
+/* Renders only light paths [0, n_light_paths) and camera paths [0, n_camera_paths) of one
+ * iteration (bounded CPU-baseline sample). Returns wall seconds through *seconds. */
+int oracle_render_sample(const upbp_scene* scene, const upbp_config* cfg, int iteration, uint32_t n_camera_paths,
+                         int n_threads, double* seconds);
+
+/* Brute-force range query (same kinds and layout as upbp_test_range_query). */
+int oracle_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries, uint32_t n_queries,
+                       float radius, uint32_t* offsets, uint32_t* indices, uint64_t max_indices,
+                       uint64_t* n_indices);
+
+/* Brute-force closest hit over triangles (9 floats each); rays: 8 floats each. */
+int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays, float* hit_t,
+                       int32_t* hit_prim);
+
+/* Camera mapping checks: raster -> primary ray direction and world -> raster, using the
+ * reference's matrix construction (gpuupbp/host/camera.hpp:33-76). */
+int oracle_camera_ray(const upbp_scene* scene, const upbp_config* cfg, const float* raster_xy, uint32_t n,
+                      float* dirs);
+int oracle_world_to_raster(const upbp_scene* scene, const upbp_config* cfg, const float* pts, uint32_t n,
+                           float* raster_xy);
+
+/* Leaf-function probes for closed-form tests. */
+float oracle_hg_pdf(float mean_cosine, float cos_theta);
+void oracle_hg_sample(float mean_cosine, float u0, float u1, float* cos_theta, float* pdf);
+float oracle_fresnel(float cos_inc, float ior);
+/* Philox4x32-10 block (Salmon et al. SC'11) used by both oracle and product streams. */
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

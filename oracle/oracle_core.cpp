@@ -1,0 +1,638 @@
+// Oracle leaf functions: scene preparation, medium sampling, BSDF, lights.
+// TEST INFRASTRUCTURE ONLY (see oracle.h). Citations: nolmoonen/gpuupbp.
+#include "oracle_core.h"
+
+namespace orc {
+
+static Mat4 mul(const Mat4& a, const Mat4& b)
+{
+    Mat4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float s = 0.f;
+            for (int k = 0; k < 4; k++) s += a.m[i * 4 + k] * b.m[k * 4 + j];
+            r.m[i * 4 + j] = s;
+        }
+    return r;
+}
+
+static Mat4 identity()
+{
+    Mat4 r;
+    memset(r.m, 0, sizeof(r.m));
+    r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.f;
+    return r;
+}
+
+static Mat4 scale(float x, float y, float z)
+{
+    Mat4 r = identity();
+    r.m[0] = x;
+    r.m[5] = y;
+    r.m[10] = z;
+    return r;
+}
+
+static Mat4 translate(float x, float y, float z)
+{
+    Mat4 r = identity();
+    r.m[3] = x;
+    r.m[7] = y;
+    r.m[11] = z;
+    return r;
+}
+
+static Mat4 inverse(const Mat4& a)
+{
+    double m[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 8; j++) m[i][j] = j < 4 ? a.m[i * 4 + j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; c++) {
+        int p = c;
+        for (int r = c + 1; r < 4; r++)
+            if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+        for (int j = 0; j < 8; j++) std::swap(m[c][j], m[p][j]);
+        const double inv = 1.0 / m[c][c];
+        for (int j = 0; j < 8; j++) m[c][j] *= inv;
+        for (int r = 0; r < 4; r++)
+            if (r != c) {
+                const double f = m[r][c];
+                for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+            }
+    }
+    Mat4 o;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) o.m[i * 4 + j] = float(m[i][j + 4]);
+    return o;
+}
+
+// shared/vec_math_ext.h:81-113 (homogeneous point transform)
+V transform_point(const Mat4& mat, V p)
+{
+    const float* m = mat.m;
+    float w = m[15];
+    w += m[12] * p.x;
+    w += m[13] * p.y;
+    w += m[14] * p.z;
+    const float inv = 1.f / w;
+    V r;
+    r.x = (m[3] + p.x * m[0] + p.y * m[1] + p.z * m[2]) * inv;
+    r.y = (m[7] + p.x * m[4] + p.y * m[5] + p.z * m[6]) * inv;
+    r.z = (m[11] + p.x * m[8] + p.y * m[9] + p.z * m[10]) * inv;
+    return r;
+}
+
+// host/camera.hpp:33-76 with the perspective matrix of shared/vec_math_ext.h:46-74
+static void init_camera(Camera& c, V origin, V target, V roll, float res_x, float res_y, float fov, int mat, int med)
+{
+    c.origin = origin;
+    c.direction = normalize(target - origin);
+    c.res_x = res_x;
+    c.res_y = res_y;
+    const V fwd = c.direction;
+    const V up = normalize(cross(roll, -fwd));
+    const V left = cross(-fwd, up);
+    const V pos{dot(up, origin), dot(left, origin), dot(-fwd, origin)};
+    Mat4 w2c = identity();
+    const V rows[3] = {up, left, -fwd};
+    const float t[3] = {-pos.x, -pos.y, -pos.z};
+    for (int r = 0; r < 3; r++) {
+        w2c.m[r * 4 + 0] = rows[r].x;
+        w2c.m[r * 4 + 1] = rows[r].y;
+        w2c.m[r * 4 + 2] = rows[r].z;
+        w2c.m[r * 4 + 3] = t[r];
+    }
+    const float nearp = 0.1f, farp = 10000.f;
+    const float f = 1.f / std::tan(fov * kPi / 360.f);
+    const float d = 1.f / (nearp - farp);
+    Mat4 persp;
+    memset(persp.m, 0, sizeof(persp.m));
+    persp.m[0] = f;
+    persp.m[5] = -f;
+    persp.m[10] = (nearp + farp) * d;
+    persp.m[11] = 2.f * nearp * farp * d;
+    persp.m[14] = -1.f;
+    const Mat4 w2n = mul(persp, w2c);
+    const Mat4 n2w = inverse(w2n);
+    const float aspect = res_x / res_y;
+    c.world_to_raster = mul(mul(scale(res_x * .5f, res_x * .5f, 0.f), translate(1.f, 1.f / aspect, 0.f)), w2n);
+    c.raster_to_world = mul(mul(n2w, translate(-1.f, -1.f / aspect, 0.f)), scale(2.f / res_x, 2.f / res_x, 0.f));
+    c.image_plane_dist = res_x / (2.f * std::tan(fov * kPi / 360.f));
+    c.mat_id = mat;
+    c.med_id = med;
+}
+
+// host/medium.hpp:39-85
+static void init_medium(Medium& m, V absorption, V emission, V scattering, float cont, float g)
+{
+    m.cont_prob = cont;
+    m.mean_cosine = g;
+    auto absv = [](V a) { return V{std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; };
+    m.absorption = absv(absorption);
+    m.emission = absv(emission);
+    m.scattering = absv(scattering);
+    m.attenuation = m.absorption + m.scattering;
+    m.min_pos_att = m.attenuation.x;
+    m.min_pos_idx = 0;
+    if (m.attenuation.y > 0.f && (m.min_pos_att == 0 || m.attenuation.y < m.min_pos_att)) {
+        m.min_pos_att = m.attenuation.y;
+        m.min_pos_idx = 1;
+    }
+    if (m.attenuation.z > 0.f && (m.min_pos_att == 0 || m.attenuation.z < m.min_pos_att)) {
+        m.min_pos_att = m.attenuation.z;
+        m.min_pos_idx = 2;
+    }
+    m.has_scattering = m.scattering.x > 0 || m.scattering.y > 0 || m.scattering.z > 0;
+    m.has_attenuation = m.attenuation.x > 0 || m.attenuation.y > 0 || m.attenuation.z > 0;
+    m.max_beam_length = m.has_attenuation ? -std::log(1e-9f) / maxc(m.attenuation) : kInf;
+}
+
+void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& s)
+{
+    auto tris = [](const upbp_triangles& t, std::vector<Tri>& out) {
+        out.resize(t.count);
+        for (uint32_t i = 0; i < t.count; i++) {
+            Tri& o = out[i];
+            o.p0 = fromp(t.verts + 9 * i);
+            o.p1 = fromp(t.verts + 9 * i + 3);
+            o.p2 = fromp(t.verts + 9 * i + 6);
+            o.n0 = fromp(t.normals + 12 * i);
+            o.n1 = fromp(t.normals + 12 * i + 3);
+            o.n2 = fromp(t.normals + 12 * i + 6);
+            o.face = fromp(t.normals + 12 * i + 9);
+            o.mat = t.mat_id[i];
+            o.light = t.light_id[i];
+        }
+    };
+    tris(in->real, s.real);
+    tris(in->imag, s.imag);
+    s.mats.resize(in->mat_count);
+    for (uint32_t i = 0; i < in->mat_count; i++) {
+        const upbp_material& m = in->materials[i];
+        s.mats[i] = {fromp(m.diffuse), fromp(m.phong), fromp(m.mirror), m.phong_exponent, m.ior, m.priority,
+                     m.real != 0, m.med_idx};
+    }
+    s.media.resize(in->med_count);
+    for (uint32_t i = 0; i < in->med_count; i++) {
+        const upbp_medium& m = in->media[i];
+        init_medium(s.media[i], fromp(m.absorption), fromp(m.emission), fromp(m.scattering), m.continuation_prob,
+                    m.mean_cosine);
+    }
+    s.lights.resize(in->light_count);
+    for (uint32_t i = 0; i < in->light_count; i++) {
+        const upbp_light& l = in->lights[i];
+        Light& o = s.lights[i];
+        o = Light();
+        o.type = l.type;
+        o.mat_id = l.mat_id;
+        o.med_id = l.med_id;
+        o.intensity = fromp(l.intensity);
+        if (l.type == UPBP_LIGHT_AREA) { // host/light.hpp:30-47
+            o.p0 = fromp(l.p0);
+            o.e1 = fromp(l.p1) - o.p0;
+            o.e2 = fromp(l.p2) - o.p0;
+            const V n = cross(o.e1, o.e2);
+            o.inv_area = 2.f / length(n);
+            o.frame.set_from_z(n);
+        } else if (l.type == UPBP_LIGHT_DIRECTIONAL) { // host/light.hpp:60-68
+            o.frame.set_from_z(fromp(l.position));
+        } else if (l.type == UPBP_LIGHT_POINT) {
+            o.position = fromp(l.position);
+        } else {
+            o.bg_color = fromp(l.intensity);
+            o.scale = l.scale;
+        }
+    }
+    s.global_medium = in->global_medium_idx;
+    s.background_light = in->background_light_idx;
+    const upbp_camera& c = in->camera;
+    init_camera(s.cam, fromp(c.origin), fromp(c.target), fromp(c.roll), float(res_x), float(res_y),
+                c.horizontal_fov_deg, c.mat_id, c.med_id);
+    s.center = fromp(in->scene_center);
+    s.radius = in->scene_radius;
+    s.inv_radius_sqr = 1.f / (s.radius * s.radius);
+}
+
+// kernel/medium.cuh:63-115
+float sample_ray(const Medium& m, float dist_to_boundary, float rnd, float* for_pdf, unsigned flags, float* rev_pdf)
+{
+    if (m.min_pos_att != 0.f && m.has_scattering) {
+        const float s = -std::log(rnd) / m.min_pos_att;
+        if (s < dist_to_boundary) {
+            const float att = std::exp(-m.min_pos_att * s);
+            *for_pdf = m.min_pos_att * att;
+            *rev_pdf = (flags & ORIGIN_IN_MEDIUM) ? *for_pdf : att;
+            return s;
+        }
+        const float att = std::exp(-m.min_pos_att * dist_to_boundary);
+        *for_pdf = att;
+        *rev_pdf = (flags & ORIGIN_IN_MEDIUM) ? m.min_pos_att * att : att;
+        return dist_to_boundary;
+    }
+    *for_pdf = 1.f;
+    *rev_pdf = 1.f;
+    return dist_to_boundary;
+}
+
+// kernel/medium.cuh:122-154
+float ray_sample_pdf(const Medium& m, float dmin, float dmax, unsigned flags, float* rev_pdf)
+{
+    float for_pdf = 1.f;
+    if (rev_pdf) *rev_pdf = 1.f;
+    if (m.min_pos_att != 0.f && m.has_scattering) {
+        const float att = std::fmax(std::exp(-m.min_pos_att * (dmax - dmin)), 1e-35f);
+        const float minatt = std::fmax(m.min_pos_att * att, 1e-35f);
+        for_pdf = (flags & END_IN_MEDIUM) ? minatt : att;
+        if (rev_pdf) *rev_pdf = (flags & ORIGIN_IN_MEDIUM) ? minatt : att;
+    }
+    return for_pdf;
+}
+
+// kernel/bsdf.cuh:143-177
+static void component_probabilities(BSDF& b, const Material& m)
+{
+    b.reflect_coeff = fresnel_dielectric(b.local_dir_fix.z, b.ior);
+    const float ad = luminance(m.diffuse);
+    const float ap = luminance(m.phong);
+    const float ar = b.reflect_coeff * luminance(m.mirror);
+    const float at = (1.f - b.reflect_coeff) * (m.ior > 0.f ? 1.f : 0.f);
+    const float total = ad + ap + ar + at;
+    if (total < 1e-9f) {
+        b.diff_prob = b.phong_prob = b.refl_prob = b.refr_prob = 0.f;
+        b.cont_prob = 0.f;
+        return;
+    }
+    b.diff_prob = ad / total;
+    b.phong_prob = ap / total;
+    b.refl_prob = ar / total;
+    b.refr_prob = at / total;
+    const V tmp = m.diffuse + m.phong + b.reflect_coeff * m.mirror;
+    b.cont_prob = maxc(tmp) + (1.f - b.reflect_coeff);
+    b.cont_prob = std::fmin(1.f, std::fmax(0.f, b.cont_prob));
+}
+
+// kernel/bsdf.cuh:179-223
+void bsdf_setup(BSDF& b, V ray_dir, const Isect& is, const Scene& sc, bool from_light, float ior)
+{
+    b = BSDF();
+    b.flags = from_light ? BSDF::FROM_LIGHT : 0;
+    if (is.on_surface()) {
+        const Material& m = sc.mats[size_t(is.mat)];
+        b.mat = is.mat;
+        b.frame.set_from_z(is.shading);
+        b.local_dir_fix = b.frame.to_local(-ray_dir);
+        b.geom_normal = b.frame.to_local(is.normal);
+        b.cos_fix_geom = dot(b.local_dir_fix, b.geom_normal);
+        if (std::fabs(b.local_dir_fix.z) < kEpsCosine || std::fabs(b.cos_fix_geom) < kEpsCosine) {
+            b.flags = 0;
+            return;
+        }
+        b.ior = ior;
+        component_probabilities(b, m);
+        if (b.diff_prob == 0.f && b.phong_prob == 0.f) b.flags |= BSDF::DELTA;
+    } else {
+        b.flags |= BSDF::IN_MEDIUM;
+        const unsigned idx = unsigned(std::fmin(float(is.med), float(sc.media.size() - 1)));
+        const Medium& m = sc.media[idx];
+        b.med = int(idx);
+        b.world_dir_fix = -ray_dir;
+        b.frame.set_from_z(-b.world_dir_fix);
+        b.cont_prob = m.cont_prob;
+        b.mean_cosine = m.mean_cosine;
+        b.scatter_coef = m.scattering;
+        b.scatter_pos = !black_or_neg(m.scattering);
+    }
+    b.flags |= BSDF::VALID;
+}
+
+// kernel/bsdf.cuh:227-249
+static V eval_diffuse(const BSDF& b, const Material& m, V lg, float* fp, float* rp)
+{
+    if (b.diff_prob == 0) return v3(0.f);
+    if (b.local_dir_fix.z < kEpsCosine || lg.z < kEpsCosine) return v3(0.f);
+    if (fp) *fp += b.diff_prob * std::fmax(0.f, lg.z * kInvPi);
+    if (rp) *rp += b.diff_prob * std::fmax(0.f, b.local_dir_fix.z * kInvPi);
+    return m.diffuse * kInvPi;
+}
+
+// kernel/bsdf.cuh:251-284
+static V eval_phong(const BSDF& b, const Material& m, V lg, float* fp, float* rp)
+{
+    if (b.phong_prob == 0) return v3(0.f);
+    if (b.local_dir_fix.z < kEpsCosine || lg.z < kEpsCosine) return v3(0.f);
+    const V r = reflect_local(b.local_dir_fix);
+    const float dr = dot(r, lg);
+    if (dr <= kEpsPhong) return v3(0.f);
+    const float pw = std::pow(dr, m.phong_exp);
+    if (!positive(pw)) return v3(0.f);
+    if (fp || rp) {
+        const float pdf = b.phong_prob * power_cos_hemisphere_pdf_w(r, lg, m.phong_exp);
+        if (fp) *fp += pdf;
+        if (rp) *rp += pdf;
+    }
+    const V rho = m.phong * (m.phong_exp + 2.f) * .5f * kInvPi;
+    return rho * pw;
+}
+
+// kernel/bsdf.cuh:290-326
+static void pdf_diffuse(const BSDF& b, V lg, float* fp, float* rp)
+{
+    if (b.diff_prob == 0) return;
+    if (fp) *fp += b.diff_prob * std::fmax(0.f, lg.z * kInvPi);
+    if (rp) *rp += b.diff_prob * std::fmax(0.f, b.local_dir_fix.z * kInvPi);
+}
+static void pdf_phong(const BSDF& b, const Material& m, V lg, float* fp, float* rp)
+{
+    if (b.phong_prob == 0) return;
+    const V r = reflect_local(b.local_dir_fix);
+    const float dr = dot(r, lg);
+    if (dr <= kEpsPhong) return;
+    const float pdf = power_cos_hemisphere_pdf_w(r, lg, m.phong_exp) * b.phong_prob;
+    if (fp) *fp += pdf;
+    if (rp) *rp += pdf;
+}
+
+// kernel/bsdf.cuh:334-396
+V bsdf_evaluate(const BSDF& b, V dir_gen, const Scene& sc, float& cos_gen, float* for_pdf, float* rev_pdf,
+                float* sin_theta)
+{
+    if (b.on_surface()) {
+        V res = v3(0.f);
+        if (for_pdf) *for_pdf = 0.f;
+        if (rev_pdf) *rev_pdf = 0.f;
+        if (sin_theta) *sin_theta = 0.f;
+        const V lg = b.frame.to_local(dir_gen);
+        const float cg = lg.z;
+        const float cgg = dot(lg, b.geom_normal);
+        cos_gen = std::fabs(cg);
+        if (cos_gen < kEpsCosine || std::fabs(cgg) < kEpsCosine) return res;
+        if (cg * b.local_dir_fix.z < 0.f || cgg * b.cos_fix_geom < 0.f) return res;
+        const Material& m = sc.mats[size_t(b.mat)];
+        res += eval_diffuse(b, m, lg, for_pdf, rev_pdf);
+        res += eval_phong(b, m, lg, for_pdf, rev_pdf);
+        return res;
+    }
+    cos_gen = 1.f;
+    if (!b.scatter_pos) {
+        if (for_pdf) *for_pdf = 0.f;
+        if (rev_pdf) *rev_pdf = 0.f;
+        if (sin_theta) *sin_theta = 0.f;
+        return v3(0.f);
+    }
+    const float p = hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, sin_theta);
+    if (for_pdf) *for_pdf = p;
+    if (rev_pdf) *rev_pdf = p;
+    return b.scatter_coef * p;
+}
+
+// kernel/bsdf.cuh:402-447
+float bsdf_pdf(const BSDF& b, V dir_gen, const Scene& sc, bool reverse)
+{
+    if (b.on_surface()) {
+        const V lg = b.frame.to_local(dir_gen);
+        const float cg = lg.z;
+        const float cgg = dot(lg, b.geom_normal);
+        if (std::fabs(cg) < kEpsCosine || std::fabs(cgg) < kEpsCosine) return 0.f;
+        if (cg * b.local_dir_fix.z < 0.f || cgg * b.cos_fix_geom < 0.f) return 0.f;
+        float fp = 0.f, rp = 0.f;
+        const Material& m = sc.mats[size_t(b.mat)];
+        pdf_diffuse(b, lg, &fp, &rp);
+        pdf_phong(b, m, lg, &fp, &rp);
+        return reverse ? rp : fp;
+    }
+    return hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, nullptr);
+}
+
+// kernel/bsdf.cuh:455-559 (component samplers)
+static V sample_diffuse(const BSDF& b, const Material& m, float u0, float u1, V& lg, float& pdf)
+{
+    if (b.local_dir_fix.z < kEpsCosine) return v3(0.f);
+    float up;
+    lg = sample_cos_hemisphere_w(u0, u1, &up);
+    pdf += up * b.diff_prob;
+    return m.diffuse * kInvPi;
+}
+static V sample_phong(const BSDF& b, const Material& m, float u0, float u1, V& lg, float& pdf)
+{
+    lg = sample_power_cos_hemisphere_w(u0, u1, m.phong_exp);
+    const V r = reflect_local(b.local_dir_fix);
+    {
+        Frame f;
+        f.set_from_z(r);
+        lg = f.to_world(lg);
+    }
+    const float dr = dot(r, lg);
+    if (dr <= kEpsPhong) return v3(0.f);
+    pdf_phong(b, m, lg, &pdf, nullptr);
+    const V rho = m.phong * (m.phong_exp + 2.f) * .5f * kInvPi;
+    return rho * std::pow(dr, m.phong_exp);
+}
+static V sample_reflect(const BSDF& b, const Material& m, V& lg, float& pdf)
+{
+    lg = reflect_local(b.local_dir_fix);
+    pdf += b.refl_prob;
+    return b.reflect_coeff * m.mirror / std::fabs(lg.z);
+}
+static V sample_refract(const BSDF& b, V& lg, float& pdf)
+{
+    if (b.ior < 0) return v3(0.f);
+    float cos_i = b.local_dir_fix.z;
+    float cos_t;
+    const float eta = b.ior;
+    if (cos_i < 0.f) {
+        cos_i = -cos_i;
+        cos_t = 1.f;
+    } else {
+        cos_t = -1.f;
+    }
+    const float sin_i2 = 1.f - cos_i * cos_i;
+    const float sin_t2 = eta * eta * sin_i2;
+    if (sin_t2 < 1.f) {
+        cos_t *= std::sqrt(std::fmax(0.f, 1.f - sin_t2));
+        lg = {-eta * b.local_dir_fix.x, -eta * b.local_dir_fix.y, cos_t};
+        pdf += b.refr_prob;
+        const float rc = 1.f - b.reflect_coeff;
+        if (!b.from_light()) return v3(rc * eta * eta / std::fabs(cos_t));
+        return v3(rc / std::fabs(cos_t));
+    }
+    return v3(0.f);
+}
+
+// kernel/bsdf.cuh:567-660
+V bsdf_sample(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pdf, float& cos_gen,
+              const Scene& sc, unsigned& event, float& sin_theta)
+{
+    if (b.on_surface()) {
+        unsigned ev;
+        if (u2 < b.diff_prob) ev = BSDF::DIFFUSE;
+        else if (u2 < b.diff_prob + b.phong_prob) ev = BSDF::PHONG;
+        else if (u2 < b.diff_prob + b.phong_prob + b.refl_prob) ev = BSDF::REFLECT;
+        else ev = BSDF::REFRACT;
+        event = ev;
+        sin_theta = 0.f;
+        pdf = 0.f;
+        V res = v3(0.f);
+        V lg{0, 0, 0};
+        const Material& m = sc.mats[size_t(b.mat)];
+        if (ev == BSDF::DIFFUSE) {
+            res += sample_diffuse(b, m, u0, u1, lg, pdf);
+            if (black_or_neg(res)) return v3(0.f);
+            res += eval_phong(b, m, lg, &pdf, nullptr);
+        } else if (ev == BSDF::PHONG) {
+            res += sample_phong(b, m, u0, u1, lg, pdf);
+            if (black_or_neg(res)) return v3(0.f);
+            res += eval_diffuse(b, m, lg, &pdf, nullptr);
+        } else if (ev == BSDF::REFLECT) {
+            res += sample_reflect(b, m, lg, pdf);
+            if (black_or_neg(res)) return v3(0.f);
+        } else {
+            res += sample_refract(b, lg, pdf);
+            if (black_or_neg(res)) return v3(0.f);
+        }
+        dir_gen = b.frame.to_world(lg);
+        lg = b.frame.to_local(dir_gen);
+        const float cg = lg.z;
+        const float cgg = dot(lg, b.geom_normal);
+        cos_gen = std::fabs(cg);
+        if (cos_gen < kEpsCosine || std::fabs(cgg) < kEpsCosine) return v3(0.f);
+        const bool refr = ev == BSDF::REFRACT;
+        if ((refr && cg * b.local_dir_fix.z > 0.f) || (!refr && cg * b.local_dir_fix.z < 0.f) ||
+            (refr && cgg * b.cos_fix_geom > 0.f) || (!refr && cgg * b.cos_fix_geom < 0.f))
+            return v3(0.f);
+        return res;
+    }
+    cos_gen = 1.f;
+    event = BSDF::SCATTER;
+    if (!b.scatter_pos) {
+        dir_gen = v3(0.f);
+        pdf = 0.f;
+        sin_theta = 0.f;
+        return v3(0.f);
+    }
+    dir_gen = hg_sample(b.world_dir_fix, b.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
+    return b.scatter_coef * pdf;
+}
+
+// ---- lights: kernel/light.cuh:41-543 ---------------------------------------
+V light_illuminate(const Light& l, const Scene& sc, V recv, float u0, float u1, V& dir, float& dist, float& for_pdf_w,
+                   float* emission_pdf_w, float* cos_at_light)
+{
+    switch (l.type) {
+    case L_AREA: { // :43-72
+        float b0, b1;
+        sample_uniform_triangle(u0, u1, b0, b1);
+        const V p = l.p0 + l.e1 * b0 + l.e2 * b1;
+        dir = p - recv;
+        const float d2 = dot(dir, dir);
+        dist = std::sqrt(d2);
+        dir = dir / dist;
+        const float cn = dot(l.frame.z, -dir);
+        if (cn < kEpsCosine) return v3(0.f);
+        for_pdf_w = l.inv_area * d2 / cn;
+        if (cos_at_light) *cos_at_light = cn;
+        if (emission_pdf_w) *emission_pdf_w = l.inv_area * cn * kInvPi;
+        return l.intensity;
+    }
+    case L_BACKGROUND: { // :128-151 (constant colour only)
+        dir = sample_uniform_sphere_w(u0, u1, &for_pdf_w);
+        const V rad = l.bg_color * l.scale;
+        dist = kInf;
+        if (emission_pdf_w) *emission_pdf_w = for_pdf_w * concentric_disc_pdf_a() * sc.inv_radius_sqr;
+        if (cos_at_light) *cos_at_light = 1.f;
+        return rad;
+    }
+    case L_DIRECTIONAL: { // :230-248
+        dir = -l.frame.z;
+        dist = kInf;
+        for_pdf_w = 1.f;
+        if (cos_at_light) *cos_at_light = 1.f;
+        if (emission_pdf_w) *emission_pdf_w = concentric_disc_pdf_a() * sc.inv_radius_sqr;
+        return l.intensity;
+    }
+    default: { // point, :301-317
+        dir = l.position - recv;
+        const float d2 = dot(dir, dir);
+        for_pdf_w = d2;
+        dist = std::sqrt(d2);
+        dir = dir / dist;
+        if (cos_at_light) *cos_at_light = 1.f;
+        if (emission_pdf_w) *emission_pdf_w = uniform_sphere_pdf_w();
+        return l.intensity;
+    }
+    }
+}
+
+V light_emit(const Light& l, const Scene& sc, float d0, float d1, float p0, float p1, V& pos, V& dir,
+             float& emission_pdf_w, float* for_pdf_a, float* cos_light)
+{
+    switch (l.type) {
+    case L_AREA: { // :74-99
+        float b0, b1;
+        sample_uniform_triangle(p0, p1, b0, b1);
+        pos = l.p0 + l.e1 * b0 + l.e2 * b1;
+        V lo = sample_cos_hemisphere_w(d0, d1, &emission_pdf_w);
+        emission_pdf_w *= l.inv_area;
+        lo.z = std::fmax(lo.z, kEpsCosine);
+        dir = l.frame.to_world(lo);
+        if (for_pdf_a) *for_pdf_a = l.inv_area;
+        if (cos_light) *cos_light = lo.z;
+        return l.intensity * lo.z;
+    }
+    case L_BACKGROUND: { // :153-190
+        float fp;
+        dir = sample_uniform_sphere_w(d0, d1, &fp);
+        const V rad = l.bg_color * l.scale;
+        float x, y;
+        sample_concentric_disc(p0, p1, x, y);
+        Frame f;
+        f.set_from_z(dir);
+        pos = sc.center + sc.radius * (-dir + f.x * x + f.y * y);
+        emission_pdf_w = fp * concentric_disc_pdf_a() * sc.inv_radius_sqr;
+        if (for_pdf_a) *for_pdf_a = fp;
+        if (cos_light) *cos_light = 1.f;
+        return rad;
+    }
+    case L_DIRECTIONAL: { // :250-272
+        float x, y;
+        sample_concentric_disc(p0, p1, x, y);
+        pos = sc.center + sc.radius * (-l.frame.z + l.frame.x * x + l.frame.y * y);
+        dir = l.frame.z;
+        emission_pdf_w = concentric_disc_pdf_a() * sc.inv_radius_sqr;
+        if (for_pdf_a) *for_pdf_a = 1.f;
+        if (cos_light) *cos_light = 1.f;
+        return l.intensity;
+    }
+    default: { // point, :319-337
+        pos = l.position;
+        dir = sample_uniform_sphere_w(d0, d1, &emission_pdf_w);
+        if (for_pdf_a) *for_pdf_a = 1.f;
+        if (cos_light) *cos_light = 1.f;
+        return l.intensity;
+    }
+    }
+}
+
+V light_radiance(const Light& l, const Scene& sc, V ray_dir, V hit, float* for_pdf_a, float* emission_pdf_w)
+{
+    (void)hit;
+    switch (l.type) {
+    case L_AREA: { // :101-122
+        const float c = std::fmax(0.f, dot(l.frame.z, -ray_dir));
+        if (c == 0.f) return v3(0.f);
+        if (for_pdf_a) *for_pdf_a = l.inv_area;
+        if (emission_pdf_w) *emission_pdf_w = cos_hemisphere_pdf_w(l.frame.z, -ray_dir) * l.inv_area;
+        return l.intensity;
+    }
+    case L_BACKGROUND: { // :192-217
+        const float fp = uniform_sphere_pdf_w();
+        const V rad = l.bg_color * l.scale;
+        const float pos_pdf = concentric_disc_pdf_a() * sc.inv_radius_sqr;
+        if (for_pdf_a) *for_pdf_a = fp;
+        if (emission_pdf_w) *emission_pdf_w = fp * pos_pdf;
+        return rad;
+    }
+    default: return v3(0.f);
+    }
+}
+
+} // namespace orc

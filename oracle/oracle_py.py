@@ -1,0 +1,118 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the CPU baseline.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+_lib = None
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("light_vertices", C.c_uint64), ("surface_vertices", C.c_uint64),
+                ("medium_vertices", C.c_uint64), ("beams", C.c_uint64), ("path_errors", C.c_uint64),
+                ("seconds", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        h = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        h.oracle_render.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(OracleStats)]
+        h.oracle_render_sample.argtypes = [vp, vp, C.c_int, C.c_uint32, C.c_int, C.POINTER(C.c_double)]
+        h.oracle_range_query.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, C.c_float, vp, vp, C.c_uint64,
+                                         C.POINTER(C.c_uint64)]
+        h.oracle_closest_hit.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
+        h.oracle_camera_ray.argtypes = [vp, vp, vp, C.c_uint32, vp]
+        h.oracle_world_to_raster.argtypes = [vp, vp, vp, C.c_uint32, vp]
+        h.oracle_hg_pdf.argtypes = [C.c_float, C.c_float]
+        h.oracle_hg_pdf.restype = C.c_float
+        h.oracle_hg_sample.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        h.oracle_fresnel.argtypes = [C.c_float, C.c_float]
+        h.oracle_fresnel.restype = C.c_float
+        h.oracle_philox.argtypes = [vp, vp, vp]
+        _lib = h
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def render(scene_ptr, cfg, first_iteration=0, n_iterations=1, threads=0, want_fb=False):
+    """Mean FP64 image (H, W, 3) over n_iterations; optionally the FP32 framebuffer of the first."""
+    out = np.zeros((cfg.res_y, cfg.res_x, 3), np.float64)
+    fb = np.zeros((cfg.res_y, cfg.res_x, 3), np.float32) if want_fb else None
+    st = OracleStats()
+    rc = lib().oracle_render(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), first_iteration,
+                             n_iterations, threads, _p(out), _p(fb) if want_fb else None, C.byref(st))
+    if rc != 0:
+        raise RuntimeError("oracle_render failed: %d" % rc)
+    return (out, fb, st) if want_fb else (out, st)
+
+
+def render_sample_seconds(scene_ptr, cfg, n_camera_paths, threads=0, iteration=0):
+    s = C.c_double()
+    rc = lib().oracle_render_sample(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), iteration,
+                                    n_camera_paths, threads, C.byref(s))
+    if rc != 0:
+        raise RuntimeError("oracle_render_sample failed: %d" % rc)
+    return s.value
+
+
+def range_query(kind, prims, queries, radius, max_indices=None):
+    prims = np.ascontiguousarray(prims, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    nq = queries.shape[0]
+    npr = prims.shape[0]
+    cap = max_indices if max_indices is not None else max(1, nq * npr)
+    offsets = np.zeros(nq + 1, np.uint32)
+    idx = np.zeros(cap, np.uint32)
+    n = C.c_uint64()
+    rc = lib().oracle_range_query(kind, _p(prims), npr, _p(queries), nq, radius, _p(offsets), _p(idx), cap,
+                                  C.byref(n))
+    if rc != 0:
+        raise RuntimeError("oracle_range_query failed: %d" % rc)
+    return offsets, idx[: n.value]
+
+
+def closest_hit(verts, rays):
+    verts = np.ascontiguousarray(verts, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    t = np.zeros(rays.shape[0], np.float32)
+    prim = np.zeros(rays.shape[0], np.int32)
+    lib().oracle_closest_hit(_p(verts), verts.shape[0], _p(rays), rays.shape[0], _p(t), _p(prim))
+    return t, prim
+
+
+def camera_rays(scene_ptr, cfg, raster_xy):
+    raster_xy = np.ascontiguousarray(raster_xy, np.float32)
+    d = np.zeros((raster_xy.shape[0], 3), np.float32)
+    lib().oracle_camera_ray(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), _p(raster_xy),
+                            raster_xy.shape[0], _p(d))
+    return d
+
+
+def world_to_raster(scene_ptr, cfg, pts):
+    pts = np.ascontiguousarray(pts, np.float32)
+    r = np.zeros((pts.shape[0], 2), np.float32)
+    lib().oracle_world_to_raster(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), _p(pts),
+                                 pts.shape[0], _p(r))
+    return r
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().oracle_philox(_p(c), _p(k), _p(o))
+    return o
