@@ -1,0 +1,146 @@
+"""Benchmark: UPBP renderer iteration on one MI355X.
+
+Workload (the plain-UPBP-at-scale configuration): seeded grid of 64 noise-displaced
+icosphere blobs (1,310,720 triangles) on a floor, three overlapping homogeneous media
+spheres with priorities 1-3, two quad lights and a point light, 1280x720, max path
+length 12, 900,000 light paths per iteration, radius 0.005, upbp_all
+(BPT + surface PM + P-P3D + P-B2D + B-B1D), FP32 transport, FP64 accumulation.
+
+One step = one 1-spp iteration (light pass, photon LBVH builds, camera pass, FP64
+accumulate) with the scene resident in HBM. Prints one JSON line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "samples/s per 1-spp iter + equal-time relMSE, learned vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+RES = (1280, 720)
+ARGS = ["-r", "%dx%d" % RES, "-l", "12", "-pcpi", "900000", "-r_initial", "0.005", "-a", "upbp_all"]
+
+
+def make_scene(workdir):
+    from gpuupbp_amd import capi
+    obj = os.path.join(workdir, "blobs.obj")
+    if not os.path.exists(obj + ".aux"):
+        capi.generate_scene("blobs", 1, obj)
+    return obj
+
+
+def cpu_baseline(scene_ptr, cfg):
+    """Oracle (scalar FP32 port of the reference estimators) on the host cores: the full light pass
+    and photon-map build, plus an evenly spread sample of camera paths whose time is scaled to
+    the full frame."""
+    from oracle import oracle_py
+    threads = min(16, os.cpu_count() or 1)
+    n_cam = 16384
+    t = oracle_py.render_sample_times(scene_ptr, cfg, n_cam, int(cfg.path_count_per_iter), threads)
+    npx = cfg.res_x * cfg.res_y
+    t_iter = t[0] + t[1] + t[2] * (npx / float(n_cam))
+    return {
+        "value": npx / t_iter,
+        "unit": "samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "one iteration: all %d light paths + photon maps (%.1f s + %.1f s), camera pass on %d of %d "
+                  "pixels (%.1f s) scaled to the frame" % (int(cfg.path_count_per_iter), t[0], t[1], n_cam, npx, t[2]),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workdir", default=None)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: the iteration-sharded multi-GPU path is not built", file=sys.stderr)
+        sys.exit(2)
+
+    from gpuupbp_amd import capi
+    from gpuupbp_amd.renderer import Renderer
+
+    workdir = a.workdir or tempfile.mkdtemp(prefix="upbp_bench_")
+    os.makedirs(workdir, exist_ok=True)
+    obj = make_scene(workdir)
+    rc, cfg = capi.parse_args(["-s", "-1", obj] + ARGS)
+    assert rc == 0
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+
+    r = Renderer(sh.ptr, cfg)
+    for i in range(a.warmup):
+        r.run_iteration(i, want_stats=False)
+    r.sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        r.run_iteration(a.warmup + i, want_stats=False)
+    r.sync()
+    t1 = time.perf_counter()
+    # one more iteration with device counters and per-kernel HIP events (untimed)
+    st = r.run_iteration(a.warmup + a.steps, want_stats=True)
+    r.close()
+
+    ms = (t1 - t0) * 1e3 / a.steps
+    npx = cfg.res_x * cfg.res_y
+    launches = max(1, st.dominant_launches)
+    dom_ms = st.ms_dominant / launches
+    achieved = (st.dominant_bytes / launches) / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": npx / (ms * 1e-3),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {
+            "workload": "blobs: 64 noise-displaced icospheres (1,310,720 tris), 3 media spheres, 2 quad + 1 point "
+                        "light, 1280x720, l=12, 900k light paths/iter, r=0.005, upbp_all",
+            "resolution": "%dx%d" % RES,
+            "max_path_length": 12,
+            "light_paths": int(cfg.path_count_per_iter),
+            "parallelism": "single GPU",
+        },
+        "paths_per_s": (npx + cfg.path_count_per_iter) / (ms * 1e-3),
+        "queries_per_s": (st.point_queries + st.segment_queries) / (ms * 1e-3),
+        "iteration_breakdown_ms": {"light": st.ms_light, "build": st.ms_build, "camera": st.ms_camera},
+        "counters": {"light_steps": st.light_extend_steps, "camera_steps": st.camera_extend_steps,
+                     "surface_points": st.surface_vertices, "medium_points": st.medium_vertices, "beams": st.beams,
+                     "bpt_connections": st.bpt_connections, "merge_accepts": st.merge_accepts,
+                     "path_errors": st.path_errors, "beam_drops": st.beam_drops},
+        "roofline": {
+            "kernel": "k_camera_bounce",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": dom_ms,
+            "algorithmic_bytes_per_launch": st.dominant_bytes / launches,
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sh.ptr, cfg)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    sh.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

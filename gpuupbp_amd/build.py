@@ -29,7 +29,7 @@ HIP_SOURCES = ["device/renderer.hip", "device/lbvh.hip", "device/light_pass.hip"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include")]
 HIP_FLAGS = COMMON + ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                      "-munsafe-fp-atomics", "-fgpu-flush-denormals-to-zero=false"]
+                      "-munsafe-fp-atomics", "-ffp-contract=off"]
 
 
 def run(cmd):

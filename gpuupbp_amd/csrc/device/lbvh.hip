@@ -1,0 +1,431 @@
+// LBVH construction on gfx950: centroid bounds -> 30-bit Morton codes -> stable
+// LSD radix sort (8-bit digits, LDS histograms, wave64 ballot ranking) -> Karras
+// (HPG 2012) hierarchy -> bottom-up box refit with one atomic flag per node.
+// Replaces the per-iteration OptiX GAS/IAS builds of nolmoonen/gpuupbp
+// gpuupbp/renderer/renderer.cpp:201-307 and the scene GAS build of
+// gpuupbp/renderer/renderer_init.cpp:31-289.
+#include "lbvh.h"
+
+#include <algorithm>
+
+using namespace upbp;
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t f2ord(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u)
+{
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// centroid bounds: bounds[0..2] = ordered min, bounds[3..5] = ordered max
+__global__ void k_centroid_bounds(const float4* lo, const float4* hi, uint32_t n, uint32_t* bounds)
+{
+    __shared__ float smin[3][256], smax[3][256];
+    const uint32_t t = threadIdx.x;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x) {
+        const float4 a = lo[i], b = hi[i];
+        const float c[3] = {(a.x + b.x) * .5f, (a.y + b.y) * .5f, (a.z + b.z) * .5f};
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fminf(mn[k], c[k]);
+            mx[k] = fmaxf(mx[k], c[k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        smin[k][t] = mn[k];
+        smax[k][t] = mx[k];
+    }
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 3; k++) {
+                smin[k][t] = fminf(smin[k][t], smin[k][t + s]);
+                smax[k][t] = fmaxf(smax[k][t], smax[k][t + s]);
+            }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&bounds[k], f2ord(smin[k][0]));
+            atomicMax(&bounds[3 + k], f2ord(smax[k][0]));
+        }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(const float4* lo, const float4* hi, uint32_t n, const uint32_t* bounds, uint32_t* keys,
+                         uint32_t* vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = lo[i], b = hi[i];
+    const float c[3] = {(a.x + b.x) * .5f, (a.y + b.y) * .5f, (a.z + b.z) * .5f};
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        const float l = ord2f(bounds[k]), h = ord2f(bounds[3 + k]);
+        const float ext = h - l;
+        const float u = ext > 0.f ? (c[k] - l) / ext : 0.5f;
+        q[k] = uint32_t(fminf(fmaxf(u * 1024.f, 0.f), 1023.f));
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = i;
+}
+
+// ---- radix sort ----------------------------------------------------------------
+__global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
+                                                             uint32_t* hist, uint32_t nblocks)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: element order within a tile is preserved (round r, then thread id).
+__global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* keys_in, const uint32_t* vals_in,
+                                                                uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                                                int shift, const uint32_t* offsets, uint32_t nblocks)
+{
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wave_hist[kWaves][256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    running[t] = offsets[t * nblocks + blockIdx.x];
+    for (int w = 0; w < kWaves; w++) wave_hist[w][t] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t i = base + r * kSortThreads + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        const uint32_t d = (key >> shift) & 0xffu;
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            match &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = __popcll(match & lt_mask);
+        const bool leader = valid && (match & lt_mask) == 0ull;
+        if (leader) wave_hist[wave][d] = __popcll(match);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = running[d] + rank;
+            for (uint32_t w = 0; w < wave; w++) off += wave_hist[w][d];
+            keys_out[off] = key;
+            vals_out[off] = val;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for (int w = 0; w < kWaves; w++) {
+            tot += wave_hist[w][t];
+            wave_hist[w][t] = 0;
+        }
+        running[t] += tot;
+        __syncthreads();
+    }
+}
+
+// ---- exclusive scan (reduce, scan of block sums, downsweep) ----------------------
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* sums)
+{
+    __shared__ uint32_t s[kScanThreads];
+    uint32_t acc = 0;
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) acc += in[base + k];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t st = kScanThreads / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+}
+
+// Scans one tile per block; adds block_offsets[blockIdx] if given. Writes the grand total of
+// block 0's tile into *total when there is a single block.
+__global__ void __launch_bounds__(kScanThreads) k_scan_tile(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                            const uint32_t* block_offsets, uint32_t* total)
+{
+    __shared__ uint32_t s[kScanThreads];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t acc = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        acc += v[k];
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over thread sums
+    for (uint32_t st = 1; st < kScanThreads; st <<= 1) {
+        const uint32_t x = threadIdx.x >= st ? s[threadIdx.x - st] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = (threadIdx.x ? s[threadIdx.x - 1] : 0u) + (block_offsets ? block_offsets[blockIdx.x] : 0u);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (total && gridDim.x == 1 && threadIdx.x == kScanThreads - 1) *total = s[kScanThreads - 1];
+}
+
+// ---- Karras hierarchy -----------------------------------------------------------
+__device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = keys[i], b = keys[j];
+    if (a == b) return 32 + __clz(uint32_t(i) ^ uint32_t(j));
+    return __clz(a ^ b);
+}
+
+__global__ void k_karras(const uint32_t* keys, int n, BNode* nodes, int* parent)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int div = 2;
+    int t = (l + div - 1) / div;
+    for (;;) {
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+        div <<= 1;
+        t = (l + div - 1) / div;
+    }
+    const int g = i + s * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    // parent[] indexes inner nodes as k and leaves as (n - 1) + leaf
+    const int c0 = (lo == g) ? ~g : g;
+    const int c1 = (hi == g + 1) ? ~(g + 1) : g + 1;
+    nodes[i].c0 = c0;
+    nodes[i].c1 = c1;
+    parent[c0 < 0 ? (n - 1) + ~c0 : c0] = i;
+    parent[c1 < 0 ? (n - 1) + ~c1 : c1] = i;
+}
+
+__device__ __forceinline__ void write_child_box(BNode* nd, int slot, float4 a, float4 b)
+{
+    if (slot == 0) {
+        nd->lx0 = a.x;
+        nd->ly0 = a.y;
+        nd->lz0 = a.z;
+        nd->ux0 = b.x;
+        nd->uy0 = b.y;
+        nd->uz0 = b.z;
+    } else {
+        nd->lx1 = a.x;
+        nd->ly1 = a.y;
+        nd->lz1 = a.z;
+        nd->ux1 = b.x;
+        nd->uy1 = b.y;
+        nd->uz1 = b.z;
+    }
+}
+
+__global__ void k_refit(const float4* lo, const float4* hi, const uint32_t* order, int n, BNode* nodes,
+                        const int* parent, uint32_t* flags)
+{
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= n) return;
+    const uint32_t prim = order[leaf];
+    float4 a = lo[prim], b = hi[prim];
+    int child = ~leaf;
+    int node = parent[(n - 1) + leaf];
+    // a Karras tree over n leaves is at most n - 1 levels deep; the bound keeps every wave finite
+    for (int level = 0; level < n; level++) {
+        BNode* nd = &nodes[node];
+        const int slot = (nd->c0 == child) ? 0 : 1;
+        write_child_box(nd, slot, a, b);
+        __threadfence();
+        if (atomicAdd(&flags[node], 1u) == 0u) return; // sibling not finished yet
+        __threadfence();
+        const volatile BNode* vn = nd;
+        a = make_float4(fminf(vn->lx0, vn->lx1), fminf(vn->ly0, vn->ly1), fminf(vn->lz0, vn->lz1), 0.f);
+        b = make_float4(fmaxf(vn->ux0, vn->ux1), fmaxf(vn->uy0, vn->uy1), fmaxf(vn->uz0, vn->uz1), 0.f);
+        if (node == 0) return;
+        child = node;
+        node = parent[node];
+    }
+}
+
+// single primitive: one inner node with leaf 0 on the left and an empty right box
+__global__ void k_single(const float4* lo, const float4* hi, BNode* nodes, uint32_t* order)
+{
+    BNode nd;
+    nd.lx0 = lo[0].x;
+    nd.ly0 = lo[0].y;
+    nd.lz0 = lo[0].z;
+    nd.ux0 = hi[0].x;
+    nd.uy0 = hi[0].y;
+    nd.uz0 = hi[0].z;
+    // the unused right slot gets a point box far outside any scene: an inverted box would pass
+    // the slab test, which is symmetric in the two planes of each axis
+    nd.lx1 = nd.ly1 = nd.lz1 = 3e38f;
+    nd.ux1 = nd.uy1 = nd.uz1 = 3e38f;
+    nd.c0 = ~0;
+    nd.c1 = ~0;
+    nd.pad0 = nd.pad1 = 0;
+    nodes[0] = nd;
+    order[0] = 0;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t tpb) { return uint32_t((n + tpb - 1) / tpb); }
+
+} // namespace
+
+namespace upbp_host {
+
+// scan_exclusive uses sc.scan as a stack of block-sum arrays, starting at word `base`
+static int scan_rec(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev, BvhScratch& sc, size_t base,
+                    hipStream_t s)
+{
+    if (n == 0) {
+        if (total_dev) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, total_dev, 0u, 1u);
+        return int(hipGetLastError());
+    }
+    const uint32_t nb = blocks_for(n, kScanTile);
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(kScanThreads), 0, s, in, out, n, nullptr, total_dev);
+        return int(hipGetLastError());
+    }
+    uint32_t* sums = sc.scan.as<uint32_t>() + base;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, n, sums);
+    int e = scan_rec(sums, sums, nb, total_dev, sc, base + nb + 1, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(kScanThreads), 0, s, in, out, n, sums, nullptr);
+    return int(hipGetLastError());
+}
+
+int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev, BvhScratch& sc, hipStream_t s)
+{
+    size_t need = 16;
+    for (uint64_t m = n; m > uint64_t(kScanTile); m = blocks_for(m, kScanTile)) need += blocks_for(m, kScanTile) + 1;
+    if (sc.scan.ensure(need * 4)) return 1;
+    return scan_rec(in, out, n, total_dev, sc, 0, s);
+}
+
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhScratch& sc, hipStream_t s)
+{
+    if (n <= 1) return 0;
+    const uint32_t nb = blocks_for(n, kSortTile);
+    if (sc.keys2.ensure(size_t(n) * 4) || sc.vals2.ensure(size_t(n) * 4) || sc.hist.ensure(size_t(256) * nb * 4))
+        return 1;
+    uint32_t* hist = sc.hist.as<uint32_t>();
+    uint32_t *ka = keys, *va = vals, *kb = sc.keys2.as<uint32_t>(), *vb = sc.vals2.as<uint32_t>();
+    int passes = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ka, n, shift, hist, nb);
+        const int e = scan_exclusive(hist, hist, 256 * nb, nullptr, sc, s);
+        if (e) return e;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, hist, nb);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+        passes++;
+    }
+    if (passes & 1) {
+        if (hipMemcpyAsync(keys, ka, size_t(n) * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 1;
+        if (hipMemcpyAsync(vals, va, size_t(n) * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 1;
+    }
+    return int(hipGetLastError());
+}
+
+int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s)
+{
+    st.n = n;
+    if (n == 0) return 0;
+    const uint32_t inner = n > 1 ? n - 1 : 1;
+    if (st.cap < n) {
+        lbvh_free(st);
+        const uint32_t c = n + n / 4;
+        if (hipMalloc(&st.nodes, size_t(std::max<uint32_t>(c - 1, inner)) * sizeof(BNode)) != hipSuccess) return 1;
+        if (hipMalloc(&st.order, size_t(c) * sizeof(uint32_t)) != hipSuccess) return 1;
+        st.cap = c;
+    }
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, lo, hi, st.nodes, st.order);
+        return int(hipGetLastError());
+    }
+    if (sc.keys.ensure(size_t(n) * 4) || sc.parent.ensure(size_t(2) * n * 4) || sc.flags.ensure(size_t(n) * 4) ||
+        sc.bounds.ensure(64))
+        return 1;
+    uint32_t* bounds = sc.bounds.as<uint32_t>();
+    uint32_t* keys = sc.keys.as<uint32_t>();
+    const uint32_t tpb = 256;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(3), 0, s, bounds, 0xffffffffu, 3u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(3), 0, s, bounds + 3, 0u, 3u);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<uint32_t>(blocks_for(n, tpb), 1024)), dim3(tpb), 0, s, lo, hi,
+                       n, bounds);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, tpb)), dim3(tpb), 0, s, lo, hi, n, bounds, keys, st.order);
+    const int e = radix_sort_pairs(keys, st.order, n, 30, sc, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, keys, int(n), st.nodes,
+                       sc.parent.as<int>());
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, sc.flags.as<uint32_t>(), 0u, n - 1);
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, tpb)), dim3(tpb), 0, s, lo, hi, st.order, int(n), st.nodes,
+                       sc.parent.as<int>(), sc.flags.as<uint32_t>());
+    return int(hipGetLastError());
+}
+
+void lbvh_free(BvhStore& st)
+{
+    if (st.nodes) (void)hipFree(st.nodes);
+    if (st.order) (void)hipFree(st.order);
+    st.nodes = nullptr;
+    st.order = nullptr;
+    st.cap = 0;
+}
+
+void lbvh_scratch_free(BvhScratch& sc)
+{
+    for (DevBuf* b : {&sc.keys, &sc.keys2, &sc.vals2, &sc.hist, &sc.scan, &sc.parent, &sc.flags, &sc.bounds})
+        b->release();
+}
+
+} // namespace upbp_host

@@ -1,0 +1,693 @@
+// Host orchestration of one UPBP iteration on the GPU and the C ABI around it.
+//
+// upbp_create replaces Renderer::init (nolmoonen/gpuupbp gpuupbp/renderer/renderer_init.cpp:675-929):
+// scene tables go to HBM, the real and imaginary triangle LBVHs are built once, and the
+// wavefront buffers are sized from N_L (max path length - 1) vertex slots per light path.
+// upbp_run_iteration replaces Renderer::run_iteration (gpuupbp/renderer/renderer.cpp:37-418):
+// radius schedule and MIS factors (:56-119), light wavefront, deterministic photon lists +
+// three photon LBVH builds, camera wavefront, FP64 accumulation on the device.
+#include "kernels.h"
+
+#include "../../../include/upbp_capi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace upbp;
+using namespace upbp_host;
+
+namespace {
+
+constexpr int kMaxBounces = 64;
+
+__global__ void k_fill(uint32_t* p, uint32_t v, size_t n)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// per light path: how many surface points, medium points and beams it contributes
+__global__ void k_count_photons(IterParams p, Bufs b, uint32_t* cs, uint32_t* cm, uint32_t* cb, int want_pts,
+                                int want_beams)
+{
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= p.n_light) return;
+    uint32_t ns = 0, nm = 0;
+    if (want_pts) {
+        const uint32_t nv = b.nvert[pid];
+        for (uint32_t k = 0; k < nv; k++) {
+            const int fl = __float_as_int(b.verts[size_t(pid) * p.vstride + k].g.z);
+            if (fl & VF_IN_MEDIUM) nm++;
+            else ns++;
+        }
+    }
+    cs[pid] = ns;
+    cm[pid] = nm;
+    cb[pid] = want_beams ? b.nbeam[pid] : 0u;
+}
+
+// writes the photon lists in light-path order together with inflated primitive boxes
+__global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const uint32_t* om, const uint32_t* ob,
+                               int want_surf, int want_med, int want_beams, uint32_t* ls, float4* slo, float4* shi,
+                               uint32_t* lm, float4* mlo, float4* mhi, uint32_t* lb, float4* blo, float4* bhi)
+{
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= p.n_light) return;
+    uint32_t js = os[pid], jm = om[pid];
+    if (want_surf || want_med) {
+        const uint32_t nv = b.nvert[pid];
+        for (uint32_t k = 0; k < nv; k++) {
+            const uint32_t slot = pid * p.vstride + k;
+            const VertexRec& v = b.verts[slot];
+            const int fl = __float_as_int(v.g.z);
+            const float4 a = v.a;
+            const float mag = fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
+            if (fl & VF_IN_MEDIUM) {
+                const float r = p.rad_point_med * (1.f + 1e-5f) + 1e-6f * mag;
+                if (want_med) {
+                    lm[jm] = slot;
+                    mlo[jm] = make_float4(a.x - r, a.y - r, a.z - r, 0.f);
+                    mhi[jm] = make_float4(a.x + r, a.y + r, a.z + r, 0.f);
+                }
+                jm++;
+            } else {
+                const float r = p.r_surf * (1.f + 1e-5f) + 1e-6f * mag;
+                if (want_surf) {
+                    ls[js] = slot;
+                    slo[js] = make_float4(a.x - r, a.y - r, a.z - r, 0.f);
+                    shi[js] = make_float4(a.x + r, a.y + r, a.z + r, 0.f);
+                }
+                js++;
+            }
+        }
+    }
+    if (want_beams) {
+        const uint32_t nb = b.nbeam[pid];
+        uint32_t jb = ob[pid];
+        for (uint32_t k = 0; k < nb; k++, jb++) {
+            const uint32_t slot = pid * p.bstride + k;
+            const BeamRec& r = b.beams[slot];
+            const float4 a = r.a, d = r.b;
+            const float ex = a.x + d.x * a.w, ey = a.y + d.y * a.w, ez = a.z + d.z * a.w;
+            const float mag = fmaxf(fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))),
+                                    fmaxf(fabsf(ex), fmaxf(fabsf(ey), fabsf(ez))));
+            const float rr = p.rad_beam * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f * a.w;
+            lb[jb] = slot;
+            blo[jb] = make_float4(fminf(a.x, ex) - rr, fminf(a.y, ey) - rr, fminf(a.z, ez) - rr, 0.f);
+            bhi[jb] = make_float4(fmaxf(a.x, ex) + rr, fmaxf(a.y, ey) + rr, fmaxf(a.z, ez) + rr, 0.f);
+        }
+    }
+}
+
+__global__ void k_gather_points(const uint32_t* order, const uint32_t* list, const VertexRec* verts, uint32_t n,
+                                float4* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t slot = list[order[i]];
+    const float4 a = verts[slot].a;
+    out[i] = make_float4(a.x, a.y, a.z, __int_as_float(int(slot)));
+}
+
+__global__ void k_gather_beams(const uint32_t* order, const uint32_t* list, const BeamRec* beams, uint32_t n,
+                               float4* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t slot = list[order[i]];
+    const float4 a = beams[slot].a, d = beams[slot].b;
+    out[2 * i] = a;
+    out[2 * i + 1] = make_float4(d.x, d.y, d.z, __int_as_float(int(slot)));
+}
+
+__global__ void k_gather_tris(const uint32_t* order, const float4* tri, const float4* nrm, const int2* rec, uint32_t n,
+                              float4* tri_s, float4* nrm_s, int2* rec_s)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = order[i];
+    for (int k = 0; k < 3; k++) tri_s[3 * i + k] = tri[3 * j + k];
+    for (int k = 0; k < 4; k++) nrm_s[4 * i + k] = nrm[4 * j + k];
+    rec_s[i] = rec[j];
+}
+
+inline uint32_t nblk(size_t n, uint32_t t = 256) { return uint32_t((n + t - 1) / t); }
+
+struct TriStore {
+    float4 *tri = nullptr, *nrm = nullptr;
+    int2* rec = nullptr;
+    BvhStore bvh;
+};
+
+} // namespace
+
+struct upbp_renderer {
+    std::string err;
+    hipStream_t stream = nullptr;
+    upbp_config cfg;
+    DScene sc;
+    std::vector<DevBuf> owned; // scene tables
+    TriStore real, imag;
+    BvhScratch scratch;
+    float r0_bb1d, r0_bp2d, r0_pb2d, r0_pp3d, r0_surf;
+    uint32_t n_light, n_pixels, bb1d_used, vstride, bstride;
+    bool want_beams;
+    Bufs bufs;
+    DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
+    DevBuf cnt, off, tot, list_s, list_m, list_b, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_b;
+    BvhStore bvh_s, bvh_m, bvh_b;
+    hipEvent_t ev[2 * kMaxBounces + 8];
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(upbp_renderer* r, const std::string& m, int code)
+{
+    if (r) r->err = m;
+    g_err = m;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                                    \
+    do {                                                                                                                \
+        hipError_t e_ = (expr);                                                                                         \
+        if (e_ != hipSuccess) return fail(r, std::string(#expr) + ": " + hipGetErrorString(e_), UPBP_E_HIP);            \
+    } while (0)
+
+template <typename T>
+T* upload(upbp_renderer* r, const std::vector<T>& v, DevBuf& buf)
+{
+    if (buf.ensure(std::max<size_t>(v.size(), 1) * sizeof(T))) return nullptr;
+    if (!v.empty() && hipMemcpy(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return nullptr;
+    return buf.as<T>();
+}
+
+f3 F3(const float* p) { return mk(p[0], p[1], p[2]); }
+
+// host/medium.hpp:39-85
+DMedium make_medium(const upbp_medium& in)
+{
+    DMedium m;
+    m.cont_prob = in.continuation_prob;
+    m.mean_cosine = in.mean_cosine;
+    m.absorption = mk(fabsf(in.absorption[0]), fabsf(in.absorption[1]), fabsf(in.absorption[2]));
+    m.emission = mk(fabsf(in.emission[0]), fabsf(in.emission[1]), fabsf(in.emission[2]));
+    m.scattering = mk(fabsf(in.scattering[0]), fabsf(in.scattering[1]), fabsf(in.scattering[2]));
+    m.attenuation = m.absorption + m.scattering;
+    m.min_pos_att = m.attenuation.x;
+    m.min_pos_idx = 0;
+    if (m.attenuation.y > 0.f && (m.min_pos_att == 0 || m.attenuation.y < m.min_pos_att)) {
+        m.min_pos_att = m.attenuation.y;
+        m.min_pos_idx = 1;
+    }
+    if (m.attenuation.z > 0.f && (m.min_pos_att == 0 || m.attenuation.z < m.min_pos_att)) {
+        m.min_pos_att = m.attenuation.z;
+        m.min_pos_idx = 2;
+    }
+    m.has_scattering = m.scattering.x > 0 || m.scattering.y > 0 || m.scattering.z > 0;
+    m.has_attenuation = m.attenuation.x > 0 || m.attenuation.y > 0 || m.attenuation.z > 0;
+    m.max_beam_length = m.has_attenuation ? -logf(1e-9f) / maxc(m.attenuation) : kInf;
+    return m;
+}
+
+int build_tris(upbp_renderer* r, const upbp_triangles& t, TriStore& out, TriSet& set)
+{
+    const uint32_t n = t.count;
+    set.bvh.n = n;
+    if (n == 0) {
+        set.tri = nullptr;
+        set.nrm = nullptr;
+        set.rec = nullptr;
+        set.bvh.nodes = nullptr;
+        set.bvh.order = nullptr;
+        return 0;
+    }
+    std::vector<float4> tri(3 * size_t(n)), nrm(4 * size_t(n)), lo(n), hi(n);
+    std::vector<int2> rec(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const float* v = t.verts + 9 * size_t(i);
+        const f3 p0 = F3(v), p1 = F3(v + 3), p2 = F3(v + 6);
+        const f3 e1 = p1 - p0, e2 = p2 - p0;
+        tri[3 * i] = make_float4(p0.x, p0.y, p0.z, 0.f);
+        tri[3 * i + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
+        tri[3 * i + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
+        for (int k = 0; k < 4; k++) {
+            const float* q = t.normals + 12 * size_t(i) + 3 * k;
+            nrm[4 * i + k] = make_float4(q[0], q[1], q[2], 0.f);
+        }
+        rec[i] = make_int2(t.mat_id[i], t.light_id[i]);
+        float mn[3], mx[3], mag = 0.f;
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(v[a], fminf(v[3 + a], v[6 + a]));
+            mx[a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a]));
+            mag = fmaxf(mag, fmaxf(fabsf(mn[a]), fabsf(mx[a])));
+        }
+        const float ext = fmaxf(mx[0] - mn[0], fmaxf(mx[1] - mn[1], mx[2] - mn[2]));
+        const float pad = 1e-5f * ext + 1e-6f * mag + 1e-7f;
+        lo[i] = make_float4(mn[0] - pad, mn[1] - pad, mn[2] - pad, 0.f);
+        hi[i] = make_float4(mx[0] + pad, mx[1] + pad, mx[2] + pad, 0.f);
+    }
+    DevBuf dtri, dnrm, drec, dlo, dhi;
+    if (!upload(r, tri, dtri) || !upload(r, nrm, dnrm) || !upload(r, rec, drec) || !upload(r, lo, dlo) ||
+        !upload(r, hi, dhi))
+        return fail(r, "triangle upload failed", UPBP_E_HIP);
+    if (lbvh_build(out.bvh, r->scratch, dlo.as<float4>(), dhi.as<float4>(), n, r->stream))
+        return fail(r, "triangle LBVH build failed", UPBP_E_HIP);
+    HIPCHK(hipMalloc(&out.tri, tri.size() * sizeof(float4)));
+    HIPCHK(hipMalloc(&out.nrm, nrm.size() * sizeof(float4)));
+    HIPCHK(hipMalloc(&out.rec, rec.size() * sizeof(int2)));
+    hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n)), dim3(256), 0, r->stream, out.bvh.order, dtri.as<float4>(),
+                       dnrm.as<float4>(), drec.as<int2>(), n, out.tri, out.nrm, out.rec);
+    HIPCHK(hipStreamSynchronize(r->stream));
+    for (DevBuf* b : {&dtri, &dnrm, &drec, &dlo, &dhi}) b->release();
+    set.tri = out.tri;
+    set.nrm = out.nrm;
+    set.rec = out.rec;
+    set.bvh.nodes = out.bvh.nodes;
+    set.bvh.order = out.bvh.order;
+    return 0;
+}
+
+// renderer.cpp:56-119 + renderer_init.cpp:713-736, 891-916
+void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
+{
+    const upbp_config& c = r->cfg;
+    const uint32_t f = c.algorithm_flags;
+    memset(&p, 0, sizeof(p));
+    p.trace_light = p.trace_camera = f != 0;
+    p.do_bpt = (f & T_BPT) != 0;
+    p.do_bb1d = (f & T_BB1D) != 0;
+    p.do_bp2d = (f & T_BP2D) != 0;
+    p.do_pb2d = (f & T_PB2D) != 0;
+    p.do_pp3d = (f & T_PP3D) != 0;
+    p.do_surf = (f & T_SURF) != 0;
+    if (f & T_SPECULAR_ONLY) {
+        p.trace_light = 0;
+        p.trace_camera = 1;
+        p.do_bpt = p.do_bb1d = p.do_bp2d = p.do_pb2d = p.do_pp3d = p.do_surf = 0;
+    }
+    p.connect_from_surf = (f & (T_PREVIOUS | T_COMPATIBLE)) == 0;
+    p.n_pixels = r->n_pixels;
+    p.n_light = r->n_light;
+    const bool merging = c.max_path_length > 1;
+    p.build_bb1d = p.trace_light && merging && p.do_bb1d;
+    p.build_bp2d = p.trace_light && merging && p.do_bp2d;
+    p.build_pb2d = p.trace_light && merging && p.do_pb2d;
+    p.build_pp3d = p.trace_light && merging && p.do_pp3d;
+    p.build_surf = p.trace_light && merging && p.do_surf;
+    p.min_len = c.min_path_length;
+    p.max_len = c.max_path_length;
+    p.pbt = c.photon_beam_type;
+    p.qbt = c.query_beam_type;
+    p.techniques = f;
+    p.bb1d_used = r->bb1d_used;
+    p.vstride = r->vstride;
+    p.bstride = r->bstride;
+    p.ignore_spec = c.ignore_fully_spec_paths != 0;
+    p.iter_key = uint64_t(iteration) + c.iteration_offset;
+    const float n_light = float(r->n_light);
+    const float bb1d_used = float(r->bb1d_used);
+    p.n_bpt = float(p.do_bpt);
+    const float n_bb1d = float(p.do_bb1d) * bb1d_used, n_bp2d = float(p.do_bp2d) * n_light,
+                n_pb2d = float(p.do_pb2d) * n_light, n_pp3d = float(p.do_pp3d) * n_light,
+                n_surf = float(p.do_surf) * n_light;
+    const float eff = 1.f + float(iteration);
+    const float r_bb1d = fmaxf(
+        r->r0_bb1d * powf(1.f + float(iteration) * bb1d_used / n_light, c.bb1d_radius_alpha - 1.f), 1e-7f);
+    const float r_bp2d = fmaxf(r->r0_bp2d * powf(eff, (c.bp2d_radius_alpha - 1.f) * .5f), 1e-7f);
+    const float r_pb2d = fmaxf(r->r0_pb2d * powf(eff, (c.pb2d_radius_alpha - 1.f) * .5f), 1e-7f);
+    const float r_pp3d = fmaxf(r->r0_pp3d * powf(eff, (c.pp3d_radius_alpha - 1.f) * (1.f / 3.f)), 1e-7f);
+    const float r_surf = fmaxf(r->r0_surf * powf(eff, (c.surf_radius_alpha - 1.f) * .5f), 1e-7f);
+    p.mf_bb1d = .5f * r_bb1d * n_bb1d;
+    p.mf_bp2d = (kPi * (r_bp2d * r_bp2d)) * n_bp2d;
+    p.mf_pb2d = (kPi * (r_pb2d * r_pb2d)) * n_pb2d;
+    p.mf_pp3d = (4.f / 3.f) * (kPi * (r_pp3d * r_pp3d * r_pp3d)) * n_pp3d;
+    p.mf_surf = (kPi * (r_surf * r_surf)) * n_surf;
+    p.norm_bb1d = 1.f / bb1d_used;
+    p.norm_bp2d = 1.f / n_light;
+    p.norm_pb2d = 1.f / n_light;
+    p.norm_pp3d = 1.f / p.mf_pp3d;
+    p.norm_surf = 1.f / p.mf_surf;
+    p.r_bb1d = r_bb1d;
+    p.r_bp2d = r_bp2d;
+    p.r_pb2d = r_pb2d;
+    p.r_pp3d = r_pp3d;
+    p.r_surf = r_surf;
+    p.rad_point_med = 0.f;
+    if (p.do_pp3d) p.rad_point_med = fmaxf(p.rad_point_med, r_pp3d);
+    if (p.do_pb2d) p.rad_point_med = fmaxf(p.rad_point_med, r_pb2d);
+    p.rad_beam = 0.f;
+    if (p.do_bb1d) p.rad_beam = fmaxf(p.rad_beam, r_bb1d);
+    if (p.do_bp2d) p.rad_beam = fmaxf(p.rad_beam, r_bp2d);
+    p.r2_bb1d = r_bb1d * r_bb1d;
+    p.r2_bp2d = r_bp2d * r_bp2d;
+    p.r2_pb2d = r_pb2d * r_pb2d;
+    p.r2_pp3d = r_pp3d * r_pp3d;
+    p.r2_surf = r_surf * r_surf;
+}
+
+} // namespace
+
+extern "C" const char* upbp_last_error(const upbp_renderer* r) { return r ? r->err.c_str() : g_err.c_str(); }
+
+extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp_renderer** out)
+{
+    upbp_renderer* r = nullptr;
+    if (!scene || !cfg || !out) return fail(r, "null argument", UPBP_E_ARG);
+    *out = nullptr;
+    if (scene->light_count == 0) return fail(r, "scene has no lights", UPBP_E_ARG);
+    if (cfg->path_count_per_iter < 1.f) return fail(r, "light path count must be positive", UPBP_E_ARG);
+    if (cfg->max_path_length >= uint32_t(kMaxBounces)) return fail(r, "max path length must be below 64", UPBP_E_ARG);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(r, "no GPU visible", UPBP_E_NODEVICE);
+    r = new upbp_renderer();
+    r->cfg = *cfg;
+    const upbp_config& c = *cfg;
+    HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    for (auto& e : r->ev) HIPCHK(hipEventCreate(&e));
+
+    // radii relative to the scene radius when negative (renderer_init.cpp:681-711)
+    const float rad = scene->scene_radius;
+    auto rel = [&](float v) { return v < 0.f ? -v * rad : v; };
+    r->r0_bb1d = rel(c.bb1d_radius_initial);
+    r->r0_bp2d = rel(c.bp2d_radius_initial);
+    r->r0_pb2d = rel(c.pb2d_radius_initial);
+    r->r0_pp3d = rel(c.pp3d_radius_initial);
+    r->r0_surf = rel(c.surf_radius_initial);
+    r->n_pixels = c.res_x * c.res_y;
+    r->n_light = uint32_t(c.path_count_per_iter);
+    float used = c.bb1d_used_light_subpath_count;
+    if (used < 0.f) used = floorf(-used * float(r->n_light));
+    r->bb1d_used = uint32_t(used);
+    r->vstride = std::max<uint32_t>(c.max_path_length - 1, 1);
+    r->want_beams = (c.algorithm_flags & (T_BB1D | T_BP2D)) != 0 && !(c.algorithm_flags & T_SPECULAR_ONLY);
+    r->bstride = r->want_beams ? 4 * r->vstride : 0;
+
+    // scene tables
+    DScene& sc = r->sc;
+    memset(&sc, 0, sizeof(sc));
+    std::vector<DMaterial> mats(scene->mat_count);
+    for (uint32_t i = 0; i < scene->mat_count; i++) {
+        const upbp_material& m = scene->materials[i];
+        mats[i] = {F3(m.diffuse), F3(m.phong), F3(m.mirror), m.phong_exponent, m.ior, m.priority, m.real, m.med_idx};
+    }
+    std::vector<DMedium> media(scene->med_count);
+    for (uint32_t i = 0; i < scene->med_count; i++) media[i] = make_medium(scene->media[i]);
+    std::vector<DLight> lights(scene->light_count);
+    for (uint32_t i = 0; i < scene->light_count; i++) {
+        const upbp_light& l = scene->lights[i];
+        DLight& o = lights[i];
+        memset(&o, 0, sizeof(o));
+        o.type = l.type;
+        o.mat_id = l.mat_id;
+        o.med_id = l.med_id;
+        o.intensity = F3(l.intensity);
+        o.scale = l.scale;
+        if (l.type == UPBP_LIGHT_AREA) { // host/light.hpp:30-47
+            o.p0 = F3(l.p0);
+            o.e1 = F3(l.p1) - o.p0;
+            o.e2 = F3(l.p2) - o.p0;
+            const f3 nrm = cross(o.e1, o.e2);
+            o.inv_area = 2.f / sqrtf(dot(nrm, nrm));
+            o.frame.set_from_z(nrm);
+        } else if (l.type == UPBP_LIGHT_DIRECTIONAL) {
+            o.frame.set_from_z(F3(l.position));
+        } else if (l.type == UPBP_LIGHT_POINT) {
+            o.position = F3(l.position);
+        }
+    }
+    r->owned.resize(3);
+    sc.mats = upload(r, mats, r->owned[0]);
+    sc.media = upload(r, media, r->owned[1]);
+    sc.lights = upload(r, lights, r->owned[2]);
+    if (!sc.mats || !sc.media || !sc.lights) {
+        upbp_destroy(r);
+        return fail(nullptr, "scene upload failed", UPBP_E_HIP);
+    }
+    sc.n_mats = scene->mat_count;
+    sc.n_media = scene->med_count;
+    sc.n_lights = scene->light_count;
+    sc.global_medium = scene->global_medium_idx;
+    sc.background_light = scene->background_light_idx;
+    sc.center = F3(scene->scene_center);
+    sc.radius = scene->scene_radius;
+    sc.inv_radius_sqr = 1.f / (sc.radius * sc.radius);
+    sc.use_shading_normal = c.use_shading_normal;
+    { // camera basis (host/camera.hpp:33-76)
+        const upbp_camera& cam = scene->camera;
+        DCamera& dc = sc.cam;
+        dc.origin = F3(cam.origin);
+        dc.forward = normalize(F3(cam.target) - dc.origin);
+        dc.right = normalize(cross(F3(cam.roll), -dc.forward));
+        dc.up = cross(-dc.forward, dc.right);
+        dc.res_x = float(c.res_x);
+        dc.res_y = float(c.res_y);
+        dc.ipd = float(c.res_x) / (2.f * tanf(cam.horizontal_fov_deg * kPi / 360.f));
+        dc.mat_id = cam.mat_id;
+        dc.med_id = cam.med_id;
+    }
+    int e = build_tris(r, scene->real, r->real, sc.real);
+    if (!e) e = build_tris(r, scene->imag, r->imag, sc.imag);
+    if (e) {
+        const std::string m = r->err;
+        upbp_destroy(r);
+        return fail(nullptr, m, e);
+    }
+
+    // wavefront buffers
+    const size_t NL = r->n_light, NP = r->n_pixels, NQ = std::max(NL, NP);
+    bool ok = !r->lstate.ensure(NL * sizeof(PathState)) && !r->lstack.ensure(NL * sizeof(StackRec)) &&
+              !r->cstate.ensure(NP * sizeof(PathState)) && !r->cstack.ensure(NP * sizeof(StackRec)) &&
+              !r->q0.ensure(NQ * 4) && !r->q1.ensure(NQ * 4) && !r->qcount.ensure(256 * 4) &&
+              !r->verts.ensure(NL * r->vstride * sizeof(VertexRec)) && !r->nvert.ensure(NL * 4) &&
+              !r->nbeam.ensure(NL * 4) && !r->fb32.ensure(NP * 12) && !r->accum.ensure(NP * 24) &&
+              !r->stats.ensure(STAT_COUNT * 8) && !r->cnt.ensure(3 * NL * 4) && !r->off.ensure(3 * NL * 4) &&
+              !r->tot.ensure(16);
+    if (ok && r->want_beams) ok = !r->beams.ensure(NL * r->bstride * sizeof(BeamRec));
+    if (!ok) {
+        upbp_destroy(r);
+        return fail(nullptr, "device allocation failed", UPBP_E_HIP);
+    }
+    HIPCHK(hipMemsetAsync(r->accum.p, 0, NP * 24, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    Bufs& b = r->bufs;
+    memset(&b, 0, sizeof(b));
+    b.lstate = r->lstate.as<PathState>();
+    b.lstack = r->lstack.as<StackRec>();
+    b.cstate = r->cstate.as<PathState>();
+    b.cstack = r->cstack.as<StackRec>();
+    b.queue[0] = r->q0.as<uint32_t>();
+    b.queue[1] = r->q1.as<uint32_t>();
+    b.qcount = r->qcount.as<uint32_t>();
+    b.verts = r->verts.as<VertexRec>();
+    b.nvert = r->nvert.as<uint32_t>();
+    b.beams = r->beams.as<BeamRec>();
+    b.nbeam = r->nbeam.as<uint32_t>();
+    b.fb32 = r->fb32.as<float>();
+    b.accum = r->accum.as<double>();
+    b.stats = r->stats.as<unsigned long long>();
+    *out = r;
+    return UPBP_OK;
+}
+
+namespace {
+
+// photon lists (path order) -> boxes -> three LBVHs -> sorted payloads
+int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
+{
+    Bufs& b = r->bufs;
+    hipStream_t s = r->stream;
+    const uint32_t NL = r->n_light;
+    const int want_surf = p.build_surf, want_med = p.build_pp3d || p.build_pb2d;
+    const int want_pts = want_surf || want_med;
+    const int want_beams = (p.build_bb1d || p.build_bp2d) && r->want_beams;
+    uint32_t* cnt = r->cnt.as<uint32_t>();
+    uint32_t* off = r->off.as<uint32_t>();
+    uint32_t* tot = r->tot.as<uint32_t>();
+    hipLaunchKernelGGL(k_count_photons, dim3(nblk(NL)), dim3(256), 0, s, p, b, cnt, cnt + NL, cnt + 2 * NL, want_pts,
+                       want_beams);
+    for (int k = 0; k < 3; k++)
+        if (scan_exclusive(cnt + size_t(k) * NL, off + size_t(k) * NL, NL, tot + k, r->scratch, s))
+            return fail(r, "photon list scan failed", UPBP_E_HIP);
+    uint32_t host_tot[3];
+    HIPCHK(hipMemcpyAsync(host_tot, tot, 12, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const uint32_t ns = want_surf ? host_tot[0] : 0, nm = want_med ? host_tot[1] : 0, nb = want_beams ? host_tot[2] : 0;
+    counts[0] = ns;
+    counts[1] = nm;
+    counts[2] = nb;
+    bool ok = !r->list_s.ensure(std::max<size_t>(ns, 1) * 4) && !r->lo_s.ensure(std::max<size_t>(ns, 1) * 16) &&
+              !r->hi_s.ensure(std::max<size_t>(ns, 1) * 16) && !r->pts_s.ensure(std::max<size_t>(ns, 1) * 16) &&
+              !r->list_m.ensure(std::max<size_t>(nm, 1) * 4) && !r->lo_m.ensure(std::max<size_t>(nm, 1) * 16) &&
+              !r->hi_m.ensure(std::max<size_t>(nm, 1) * 16) && !r->pts_m.ensure(std::max<size_t>(nm, 1) * 16) &&
+              !r->list_b.ensure(std::max<size_t>(nb, 1) * 4) && !r->lo_b.ensure(std::max<size_t>(nb, 1) * 16) &&
+              !r->hi_b.ensure(std::max<size_t>(nb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(nb, 1) * 32);
+    if (!ok) return fail(r, "photon list allocation failed", UPBP_E_HIP);
+    hipLaunchKernelGGL(k_emit_photons, dim3(nblk(NL)), dim3(256), 0, s, p, b, off, off + NL, off + 2 * NL, want_surf,
+                       want_med, want_beams, r->list_s.as<uint32_t>(), r->lo_s.as<float4>(), r->hi_s.as<float4>(),
+                       r->list_m.as<uint32_t>(), r->lo_m.as<float4>(), r->hi_m.as<float4>(), r->list_b.as<uint32_t>(),
+                       r->lo_b.as<float4>(), r->hi_b.as<float4>());
+    PhotonMaps& M = b.maps;
+    if (lbvh_build(r->bvh_s, r->scratch, r->lo_s.as<float4>(), r->hi_s.as<float4>(), ns, s) ||
+        lbvh_build(r->bvh_m, r->scratch, r->lo_m.as<float4>(), r->hi_m.as<float4>(), nm, s) ||
+        lbvh_build(r->bvh_b, r->scratch, r->lo_b.as<float4>(), r->hi_b.as<float4>(), nb, s))
+        return fail(r, "photon LBVH build failed", UPBP_E_HIP);
+    if (ns)
+        hipLaunchKernelGGL(k_gather_points, dim3(nblk(ns)), dim3(256), 0, s, r->bvh_s.order, r->list_s.as<uint32_t>(),
+                           b.verts, ns, r->pts_s.as<float4>());
+    if (nm)
+        hipLaunchKernelGGL(k_gather_points, dim3(nblk(nm)), dim3(256), 0, s, r->bvh_m.order, r->list_m.as<uint32_t>(),
+                           b.verts, nm, r->pts_m.as<float4>());
+    if (nb)
+        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(nb)), dim3(256), 0, s, r->bvh_b.order, r->list_b.as<uint32_t>(),
+                           b.beams, nb, r->geo_b.as<float4>());
+    M.surf = {r->bvh_s.nodes, r->bvh_s.order, ns};
+    M.med = {r->bvh_m.nodes, r->bvh_m.order, nm};
+    M.beam = {r->bvh_b.nodes, r->bvh_b.order, nb};
+    M.surf_pts = r->pts_s.as<float4>();
+    M.med_pts = r->pts_m.as<float4>();
+    M.beam_geo = r->geo_b.as<float4>();
+    return 0;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+} // namespace
+
+extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
+{
+    if (!r) return fail(r, "null renderer", UPBP_E_ARG);
+    IterParams p;
+    setup_params(r, iteration, p);
+    hipStream_t s = r->stream;
+    Bufs& b = r->bufs;
+    const int L = int(r->cfg.max_path_length);
+    hipEvent_t* ev = r->ev;
+    HIPCHK(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, b.qcount, 0u, size_t(256));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(3 * size_t(r->n_pixels))), dim3(256), 0, s, reinterpret_cast<uint32_t*>(b.fb32),
+                       0u, 3 * size_t(r->n_pixels));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(b.stats), 0u,
+                       size_t(2 * STAT_COUNT));
+    if (p.trace_light) {
+        launch_light_generate(r->sc, p, b, s);
+        for (int k = 0; k < L; k++) launch_light_bounce(r->sc, p, b, k, s);
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(nblk(r->n_light)), dim3(256), 0, s, b.nvert, 0u, size_t(r->n_light));
+        hipLaunchKernelGGL(k_fill, dim3(nblk(r->n_light)), dim3(256), 0, s, b.nbeam, 0u, size_t(r->n_light));
+    }
+    HIPCHK(hipEventRecord(ev[1], s));
+    uint64_t counts[3] = {0, 0, 0};
+    if (p.trace_light) {
+        const int e = build_photon_maps(r, p, counts);
+        if (e) return e;
+    } else {
+        b.maps = PhotonMaps();
+        memset(&b.maps, 0, sizeof(b.maps));
+    }
+    HIPCHK(hipEventRecord(ev[2], s));
+    if (p.trace_camera) {
+        launch_camera_generate(r->sc, p, b, s);
+        for (int k = 0; k < L; k++) {
+            HIPCHK(hipEventRecord(ev[8 + 2 * k], s));
+            launch_camera_bounce(r->sc, p, b, k, s);
+            HIPCHK(hipEventRecord(ev[9 + 2 * k], s));
+        }
+    }
+    launch_camera_finish(p, b, float(r->cfg.res_x), float(r->cfg.res_y), s);
+    HIPCHK(hipEventRecord(ev[3], s));
+    HIPCHK(hipGetLastError());
+    if (stats) {
+        HIPCHK(hipEventSynchronize(ev[3]));
+        memset(stats, 0, sizeof(*stats));
+        uint32_t q[256];
+        unsigned long long st[STAT_COUNT];
+        HIPCHK(hipMemcpy(q, b.qcount, sizeof(q), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(st, b.stats, sizeof(st), hipMemcpyDeviceToHost));
+        for (int k = 0; k < L; k++) {
+            stats->light_extend_steps += q[k];
+            stats->camera_extend_steps += q[64 + k];
+        }
+        stats->ms_light = elapsed(ev[0], ev[1]);
+        stats->ms_build = elapsed(ev[1], ev[2]);
+        stats->ms_camera = elapsed(ev[2], ev[3]);
+        stats->ms_total = elapsed(ev[0], ev[3]);
+        if (p.trace_camera)
+            for (int k = 0; k < L; k++) stats->ms_dominant += elapsed(ev[8 + 2 * k], ev[9 + 2 * k]);
+        stats->dominant_launches = p.trace_camera ? uint64_t(L) : 0;
+        stats->surface_vertices = counts[0];
+        stats->medium_vertices = counts[1];
+        stats->beams = counts[2];
+        stats->bpt_connections = st[STAT_BPT_CONN];
+        stats->point_queries = st[STAT_POINT_Q];
+        stats->segment_queries = st[STAT_SEG_Q];
+        stats->merge_accepts = st[STAT_ACCEPT];
+        stats->vertex_drops = st[STAT_VDROP];
+        stats->beam_drops = st[STAT_BDROP];
+        stats->path_errors = st[STAT_ERRORS];
+        // compulsory HBM bytes of the camera bounce kernels: 128-byte state read and write
+        // plus 64 bytes of stack per live path-step, 112 bytes per light vertex read by a
+        // connection and 120 bytes (vertex or beam) per accepted merge candidate
+        stats->dominant_bytes = 320.0 * double(stats->camera_extend_steps) + 112.0 * double(st[STAT_BPT_CONN]) +
+                                120.0 * double(st[STAT_ACCEPT]);
+        stats->algorithmic_bytes = stats->dominant_bytes + 320.0 * double(stats->light_extend_steps) +
+                                   112.0 * double(counts[0] + counts[1]) + 128.0 * double(counts[2]) +
+                                   88.0 * double(counts[0] + counts[1] + counts[2]) + 60.0 * double(r->n_pixels);
+    }
+    return UPBP_OK;
+}
+
+extern "C" int upbp_read_accum(upbp_renderer* r, double* rgb, int iters_done)
+{
+    if (!r || !rgb || iters_done < 1) return fail(r, "invalid argument", UPBP_E_ARG);
+    HIPCHK(hipStreamSynchronize(r->stream));
+    const size_t n = 3 * size_t(r->n_pixels);
+    HIPCHK(hipMemcpy(rgb, r->bufs.accum, n * sizeof(double), hipMemcpyDeviceToHost));
+    const double inv = 1.0 / double(iters_done);
+    for (size_t i = 0; i < n; i++) rgb[i] *= inv;
+    return UPBP_OK;
+}
+
+extern "C" int upbp_sync(upbp_renderer* r)
+{
+    if (!r) return fail(r, "null renderer", UPBP_E_ARG);
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return UPBP_OK;
+}
+
+extern "C" void upbp_destroy(upbp_renderer* r)
+{
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    for (DevBuf& d : r->owned) d.release();
+    for (TriStore* t : {&r->real, &r->imag}) {
+        if (t->tri) (void)hipFree(t->tri);
+        if (t->nrm) (void)hipFree(t->nrm);
+        if (t->rec) (void)hipFree(t->rec);
+        lbvh_free(t->bvh);
+    }
+    for (DevBuf* d : {&r->lstate, &r->lstack, &r->cstate, &r->cstack, &r->q0, &r->q1, &r->qcount, &r->verts, &r->nvert,
+                      &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
+                      &r->list_m, &r->list_b, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
+                      &r->pts_m, &r->geo_b})
+        d->release();
+    lbvh_free(r->bvh_s);
+    lbvh_free(r->bvh_m);
+    lbvh_free(r->bvh_b);
+    lbvh_scratch_free(r->scratch);
+    for (auto& e : r->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}

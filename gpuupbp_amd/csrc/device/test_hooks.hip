@@ -1,0 +1,254 @@
+// Test hooks: LBVH range queries and closest-hit queries on caller-provided
+// primitives, returning exact index sets for comparison with brute force.
+// The acceptance predicates are those of nolmoonen/gpuupbp gpuupbp/kernel/intersection.cuh:33-173,
+// applied as in gpuupbp/kernel/pde/program_{surf,pp3d,pb2d,bb1d,bp2d}.cu.
+#include "kernels.h"
+
+#include "../../../include/upbp_capi.h"
+
+#include <algorithm>
+#include <vector>
+
+using namespace upbp;
+using namespace upbp_host;
+
+namespace {
+
+__global__ void k_prim_boxes(int kind, const float* prims, uint32_t n, float r, float4* lo, float4* hi)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (kind <= 1) {
+        const float* q = prims + 3 * size_t(i);
+        const float mag = fmaxf(fabsf(q[0]), fmaxf(fabsf(q[1]), fabsf(q[2])));
+        const float rr = r * (1.f + 1e-5f) + 1e-6f * mag;
+        lo[i] = make_float4(q[0] - rr, q[1] - rr, q[2] - rr, 0.f);
+        hi[i] = make_float4(q[0] + rr, q[1] + rr, q[2] + rr, 0.f);
+    } else {
+        const float* q = prims + 7 * size_t(i);
+        const float e[3] = {q[0] + q[3] * q[6], q[1] + q[4] * q[6], q[2] + q[5] * q[6]};
+        float mag = 0.f;
+        for (int a = 0; a < 3; a++) mag = fmaxf(mag, fmaxf(fabsf(q[a]), fabsf(e[a])));
+        const float rr = r * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f * q[6];
+        lo[i] = make_float4(fminf(q[0], e[0]) - rr, fminf(q[1], e[1]) - rr, fminf(q[2], e[2]) - rr, 0.f);
+        hi[i] = make_float4(fmaxf(q[0], e[0]) + rr, fmaxf(q[1], e[1]) + rr, fmaxf(q[2], e[2]) + rr, 0.f);
+    }
+}
+
+// pass 0 counts accepted primitives per query, pass 1 writes them at offsets[q]
+__global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const float* queries, uint32_t nq, float r2,
+                        uint32_t* counts, const uint32_t* offsets, uint32_t* out, uint32_t* stack_fail)
+{
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    uint32_t c = 0;
+    const uint32_t base = pass ? offsets[qi] : 0u;
+    auto accept = [&](uint32_t prim) {
+        if (pass) out[base + c] = prim;
+        c++;
+    };
+    bool ok = true;
+    if (kind == 0 || kind == 2) {
+        const float* q = queries + 3 * size_t(qi);
+        const f3 P = mk(q[0], q[1], q[2]);
+        ok = bvh_point_query(bvh, P, [&](uint32_t k) {
+            const uint32_t prim = bvh.order[k];
+            if (kind == 0) {
+                const float* x = prims + 3 * size_t(prim);
+                if (pred_point_point(P, mk(x[0], x[1], x[2]), r2)) accept(prim);
+            } else {
+                const float* x = prims + 7 * size_t(prim);
+                if (pred_point_beam(mk(x[0], x[1], x[2]), mk(x[3], x[4], x[5]), x[6], P, r2)) accept(prim);
+            }
+        });
+    } else {
+        const float* q = queries + 8 * size_t(qi);
+        const f3 O = mk(q[0], q[1], q[2]), D = mk(q[3], q[4], q[5]);
+        const float t0 = q[6], t1 = q[7];
+        ok = bvh_ray_query(bvh, O, D, t0, t1, [&](uint32_t k, float&) {
+            const uint32_t prim = bvh.order[k];
+            if (kind == 1) {
+                const float* x = prims + 3 * size_t(prim);
+                const f3 X = mk(x[0], x[1], x[2]);
+                float d, h;
+                if (pred_point_beam(O, D, t1, X, r2) && pred_bre(d, h, O, D, t0, t1, X, r2)) accept(prim);
+            } else {
+                const float* x = prims + 7 * size_t(prim);
+                float dd, st, ta, tb;
+                if (pred_beam_beam(O, D, t0, t1, mk(x[0], x[1], x[2]), mk(x[3], x[4], x[5]), 0.f, x[6], r2, dd, st, ta,
+                                   tb))
+                    accept(prim);
+            }
+        });
+    }
+    if (!ok) atomicAdd(stack_fail, 1u);
+    if (!pass) counts[qi] = c;
+}
+
+__global__ void k_closest(TriSet T, const float* rays, uint32_t n, float* ht, int32_t* hp, uint32_t* stack_fail)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* R = rays + 8 * size_t(i);
+    const f3 o = mk(R[0], R[1], R[2]), d = mk(R[3], R[4], R[5]);
+    const float tmin = R[6];
+    int best = -1;
+    float bt = INFINITY;
+    const bool ok = bvh_ray_query(T.bvh, o, d, tmin, R[7], [&](uint32_t k, float& thi) {
+        float t, u, v;
+        if (tri_test(T.tri[3 * k], T.tri[3 * k + 1], T.tri[3 * k + 2], o, d, tmin, thi, t, u, v)) {
+            best = int(T.bvh.order[k]);
+            bt = t;
+            thi = t;
+        }
+    });
+    if (!ok) atomicAdd(stack_fail, 1u);
+    ht[i] = bt;
+    hp[i] = best;
+}
+
+__global__ void k_sort_tris(const uint32_t* order, const float4* tri, uint32_t n, float4* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 3; k++) out[3 * i + k] = tri[3 * order[i] + k];
+}
+
+struct Tmp {
+    std::vector<void*> ptrs;
+    ~Tmp()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    T* alloc(size_t n)
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+inline uint32_t nb(size_t n) { return uint32_t((n + 255) / 256); }
+
+} // namespace
+
+extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries,
+                                     uint32_t n_queries, float radius, uint32_t* offsets, uint32_t* indices,
+                                     uint64_t max_indices, uint64_t* n_indices)
+{
+    if (kind < 0 || kind > 3 || !offsets || (n_prims && !prims) || (n_queries && !queries)) return UPBP_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
+    const size_t pw = (kind <= 1) ? 3 : 7, qw = (kind == 0 || kind == 2) ? 3 : 8;
+    Tmp t;
+    float* dp = t.alloc<float>(n_prims * pw);
+    float* dq = t.alloc<float>(n_queries * qw);
+    float4* lo = t.alloc<float4>(n_prims);
+    float4* hi = t.alloc<float4>(n_prims);
+    uint32_t* cnt = t.alloc<uint32_t>(n_queries + 1);
+    uint32_t* off = t.alloc<uint32_t>(n_queries + 1);
+    uint32_t* fail_count = t.alloc<uint32_t>(1);
+    if (!dp || !dq || !lo || !hi || !cnt || !off || !fail_count) return UPBP_E_HIP;
+    hipStream_t s = nullptr;
+    if (n_prims) (void)hipMemcpy(dp, prims, n_prims * pw * 4, hipMemcpyHostToDevice);
+    if (n_queries) (void)hipMemcpy(dq, queries, n_queries * qw * 4, hipMemcpyHostToDevice);
+    (void)hipMemset(fail_count, 0, 4);
+    BvhStore st;
+    BvhScratch sc;
+    if (n_prims) hipLaunchKernelGGL(k_prim_boxes, dim3(nb(n_prims)), dim3(256), 0, s, kind, dp, n_prims, radius, lo, hi);
+    if (lbvh_build(st, sc, lo, hi, n_prims, s)) {
+        lbvh_free(st);
+        lbvh_scratch_free(sc);
+        return UPBP_E_HIP;
+    }
+    const Bvh bvh{st.nodes, st.order, n_prims};
+    const float r2 = radius * radius;
+    if (n_queries)
+        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 0, bvh, dp, dq, n_queries, r2, cnt, off,
+                           nullptr, fail_count);
+    uint32_t* total = off + n_queries;
+    int rc = scan_exclusive(cnt, off, n_queries, total, sc, s);
+    uint32_t host_total = 0, fails = 0;
+    if (!rc) rc = int(hipMemcpy(&host_total, total, 4, hipMemcpyDeviceToHost));
+    uint32_t* dout = rc ? nullptr : t.alloc<uint32_t>(host_total);
+    if (!rc && n_queries && dout)
+        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 1, bvh, dp, dq, n_queries, r2, cnt, off,
+                           dout, fail_count);
+    if (!rc) rc = int(hipDeviceSynchronize());
+    if (!rc) rc = int(hipMemcpy(offsets, off, (size_t(n_queries) + 1) * 4, hipMemcpyDeviceToHost));
+    if (!rc && indices && host_total)
+        rc = int(hipMemcpy(indices, dout, std::min<uint64_t>(host_total, max_indices) * 4, hipMemcpyDeviceToHost));
+    if (!rc) rc = int(hipMemcpy(&fails, fail_count, 4, hipMemcpyDeviceToHost));
+    lbvh_free(st);
+    lbvh_scratch_free(sc);
+    if (n_queries == 0) offsets[0] = 0;
+    if (n_indices) *n_indices = host_total;
+    if (rc) return UPBP_E_HIP;
+    if (fails) return UPBP_E_CAPACITY;
+    return host_total <= max_indices ? UPBP_OK : UPBP_E_CAPACITY;
+}
+
+extern "C" int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays,
+                                     float* hit_t, int32_t* hit_prim)
+{
+    if ((n_tris && !verts) || (n_rays && (!rays || !hit_t || !hit_prim))) return UPBP_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
+    std::vector<float4> tri(3 * size_t(n_tris)), lo(n_tris), hi(n_tris);
+    for (uint32_t i = 0; i < n_tris; i++) {
+        const float* v = verts + 9 * size_t(i);
+        tri[3 * i] = make_float4(v[0], v[1], v[2], 0.f);
+        tri[3 * i + 1] = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.f);
+        tri[3 * i + 2] = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.f);
+        float mn[3], mx[3], mag = 0.f;
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(v[a], fminf(v[3 + a], v[6 + a]));
+            mx[a] = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a]));
+            mag = fmaxf(mag, fmaxf(fabsf(mn[a]), fabsf(mx[a])));
+        }
+        const float ext = fmaxf(mx[0] - mn[0], fmaxf(mx[1] - mn[1], mx[2] - mn[2]));
+        const float pad = 1e-5f * ext + 1e-6f * mag + 1e-7f;
+        lo[i] = make_float4(mn[0] - pad, mn[1] - pad, mn[2] - pad, 0.f);
+        hi[i] = make_float4(mx[0] + pad, mx[1] + pad, mx[2] + pad, 0.f);
+    }
+    Tmp t;
+    float4* dtri = t.alloc<float4>(tri.size());
+    float4* dsorted = t.alloc<float4>(tri.size());
+    float4* dlo = t.alloc<float4>(n_tris);
+    float4* dhi = t.alloc<float4>(n_tris);
+    float* drays = t.alloc<float>(8 * size_t(n_rays));
+    float* dt = t.alloc<float>(n_rays);
+    int32_t* dp = t.alloc<int32_t>(n_rays);
+    uint32_t* fail_count = t.alloc<uint32_t>(1);
+    if (!dtri || !dsorted || !dlo || !dhi || !drays || !dt || !dp || !fail_count) return UPBP_E_HIP;
+    (void)hipMemcpy(dtri, tri.data(), tri.size() * 16, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dlo, lo.data(), lo.size() * 16, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dhi, hi.data(), hi.size() * 16, hipMemcpyHostToDevice);
+    (void)hipMemcpy(drays, rays, 8 * size_t(n_rays) * 4, hipMemcpyHostToDevice);
+    (void)hipMemset(fail_count, 0, 4);
+    BvhStore st;
+    BvhScratch sc;
+    if (lbvh_build(st, sc, dlo, dhi, n_tris, nullptr)) {
+        lbvh_free(st);
+        lbvh_scratch_free(sc);
+        return UPBP_E_HIP;
+    }
+    if (n_tris) hipLaunchKernelGGL(k_sort_tris, dim3(nb(n_tris)), dim3(256), 0, 0, st.order, dtri, n_tris, dsorted);
+    TriSet T;
+    T.tri = dsorted;
+    T.nrm = nullptr;
+    T.rec = nullptr;
+    T.bvh = {st.nodes, st.order, n_tris};
+    if (n_rays) hipLaunchKernelGGL(k_closest, dim3(nb(n_rays)), dim3(256), 0, 0, T, drays, n_rays, dt, dp, fail_count);
+    int rc = int(hipDeviceSynchronize());
+    uint32_t fails = 0;
+    if (!rc) rc = int(hipMemcpy(hit_t, dt, size_t(n_rays) * 4, hipMemcpyDeviceToHost));
+    if (!rc) rc = int(hipMemcpy(hit_prim, dp, size_t(n_rays) * 4, hipMemcpyDeviceToHost));
+    if (!rc) rc = int(hipMemcpy(&fails, fail_count, 4, hipMemcpyDeviceToHost));
+    lbvh_free(st);
+    lbvh_scratch_free(sc);
+    if (rc) return UPBP_E_HIP;
+    return fails ? UPBP_E_CAPACITY : UPBP_OK;
+}

@@ -1,0 +1,118 @@
+"""Python host-side mirror of the reference's renderer interface.
+
+Mirrors Renderer::init / run_iteration / cleanup (gpuupbp/renderer/renderer.hpp:46-55)
+and the main loop of gpuupbp/gpuupbp.cpp:57-148 over the C ABI of libupbp.so.
+Raises if the HIP library is not built or no GPU is visible: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import capi
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Renderer:
+    def __init__(self, scene_ptr, cfg):
+        self.cfg = cfg
+        self.h = C.c_void_p()
+        rc = capi.lib().upbp_create(scene_ptr, C.byref(cfg), C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("upbp_create failed (%d): %s" % (rc, capi.lib().upbp_last_error(None).decode()))
+        self.iterations = 0
+
+    def run_iteration(self, iteration=None, want_stats=True):
+        it = self.iterations if iteration is None else iteration
+        st = capi.IterStats() if want_stats else None
+        rc = capi.lib().upbp_run_iteration(self.h, it, C.byref(st) if want_stats else None)
+        if rc != 0:
+            raise RuntimeError("upbp_run_iteration failed (%d): %s" % (rc, capi.lib().upbp_last_error(self.h).decode()))
+        self.iterations += 1
+        return st
+
+    def sync(self):
+        rc = capi.lib().upbp_sync(self.h)
+        if rc != 0:
+            raise RuntimeError("upbp_sync failed: %s" % capi.lib().upbp_last_error(self.h).decode())
+
+    def read_accum(self, iters_done=None):
+        n = self.iterations if iters_done is None else iters_done
+        img = np.zeros((self.cfg.res_y, self.cfg.res_x, 3), np.float64)
+        rc = capi.lib().upbp_read_accum(self.h, _p(img), n)
+        if rc != 0:
+            raise RuntimeError("upbp_read_accum failed: %s" % capi.lib().upbp_last_error(self.h).decode())
+        return img
+
+    def close(self):
+        if self.h:
+            capi.lib().upbp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def range_query(kind, prims, queries, radius, max_indices=None):
+    """Exact accepted index sets of an LBVH range query (CSR: offsets, indices)."""
+    prims = np.ascontiguousarray(prims, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    nq = queries.shape[0]
+    cap = max_indices if max_indices is not None else max(1, nq * max(prims.shape[0], 1))
+    offsets = np.zeros(nq + 1, np.uint32)
+    idx = np.zeros(cap, np.uint32)
+    n = C.c_uint64()
+    rc = capi.lib().upbp_test_range_query(kind, _p(prims), prims.shape[0], _p(queries), nq, radius, _p(offsets),
+                                          _p(idx), cap, C.byref(n))
+    if rc != 0:
+        raise RuntimeError("upbp_test_range_query failed: %d" % rc)
+    return offsets, idx[: n.value]
+
+
+def closest_hit(verts, rays):
+    verts = np.ascontiguousarray(verts, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    t = np.zeros(rays.shape[0], np.float32)
+    prim = np.zeros(rays.shape[0], np.int32)
+    rc = capi.lib().upbp_test_closest_hit(_p(verts), verts.shape[0], _p(rays), rays.shape[0], _p(t), _p(prim))
+    if rc != 0:
+        raise RuntimeError("upbp_test_closest_hit failed: %d" % rc)
+    return t, prim
+
+
+def save_exr(path, rgb):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = rgb.shape
+    rc = capi.lib().upbp_save_exr(path.encode(), _p(rgb), w, h)
+    if rc != 0:
+        raise RuntimeError("upbp_save_exr failed: %d" % rc)
+
+
+def render(obj_path, argv, iterations=None, want_stats=False):
+    """Parse argv like the CLI, load the OBJ scene, render, return the mean image."""
+    rc, cfg = capi.parse_args(["-s", "-1", obj_path] + list(argv))
+    if rc != 0:
+        raise RuntimeError("command line rejected")
+    sh = capi.SceneHandle(obj_path, cfg.res_x, cfg.res_y)
+    n = iterations if iterations is not None else cfg.iteration_count
+    stats = []
+    with Renderer(sh.ptr, cfg) as r:
+        for i in range(n):
+            s = r.run_iteration(i, want_stats)
+            if want_stats:
+                stats.append(s)
+        img = r.read_accum(n)
+    sh.close()
+    return (img, stats) if want_stats else img

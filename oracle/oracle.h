@@ -36,10 +36,11 @@ typedef struct oracle_stats {
 int oracle_render(const upbp_scene* scene, const upbp_config* cfg, int first_iteration, int n_iterations,
                   int n_threads, double* out_rgb, float* fb_first, oracle_stats* stats);
 
-/* Renders only light paths [0, n_light_paths) and camera paths [0, n_camera_paths) of one
- * iteration (bounded CPU-baseline sample). Returns wall seconds through *seconds. */
+/* Renders light paths [0, n_light_paths) and camera paths [0, n_camera_paths) of one
+ * iteration (bounded CPU-baseline sample). times[0..2] receive the wall seconds of the light
+ * pass, the photon-map build and the camera pass. */
 int oracle_render_sample(const upbp_scene* scene, const upbp_config* cfg, int iteration, uint32_t n_camera_paths,
-                         int n_threads, double* seconds);
+                         uint32_t n_light_paths, int n_threads, double* times);
 
 /* Brute-force range query (same kinds and layout as upbp_test_range_query). */
 int oracle_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries, uint32_t n_queries,

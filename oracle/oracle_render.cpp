@@ -1591,8 +1591,10 @@ static Box point_box(V p, float r)
 
 // One iteration into fb (W*H*3 floats, zeroed by the caller).
 static void render_iteration(Ctx& c, const upbp_config& cfg, int iteration, int threads, std::vector<float>& fb,
-                             uint32_t n_camera, uint32_t n_light_limit, oracle_stats* st)
+                             uint32_t n_camera, uint32_t n_light_limit, oracle_stats* st, double* times = nullptr)
 {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     const Scene& sc = *c.sc;
     setup_params(sc, cfg, iteration, c.p);
     Params& p = c.p;
@@ -1623,6 +1625,7 @@ static void render_iteration(Ctx& c, const upbp_config& cfg, int iteration, int 
         }
         for (uint32_t i = nl; i < p.n_light; i++) c.path_begin[i] = c.path_end[i] = uint32_t(c.verts.size());
     }
+    const auto t1 = clk::now();
     std::vector<Box> sb, mb, bb;
     for (uint32_t i = 0; i < c.verts.size(); i++) {
         if (c.verts[i].in_medium) {
@@ -1648,6 +1651,7 @@ static void render_iteration(Ctx& c, const upbp_config& cfg, int iteration, int 
     c.bvh_surf.build(sb);
     c.bvh_med.build(mb);
     c.bvh_beam.build(bb);
+    const auto t2 = clk::now();
     if (st) {
         st->light_vertices += c.verts.size();
         st->surface_vertices += c.surf_idx.size();
@@ -1658,10 +1662,19 @@ static void render_iteration(Ctx& c, const upbp_config& cfg, int iteration, int 
         const uint32_t nc = std::min<uint32_t>(p.n_pixels, n_camera);
         std::vector<V> col(nc);
         std::vector<float> sxs(nc), sys(nc);
-        parallel_for(nc, threads, [&](int, uint32_t i) { col[i] = trace_camera_path(c, i, sxs[i], sys[i]); });
+        // a bounded sample takes pixels spread evenly over the image
+        parallel_for(nc, threads, [&](int, uint32_t i) {
+            const uint32_t pix = nc == p.n_pixels ? i : uint32_t(uint64_t(i) * p.n_pixels / nc);
+            col[i] = trace_camera_path(c, pix, sxs[i], sys[i]);
+        });
         for (uint32_t i = 0; i < nc; i++) splat(fb, sc.cam, sxs[i], sys[i], col[i]);
     }
     if (st) st->path_errors += c.errors.exchange(0);
+    if (times) {
+        times[0] = std::chrono::duration<double>(t1 - t0).count();
+        times[1] = std::chrono::duration<double>(t2 - t1).count();
+        times[2] = std::chrono::duration<double>(clk::now() - t2).count();
+    }
 }
 
 } // namespace orc
@@ -1702,9 +1715,9 @@ extern "C" int oracle_render(const upbp_scene* scene, const upbp_config* cfg, in
 }
 
 extern "C" int oracle_render_sample(const upbp_scene* scene, const upbp_config* cfg, int iteration,
-                                    uint32_t n_camera_paths, int n_threads, double* seconds)
+                                    uint32_t n_camera_paths, uint32_t n_light_paths, int n_threads, double* times)
 {
-    if (!scene || !cfg) return UPBP_E_ARG;
+    if (!scene || !cfg || !times) return UPBP_E_ARG;
     Scene sc;
     prepare_scene(scene, cfg->res_x, cfg->res_y, sc);
     Ctx c;
@@ -1716,12 +1729,8 @@ extern "C" int oracle_render_sample(const upbp_scene* scene, const upbp_config* 
     c.bvh_imag.build(ib);
     const int threads = n_threads > 0 ? n_threads : int(std::max(1u, std::thread::hardware_concurrency()));
     std::vector<float> fb(3 * size_t(cfg->res_x) * cfg->res_y, 0.f);
-    // light paths scale with the camera sample so that the photon maps keep their density ratio
-    const uint64_t npx = uint64_t(cfg->res_x) * cfg->res_y;
-    const uint32_t nl = uint32_t(std::max<uint64_t>(1, uint64_t(double(cfg->path_count_per_iter) * n_camera_paths / double(npx))));
-    const auto t0 = std::chrono::steady_clock::now();
-    render_iteration(c, *cfg, iteration, threads, fb, n_camera_paths, nl, nullptr);
-    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // times[0]: light pass, times[1]: photon-map build, times[2]: camera pass (wall seconds)
+    render_iteration(c, *cfg, iteration, threads, fb, n_camera_paths, n_light_paths, nullptr, times);
     return UPBP_OK;
 }
 
