@@ -1,0 +1,134 @@
+"""Pins of the CPU oracle against known answers, closed forms and invariants (no GPU).
+
+Each check is chosen so that a plausible restatement mistake (dropped term, wrong sign,
+swapped operand) fails it: Philox known-answer vectors from the Random123 paper's test
+suite, predicate boundaries from gpuupbp/kernel/intersection.cuh:33-173, Henyey-Greenstein
+normalisation and mean cosine, Fresnel at normal incidence, the camera mapping against its
+inverse, radiance seen directly (no MIS), and agreement of every technique subset.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import box_args
+from gpuupbp_amd import capi
+from oracle import oracle_py
+
+
+def test_philox_known_answers():
+    # Philox4x32-10 known-answer vectors (Random123 kat_vectors)
+    assert list(oracle_py.philox([0, 0, 0, 0], [0, 0])) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(oracle_py.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                                             0x6D5451FD]
+    assert list(oracle_py.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0])) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def _q(kind, prims, queries, r):
+    off, idx = oracle_py.range_query(kind, np.array(prims, np.float32), np.array(queries, np.float32), r)
+    return [sorted(idx[off[i]:off[i + 1]].tolist()) for i in range(len(off) - 1)]
+
+
+def test_point_point_is_closed():
+    # point_point: |p0 - p1|^2 <= r^2 (intersection.cuh:71-75); 0.25 is exact in binary
+    assert _q(0, [[0.25, 0, 0], [0.2500001, 0, 0]], [[0, 0, 0]], 0.25) == [[0]]
+
+
+def test_point_beam_open_ends():
+    # point_beam: t in (0, rt) open, h <= r^2 closed (intersection.cuh:55-68), beam along +x of length 1
+    beam = [[0, 0, 0, 1, 0, 0, 1.0]]
+    q = [[0.0, 0.1, 0], [1.0, 0.1, 0], [0.5, 0.25, 0], [0.5, 0.26, 0]]
+    assert _q(2, beam, q, 0.25) == [[], [], [0], []]
+
+
+def test_segment_point_window():
+    # P-B2D acceptance: min_t < t < max_t (test_intersection_bre, intersection.cuh:88-110)
+    pts = [[0.2, 0.01, 0], [0.6, 0.0, 0], [1.2, 0.0, 0]]
+    q = [[0, 0, 0, 1, 0, 0, 0.2, 1.0]]
+    assert _q(1, pts, q, 0.05) == [[1]]
+
+
+def test_beam_beam_rules():
+    # oblique beam passing 0.01 from the query at t1 = 0.5 (t2 = 0.5) is accepted; a parallel beam
+    # is rejected (|cos^2 - 1| < 1e-5); an exactly perpendicular beam is rejected too, because
+    # t2 = (t1 + d1.o1 - d1.o2) / (d1.d2) is 0/0 there (intersection.cuh:160-161)
+    oblique = [0.2, -0.4, 0.01, 0.6, 0.8, 0, 1.0]
+    parallel = [0, 0.005, 0, 1, 0, 0, 1.0]
+    perpendicular = [0.5, -0.5, 0.01, 0, 1, 0, 1.0]
+    q = [[0, 0, 0, 1, 0, 0, 0.0, 1.0]]
+    assert _q(3, [oblique, parallel, perpendicular], q, 0.02) == [[0]]
+    # t1 outside the query window, and t2 outside the beam
+    assert _q(3, [oblique], [[0, 0, 0, 1, 0, 0, 0.6, 1.0]], 0.02) == [[]]
+    assert _q(3, [[0.2, -0.4, 0.01, 0.6, 0.8, 0, 0.4]], q, 0.02) == [[]]
+
+
+@pytest.mark.parametrize("g", [0.0, 0.5, 0.9, -0.3])
+def test_henyey_greenstein_normalisation_and_mean_cosine(g):
+    n = 20001
+    mu = np.linspace(-1.0, 1.0, n)
+    pdf = np.array([oracle_py.lib().oracle_hg_pdf(g, float(m)) for m in mu])
+    w = np.full(n, 2.0 / (n - 1))
+    w[0] = w[-1] = 1.0 / (n - 1)
+    assert abs(np.sum(2 * math.pi * pdf * w) - 1.0) < 2e-3
+    assert abs(np.sum(2 * math.pi * pdf * mu * w) - g) < 2e-3
+
+
+@pytest.mark.parametrize("g", [0.0, 0.6])
+def test_henyey_greenstein_sampler_matches_pdf(g):
+    import ctypes as C
+    rng = np.random.default_rng(1)
+    cs = []
+    for u0, u1 in rng.uniform(0, 1, (4000, 2)):
+        c, p = C.c_float(), C.c_float()
+        oracle_py.lib().oracle_hg_sample(g, float(u0), float(u1), C.byref(c), C.byref(p))
+        assert abs(p.value - oracle_py.lib().oracle_hg_pdf(g, c.value)) <= 1e-4 * p.value + 1e-6
+        cs.append(c.value)
+    assert abs(np.mean(cs) - g) < 0.03
+
+
+@pytest.mark.parametrize("n", [1.3, 1.5, 2.0])
+def test_fresnel_normal_incidence(n):
+    assert abs(oracle_py.lib().oracle_fresnel(1.0, n) - ((n - 1) / (n + 1)) ** 2) < 1e-6
+    assert oracle_py.lib().oracle_fresnel(0.3, -1.0) == 1.0
+
+
+def test_camera_roundtrip(box_obj):
+    rc, cfg = capi.parse_args(["-s", "-1", box_obj] + box_args("96x64"))
+    sh = capi.SceneHandle(box_obj, cfg.res_x, cfg.res_y)
+    xy = np.random.default_rng(2).uniform([0, 0], [96, 64], (200, 2)).astype(np.float32)
+    d = oracle_py.camera_rays(sh.ptr, cfg, xy)
+    o = np.array(sh.scene.camera.origin[:])
+    back = oracle_py.world_to_raster(sh.ptr, cfg, o + 3.0 * d)
+    assert np.max(np.abs(back - xy)) < 2e-3
+    # image-plane distance: W / (2 tan(fov/2)) (host/camera.hpp:71-72); centre ray is the view axis
+    c = oracle_py.camera_rays(sh.ptr, cfg, np.array([[48.0, 32.0]], np.float32))[0]
+    assert np.allclose(c, [0, 0, -1], atol=1e-5)
+
+
+def test_direct_view_of_light_is_exact(box_obj):
+    # camera sub-path of length 1 hitting the light returns its radiance without MIS
+    # (functs_camera.cuh:130); pixels fully inside the light's image equal Ke = 10
+    rc, cfg = capi.parse_args(["-s", "-1", box_obj] + box_args("64x64"))
+    sh = capi.SceneHandle(box_obj, cfg.res_x, cfg.res_y)
+    img, st = oracle_py.render(sh.ptr, cfg, 0, 1)
+    assert st.path_errors == 0
+    top = img[:16]
+    assert np.sum(np.all(np.abs(top - 10.0) < 1e-4, axis=2)) >= 4
+
+
+def test_technique_subsets_agree(box_obj):
+    # every subset of unbiased-in-the-limit techniques converges to the same image; a wrong MIS
+    # weight in any estimator shifts its subset's mean
+    means = {}
+    for alg in ["upbp_bpt", "upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d", "upbp_surf+pp3d", "upbp_bpt+bb1d",
+                "upbp_bpt+pb2d", "upbp_bpt+bp2d"]:
+        # radius 0.04 keeps the photon-mapping-only subset's variance low at this light-path count
+        args = ["-r", "48x48", "-l", "8", "-a", alg, "-r_initial", "0.04", "-r_alpha", "1"]
+        rc, cfg = capi.parse_args(["-s", "-1", box_obj] + args)
+        sh = capi.SceneHandle(box_obj, cfg.res_x, cfg.res_y)
+        img, _ = oracle_py.render(sh.ptr, cfg, 0, 64)
+        means[alg] = img[6:42, 6:42].mean()
+    ref = means["upbp_bpt"]
+    for alg, m in means.items():
+        assert abs(m - ref) / ref < 0.02, (alg, m, ref)
