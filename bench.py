@@ -78,6 +78,7 @@ def main():
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
 
     r = Renderer(sh.ptr, cfg)
+    print("bench: scene loaded, %d warm-up + %d timed iterations" % (a.warmup, a.steps), file=sys.stderr, flush=True)
     for i in range(a.warmup):
         r.run_iteration(i, want_stats=False)
     r.sync()
@@ -118,13 +119,19 @@ def main():
         },
         "paths_per_s": (npx + cfg.path_count_per_iter) / (ms * 1e-3),
         "queries_per_s": (st.point_queries + st.segment_queries) / (ms * 1e-3),
-        "iteration_breakdown_ms": {"light": st.ms_light, "build": st.ms_build, "camera": st.ms_camera},
+        "iteration_breakdown_ms": {"light": st.ms_light, "build": st.ms_build, "camera": st.ms_camera,
+                                   "camera_bounce": st.ms_bounce, "merge_pb2d": st.ms_merge_pb2d,
+                                   "merge_bb1d": st.ms_merge_bb1d},
         "counters": {"light_steps": st.light_extend_steps, "camera_steps": st.camera_extend_steps,
                      "surface_points": st.surface_vertices, "medium_points": st.medium_vertices, "beams": st.beams,
-                     "bpt_connections": st.bpt_connections, "merge_accepts": st.merge_accepts,
-                     "path_errors": st.path_errors, "beam_drops": st.beam_drops},
+                     "subbeams": st.subbeams, "bpt_connections": st.bpt_connections,
+                     "point_queries": st.point_queries, "segment_queries": st.segment_queries,
+                     "merge_accepts": st.merge_accepts, "pb2d_accepts": st.pb2d_accepts,
+                     "bb1d_accepts": st.bb1d_accepts, "seg_nodes": st.seg_nodes,
+                     "seg_candidates": st.seg_candidates, "path_errors": st.path_errors,
+                     "beam_drops": st.beam_drops},
         "roofline": {
-            "kernel": "k_camera_bounce",
+            "kernel": st.dominant_kernel.decode(),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -135,6 +142,7 @@ def main():
             "algorithmic_bytes_per_launch": st.dominant_bytes / launches,
         },
     }
+    print("bench: GPU done, %.1f ms per iteration" % ms, file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sh.ptr, cfg)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -79,13 +79,21 @@ class Scene(C.Structure):
                 ("scene_radius", C.c_float)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("subbeam_factor", C.c_float), ("merge_blocks_per_cu", C.c_int32)]
+
+
 class IterStats(C.Structure):
     _fields_ = [("ms_light", C.c_double), ("ms_build", C.c_double), ("ms_camera", C.c_double),
-                ("ms_total", C.c_double), ("ms_dominant", C.c_double), ("dominant_launches", C.c_uint64),
+                ("ms_total", C.c_double), ("ms_bounce", C.c_double), ("ms_merge_pb2d", C.c_double),
+                ("ms_merge_bb1d", C.c_double), ("ms_dominant", C.c_double), ("dominant_launches", C.c_uint64),
+                ("dominant_kernel", C.c_char * 32),
                 ("light_vertices", C.c_uint64), ("surface_vertices", C.c_uint64),
-                ("medium_vertices", C.c_uint64), ("beams", C.c_uint64), ("light_extend_steps", C.c_uint64),
-                ("camera_extend_steps", C.c_uint64), ("bpt_connections", C.c_uint64),
-                ("point_queries", C.c_uint64), ("segment_queries", C.c_uint64), ("merge_accepts", C.c_uint64),
+                ("medium_vertices", C.c_uint64), ("beams", C.c_uint64), ("subbeams", C.c_uint64),
+                ("light_extend_steps", C.c_uint64), ("camera_extend_steps", C.c_uint64),
+                ("bpt_connections", C.c_uint64), ("point_queries", C.c_uint64), ("segment_queries", C.c_uint64),
+                ("merge_accepts", C.c_uint64), ("pb2d_accepts", C.c_uint64), ("bb1d_accepts", C.c_uint64),
+                ("seg_nodes", C.c_uint64), ("seg_candidates", C.c_uint64),
                 ("vertex_drops", C.c_uint64), ("beam_drops", C.c_uint64), ("path_errors", C.c_uint64),
                 ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double)]
 
@@ -109,7 +117,9 @@ _SIGS = [
     ("upbp_last_error", C.c_char_p, [C.c_void_p]),
     ("upbp_sync", C.c_int, [C.c_void_p]),
     ("upbp_test_range_query", C.c_int, [C.c_int, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_float,
-                                        C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+                                        C.c_float, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("upbp_tuning_defaults", None, [C.c_void_p]),
+    ("upbp_set_tuning", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
 ]
 

@@ -27,6 +27,8 @@ struct IterParams {
     float norm_bb1d, norm_bp2d, norm_pb2d, norm_pp3d, norm_surf;
     float r_bb1d, r_bp2d, r_pb2d, r_pp3d, r_surf, rad_point_med, rad_beam;
     float r2_bb1d, r2_bp2d, r2_pb2d, r2_pp3d, r2_surf;
+    float sub_len;   // target sub-beam length of the beam map
+    float box_slack; // extra beam-box inflation covering FP32 error of the beam predicates
     uint64_t iter_key;
 };
 
@@ -289,11 +291,58 @@ __device__ __forceinline__ void load_beam(const BeamRec* r, LBeam& x)
 }
 
 // ---- device buffers of one renderer --------------------------------------------------
+// Long beams are cut into sub-beams so that their boxes stay tight. Every sub-beam leaf
+// carries the whole beam (the predicates run on it unchanged) plus the parameter range
+// [lo, hi) it owns; a pair is accepted only by the sub-beam whose range holds the
+// predicate's own beam parameter, so each accepted beam is reported exactly once.
+constexpr uint32_t kMaxSubBeams = 1024;
+constexpr int SB_BB1D = 1; // sub-beam flag: beam belongs to the B-B1D subset (path < bb1d_used)
+
+UPBP_HD uint32_t subbeam_count(float len, float sub_len)
+{
+    if (!(sub_len > 0.f) || !(len > sub_len)) return 1u;
+    const float n = ceilf(len / sub_len);
+    return n >= float(kMaxSubBeams) ? kMaxSubBeams : uint32_t(n);
+}
+
+// sub-beam j of n over [0, len): owned parameter range and the box of its beam piece
+UPBP_HD void subbeam_range(float len, uint32_t j, uint32_t n, float& lo, float& hi, float& ta, float& tb)
+{
+    const float step = len / float(n);
+    lo = j == 0 ? -kInf : float(j) * step;
+    hi = j + 1 == n ? kInf : float(j + 1) * step;
+    ta = j == 0 ? 0.f : lo;
+    tb = j + 1 == n ? len : hi;
+}
+
+UPBP_HD void subbeam_box(f3 o, f3 d, float len, float ta, float tb, float r, float slack, float4& blo, float4& bhi)
+{
+    const f3 a = o + d * ta, e = o + d * tb;
+    const float mag = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(e.x))),
+                            fmaxf(fmaxf(fabsf(e.y), fabsf(e.z)), fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)))));
+    const float rr = r * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f * len + slack;
+    blo = make_float4(fminf(a.x, e.x) - rr, fminf(a.y, e.y) - rr, fminf(a.z, e.z) - rr, 0.f);
+    bhi = make_float4(fmaxf(a.x, e.x) + rr, fmaxf(a.y, e.y) + rr, fmaxf(a.z, e.z) + rr, 0.f);
+}
+
 struct PhotonMaps {
     Bvh surf, med, beam;
     const float4* surf_pts; // sorted: (hit.xyz, vertex slot)
     const float4* med_pts;
-    const float4* beam_geo; // sorted: 2 per beam: (origin.xyz, length), (dir.xyz, beam slot)
+    const float4* beam_geo; // sorted: 3 per sub-beam: (origin.xyz, length), (dir.xyz, beam slot),
+                            // (lo, hi, medium, SB_* flags)
+};
+
+// One camera-segment query (a scattering medium segment of one camera ray), handed from
+// the camera bounce kernel to the P-B2D / B-B1D merge kernels: 7 x float4 = 112 bytes.
+struct __align__(16) SegRec {
+    float4 a; // ro.xyz, dmin
+    float4 b; // rd.xyz, dmax
+    float4 c; // throughput * attenuation before the segment (xyz), pixel
+    float4 d; // rsf, rsr, medium, bits: ray-sampling flags 0-1, prev medium 2, prev specular 3, cam_len << 8
+    float4 e; // d_shared, d_bpt, d_pde, for_inv
+    float4 f; // rev_inv, for_ratio, rev_ratio, last_sin
+    float4 g; // d_bpt_a, d_bpt_b, d_pde_a, d_pde_b
 };
 
 struct Bufs {
@@ -302,19 +351,25 @@ struct Bufs {
     PathState* cstate;
     StackRec* cstack;
     uint32_t* queue[2];
-    uint32_t* qcount; // qcount[k]: paths alive at bounce k (light 0..63, camera 64..127)
+    uint32_t* qcount; // paths alive at bounce k: light [k], camera [64 + k]; segment queries
+                      // [128 + k]; merge work heads [192 + 2k + kind]
     VertexRec* verts;
     uint32_t* nvert; // per light path
     BeamRec* beams;
     uint32_t* nbeam;
     float* fb32;
+    float* mcolor;  // per camera path: P-B2D / B-B1D contributions of the merge kernels
+    SegRec* segq;   // segment queries of the current camera bounce
+    uint32_t seg_cap;
     double* accum;
     unsigned long long* stats; // see STAT_*
     PhotonMaps maps;
 };
 
 enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_POINT_Q, STAT_SEG_Q, STAT_ACCEPT,
-                  STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_COUNT };
+                  STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,
+                  STAT_COUNT };
+constexpr int kQcountWords = 512;
 
 // wave-aggregated counter increment
 __device__ __forceinline__ void stat_add(unsigned long long* stats, int k, uint32_t v)

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -44,15 +45,20 @@ __global__ void k_count_photons(IterParams p, Bufs b, uint32_t* cs, uint32_t* cm
             else ns++;
         }
     }
+    uint32_t nb = 0;
+    if (want_beams) {
+        const uint32_t n = b.nbeam[pid];
+        for (uint32_t k = 0; k < n; k++) nb += subbeam_count(b.beams[size_t(pid) * p.bstride + k].a.w, p.sub_len);
+    }
     cs[pid] = ns;
     cm[pid] = nm;
-    cb[pid] = want_beams ? b.nbeam[pid] : 0u;
+    cb[pid] = nb;
 }
 
 // writes the photon lists in light-path order together with inflated primitive boxes
 __global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const uint32_t* om, const uint32_t* ob,
                                int want_surf, int want_med, int want_beams, uint32_t* ls, float4* slo, float4* shi,
-                               uint32_t* lm, float4* mlo, float4* mhi, uint32_t* lb, float4* blo, float4* bhi)
+                               uint32_t* lm, float4* mlo, float4* mhi, float4* geo, float4* blo, float4* bhi)
 {
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= p.n_light) return;
@@ -84,20 +90,24 @@ __global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const u
             }
         }
     }
-    if (want_beams) {
+    if (want_beams) { // sub-beams: box and (beam, owned parameter range) payload, unsorted
         const uint32_t nb = b.nbeam[pid];
         uint32_t jb = ob[pid];
-        for (uint32_t k = 0; k < nb; k++, jb++) {
+        for (uint32_t k = 0; k < nb; k++) {
             const uint32_t slot = pid * p.bstride + k;
             const BeamRec& r = b.beams[slot];
-            const float4 a = r.a, d = r.b;
-            const float ex = a.x + d.x * a.w, ey = a.y + d.y * a.w, ez = a.z + d.z * a.w;
-            const float mag = fmaxf(fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))),
-                                    fmaxf(fabsf(ex), fmaxf(fabsf(ey), fabsf(ez))));
-            const float rr = p.rad_beam * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f * a.w;
-            lb[jb] = slot;
-            blo[jb] = make_float4(fminf(a.x, ex) - rr, fminf(a.y, ey) - rr, fminf(a.z, ez) - rr, 0.f);
-            bhi[jb] = make_float4(fmaxf(a.x, ex) + rr, fmaxf(a.y, ey) + rr, fmaxf(a.z, ez) + rr, 0.f);
+            const float4 a = r.a, d = r.b, dd = r.d;
+            const int flags = uint32_t(__float_as_int(dd.z)) < p.bb1d_used ? SB_BB1D : 0;
+            const f3 o = mk(a.x, a.y, a.z), dir = mk(d.x, d.y, d.z);
+            const uint32_t n = subbeam_count(a.w, p.sub_len);
+            for (uint32_t j = 0; j < n; j++, jb++) {
+                float lo, hi, ta, tb;
+                subbeam_range(a.w, j, n, lo, hi, ta, tb);
+                subbeam_box(o, dir, a.w, ta, tb, p.rad_beam, p.box_slack, blo[jb], bhi[jb]);
+                geo[3 * size_t(jb)] = a;
+                geo[3 * size_t(jb) + 1] = make_float4(d.x, d.y, d.z, __int_as_float(int(slot)));
+                geo[3 * size_t(jb) + 2] = make_float4(lo, hi, d.w, __int_as_float(flags));
+            }
         }
     }
 }
@@ -112,17 +122,15 @@ __global__ void k_gather_points(const uint32_t* order, const uint32_t* list, con
     out[i] = make_float4(a.x, a.y, a.z, __int_as_float(int(slot)));
 }
 
-__global__ void k_gather_beams(const uint32_t* order, const uint32_t* list, const BeamRec* beams, uint32_t n,
-                               float4* out)
+__global__ void k_gather_beams(const uint32_t* order, const float4* geo_u, uint32_t n, float4* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t slot = list[order[i]];
-    const float4 a = beams[slot].a, d = beams[slot].b;
-    out[2 * i] = a;
-    out[2 * i + 1] = make_float4(d.x, d.y, d.z, __int_as_float(int(slot)));
+    const size_t j = order[i];
+    out[3 * size_t(i)] = geo_u[3 * j];
+    out[3 * size_t(i) + 1] = geo_u[3 * j + 1];
+    out[3 * size_t(i) + 2] = geo_u[3 * j + 2];
 }
-
 __global__ void k_gather_tris(const uint32_t* order, const float4* tri, const float4* nrm, const int2* rec, uint32_t n,
                               float4* tri_s, float4* nrm_s, int2* rec_s)
 {
@@ -157,9 +165,12 @@ struct upbp_renderer {
     bool want_beams;
     Bufs bufs;
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
-    DevBuf cnt, off, tot, list_s, list_m, list_b, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_b;
+    DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
+    DevBuf mcolor, segq;
     BvhStore bvh_s, bvh_m, bvh_b;
-    hipEvent_t ev[2 * kMaxBounces + 8];
+    float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
+    upbp_tuning tune;
+    hipEvent_t ev[4 * kMaxBounces + 8];
 };
 
 namespace {
@@ -345,6 +356,10 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.rad_beam = 0.f;
     if (p.do_bb1d) p.rad_beam = fmaxf(p.rad_beam, r_bb1d);
     if (p.do_bp2d) p.rad_beam = fmaxf(p.rad_beam, r_bp2d);
+    // beam map: sub-beam length and the box slack for the FP32 error of the beam predicates'
+    // line parameters near the parallel cut-off, which grows with the coordinate magnitude
+    p.sub_len = r->tune.subbeam_factor > 0.f ? r->tune.subbeam_factor * p.rad_beam : 0.f;
+    p.box_slack = 2e-4f * r->scene_mag;
     p.r2_bb1d = r_bb1d * r_bb1d;
     p.r2_bp2d = r_bp2d * r_bp2d;
     p.r2_pb2d = r_pb2d * r_pb2d;
@@ -452,6 +467,14 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
         dc.mat_id = cam.mat_id;
         dc.med_id = cam.med_id;
     }
+    { // largest coordinate magnitude of geometry and camera (scales the beam-box slack)
+        float mag = 0.f;
+        for (const upbp_triangles* t : {&scene->real, &scene->imag})
+            for (size_t i = 0; i < size_t(t->count) * 9; i++) mag = fmaxf(mag, fabsf(t->verts[i]));
+        for (int a = 0; a < 3; a++) mag = fmaxf(mag, fabsf(scene->camera.origin[a]));
+        r->scene_mag = fmaxf(mag, 1e-3f);
+    }
+    upbp_tuning_defaults(&r->tune);
     int e = build_tris(r, scene->real, r->real, sc.real);
     if (!e) e = build_tris(r, scene->imag, r->imag, sc.imag);
     if (e) {
@@ -464,12 +487,17 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     const size_t NL = r->n_light, NP = r->n_pixels, NQ = std::max(NL, NP);
     bool ok = !r->lstate.ensure(NL * sizeof(PathState)) && !r->lstack.ensure(NL * sizeof(StackRec)) &&
               !r->cstate.ensure(NP * sizeof(PathState)) && !r->cstack.ensure(NP * sizeof(StackRec)) &&
-              !r->q0.ensure(NQ * 4) && !r->q1.ensure(NQ * 4) && !r->qcount.ensure(256 * 4) &&
+              !r->q0.ensure(NQ * 4) && !r->q1.ensure(NQ * 4) && !r->qcount.ensure(kQcountWords * 4) &&
+              !r->mcolor.ensure(NP * 12) &&
               !r->verts.ensure(NL * r->vstride * sizeof(VertexRec)) && !r->nvert.ensure(NL * 4) &&
               !r->nbeam.ensure(NL * 4) && !r->fb32.ensure(NP * 12) && !r->accum.ensure(NP * 24) &&
               !r->stats.ensure(STAT_COUNT * 8) && !r->cnt.ensure(3 * NL * 4) && !r->off.ensure(3 * NL * 4) &&
               !r->tot.ensure(16);
     if (ok && r->want_beams) ok = !r->beams.ensure(NL * r->bstride * sizeof(BeamRec));
+    // segment queries of one camera bounce: at most kMaxSegments per live camera path
+    const bool seg_merges = (c.algorithm_flags & (T_PB2D | T_BB1D)) != 0;
+    const size_t seg_cap = seg_merges ? NP * kMaxSegments : 1;
+    if (ok) ok = !r->segq.ensure(seg_cap * sizeof(SegRec));
     if (!ok) {
         upbp_destroy(r);
         return fail(nullptr, "device allocation failed", UPBP_E_HIP);
@@ -490,6 +518,9 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     b.beams = r->beams.as<BeamRec>();
     b.nbeam = r->nbeam.as<uint32_t>();
     b.fb32 = r->fb32.as<float>();
+    b.mcolor = r->mcolor.as<float>();
+    b.segq = r->segq.as<SegRec>();
+    b.seg_cap = uint32_t(seg_cap);
     b.accum = r->accum.as<double>();
     b.stats = r->stats.as<unsigned long long>();
     *out = r;
@@ -526,12 +557,12 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
               !r->hi_s.ensure(std::max<size_t>(ns, 1) * 16) && !r->pts_s.ensure(std::max<size_t>(ns, 1) * 16) &&
               !r->list_m.ensure(std::max<size_t>(nm, 1) * 4) && !r->lo_m.ensure(std::max<size_t>(nm, 1) * 16) &&
               !r->hi_m.ensure(std::max<size_t>(nm, 1) * 16) && !r->pts_m.ensure(std::max<size_t>(nm, 1) * 16) &&
-              !r->list_b.ensure(std::max<size_t>(nb, 1) * 4) && !r->lo_b.ensure(std::max<size_t>(nb, 1) * 16) &&
-              !r->hi_b.ensure(std::max<size_t>(nb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(nb, 1) * 32);
+              !r->geo_u.ensure(std::max<size_t>(nb, 1) * 48) && !r->lo_b.ensure(std::max<size_t>(nb, 1) * 16) &&
+              !r->hi_b.ensure(std::max<size_t>(nb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(nb, 1) * 48);
     if (!ok) return fail(r, "photon list allocation failed", UPBP_E_HIP);
     hipLaunchKernelGGL(k_emit_photons, dim3(nblk(NL)), dim3(256), 0, s, p, b, off, off + NL, off + 2 * NL, want_surf,
                        want_med, want_beams, r->list_s.as<uint32_t>(), r->lo_s.as<float4>(), r->hi_s.as<float4>(),
-                       r->list_m.as<uint32_t>(), r->lo_m.as<float4>(), r->hi_m.as<float4>(), r->list_b.as<uint32_t>(),
+                       r->list_m.as<uint32_t>(), r->lo_m.as<float4>(), r->hi_m.as<float4>(), r->geo_u.as<float4>(),
                        r->lo_b.as<float4>(), r->hi_b.as<float4>());
     PhotonMaps& M = b.maps;
     if (lbvh_build(r->bvh_s, r->scratch, r->lo_s.as<float4>(), r->hi_s.as<float4>(), ns, s) ||
@@ -545,8 +576,8 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
         hipLaunchKernelGGL(k_gather_points, dim3(nblk(nm)), dim3(256), 0, s, r->bvh_m.order, r->list_m.as<uint32_t>(),
                            b.verts, nm, r->pts_m.as<float4>());
     if (nb)
-        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(nb)), dim3(256), 0, s, r->bvh_b.order, r->list_b.as<uint32_t>(),
-                           b.beams, nb, r->geo_b.as<float4>());
+        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(nb)), dim3(256), 0, s, r->bvh_b.order, r->geo_u.as<float4>(), nb,
+                           r->geo_b.as<float4>());
     M.surf = {r->bvh_s.nodes, r->bvh_s.order, ns};
     M.med = {r->bvh_m.nodes, r->bvh_m.order, nm};
     M.beam = {r->bvh_b.nodes, r->bvh_b.order, nb};
@@ -573,11 +604,14 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     hipStream_t s = r->stream;
     Bufs& b = r->bufs;
     const int L = int(r->cfg.max_path_length);
+    const bool timed = stats != nullptr; // per-kernel events only when statistics are asked for
     hipEvent_t* ev = r->ev;
     HIPCHK(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, b.qcount, 0u, size_t(256));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(kQcountWords)), dim3(256), 0, s, b.qcount, 0u, size_t(kQcountWords));
     hipLaunchKernelGGL(k_fill, dim3(nblk(3 * size_t(r->n_pixels))), dim3(256), 0, s, reinterpret_cast<uint32_t*>(b.fb32),
                        0u, 3 * size_t(r->n_pixels));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(3 * size_t(r->n_pixels))), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(b.mcolor), 0u, 3 * size_t(r->n_pixels));
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(b.stats), 0u,
                        size_t(2 * STAT_COUNT));
     if (p.trace_light) {
@@ -600,9 +634,14 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     if (p.trace_camera) {
         launch_camera_generate(r->sc, p, b, s);
         for (int k = 0; k < L; k++) {
-            HIPCHK(hipEventRecord(ev[8 + 2 * k], s));
+            hipEvent_t* e = ev + 8 + 4 * k;
+            if (timed) HIPCHK(hipEventRecord(e[0], s));
             launch_camera_bounce(r->sc, p, b, k, s);
-            HIPCHK(hipEventRecord(ev[9 + 2 * k], s));
+            if (timed) HIPCHK(hipEventRecord(e[1], s));
+            launch_seg_merge(r->sc, p, b, 0, k, s);
+            if (timed) HIPCHK(hipEventRecord(e[2], s));
+            launch_seg_merge(r->sc, p, b, 1, k, s);
+            if (timed) HIPCHK(hipEventRecord(e[3], s));
         }
     }
     launch_camera_finish(p, b, float(r->cfg.res_x), float(r->cfg.res_y), s);
@@ -611,7 +650,7 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     if (stats) {
         HIPCHK(hipEventSynchronize(ev[3]));
         memset(stats, 0, sizeof(*stats));
-        uint32_t q[256];
+        uint32_t q[kQcountWords];
         unsigned long long st[STAT_COUNT];
         HIPCHK(hipMemcpy(q, b.qcount, sizeof(q), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(st, b.stats, sizeof(st), hipMemcpyDeviceToHost));
@@ -624,25 +663,61 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         stats->ms_camera = elapsed(ev[2], ev[3]);
         stats->ms_total = elapsed(ev[0], ev[3]);
         if (p.trace_camera)
-            for (int k = 0; k < L; k++) stats->ms_dominant += elapsed(ev[8 + 2 * k], ev[9 + 2 * k]);
-        stats->dominant_launches = p.trace_camera ? uint64_t(L) : 0;
+            for (int k = 0; k < L; k++) {
+                const hipEvent_t* e = ev + 8 + 4 * k;
+                stats->ms_bounce += elapsed(e[0], e[1]);
+                stats->ms_merge_pb2d += elapsed(e[1], e[2]);
+                stats->ms_merge_bb1d += elapsed(e[2], e[3]);
+            }
         stats->surface_vertices = counts[0];
         stats->medium_vertices = counts[1];
-        stats->beams = counts[2];
+        stats->light_vertices = counts[0] + counts[1];
+        stats->subbeams = counts[2];
+        stats->beams = 0;
+        {
+            std::vector<uint32_t> nbeam(r->want_beams ? r->n_light : 0);
+            if (!nbeam.empty()) {
+                HIPCHK(hipMemcpy(nbeam.data(), b.nbeam, nbeam.size() * 4, hipMemcpyDeviceToHost));
+                for (uint32_t v : nbeam) stats->beams += v;
+            }
+        }
         stats->bpt_connections = st[STAT_BPT_CONN];
         stats->point_queries = st[STAT_POINT_Q];
         stats->segment_queries = st[STAT_SEG_Q];
         stats->merge_accepts = st[STAT_ACCEPT];
+        stats->pb2d_accepts = st[STAT_ACC_PB2D];
+        stats->bb1d_accepts = st[STAT_ACC_BB1D];
+        stats->seg_nodes = st[STAT_SEG_NODES];
+        stats->seg_candidates = st[STAT_SEG_CANDS];
         stats->vertex_drops = st[STAT_VDROP];
         stats->beam_drops = st[STAT_BDROP];
         stats->path_errors = st[STAT_ERRORS];
-        // compulsory HBM bytes of the camera bounce kernels: 128-byte state read and write
-        // plus 64 bytes of stack per live path-step, 112 bytes per light vertex read by a
-        // connection and 120 bytes (vertex or beam) per accepted merge candidate
-        stats->dominant_bytes = 320.0 * double(stats->camera_extend_steps) + 112.0 * double(st[STAT_BPT_CONN]) +
-                                120.0 * double(st[STAT_ACCEPT]);
-        stats->algorithmic_bytes = stats->dominant_bytes + 320.0 * double(stats->light_extend_steps) +
-                                   112.0 * double(counts[0] + counts[1]) + 128.0 * double(counts[2]) +
+        // Compulsory HBM bytes. Camera bounce kernels: 128-byte state read and write plus 64
+        // bytes of stack per live path-step, 112 bytes per light vertex read by a connection or
+        // an accepted point merge, 112 bytes per segment query written. Segment merge kernels:
+        // the 112-byte query read once per kernel, 112 bytes per accepted medium point (P-B2D),
+        // 128 bytes per accepted beam (B-B1D).
+        const double point_acc = double(st[STAT_ACCEPT] - st[STAT_ACC_PB2D] - st[STAT_ACC_BB1D]);
+        const double nq = double(st[STAT_SEG_Q]);
+        const double b_bounce = 320.0 * double(stats->camera_extend_steps) + 112.0 * double(st[STAT_BPT_CONN]) +
+                                112.0 * point_acc + 112.0 * nq;
+        const double b_pb2d = (p.build_pb2d ? 112.0 * nq : 0.0) + 112.0 * double(st[STAT_ACC_PB2D]);
+        const double b_bb1d = (p.build_bb1d ? 112.0 * nq : 0.0) + 128.0 * double(st[STAT_ACC_BB1D]);
+        struct Group {
+            const char* name;
+            double ms, bytes;
+        } g[3] = {{"k_camera_bounce", stats->ms_bounce, b_bounce},
+                  {"k_seg_merge<0> (P-B2D)", stats->ms_merge_pb2d, b_pb2d},
+                  {"k_seg_merge<1> (B-B1D)", stats->ms_merge_bb1d, b_bb1d}};
+        int dom = 0;
+        for (int k = 1; k < 3; k++)
+            if (g[k].ms > g[dom].ms) dom = k;
+        stats->ms_dominant = g[dom].ms;
+        stats->dominant_bytes = g[dom].bytes;
+        stats->dominant_launches = p.trace_camera ? uint64_t(L) : 0;
+        snprintf(stats->dominant_kernel, sizeof(stats->dominant_kernel), "%s", g[dom].name);
+        stats->algorithmic_bytes = b_bounce + b_pb2d + b_bb1d + 320.0 * double(stats->light_extend_steps) +
+                                   112.0 * double(counts[0] + counts[1]) + 128.0 * double(stats->beams) +
                                    88.0 * double(counts[0] + counts[1] + counts[2]) + 60.0 * double(r->n_pixels);
     }
     return UPBP_OK;
@@ -656,6 +731,22 @@ extern "C" int upbp_read_accum(upbp_renderer* r, double* rgb, int iters_done)
     HIPCHK(hipMemcpy(rgb, r->bufs.accum, n * sizeof(double), hipMemcpyDeviceToHost));
     const double inv = 1.0 / double(iters_done);
     for (size_t i = 0; i < n; i++) rgb[i] *= inv;
+    return UPBP_OK;
+}
+
+extern "C" void upbp_tuning_defaults(upbp_tuning* t)
+{
+    if (!t) return;
+    memset(t, 0, sizeof(*t));
+    t->subbeam_factor = 16.f;
+    t->merge_blocks_per_cu = 0;
+}
+
+extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
+{
+    if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
+    if (!(t->subbeam_factor >= 0.f) || t->merge_blocks_per_cu < 0) return fail(r, "invalid tuning", UPBP_E_ARG);
+    r->tune = *t;
     return UPBP_OK;
 }
 
@@ -679,8 +770,8 @@ extern "C" void upbp_destroy(upbp_renderer* r)
     }
     for (DevBuf* d : {&r->lstate, &r->lstack, &r->cstate, &r->cstack, &r->q0, &r->q1, &r->qcount, &r->verts, &r->nvert,
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
-                      &r->list_m, &r->list_b, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
-                      &r->pts_m, &r->geo_b})
+                      &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
+                      &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq})
         d->release();
     lbvh_free(r->bvh_s);
     lbvh_free(r->bvh_m);

@@ -14,30 +14,23 @@ using namespace upbp_host;
 
 namespace {
 
-__global__ void k_prim_boxes(int kind, const float* prims, uint32_t n, float r, float4* lo, float4* hi)
+__global__ void k_point_boxes(const float* prims, uint32_t n, float r, float4* lo, float4* hi)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (kind <= 1) {
-        const float* q = prims + 3 * size_t(i);
-        const float mag = fmaxf(fabsf(q[0]), fmaxf(fabsf(q[1]), fabsf(q[2])));
-        const float rr = r * (1.f + 1e-5f) + 1e-6f * mag;
-        lo[i] = make_float4(q[0] - rr, q[1] - rr, q[2] - rr, 0.f);
-        hi[i] = make_float4(q[0] + rr, q[1] + rr, q[2] + rr, 0.f);
-    } else {
-        const float* q = prims + 7 * size_t(i);
-        const float e[3] = {q[0] + q[3] * q[6], q[1] + q[4] * q[6], q[2] + q[5] * q[6]};
-        float mag = 0.f;
-        for (int a = 0; a < 3; a++) mag = fmaxf(mag, fmaxf(fabsf(q[a]), fabsf(e[a])));
-        const float rr = r * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f * q[6];
-        lo[i] = make_float4(fminf(q[0], e[0]) - rr, fminf(q[1], e[1]) - rr, fminf(q[2], e[2]) - rr, 0.f);
-        hi[i] = make_float4(fmaxf(q[0], e[0]) + rr, fmaxf(q[1], e[1]) + rr, fmaxf(q[2], e[2]) + rr, 0.f);
-    }
+    const float* q = prims + 3 * size_t(i);
+    const float mag = fmaxf(fabsf(q[0]), fmaxf(fabsf(q[1]), fabsf(q[2])));
+    const float rr = r * (1.f + 1e-5f) + 1e-6f * mag;
+    lo[i] = make_float4(q[0] - rr, q[1] - rr, q[2] - rr, 0.f);
+    hi[i] = make_float4(q[0] + rr, q[1] + rr, q[2] + rr, 0.f);
 }
 
 // pass 0 counts accepted primitives per query, pass 1 writes them at offsets[q]
-__global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const float* queries, uint32_t nq, float r2,
-                        uint32_t* counts, const uint32_t* offsets, uint32_t* out, uint32_t* stack_fail)
+// Beams (kinds 2, 3) enter the LBVH as sub-beams: leaf -> (beam, owned parameter range), and a
+// pair counts only for the sub-beam whose range holds the predicate's beam parameter.
+__global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const uint32_t* sub_prim,
+                        const float2* sub_rng, const float* queries, uint32_t nq, float r2, uint32_t* counts,
+                        const uint32_t* offsets, uint32_t* out, uint32_t* stack_fail)
 {
     const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= nq) return;
@@ -52,13 +45,19 @@ __global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const f
         const float* q = queries + 3 * size_t(qi);
         const f3 P = mk(q[0], q[1], q[2]);
         ok = bvh_point_query(bvh, P, [&](uint32_t k) {
-            const uint32_t prim = bvh.order[k];
             if (kind == 0) {
+                const uint32_t prim = bvh.order[k];
                 const float* x = prims + 3 * size_t(prim);
                 if (pred_point_point(P, mk(x[0], x[1], x[2]), r2)) accept(prim);
             } else {
+                const uint32_t prim = sub_prim[bvh.order[k]];
+                const float2 rg = sub_rng[bvh.order[k]];
                 const float* x = prims + 7 * size_t(prim);
-                if (pred_point_beam(mk(x[0], x[1], x[2]), mk(x[3], x[4], x[5]), x[6], P, r2)) accept(prim);
+                const f3 bo = mk(x[0], x[1], x[2]), bd = mk(x[3], x[4], x[5]);
+                float t, h;
+                if (pred_point_beam(bo, bd, x[6], P, r2) && pred_bre(t, h, bo, bd, 0.f, x[6], P, r2) && t >= rg.x &&
+                    t < rg.y)
+                    accept(prim);
             }
         });
     } else {
@@ -66,17 +65,20 @@ __global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const f
         const f3 O = mk(q[0], q[1], q[2]), D = mk(q[3], q[4], q[5]);
         const float t0 = q[6], t1 = q[7];
         ok = bvh_ray_query(bvh, O, D, t0, t1, [&](uint32_t k, float&) {
-            const uint32_t prim = bvh.order[k];
             if (kind == 1) {
+                const uint32_t prim = bvh.order[k];
                 const float* x = prims + 3 * size_t(prim);
                 const f3 X = mk(x[0], x[1], x[2]);
                 float d, h;
                 if (pred_point_beam(O, D, t1, X, r2) && pred_bre(d, h, O, D, t0, t1, X, r2)) accept(prim);
             } else {
+                const uint32_t prim = sub_prim[bvh.order[k]];
+                const float2 rg = sub_rng[bvh.order[k]];
                 const float* x = prims + 7 * size_t(prim);
                 float dd, st, ta, tb;
                 if (pred_beam_beam(O, D, t0, t1, mk(x[0], x[1], x[2]), mk(x[3], x[4], x[5]), 0.f, x[6], r2, dd, st, ta,
-                                   tb))
+                                   tb) &&
+                    tb >= rg.x && tb < rg.y)
                     accept(prim);
             }
         });
@@ -135,47 +137,82 @@ inline uint32_t nb(size_t n) { return uint32_t((n + 255) / 256); }
 } // namespace
 
 extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries,
-                                     uint32_t n_queries, float radius, uint32_t* offsets, uint32_t* indices,
-                                     uint64_t max_indices, uint64_t* n_indices)
+                                     uint32_t n_queries, float radius, float subbeam_len, uint32_t* offsets,
+                                     uint32_t* indices, uint64_t max_indices, uint64_t* n_indices)
 {
     if (kind < 0 || kind > 3 || !offsets || (n_prims && !prims) || (n_queries && !queries)) return UPBP_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
     const size_t pw = (kind <= 1) ? 3 : 7, qw = (kind == 0 || kind == 2) ? 3 : 8;
+    // beams: sub-beam boxes, owning beam and parameter range, built on the host
+    std::vector<float4> blo, bhi;
+    std::vector<uint32_t> sprim;
+    std::vector<float2> srng;
+    if (kind >= 2) {
+        float mag = 0.f;
+        for (size_t i = 0; i < n_prims * pw; i++) mag = fmaxf(mag, fabsf(prims[i]));
+        for (size_t i = 0; i < n_queries * qw; i++) mag = fmaxf(mag, fabsf(queries[i]));
+        const float slack = 2e-4f * mag;
+        for (uint32_t i = 0; i < n_prims; i++) {
+            const float* x = prims + 7 * size_t(i);
+            const f3 o = mk(x[0], x[1], x[2]), d = mk(x[3], x[4], x[5]);
+            const uint32_t n = subbeam_count(x[6], subbeam_len);
+            for (uint32_t j = 0; j < n; j++) {
+                float lo, hi, ta, tb;
+                float4 a, b;
+                subbeam_range(x[6], j, n, lo, hi, ta, tb);
+                subbeam_box(o, d, x[6], ta, tb, radius, slack, a, b);
+                blo.push_back(a);
+                bhi.push_back(b);
+                sprim.push_back(i);
+                srng.push_back(make_float2(lo, hi));
+            }
+        }
+    }
+    const uint32_t n_leaves = kind >= 2 ? uint32_t(sprim.size()) : n_prims;
     Tmp t;
     float* dp = t.alloc<float>(n_prims * pw);
     float* dq = t.alloc<float>(n_queries * qw);
-    float4* lo = t.alloc<float4>(n_prims);
-    float4* hi = t.alloc<float4>(n_prims);
+    float4* lo = t.alloc<float4>(n_leaves);
+    float4* hi = t.alloc<float4>(n_leaves);
+    uint32_t* dsp = t.alloc<uint32_t>(n_leaves);
+    float2* dsr = t.alloc<float2>(n_leaves);
     uint32_t* cnt = t.alloc<uint32_t>(n_queries + 1);
     uint32_t* off = t.alloc<uint32_t>(n_queries + 1);
     uint32_t* fail_count = t.alloc<uint32_t>(1);
-    if (!dp || !dq || !lo || !hi || !cnt || !off || !fail_count) return UPBP_E_HIP;
+    if (!dp || !dq || !lo || !hi || !dsp || !dsr || !cnt || !off || !fail_count) return UPBP_E_HIP;
     hipStream_t s = nullptr;
     if (n_prims) (void)hipMemcpy(dp, prims, n_prims * pw * 4, hipMemcpyHostToDevice);
     if (n_queries) (void)hipMemcpy(dq, queries, n_queries * qw * 4, hipMemcpyHostToDevice);
+    if (kind >= 2 && n_leaves) {
+        (void)hipMemcpy(lo, blo.data(), n_leaves * 16, hipMemcpyHostToDevice);
+        (void)hipMemcpy(hi, bhi.data(), n_leaves * 16, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dsp, sprim.data(), n_leaves * 4, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dsr, srng.data(), n_leaves * 8, hipMemcpyHostToDevice);
+    }
     (void)hipMemset(fail_count, 0, 4);
     BvhStore st;
     BvhScratch sc;
-    if (n_prims) hipLaunchKernelGGL(k_prim_boxes, dim3(nb(n_prims)), dim3(256), 0, s, kind, dp, n_prims, radius, lo, hi);
-    if (lbvh_build(st, sc, lo, hi, n_prims, s)) {
+    if (kind <= 1 && n_prims)
+        hipLaunchKernelGGL(k_point_boxes, dim3(nb(n_prims)), dim3(256), 0, s, dp, n_prims, radius, lo, hi);
+    if (lbvh_build(st, sc, lo, hi, n_leaves, s)) {
         lbvh_free(st);
         lbvh_scratch_free(sc);
         return UPBP_E_HIP;
     }
-    const Bvh bvh{st.nodes, st.order, n_prims};
+    const Bvh bvh{st.nodes, st.order, n_leaves};
     const float r2 = radius * radius;
     if (n_queries)
-        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 0, bvh, dp, dq, n_queries, r2, cnt, off,
-                           nullptr, fail_count);
+        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 0, bvh, dp, dsp, dsr, dq, n_queries,
+                           r2, cnt, off, nullptr, fail_count);
     uint32_t* total = off + n_queries;
     int rc = scan_exclusive(cnt, off, n_queries, total, sc, s);
     uint32_t host_total = 0, fails = 0;
     if (!rc) rc = int(hipMemcpy(&host_total, total, 4, hipMemcpyDeviceToHost));
     uint32_t* dout = rc ? nullptr : t.alloc<uint32_t>(host_total);
     if (!rc && n_queries && dout)
-        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 1, bvh, dp, dq, n_queries, r2, cnt, off,
-                           dout, fail_count);
+        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 1, bvh, dp, dsp, dsr, dq, n_queries,
+                           r2, cnt, off, dout, fail_count);
     if (!rc) rc = int(hipDeviceSynchronize());
     if (!rc) rc = int(hipMemcpy(offsets, off, (size_t(n_queries) + 1) * 4, hipMemcpyDeviceToHost));
     if (!rc && indices && host_total)

@@ -140,12 +140,18 @@ typedef struct upbp_scene {
 /* Per-iteration statistics (device counters plus HIP-event times). */
 typedef struct upbp_iter_stats {
     double ms_light, ms_build, ms_camera, ms_total;
+    double ms_bounce;              /* camera bounce kernels (intersector, BPT, point merges) */
+    double ms_merge_pb2d;          /* P-B2D segment merge kernels */
+    double ms_merge_bb1d;          /* B-B1D segment merge kernels */
     double ms_dominant;            /* summed duration of the dominant kernel in this iteration */
     uint64_t dominant_launches;    /* its launch count */
-    uint64_t light_vertices, surface_vertices, medium_vertices, beams;
+    char dominant_kernel[32];      /* its name */
+    uint64_t light_vertices, surface_vertices, medium_vertices, beams, subbeams;
     uint64_t light_extend_steps, camera_extend_steps;
     uint64_t bpt_connections, point_queries, segment_queries;
-    uint64_t merge_accepts, vertex_drops, beam_drops, path_errors;
+    uint64_t merge_accepts, pb2d_accepts, bb1d_accepts;
+    uint64_t seg_nodes, seg_candidates; /* LBVH nodes visited / leaves tested by segment queries */
+    uint64_t vertex_drops, beam_drops, path_errors;
     double algorithmic_bytes;      /* compulsory HBM bytes of this iteration */
     double dominant_bytes;         /* compulsory HBM bytes of the dominant kernel */
 } upbp_iter_stats;
@@ -197,6 +203,17 @@ int upbp_generate_scene(const char* kind, uint32_t seed, const char* obj_path);
 /* Blocks until all work queued by r has finished. */
 int upbp_sync(upbp_renderer* r);
 
+/* Launch and data-structure knobs of one renderer (no reference counterpart: the OptiX
+ * build had none). They change speed only, never which photons a query accepts. */
+typedef struct upbp_tuning {
+    float subbeam_factor;     /* beams are cut into pieces of about subbeam_factor x the beam
+                                 radius for the beam LBVH; 0 keeps whole beams (default 16) */
+    int32_t merge_blocks_per_cu; /* resident blocks per CU of the segment merge kernels; 0 = as
+                                    many as fit (default) */
+} upbp_tuning;
+void upbp_tuning_defaults(upbp_tuning* t);
+int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
+
 /* ---- test hooks (parity tests and autotuning only) ----------------------
  * Range queries over a device LBVH, returning the accepted index set of every
  * query in CSR form (offsets[n_queries+1], indices). They replace the OptiX
@@ -208,10 +225,12 @@ int upbp_sync(upbp_renderer* r);
  *   kind 3: segment query over beams, test_intersection_beam_beam   (B-B1D)
  * points: n_prims*3 floats; beams: n_prims*7 floats (origin, unit dir, length);
  * queries: kind 0/2: 3 floats, kind 1/3: 8 floats (origin, unit dir, t0, t1).
+ * Beams are cut into sub-beams of length about subbeam_len (0: whole beams) as in the
+ * renderer's beam map; each accepted beam is still reported once.
  * indices has room for max_indices entries; *n_indices receives the total. */
 int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries,
-                          uint32_t n_queries, float radius, uint32_t* offsets, uint32_t* indices,
-                          uint64_t max_indices, uint64_t* n_indices);
+                          uint32_t n_queries, float radius, float subbeam_len, uint32_t* offsets,
+                          uint32_t* indices, uint64_t max_indices, uint64_t* n_indices);
 
 /* Closest-hit / all-hits ray queries against a triangle LBVH (replaces the optixTrace calls of
  * gpuupbp/kernel/intersector.cuh:120,214). rays: 8 floats each (origin, dir, tmin, tmax).

@@ -59,6 +59,42 @@ def test_range_query_matches_brute_force(kind, radius):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("kind,sub", [(2, 0.0), (2, 0.07), (3, 0.0), (3, 0.03), (3, 0.31)])
+def test_range_query_subbeams_report_each_beam_once(kind, sub):
+    # beams cut into sub-beams (as the renderer's beam map does) give the brute-force index set
+    rng = np.random.default_rng(200 + kind)
+    prims, q = _cases(kind, rng, 5000, 1200)
+    go, gi = gpu.range_query(kind, prims, q, 0.02, subbeam_len=sub)
+    oo, oi = oracle_py.range_query(kind, prims, q, 0.02)
+    assert len(gi) == len(oi) and len(gi) > 50
+    for a, b in zip(_sets(go, gi), _sets(oo, oi)):
+        assert np.array_equal(a, b)
+
+
+def test_range_query_near_parallel_beams():
+    # beams within a few degrees of the query direction, around the parallel cut-off
+    # |cos^2 - 1| < 1e-5 of test_intersection_beam_beam (intersection.cuh:153), far from the origin
+    rng = np.random.default_rng(11)
+    n = 4000
+    qd = _unit(rng.normal(size=(300, 3)))
+    qo = rng.uniform(2.0, 6.0, (300, 3))
+    q = np.hstack([qo, qd, np.zeros((300, 1)), np.full((300, 1), 4.0)]).astype(np.float32)
+    k = rng.integers(0, 300, n)
+    ang = 10.0 ** rng.uniform(-3.3, -1.5, n)
+    perp = _unit(np.cross(qd[k], rng.normal(size=(n, 3))))
+    d = _unit(qd[k] + perp * ang[:, None])
+    t = rng.uniform(0.0, 4.0, (n, 1))
+    off = _unit(rng.normal(size=(n, 3))) * rng.uniform(0.0, 0.03, (n, 1))
+    o = qo[k] + qd[k] * t + off - d * rng.uniform(0.1, 1.0, (n, 1))
+    prims = np.hstack([o, d, rng.uniform(0.2, 2.0, (n, 1))]).astype(np.float32)
+    for sub in (0.0, 0.05):
+        go, gi = gpu.range_query(3, prims, q, 0.02, subbeam_len=sub)
+        oo, oi = oracle_py.range_query(3, prims, q, 0.02)
+        assert len(gi) == len(oi) and len(gi) > 100
+        for a, b in zip(_sets(go, gi), _sets(oo, oi)):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_range_query_edge_sizes(kind):
     rng = np.random.default_rng(7)

@@ -210,4 +210,4 @@ def test_no_gpu_reports_error():
     cfg = capi.Config()
     capi.lib().upbp_config_defaults(C.byref(cfg))
     out = C.c_void_p()
-    assert capi.lib().upbp_test_range_query(0, None, 0, None, 0, 0.1, (C.c_uint32 * 1)(), None, 0, None) < 0
+    assert capi.lib().upbp_test_range_query(0, None, 0, None, 0, 0.1, 0.0, (C.c_uint32 * 1)(), None, 0, None) < 0
