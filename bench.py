@@ -62,6 +62,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--subbeam", type=float, default=None, help="sub-beam length in beam radii")
+    ap.add_argument("--piece", type=float, default=None, help="query piece length in merge radii")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the plain copy kernel run once for bandwidth and counter calibration (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the iteration-sharded multi-GPU path is not built", file=sys.stderr)
@@ -77,7 +81,15 @@ def main():
     assert rc == 0
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
 
+    copy = None
+    if a.copy_mib > 0:
+        import ctypes as C
+        cms = C.c_double()
+        nbytes = a.copy_mib << 20
+        if capi.lib().upbp_test_copy(nbytes, 3, C.byref(cms)) == 0:
+            copy = {"bytes": nbytes, "ms": cms.value, "GB_per_s": 2.0 * nbytes / (cms.value * 1e-3) / 1e9}
     r = Renderer(sh.ptr, cfg)
+    tune = r.set_tuning(subbeam_factor=a.subbeam, query_piece_factor=a.piece)
     print("bench: scene loaded, %d warm-up + %d timed iterations" % (a.warmup, a.steps), file=sys.stderr, flush=True)
     for i in range(a.warmup):
         r.run_iteration(i, want_stats=False)
@@ -116,6 +128,7 @@ def main():
             "max_path_length": 12,
             "light_paths": int(cfg.path_count_per_iter),
             "parallelism": "single GPU",
+            "tuning": {"subbeam_factor": tune.subbeam_factor, "query_piece_factor": tune.query_piece_factor},
         },
         "paths_per_s": (npx + cfg.path_count_per_iter) / (ms * 1e-3),
         "queries_per_s": (st.point_queries + st.segment_queries) / (ms * 1e-3),
@@ -128,7 +141,8 @@ def main():
                      "point_queries": st.point_queries, "segment_queries": st.segment_queries,
                      "merge_accepts": st.merge_accepts, "pb2d_accepts": st.pb2d_accepts,
                      "bb1d_accepts": st.bb1d_accepts, "seg_nodes": st.seg_nodes,
-                     "seg_candidates": st.seg_candidates, "path_errors": st.path_errors,
+                     "seg_candidates": st.seg_candidates, "seg_pieces": st.seg_pieces,
+                     "path_errors": st.path_errors,
                      "beam_drops": st.beam_drops},
         "roofline": {
             "kernel": st.dominant_kernel.decode(),
@@ -140,6 +154,7 @@ def main():
             "traffic": None,
             "avg_launch_ms": dom_ms,
             "algorithmic_bytes_per_launch": st.dominant_bytes / launches,
+            "measured_copy": copy,
         },
     }
     print("bench: GPU done, %.1f ms per iteration" % ms, file=sys.stderr, flush=True)

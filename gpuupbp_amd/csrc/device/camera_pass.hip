@@ -14,15 +14,21 @@ namespace upbp {
 // kernel/pde/functs_pb2d.cuh:226-342 and functs_bb1d.cuh:273-397: one P-B2D / B-B1D query per
 // scattering medium segment of the camera ray, carrying the camera state the merge needs
 // (throughput times the attenuation before the segment, the accumulated ray-sampling pdfs
-// and the MIS quantities of the segment's start). Counts them, or writes them to out.
-template <bool WRITE>
-__device__ __forceinline__ uint32_t segment_queries(const DScene& sc, const VolSeg* segs, const LiteSeg* lite, int n,
-                                                    unsigned flags, uint32_t qbt, const Sub& s, uint32_t pix,
-                                                    SegRec* out)
+// and the MIS quantities of the segment's start). MODE 0 counts the queries and sums their
+// lengths, MODE 1 also counts their pieces, MODE 2 writes queries and pieces.
+struct SegCount {
+    uint32_t recs = 0, pieces = 0;
+    float len = 0.f;
+};
+
+template <int MODE>
+__device__ __forceinline__ SegCount segment_queries(const DScene& sc, const VolSeg* segs, const LiteSeg* lite, int n,
+                                                    unsigned flags, uint32_t qbt, float piece_len, const Sub& s,
+                                                    uint32_t pix, SegRec* out, uint32_t rec_base, uint2* pout)
 {
     f3 att = mk(1.f);
     float rsf = 1.f, rsr = 1.f;
-    uint32_t c = 0;
+    SegCount c;
     for (int i = 0; i < n; i++) {
         const float dmin = segs ? segs[i].dmin : lite[i].dmin;
         const float dmax = segs ? segs[i].dmax : lite[i].dmax;
@@ -30,12 +36,16 @@ __device__ __forceinline__ uint32_t segment_queries(const DScene& sc, const VolS
         if (mid < 0) continue;
         const DMedium& m = sc.media[mid];
         if (m.has_scattering) {
-            if (WRITE) {
+            const uint32_t np = MODE == 0 ? 0u : piece_count(dmax - dmin, piece_len);
+            if (MODE == 2) {
                 const SegQ q{s.origin, s.dir, dmin, dmax, mid, rsf, rsr, kEndInMedium | (i == 0 ? flags : 0u),
                              s.path_length};
-                store_segrec(out + c, q, s.sw, s.throughput * att, pix);
+                store_segrec(out + c.recs, q, s.sw, s.throughput * att, pix);
+                for (uint32_t k = 0; k < np; k++) pout[c.pieces + k] = make_uint2(rec_base + c.recs, k | (np << 16));
             }
-            c++;
+            c.recs++;
+            c.pieces += np;
+            c.len += dmax - dmin;
         }
         if (segs) {
             if (qbt == T_BEAM_SHORT) att *= segs[i].att / segs[i].pf;
@@ -55,13 +65,14 @@ __device__ __forceinline__ uint32_t segment_queries(const DScene& sc, const VolS
     return c;
 }
 
-// Wave-aggregated allocation of c < 16 slots per lane: one atomic per wave.
+// Wave-aggregated allocation of c < 2^BITS slots per lane: one atomic per wave.
+template <int BITS>
 __device__ __forceinline__ uint32_t wave_alloc(uint32_t c, uint32_t* counter)
 {
     const uint32_t lane = __lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t pre = 0, tot = 0;
-    for (int bit = 0; bit < 4; bit++) {
+    for (int bit = 0; bit < BITS; bit++) {
         const uint64_t m = __ballot((c >> bit) & 1u);
         pre += uint32_t(__popcll(m & lt)) << bit;
         tot += uint32_t(__popcll(m)) << bit;
@@ -125,7 +136,7 @@ __global__ void __launch_bounds__(256) k_camera_generate(DScene sc, IterParams p
 }
 
 struct CamCounters {
-    uint32_t conn = 0, pq = 0, sq = 0, acc = 0;
+    uint32_t conn = 0, pq = 0, sq = 0, acc = 0, pieces = 0;
 };
 
 // One segment of kernel/trace_camera.cu:79-382. Returns whether the path continues.
@@ -151,15 +162,21 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         const VolSeg* sg = use_full ? segs : nullptr;
         const LiteSeg* lt = use_full ? nullptr : lite;
         const int ns = use_full ? nseg : nlite;
-        const uint32_t c = segment_queries<false>(sc, sg, lt, ns, rf, p.qbt, s, pix, nullptr);
-        const uint32_t at = wave_alloc(c, &b.qcount[128 + bounce]);
-        if (c) {
-            if (at + c > b.seg_cap) {
+        const SegCount c0 = segment_queries<0>(sc, sg, lt, ns, rf, p.qbt, 0.f, s, pix, nullptr, 0, nullptr);
+        // piece length: the tuned length, stretched so that one ray makes at most kPiecesPerRay
+        // pieces plus one per segment
+        const float plen = p.piece_len > 0.f ? fmaxf(p.piece_len, c0.len / float(kPiecesPerRay)) : 0.f;
+        const SegCount c1 = segment_queries<1>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, nullptr, 0, nullptr);
+        const uint32_t at = wave_alloc<4>(c1.recs, &b.qcount[128 + bounce]);
+        const uint32_t pat = wave_alloc<6>(c1.pieces, &b.qcount[320 + bounce]);
+        if (c1.recs) {
+            if (at + c1.recs > b.seg_cap || pat + c1.pieces > b.piece_cap) {
                 ret = S_ERR_SEGMENTS;
                 return false;
             }
-            segment_queries<true>(sc, sg, lt, ns, rf, p.qbt, s, pix, b.segq + at);
-            cc.sq += c;
+            segment_queries<2>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, b.segq + at, at, b.pieces + pat);
+            cc.sq += c1.recs;
+            cc.pieces += c1.pieces;
         }
     }
     if (!hit) {
@@ -334,6 +351,7 @@ __global__ void __launch_bounds__(256) k_camera_bounce(DScene sc, IterParams p, 
     stat_add(b.stats, STAT_BPT_CONN, cc.conn);
     stat_add(b.stats, STAT_POINT_Q, cc.pq);
     stat_add(b.stats, STAT_SEG_Q, cc.sq);
+    stat_add(b.stats, STAT_SEG_PIECES, cc.pieces);
     stat_add(b.stats, STAT_ACCEPT, cc.acc);
     st.color = color;
     if (cont) {

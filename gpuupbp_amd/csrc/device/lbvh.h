@@ -184,7 +184,7 @@ struct DevBuf {
 };
 
 struct BvhScratch {
-    DevBuf keys, keys2, vals2, hist, scan, parent, flags, bounds;
+    DevBuf keys, keys2, vals2, hist, scan, lvl, meta, bounds;
 };
 
 // Primitive boxes are given as lo/hi float4 arrays (w unused), already inflated.

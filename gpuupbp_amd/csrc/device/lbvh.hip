@@ -217,7 +217,7 @@ __device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j)
     return __clz(a ^ b);
 }
 
-__global__ void k_karras(const uint32_t* keys, int n, BNode* nodes, int* parent)
+__global__ void k_karras(const uint32_t* keys, int n, BNode* nodes)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -241,13 +241,8 @@ __global__ void k_karras(const uint32_t* keys, int n, BNode* nodes, int* parent)
     }
     const int g = i + s * d + (d < 0 ? -1 : 0);
     const int lo = i < j ? i : j, hi = i < j ? j : i;
-    // parent[] indexes inner nodes as k and leaves as (n - 1) + leaf
-    const int c0 = (lo == g) ? ~g : g;
-    const int c1 = (hi == g + 1) ? ~(g + 1) : g + 1;
-    nodes[i].c0 = c0;
-    nodes[i].c1 = c1;
-    parent[c0 < 0 ? (n - 1) + ~c0 : c0] = i;
-    parent[c1 < 0 ? (n - 1) + ~c1 : c1] = i;
+    nodes[i].c0 = (lo == g) ? ~g : g;
+    nodes[i].c1 = (hi == g + 1) ? ~(g + 1) : g + 1;
 }
 
 __device__ __forceinline__ void write_child_box(BNode* nd, int slot, float4 a, float4 b)
@@ -269,29 +264,82 @@ __device__ __forceinline__ void write_child_box(BNode* nd, int slot, float4 a, f
     }
 }
 
-__global__ void k_refit(const float4* lo, const float4* hi, const uint32_t* order, int n, BNode* nodes,
-                        const int* parent, uint32_t* flags)
+// Level-synchronous refit. A breadth-first pass lists the inner nodes level by level
+// (meta[l] = first entry of level l in lvl[], meta[kMaxLevels + 1 + l] = its size), then
+// one launch per level, deepest first, writes each node's two child boxes from the leaf
+// boxes or from the already refitted child nodes. Kernel boundaries order the levels, so
+// no device-wide fences or atomics on the boxes are needed. A Karras tree over 30-bit
+// codes with index tie-breaks is at most 30 + 32 levels deep.
+constexpr int kMaxLevels = 64;
+
+__global__ void k_level_init(uint32_t* meta, uint32_t* lvl)
 {
-    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
-    if (leaf >= n) return;
-    const uint32_t prim = order[leaf];
-    float4 a = lo[prim], b = hi[prim];
-    int child = ~leaf;
-    int node = parent[(n - 1) + leaf];
-    // a Karras tree over n leaves is at most n - 1 levels deep; the bound keeps every wave finite
-    for (int level = 0; level < n; level++) {
-        BNode* nd = &nodes[node];
-        const int slot = (nd->c0 == child) ? 0 : 1;
-        write_child_box(nd, slot, a, b);
-        __threadfence();
-        if (atomicAdd(&flags[node], 1u) == 0u) return; // sibling not finished yet
-        __threadfence();
-        const volatile BNode* vn = nd;
-        a = make_float4(fminf(vn->lx0, vn->lx1), fminf(vn->ly0, vn->ly1), fminf(vn->lz0, vn->lz1), 0.f);
-        b = make_float4(fmaxf(vn->ux0, vn->ux1), fmaxf(vn->uy0, vn->uy1), fmaxf(vn->uz0, vn->uz1), 0.f);
-        if (node == 0) return;
-        child = node;
-        node = parent[node];
+    const int t = threadIdx.x;
+    if (t <= kMaxLevels) {
+        meta[t] = 0;
+        meta[kMaxLevels + 1 + t] = t == 0 ? 1u : 0u;
+    }
+    if (t == 0) lvl[0] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_bfs_level(const BNode* nodes, uint32_t* lvl, uint32_t* meta, int level)
+{
+    uint32_t* count = meta + kMaxLevels + 1;
+    const uint32_t s = meta[level], n = count[level];
+    const uint32_t next = s + n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta[level + 1] = next;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        int c0 = -1, c1 = -1;
+        if (i < n) {
+            const BNode& nd = nodes[lvl[s + i]];
+            c0 = nd.c0;
+            c1 = nd.c1;
+        }
+        const bool k0 = c0 >= 0, k1 = c1 >= 0;
+        const uint64_t m0 = __ballot(k0), m1 = __ballot(k1);
+        const uint32_t tot = uint32_t(__popcll(m0) + __popcll(m1));
+        uint32_t at = 0;
+        if (lane == 0 && tot) at = atomicAdd(&count[level + 1], tot);
+        at = __shfl(at, 0) + next;
+        const uint32_t p0 = uint32_t(__popcll(m0 & lt)), p1 = uint32_t(__popcll(m0)) + uint32_t(__popcll(m1 & lt));
+        if (k0) lvl[at + p0] = uint32_t(c0);
+        if (k1) lvl[at + p1] = uint32_t(c1);
+    }
+}
+
+__device__ __forceinline__ void child_box(const float4* lo, const float4* hi, const uint32_t* order,
+                                          const BNode* nodes, int c, float4& a, float4& b)
+{
+    if (c < 0) {
+        const uint32_t p = order[~c];
+        a = lo[p];
+        b = hi[p];
+    } else {
+        const BNode& cn = nodes[c];
+        a = make_float4(fminf(cn.lx0, cn.lx1), fminf(cn.ly0, cn.ly1), fminf(cn.lz0, cn.lz1), 0.f);
+        b = make_float4(fmaxf(cn.ux0, cn.ux1), fmaxf(cn.uy0, cn.uy1), fmaxf(cn.uz0, cn.uz1), 0.f);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_refit_level(const float4* lo, const float4* hi, const uint32_t* order,
+                                                     BNode* nodes, const uint32_t* lvl, const uint32_t* meta,
+                                                     int level)
+{
+    const uint32_t s = meta[level], n = meta[kMaxLevels + 1 + level];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        BNode* nd = &nodes[lvl[s + i]];
+        float4 a0, b0, a1, b1;
+        child_box(lo, hi, order, nodes, nd->c0, a0, b0);
+        child_box(lo, hi, order, nodes, nd->c1, a1, b1);
+        float4* f = reinterpret_cast<float4*>(nd);
+        f[0] = make_float4(a0.x, a0.y, a0.z, b0.x);
+        f[1] = make_float4(b0.y, b0.z, a1.x, a1.y);
+        f[2] = make_float4(a1.z, b1.x, b1.y, b1.z);
     }
 }
 
@@ -392,7 +440,7 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
         hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, lo, hi, st.nodes, st.order);
         return int(hipGetLastError());
     }
-    if (sc.keys.ensure(size_t(n) * 4) || sc.parent.ensure(size_t(2) * n * 4) || sc.flags.ensure(size_t(n) * 4) ||
+    if (sc.keys.ensure(size_t(n) * 4) || sc.lvl.ensure(size_t(n) * 4) || sc.meta.ensure(2 * (kMaxLevels + 1) * 4) ||
         sc.bounds.ensure(64))
         return 1;
     uint32_t* bounds = sc.bounds.as<uint32_t>();
@@ -405,11 +453,19 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, tpb)), dim3(tpb), 0, s, lo, hi, n, bounds, keys, st.order);
     const int e = radix_sort_pairs(keys, st.order, n, 30, sc, s);
     if (e) return e;
-    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, keys, int(n), st.nodes,
-                       sc.parent.as<int>());
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, sc.flags.as<uint32_t>(), 0u, n - 1);
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, tpb)), dim3(tpb), 0, s, lo, hi, st.order, int(n), st.nodes,
-                       sc.parent.as<int>(), sc.flags.as<uint32_t>());
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, keys, int(n), st.nodes);
+    uint32_t* lvl = sc.lvl.as<uint32_t>();
+    uint32_t* meta = sc.meta.as<uint32_t>();
+    // levels deeper than a tree over n leaves can reach are skipped on the host
+    int levels = 1;
+    while (levels < kMaxLevels && (uint64_t(1) << std::min(levels - 1, 62)) < n) levels++;
+    levels = std::min(kMaxLevels, levels + 31);
+    const uint32_t grid = std::min<uint32_t>(blocks_for(n, tpb), 2048);
+    hipLaunchKernelGGL(k_level_init, dim3(1), dim3(kMaxLevels + 1), 0, s, meta, lvl);
+    for (int l = 0; l + 1 < levels; l++)
+        hipLaunchKernelGGL(k_bfs_level, dim3(grid), dim3(tpb), 0, s, st.nodes, lvl, meta, l);
+    for (int l = levels - 1; l >= 0; l--)
+        hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(tpb), 0, s, lo, hi, st.order, st.nodes, lvl, meta, l);
     return int(hipGetLastError());
 }
 
@@ -424,7 +480,7 @@ void lbvh_free(BvhStore& st)
 
 void lbvh_scratch_free(BvhScratch& sc)
 {
-    for (DevBuf* b : {&sc.keys, &sc.keys2, &sc.vals2, &sc.hist, &sc.scan, &sc.parent, &sc.flags, &sc.bounds})
+    for (DevBuf* b : {&sc.keys, &sc.keys2, &sc.vals2, &sc.hist, &sc.scan, &sc.lvl, &sc.meta, &sc.bounds})
         b->release();
 }
 

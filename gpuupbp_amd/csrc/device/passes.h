@@ -28,7 +28,9 @@ struct IterParams {
     float r_bb1d, r_bp2d, r_pb2d, r_pp3d, r_surf, rad_point_med, rad_beam;
     float r2_bb1d, r2_bp2d, r2_pb2d, r2_pp3d, r2_surf;
     float sub_len;   // target sub-beam length of the beam map
+    float piece_len; // target length of one camera-segment query piece
     float box_slack; // extra beam-box inflation covering FP32 error of the beam predicates
+    int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
     uint64_t iter_key;
 };
 
@@ -333,6 +335,28 @@ struct PhotonMaps {
                             // (lo, hi, medium, SB_* flags)
 };
 
+// A camera-segment query is walked in pieces, each an independent work item of the merge
+// kernels: piece k of n owns the camera parameters t in [lo, hi) and traverses only
+// [ta, tb], while the predicates still see the whole segment, so an accepted pair is found
+// by exactly one piece. A ray's pieces number at most kPiecesPerRay plus one per segment.
+constexpr uint32_t kPiecesPerRay = 32;
+
+UPBP_HD uint32_t piece_count(float len, float piece_len)
+{
+    if (!(piece_len > 0.f) || !(len > piece_len)) return 1u;
+    const float n = ceilf(len / piece_len);
+    return n >= float(kPiecesPerRay) ? kPiecesPerRay : uint32_t(n);
+}
+
+UPBP_HD void piece_range(float dmin, float dmax, uint32_t k, uint32_t n, float& lo, float& hi, float& ta, float& tb)
+{
+    const float step = (dmax - dmin) / float(n);
+    lo = k == 0 ? -kInf : dmin + float(k) * step;
+    hi = k + 1 == n ? kInf : dmin + float(k + 1) * step;
+    ta = k == 0 ? dmin : lo;
+    tb = k + 1 == n ? dmax : hi;
+}
+
 // One camera-segment query (a scattering medium segment of one camera ray), handed from
 // the camera bounce kernel to the P-B2D / B-B1D merge kernels: 7 x float4 = 112 bytes.
 struct __align__(16) SegRec {
@@ -352,7 +376,7 @@ struct Bufs {
     StackRec* cstack;
     uint32_t* queue[2];
     uint32_t* qcount; // paths alive at bounce k: light [k], camera [64 + k]; segment queries
-                      // [128 + k]; merge work heads [192 + 2k + kind]
+                      // [128 + k]; merge work heads [192 + 2k + kind]; query pieces [320 + k]
     VertexRec* verts;
     uint32_t* nvert; // per light path
     BeamRec* beams;
@@ -360,7 +384,8 @@ struct Bufs {
     float* fb32;
     float* mcolor;  // per camera path: P-B2D / B-B1D contributions of the merge kernels
     SegRec* segq;   // segment queries of the current camera bounce
-    uint32_t seg_cap;
+    uint2* pieces;  // their pieces: (query, k | n << 16)
+    uint32_t seg_cap, piece_cap;
     double* accum;
     unsigned long long* stats; // see STAT_*
     PhotonMaps maps;
@@ -368,7 +393,7 @@ struct Bufs {
 
 enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_POINT_Q, STAT_SEG_Q, STAT_ACCEPT,
                   STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,
-                  STAT_COUNT };
+                  STAT_SEG_PIECES, STAT_COUNT };
 constexpr int kQcountWords = 512;
 
 // wave-aggregated counter increment

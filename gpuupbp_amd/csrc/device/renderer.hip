@@ -166,7 +166,7 @@ struct upbp_renderer {
     Bufs bufs;
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
-    DevBuf mcolor, segq;
+    DevBuf mcolor, segq, pieces;
     BvhStore bvh_s, bvh_m, bvh_b;
     float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
     upbp_tuning tune;
@@ -360,6 +360,13 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     // line parameters near the parallel cut-off, which grows with the coordinate magnitude
     p.sub_len = r->tune.subbeam_factor > 0.f ? r->tune.subbeam_factor * p.rad_beam : 0.f;
     p.box_slack = 2e-4f * r->scene_mag;
+    p.merge_bpc = r->tune.merge_blocks_per_cu;
+    {
+        float rq = 0.f;
+        if (p.do_pb2d) rq = fmaxf(rq, r_pb2d);
+        if (p.do_bb1d) rq = fmaxf(rq, r_bb1d);
+        p.piece_len = r->tune.query_piece_factor > 0.f ? r->tune.query_piece_factor * rq : 0.f;
+    }
     p.r2_bb1d = r_bb1d * r_bb1d;
     p.r2_bp2d = r_bp2d * r_bp2d;
     p.r2_pb2d = r_pb2d * r_pb2d;
@@ -497,7 +504,8 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     // segment queries of one camera bounce: at most kMaxSegments per live camera path
     const bool seg_merges = (c.algorithm_flags & (T_PB2D | T_BB1D)) != 0;
     const size_t seg_cap = seg_merges ? NP * kMaxSegments : 1;
-    if (ok) ok = !r->segq.ensure(seg_cap * sizeof(SegRec));
+    const size_t piece_cap = seg_merges ? NP * (kPiecesPerRay + kMaxSegments + 1) : 1;
+    if (ok) ok = !r->segq.ensure(seg_cap * sizeof(SegRec)) && !r->pieces.ensure(piece_cap * sizeof(uint2));
     if (!ok) {
         upbp_destroy(r);
         return fail(nullptr, "device allocation failed", UPBP_E_HIP);
@@ -521,6 +529,8 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     b.mcolor = r->mcolor.as<float>();
     b.segq = r->segq.as<SegRec>();
     b.seg_cap = uint32_t(seg_cap);
+    b.pieces = r->pieces.as<uint2>();
+    b.piece_cap = uint32_t(piece_cap);
     b.accum = r->accum.as<double>();
     b.stats = r->stats.as<unsigned long long>();
     *out = r;
@@ -689,6 +699,7 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         stats->bb1d_accepts = st[STAT_ACC_BB1D];
         stats->seg_nodes = st[STAT_SEG_NODES];
         stats->seg_candidates = st[STAT_SEG_CANDS];
+        stats->seg_pieces = st[STAT_SEG_PIECES];
         stats->vertex_drops = st[STAT_VDROP];
         stats->beam_drops = st[STAT_BDROP];
         stats->path_errors = st[STAT_ERRORS];
@@ -739,13 +750,15 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     if (!t) return;
     memset(t, 0, sizeof(*t));
     t->subbeam_factor = 16.f;
+    t->query_piece_factor = 64.f;
     t->merge_blocks_per_cu = 0;
 }
 
 extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
-    if (!(t->subbeam_factor >= 0.f) || t->merge_blocks_per_cu < 0) return fail(r, "invalid tuning", UPBP_E_ARG);
+    if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0)
+        return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     return UPBP_OK;
 }
@@ -771,7 +784,7 @@ extern "C" void upbp_destroy(upbp_renderer* r)
     for (DevBuf* d : {&r->lstate, &r->lstack, &r->cstate, &r->cstack, &r->q0, &r->q1, &r->qcount, &r->verts, &r->nvert,
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
-                      &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq})
+                      &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces})
         d->release();
     lbvh_free(r->bvh_s);
     lbvh_free(r->bvh_m);

@@ -134,6 +134,14 @@ struct Tmp {
 
 inline uint32_t nb(size_t n) { return uint32_t((n + 255) / 256); }
 
+// plain streaming copy: the reference kernel for HBM counter calibration and bandwidth
+__global__ void __launch_bounds__(256) k_copy_calibration(const float4* __restrict__ in, float4* __restrict__ out,
+                                                          size_t n)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
+}
+
 } // namespace
 
 extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries,
@@ -288,4 +296,34 @@ extern "C" int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const 
     lbvh_scratch_free(sc);
     if (rc) return UPBP_E_HIP;
     return fails ? UPBP_E_CAPACITY : UPBP_OK;
+}
+
+extern "C" int upbp_test_copy(uint64_t bytes, int reps, double* ms_per_copy)
+{
+    if (bytes < 16 || reps < 1 || !ms_per_copy) return UPBP_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
+    const size_t n = size_t(bytes / 16);
+    Tmp t;
+    float4* a = t.alloc<float4>(n);
+    float4* b = t.alloc<float4>(n);
+    if (!a || !b) return UPBP_E_HIP;
+    if (hipMemset(a, 0, n * 16) != hipSuccess) return UPBP_E_HIP;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return UPBP_E_HIP;
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t blocks = uint32_t(std::max(1, cus) * 8);
+    hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e0, 0);
+    for (int k = 0; k < reps; k++) hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    int rc = int(hipEventSynchronize(e1));
+    float ms = 0.f;
+    if (!rc) rc = int(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = double(ms) / reps;
+    return rc ? UPBP_E_HIP : UPBP_OK;
 }

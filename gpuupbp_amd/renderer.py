@@ -34,6 +34,19 @@ class Renderer:
         self.iterations += 1
         return st
 
+    def set_tuning(self, **kw):
+        """Speed knobs of the C ABI's upbp_tuning (subbeam_factor, query_piece_factor,
+        merge_blocks_per_cu); unnamed fields keep their defaults."""
+        t = capi.Tuning()
+        capi.lib().upbp_tuning_defaults(C.byref(t))
+        for k, v in kw.items():
+            if v is not None:
+                setattr(t, k, v)
+        rc = capi.lib().upbp_set_tuning(self.h, C.byref(t))
+        if rc != 0:
+            raise RuntimeError("upbp_set_tuning failed: %s" % capi.lib().upbp_last_error(self.h).decode())
+        return t
+
     def sync(self):
         rc = capi.lib().upbp_sync(self.h)
         if rc != 0:

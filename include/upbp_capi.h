@@ -151,6 +151,7 @@ typedef struct upbp_iter_stats {
     uint64_t bpt_connections, point_queries, segment_queries;
     uint64_t merge_accepts, pb2d_accepts, bb1d_accepts;
     uint64_t seg_nodes, seg_candidates; /* LBVH nodes visited / leaves tested by segment queries */
+    uint64_t seg_pieces;           /* pieces the segment queries were walked in */
     uint64_t vertex_drops, beam_drops, path_errors;
     double algorithmic_bytes;      /* compulsory HBM bytes of this iteration */
     double dominant_bytes;         /* compulsory HBM bytes of the dominant kernel */
@@ -208,6 +209,9 @@ int upbp_sync(upbp_renderer* r);
 typedef struct upbp_tuning {
     float subbeam_factor;     /* beams are cut into pieces of about subbeam_factor x the beam
                                  radius for the beam LBVH; 0 keeps whole beams (default 16) */
+    float query_piece_factor; /* camera-segment queries are walked in pieces of about this
+                                 factor x the merge radius, at most 32 per ray plus one per
+                                 segment; 0 keeps whole segments (default 64) */
     int32_t merge_blocks_per_cu; /* resident blocks per CU of the segment merge kernels; 0 = as
                                     many as fit (default) */
 } upbp_tuning;
@@ -237,6 +241,11 @@ int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const 
  * Closest: hit_t[i] (INFINITY on miss), hit_prim[i] (-1 on miss). */
 int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays,
                           float* hit_t, int32_t* hit_prim);
+
+/* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
+ * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM
+ * traffic-counter calibration and the measured copy bandwidth. */
+int upbp_test_copy(uint64_t bytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }
