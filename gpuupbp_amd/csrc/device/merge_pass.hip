@@ -4,13 +4,16 @@
 // Semantics: nolmoonen/gpuupbp gpuupbp/kernel/pde/functs_pb2d.cuh:57-342 with
 // program_pb2d.cu:31-46, and functs_bb1d.cuh:57-397 with program_bb1d.cu:31-48.
 //
-// Execution: a persistent grid. Each lane owns one query at a time and walks the LBVH one
-// node per loop trip; a lane whose query is done takes the next one from a global work
-// counter (one atomic per wave), so lanes stay busy while queries differ in cost by orders
-// of magnitude. Accepted (query, photon) pairs go to a per-wave LDS queue; whenever 64 are
-// queued the whole wave evaluates them together, so the long estimator body (attenuation,
-// phase function, both-side MIS updates) runs without divergence. Each evaluated pair adds
-// its weighted contribution to the query's camera path with float atomics.
+// Execution: a persistent grid over query pieces (a long camera segment is walked as several
+// independent pieces, so no single query serialises the launch). Each lane owns one piece at
+// a time and walks the LBVH one node per loop trip; a lane whose piece is done takes the
+// next one from a global work counter (one atomic per wave), so lanes stay busy while
+// pieces differ in cost by orders of magnitude. The work is split in three stages joined
+// by per-wave LDS queues, so that each stage runs with the whole wave converged: the node
+// step (box tests, stack) appends the leaves it hits as candidates; 64 candidates at a time
+// are tested with the exact predicate and piece ownership; 64 accepted (query, photon) pairs
+// at a time are evaluated (attenuation, phase function, both-side MIS updates) and added to
+// the query's camera path with float atomics.
 #include "estimators.h"
 #include "kernels.h"
 
@@ -22,7 +25,20 @@ namespace {
 
 constexpr int kMergeBlock = 256;
 constexpr int kMergeWaves = kMergeBlock / 64;
-constexpr uint32_t kPairCap = 192; // 64 pending + up to 2 accepts per lane in one trip
+constexpr uint32_t kCandCap = 192; // < 64 pending + up to 2 leaves per lane in one node step
+constexpr uint32_t kPairCap = 128; // < 64 pending + at most one accept per lane in a test batch
+
+struct CandQueue {
+    uint32_t q[kCandCap];    // segment query index
+    uint32_t leaf[kCandCap]; // sorted LBVH leaf
+    float lo[kCandCap], hi[kCandCap]; // camera parameters the query piece owns
+};
+
+// subtrees handed from busy lanes to idle lanes of the same wave
+struct Steal {
+    uint32_t q[64], node[64];
+    float lo[64], hi[64], ta[64], tb[64];
+};
 
 struct PairQueue {
     uint32_t q[kPairCap];    // segment query index
@@ -60,6 +76,20 @@ __device__ __forceinline__ void pq_push(PairQueue& Q, uint32_t& qn, bool keep, u
     qn += uint32_t(__popcll(m));
 }
 
+__device__ __forceinline__ void cq_push(CandQueue& C, uint32_t& cn, bool keep, uint32_t q, uint32_t leaf, float lo,
+                                        float hi)
+{
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+        const uint32_t at = cn + uint32_t(__popcll(m & lanes_below()));
+        C.q[at] = q;
+        C.leaf[at] = leaf;
+        C.lo[at] = lo;
+        C.hi[at] = hi;
+    }
+    cn += uint32_t(__popcll(m));
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -68,16 +98,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 
 // Leaf test of one query against one photon (sorted leaf k). Returns whether the pair is
 // accepted and its parameters: P-B2D (idist, ir2); B-B1D (t camera, t beam, distance, sin).
+// Only the query piece owning the pair's camera parameter [lo, hi) accepts it.
 template <int KIND>
 __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMaps& M, uint32_t k, f3 ro, f3 rd,
-                                            float dmin, float dmax, int med, uint32_t& slot, float& x, float& y,
-                                            float& z, float& w)
+                                            float dmin, float dmax, float lo, float hi, int med, uint32_t& slot,
+                                            float& x, float& y, float& z, float& w)
 {
     if (KIND == 0) { // program_pb2d.cu:31-46
         const float4 pt = M.med_pts[k];
         const f3 ce = mk(pt.x, pt.y, pt.z);
         if (!pred_point_beam(ro, rd, dmax, ce, p.r2_pb2d)) return false;
         if (!pred_bre(x, y, ro, rd, dmin, dmax, ce, p.r2_pb2d)) return false;
+        if (!(x >= lo && x < hi)) return false;
         slot = uint32_t(__float_as_int(pt.w));
         z = w = 0.f;
         return true;
@@ -86,6 +118,7 @@ __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMap
         if (!pred_beam_beam(ro, rd, dmin, dmax, mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), 0.f, g0.w, p.r2_bb1d, z,
                             w, x, y))
             return false;
+        if (!(x >= lo && x < hi)) return false;
         const float4 g2 = M.beam_geo[3 * k + 2];
         if (!(y >= g2.x && y < g2.y)) return false;
         // B-B1D subset of the light paths and the same-medium test (functs_bb1d.cuh:65)
@@ -98,27 +131,30 @@ __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMap
 template <int KIND>
 __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams p, Bufs b, int bounce)
 {
+    __shared__ CandQueue cq[kMergeWaves];
     __shared__ PairQueue pq[kMergeWaves];
+    __shared__ Steal stl[kMergeWaves];
+    CandQueue& C = cq[threadIdx.x >> 6];
     PairQueue& Q = pq[threadIdx.x >> 6];
+    Steal& S = stl[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
-    const uint32_t total = min(b.qcount[128 + bounce], b.seg_cap);
+    // B-B1D walks query pieces; P-B2D (few candidates per query) walks whole segments
+    const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : min(b.qcount[128 + bounce], b.seg_cap);
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
     const Bvh& bvh = KIND == 0 ? b.maps.med : b.maps.beam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
 
-    uint32_t qn = 0;        // pairs queued in Q (same value in every lane)
-    bool exhausted = false; // no queries left to take (same value in every lane)
-    int cur = -1;           // this lane's query
-    f3 ro = mk(0.f), rd = mk(0.f);
-    float dmin = 0.f, dmax = 0.f;
-    int med = -1;
+    uint32_t cn = 0, qn = 0; // candidates / pairs queued (same value in every lane)
+    bool exhausted = false;  // no pieces left to take (same value in every lane)
+    int cur = -1;            // query of this lane's current piece
+    float lo = 0.f, hi = 0.f, ta = 0.f, tb = 0.f;
     RayBox rb = make_raybox(mk(0.f), mk(1.f), 0.f, 0.f);
-    int node = 0, sp = 0;
+    int node = 0, sb = 0, sp = 0; // pending subtrees: stack[sb, sp), oldest (largest) at sb
     int stack[kBvhStack];
     uint32_t n_nodes = 0, n_cands = 0, n_acc = 0, n_err = 0;
 
     for (;;) {
-        if (!exhausted) { // lanes without a query take the next ones
+        if (!exhausted) { // lanes without work take the next query pieces
             const bool need = cur < 0;
             const uint64_t m = __ballot(need);
             if (m) {
@@ -127,30 +163,61 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 if (int(lane) == leader) base = atomicAdd(head, uint32_t(__popcll(m)));
                 base = __shfl(base, leader);
                 if (need) {
-                    const uint32_t qi = base + uint32_t(__popcll(m & lanes_below()));
-                    if (qi < total) {
-                        const SegRec* r = b.segq + qi;
-                        const float4 a = r->a, bb = r->b, d = r->d;
-                        ro = mk(a.x, a.y, a.z);
-                        dmin = a.w;
-                        rd = mk(bb.x, bb.y, bb.z);
-                        dmax = bb.w;
-                        med = __float_as_int(d.z);
-                        rb = make_raybox(ro, rd, dmin, dmax);
+                    const uint32_t wi = base + uint32_t(__popcll(m & lanes_below()));
+                    if (wi < total) {
+                        const uint2 pc = KIND == 1 ? b.pieces[wi] : make_uint2(wi, 1u << 16);
+                        const SegRec* r = b.segq + pc.x;
+                        const float4 a = r->a, bb = r->b;
+                        piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
+                        rb = make_raybox(mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), ta, tb);
                         node = 0;
-                        sp = 0;
-                        cur = int(qi);
+                        sb = sp = 0;
+                        cur = int(pc.x);
                     }
                 }
                 if (base + uint32_t(__popcll(m)) >= total) exhausted = true;
             }
+        } else { // no pieces left: idle lanes take the oldest pending subtree of a busy lane
+            const bool idle = cur < 0, donor = cur >= 0 && sp > sb;
+            const uint64_t mi = __ballot(idle), md = __ballot(donor);
+            const uint32_t t = min(uint32_t(__popcll(mi)), uint32_t(__popcll(md)));
+            if (t) {
+                if (donor) {
+                    const uint32_t k = uint32_t(__popcll(md & lanes_below()));
+                    if (k < t) {
+                        S.q[k] = uint32_t(cur);
+                        S.node[k] = uint32_t(stack[sb++]);
+                        S.lo[k] = lo;
+                        S.hi[k] = hi;
+                        S.ta[k] = ta;
+                        S.tb[k] = tb;
+                    }
+                }
+                wave_lds_sync();
+                if (idle) {
+                    const uint32_t k = uint32_t(__popcll(mi & lanes_below()));
+                    if (k < t) {
+                        cur = int(S.q[k]);
+                        node = int(S.node[k]);
+                        lo = S.lo[k];
+                        hi = S.hi[k];
+                        ta = S.ta[k];
+                        tb = S.tb[k];
+                        const SegRec* r = b.segq + cur;
+                        const float4 a = r->a, bb = r->b;
+                        rb = make_raybox(mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), ta, tb);
+                        sb = sp = 0;
+                    }
+                }
+                wave_lds_sync();
+            }
         }
         const bool active = cur >= 0;
         const bool any_active = __ballot(active) != 0;
-        bool acc0 = false, acc1 = false;
-        uint32_t s0 = 0, s1 = 0, qnow = 0;
-        float x0 = 0.f, y0 = 0.f, z0 = 0.f, w0 = 0.f, x1 = 0.f, y1 = 0.f, z1 = 0.f, w1 = 0.f;
-        if (active) { // one node of this lane's traversal
+        // one node of this lane's traversal; leaves hit become candidates
+        int l0 = -1, l1 = -1;
+        uint32_t qnow = 0;
+        if (active) {
             const BNode nd = bvh.nodes[node];
             n_nodes++;
             float e0, e1;
@@ -158,99 +225,131 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
             const bool h1 = slab(rb, nd.lx1, nd.ly1, nd.lz1, nd.ux1, nd.uy1, nd.uz1, rb.tmax, e1);
             int next = -1;
             if (h0) {
-                if (nd.c0 < 0) {
-                    n_cands++;
-                    acc0 = leaf_accept<KIND>(p, b.maps, uint32_t(~nd.c0), ro, rd, dmin, dmax, med, s0, x0, y0, z0, w0);
-                } else {
-                    next = nd.c0;
-                }
+                if (nd.c0 < 0) l0 = ~nd.c0;
+                else next = nd.c0;
             }
             if (h1) {
                 if (nd.c1 < 0) {
-                    n_cands++;
-                    acc1 = leaf_accept<KIND>(p, b.maps, uint32_t(~nd.c1), ro, rd, dmin, dmax, med, s1, x1, y1, z1, w1);
+                    l1 = ~nd.c1;
                 } else if (next < 0) {
                     next = nd.c1;
-                } else if (sp < kBvhStack) {
-                    stack[sp++] = nd.c1;
                 } else {
-                    n_err++; // traversal stack exhausted: the query is abandoned and counted
-                    next = -1;
-                    sp = 0;
+                    if (sp == kBvhStack && sb > 0) { // slide the pending entries down
+                        for (int k = sb; k < sp; k++) stack[k - sb] = stack[k];
+                        sp -= sb;
+                        sb = 0;
+                    }
+                    if (sp < kBvhStack) {
+                        stack[sp++] = nd.c1;
+                    } else {
+                        n_err++; // traversal stack exhausted: the piece is abandoned and counted
+                        next = -1;
+                        sb = sp = 0;
+                    }
                 }
             }
-            if (next < 0 && sp > 0) next = stack[--sp];
+            if (next < 0 && sp > sb) next = stack[--sp];
             node = next;
             qnow = uint32_t(cur);
             if (next < 0) cur = -1;
-            n_acc += uint32_t(acc0) + uint32_t(acc1);
         }
         // all lanes take part in the queue appends (ballots over the whole wave)
-        pq_push(Q, qn, acc0, qnow, s0, x0, y0, z0, w0);
-        pq_push(Q, qn, acc1, qnow, s1, x1, y1, z1, w1);
-        // evaluate 64 queued pairs at a time; at the end, whatever is left
-        while (qn >= 64u || (!any_active && qn > 0u)) {
-            const uint32_t n = qn < 64u ? qn : 64u;
-            wave_lds_sync();
-            if (lane < n) {
-                const uint32_t e = qn - n + lane;
-                const uint32_t qi = Q.q[e], slot = Q.slot[e];
-                const float x = Q.x[e], y = Q.y[e], z = Q.z[e], w = Q.w[e];
-                SegQ q;
-                StateWeights cw;
-                f3 tatt;
-                uint32_t pix;
-                load_segrec(b.segq + qi, q, cw, tatt, pix);
-                f3 acc = mk(0.f);
-                if (KIND == 0) {
-                    LVertex lv;
-                    load_vertex(&b.verts[slot], lv);
-                    eval_pb2d(sc, p, acc, q, cw, lv, x, y);
-                } else {
-                    LBeam lb;
-                    load_beam(&b.beams[slot], lb);
-                    eval_bb1d(sc, p, acc, q, cw, lb, z, w, x, y);
+        cq_push(C, cn, l0 >= 0, qnow, uint32_t(l0), lo, hi);
+        cq_push(C, cn, l1 >= 0, qnow, uint32_t(l1), lo, hi);
+        // Evaluate 64 accepted pairs at a time whenever 64 are queued (this keeps fewer than 64
+        // pending before each candidate test), else test 64 candidates at a time. Once no lane
+        // has work left, drain both queues: candidates first, then pairs.
+        for (;;) {
+            const bool eval_p = qn >= 64u || (!any_active && cn == 0u && qn > 0u);
+            const bool test_c = !eval_p && (cn >= 64u || (!any_active && cn > 0u));
+            if (!eval_p && !test_c) break;
+            if (eval_p) {
+                const uint32_t m = qn < 64u ? qn : 64u;
+                wave_lds_sync();
+                if (lane < m) {
+                    const uint32_t e = qn - m + lane;
+                    const uint32_t pqi = Q.q[e], pslot = Q.slot[e];
+                    const float px = Q.x[e], py = Q.y[e], pz = Q.z[e], pw = Q.w[e];
+                    SegQ q;
+                    StateWeights cw;
+                    f3 tatt;
+                    uint32_t pix;
+                    load_segrec(b.segq + pqi, q, cw, tatt, pix);
+                    f3 sum = mk(0.f);
+                    if (KIND == 0) {
+                        LVertex lv;
+                        load_vertex(&b.verts[pslot], lv);
+                        eval_pb2d(sc, p, sum, q, cw, lv, px, py);
+                    } else {
+                        LBeam lb;
+                        load_beam(&b.beams[pslot], lb);
+                        eval_bb1d(sc, p, sum, q, cw, lb, pz, pw, px, py);
+                    }
+                    const f3 c = (tatt * norm) * sum;
+                    if (c.x != 0.f) atomicAdd(&b.mcolor[3 * pix], c.x);
+                    if (c.y != 0.f) atomicAdd(&b.mcolor[3 * pix + 1], c.y);
+                    if (c.z != 0.f) atomicAdd(&b.mcolor[3 * pix + 2], c.z);
                 }
-                const f3 c = (tatt * norm) * acc;
-                if (c.x != 0.f) atomicAdd(&b.mcolor[3 * pix], c.x);
-                if (c.y != 0.f) atomicAdd(&b.mcolor[3 * pix + 1], c.y);
-                if (c.z != 0.f) atomicAdd(&b.mcolor[3 * pix + 2], c.z);
+                wave_lds_sync();
+                qn -= m;
+            } else {
+                const uint32_t n = cn < 64u ? cn : 64u;
+                wave_lds_sync();
+                bool acc = false;
+                uint32_t qi = 0, slot = 0;
+                float x = 0.f, y = 0.f, z = 0.f, w = 0.f;
+                if (lane < n) {
+                    const uint32_t e = cn - n + lane;
+                    qi = C.q[e];
+                    const uint32_t leaf = C.leaf[e];
+                    const float clo = C.lo[e], chi = C.hi[e];
+                    const SegRec* r = b.segq + qi;
+                    const float4 a = r->a, bb = r->b, d = r->d;
+                    acc = leaf_accept<KIND>(p, b.maps, leaf, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), a.w, bb.w, clo,
+                                            chi, __float_as_int(d.z), slot, x, y, z, w);
+                }
+                wave_lds_sync();
+                cn -= n;
+                n_cands += lane < n ? 1u : 0u;
+                n_acc += acc ? 1u : 0u;
+                pq_push(Q, qn, acc, qi, slot, x, y, z, w);
             }
-            wave_lds_sync();
-            qn -= n;
         }
         if (!any_active && exhausted) break;
     }
-    const uint32_t tn = wave_sum(n_nodes), tc = wave_sum(n_cands), ta = wave_sum(n_acc), te = wave_sum(n_err);
+    const uint32_t s_nodes = wave_sum(n_nodes), s_cands = wave_sum(n_cands), s_acc = wave_sum(n_acc),
+                   s_err = wave_sum(n_err);
     if (lane == 0) {
-        stat_add(b.stats, STAT_SEG_NODES, tn);
-        stat_add(b.stats, STAT_SEG_CANDS, tc);
-        stat_add(b.stats, STAT_ACCEPT, ta);
-        stat_add(b.stats, KIND == 0 ? STAT_ACC_PB2D : STAT_ACC_BB1D, ta);
-        stat_add(b.stats, STAT_ERRORS, te);
+        stat_add(b.stats, STAT_SEG_NODES, s_nodes);
+        stat_add(b.stats, STAT_SEG_CANDS, s_cands);
+        stat_add(b.stats, STAT_ACCEPT, s_acc);
+        stat_add(b.stats, KIND == 0 ? STAT_ACC_PB2D : STAT_ACC_BB1D, s_acc);
+        stat_add(b.stats, STAT_ERRORS, s_err);
     }
 }
 
 } // namespace
 
-int merge_grid_blocks()
+// blocks of the persistent merge grid: CUs x resident blocks per CU (at most the occupancy
+// limit; bpc > 0 asks for fewer)
+int merge_grid_blocks(int bpc)
 {
-    static int blocks = 0;
-    if (blocks == 0) {
-        int dev = 0, cus = 0, per0 = 0, per1 = 0;
+    static int cus = 0, per = 0;
+    if (cus == 0) {
+        int dev = 0, per0 = 0, per1 = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per0, k_seg_merge<0>, kMergeBlock, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per1, k_seg_merge<1>, kMergeBlock, 0);
-        const int per = std::max(1, std::min(per0, per1));
-        blocks = std::max(1, cus) * per;
+        per = std::max(1, std::min(per0, per1));
+        cus = std::max(1, cus);
     }
-    return blocks;
+    return cus * (bpc > 0 ? std::min(bpc, per) : per);
 }
 
 void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int kind, int bounce, hipStream_t s)
 {
-    const int blocks = merge_grid_blocks();
+    const int blocks = merge_grid_blocks(p.merge_bpc);
     if (kind == 0 && p.build_pb2d && b.maps.med.n > 0)
         hipLaunchKernelGGL(k_seg_merge<0>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
     if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0)

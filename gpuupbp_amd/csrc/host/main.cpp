@@ -1,0 +1,136 @@
+// gpuupbp command-line driver over libupbp: the drop-in for the reference's main
+// (nolmoonen/gpuupbp gpuupbp/gpuupbp.cpp:57-148). Same options (upbp_parse_commandline),
+// same iteration- or time-based loop with percentage lines and continuous output
+// (gpuupbp.cpp:33-55), same final image. Times are wall-clock (std::chrono) rather
+// than the reference's clock() CPU time, so -t means seconds of rendering.
+#include "upbp_capi.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::string describe(const upbp_config& c)
+{
+    std::string s = "techniques from upbp:";
+    const struct {
+        uint32_t bit;
+        const char* name;
+    } tech[] = {{UPBP_BPT, "bpt"},   {UPBP_SURF, "surf"}, {UPBP_PP3D, "pp3d"},
+                {UPBP_PB2D, "pb2d"}, {UPBP_BB1D, "bb1d"}, {UPBP_BP2D, "bp2d"}};
+    for (const auto& t : tech)
+        if (c.algorithm_flags & t.bit) s += std::string(" ") + t.name;
+    char line[160];
+    snprintf(line, sizeof(line), "\n          resolution:        %dx%d\n          max path length:   %u\n", c.res_x,
+             c.res_y, c.max_path_length);
+    s += line;
+    const struct {
+        uint32_t bit;
+        const char* name;
+        float r0, alpha;
+    } rad[] = {{UPBP_BB1D, "bb1d", c.bb1d_radius_initial, c.bb1d_radius_alpha},
+               {UPBP_PP3D, "pp3d", c.pp3d_radius_initial, c.pp3d_radius_alpha},
+               {UPBP_SURF, "surf", c.surf_radius_initial, c.surf_radius_alpha},
+               {UPBP_PB2D, "pb2d", c.pb2d_radius_initial, c.pb2d_radius_alpha},
+               {UPBP_BP2D, "bp2d", c.bp2d_radius_initial, c.bp2d_radius_alpha}};
+    for (const auto& r : rad) {
+        if (!(c.algorithm_flags & r.bit)) continue;
+        snprintf(line, sizeof(line), "          %s radius init:  %g\n               radius alpha: %g\n", r.name,
+                 double(r.r0), double(r.alpha));
+        s += line;
+    }
+    return s;
+}
+
+// mean image so far (FP64 on the device) -> FP32 rows -> file
+int save_mean(upbp_renderer* r, const upbp_config& c, int iters, const std::string& path)
+{
+    const size_t n = size_t(c.res_x) * size_t(c.res_y) * 3;
+    std::vector<double> acc(n);
+    int e = upbp_read_accum(r, acc.data(), iters);
+    if (e) return e;
+    std::vector<float> rgb(n);
+    for (size_t i = 0; i < n; i++) rgb[i] = float(acc[i]);
+    const std::string ext = path.size() >= 3 ? path.substr(path.size() - 3) : "";
+    if (ext != "exr") {
+        fprintf(stderr, "Error: output extension %s is not supported by this build (exr only)\n", ext.c_str());
+        return UPBP_E_ARG;
+    }
+    return upbp_save_exr(path.c_str(), rgb.data(), uint32_t(c.res_x), uint32_t(c.res_y));
+}
+
+} // namespace
+
+int main(int argc, const char* argv[])
+{
+    upbp_config cfg;
+    // the reference exits with 0 both on parse errors and after printing help
+    if (upbp_parse_commandline(argc, argv, &cfg) != UPBP_OK) return 0;
+    upbp_scene_handle* sh = nullptr;
+    int e = upbp_scene_load_obj(cfg.scene_obj_file, uint32_t(cfg.res_x), uint32_t(cfg.res_y), &sh);
+    if (e) {
+        fprintf(stderr, "Error: %s\n", upbp_global_error());
+        return 2;
+    }
+    printf("Scene:    %s\n", cfg.scene_obj_file);
+    if (cfg.max_time > 0.f) printf("Target:   %g seconds render time\n", double(cfg.max_time));
+    else printf("Target:   %d iteration(s)\n", cfg.iteration_count);
+    printf("Running:  %s", describe(cfg).c_str());
+
+    upbp_renderer* r = nullptr;
+    e = upbp_create(upbp_scene_get(sh), &cfg, &r);
+    if (e) {
+        fprintf(stderr, "Error: %s\n", upbp_last_error(nullptr));
+        upbp_scene_free(sh);
+        return 1;
+    }
+    const std::string out = cfg.output_name;
+    const std::string name = out.substr(0, out.size() >= 4 ? out.size() - 4 : 0);
+    const std::string ext = out.size() >= 3 ? out.substr(out.size() - 3) : "";
+    auto continuous = [&](int iter) { // gpuupbp.cpp:33-55
+        if (cfg.continuous_output > 0 && iter % cfg.continuous_output == 0) {
+            const std::string fn = name + "-" + std::to_string(iter) + "." + ext;
+            if (save_mean(r, cfg, iter, fn)) fprintf(stderr, "Error: could not write %s\n", fn.c_str());
+        }
+    };
+
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    int iter = 0;
+    if (cfg.max_time > 0.f) {
+        const auto end = t0 + std::chrono::duration<double>(cfg.max_time);
+        while (clk::now() < end) {
+            if ((e = upbp_run_iteration(r, iter, nullptr))) break;
+            iter++;
+            continuous(iter);
+            if (cfg.continuous_output <= 0) upbp_sync(r); // keep the time check close to the device
+        }
+    } else {
+        int percent = -1;
+        for (iter = 0; iter < cfg.iteration_count; iter++) {
+            if ((e = upbp_run_iteration(r, iter, nullptr))) break;
+            const int p = int((float(iter + 1) / float(cfg.iteration_count)) * 100.f);
+            if (p != percent) {
+                percent = p;
+                printf("%d%%\n", p);
+                fflush(stdout);
+            }
+            continuous(iter + 1);
+        }
+    }
+    if (!e) e = upbp_sync(r);
+    const double secs = std::chrono::duration<double>(clk::now() - t0).count();
+    if (e) fprintf(stderr, "Error: %s\n", upbp_last_error(r));
+    printf("done in %.2f s (%i iterations)\n", secs, iter);
+    int rc = 0;
+    if (!e && iter > 0 && save_mean(r, cfg, iter, out)) {
+        fprintf(stderr, "Error: could not write %s\n", out.c_str());
+        rc = 2;
+    }
+    upbp_destroy(r);
+    upbp_scene_free(sh);
+    return e ? 1 : rc;
+}

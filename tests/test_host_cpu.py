@@ -211,3 +211,16 @@ def test_no_gpu_reports_error():
     capi.lib().upbp_config_defaults(C.byref(cfg))
     out = C.c_void_p()
     assert capi.lib().upbp_test_range_query(0, None, 0, None, 0, 0.1, 0.0, (C.c_uint32 * 1)(), None, 0, None) < 0
+
+
+def test_cli_help_and_no_gpu_error(box_obj):
+    import subprocess
+    exe = os.path.join(ROOT, "gpuupbp_amd", "gpuupbp")
+    if not os.path.exists(exe):
+        pytest.skip("CLI not built")
+    h = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
+    assert h.returncode == 0 and "Usage:" in h.stdout
+    if not os.path.exists("/dev/kfd"):
+        r = subprocess.run([exe, "-s", "-1", box_obj, "-r", "16x16", "-i", "1"], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 1 and "no GPU visible" in r.stderr
