@@ -95,6 +95,7 @@ class IterStats(C.Structure):
                 ("bpt_connections", C.c_uint64), ("point_queries", C.c_uint64), ("segment_queries", C.c_uint64),
                 ("merge_accepts", C.c_uint64), ("pb2d_accepts", C.c_uint64), ("bb1d_accepts", C.c_uint64),
                 ("seg_nodes", C.c_uint64), ("seg_candidates", C.c_uint64), ("seg_pieces", C.c_uint64),
+                ("seg_wave_trips", C.c_uint64), ("seg_cand_batches", C.c_uint64), ("seg_pair_batches", C.c_uint64),
                 ("vertex_drops", C.c_uint64), ("beam_drops", C.c_uint64), ("path_errors", C.c_uint64),
                 ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double)]
 
@@ -119,6 +120,9 @@ _SIGS = [
     ("upbp_sync", C.c_int, [C.c_void_p]),
     ("upbp_test_range_query", C.c_int, [C.c_int, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_float,
                                         C.c_float, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("upbp_test_range_query_wide", C.c_int, [C.c_int, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_float,
+                                             C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64,
+                                             C.POINTER(C.c_uint64)]),
     ("upbp_test_copy", C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     ("upbp_tuning_defaults", None, [C.c_void_p]),
     ("upbp_set_tuning", C.c_int, [C.c_void_p, C.c_void_p]),

@@ -120,6 +120,44 @@ __device__ __forceinline__ bool bvh_ray_query(const Bvh& b, f3 o, f3 d, float tm
     }
 }
 
+// Visits every leaf of a 4-wide LBVH whose box meets the segment [tmin, tmax] of the ray.
+template <typename F>
+__device__ __forceinline__ bool wbvh_ray_query(const WBvh& b, f3 o, f3 d, float tmin, float tmax, F&& fn)
+{
+    if (b.n == 0) return true;
+    const RayBox r = make_raybox(o, d, tmin, tmax);
+    int stack[kBvhStack];
+    int sp = 0;
+    int node = 0;
+    for (;;) {
+        const WNode nd = b.nodes[node];
+        const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
+        const int nc = int(nd.meta >> 24);
+        int next = -1;
+        for (int k = 0; k < nc; k++) {
+            float e;
+            if (!slab(r, wnode_decode(nd.ox, nd.q[0], k, sx), wnode_decode(nd.oy, nd.q[1], k, sy),
+                      wnode_decode(nd.oz, nd.q[2], k, sz), wnode_decode(nd.ox, nd.q[3], k, sx),
+                      wnode_decode(nd.oy, nd.q[4], k, sy), wnode_decode(nd.oz, nd.q[5], k, sz), tmax, e))
+                continue;
+            const int ch = nd.child[k];
+            if (ch < 0) {
+                fn(uint32_t(~ch));
+            } else if (next < 0) {
+                next = ch;
+            } else {
+                if (sp == kBvhStack) return false;
+                stack[sp++] = ch;
+            }
+        }
+        if (next < 0) {
+            if (sp == 0) return true;
+            next = stack[--sp];
+        }
+        node = next;
+    }
+}
+
 // Moller-Trumbore (Moller & Trumbore 1997); replaces OptiX's built-in triangle test.
 UPBP_HD bool tri_test(float4 p0, float4 e1, float4 e2, f3 o, f3 d, float tmin, float tmax, float& th, float& bu,
                       float& bv)
@@ -187,10 +225,21 @@ struct BvhScratch {
     DevBuf keys, keys2, vals2, hist, scan, lvl, meta, bounds;
 };
 
+// Device storage of a 4-wide LBVH collapsed from a binary one (at most n-1 nodes) and the
+// wide-node -> binary-node map used while building it.
+struct WBvhStore {
+    upbp::WNode* nodes = nullptr;
+    uint32_t* bin = nullptr;
+    uint32_t n = 0, cap = 0;
+};
+
 // Primitive boxes are given as lo/hi float4 arrays (w unused), already inflated.
 // Builds into st on stream s; returns 0 or a hipError_t value.
 int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s);
 void lbvh_free(BvhStore& st);
+// Collapses the refitted binary LBVH `bin` (n leaves) into a 4-wide one; same leaf order.
+int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s);
+void wbvh_free(WBvhStore& w);
 void lbvh_scratch_free(BvhScratch& sc);
 
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits; keys/vals updated in place.

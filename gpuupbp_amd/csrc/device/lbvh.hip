@@ -344,6 +344,167 @@ __global__ void __launch_bounds__(256) k_refit_level(const float4* lo, const flo
 }
 
 // single primitive: one inner node with leaf 0 on the left and an empty right box
+// ---- 4-wide collapse --------------------------------------------------------------
+// Each wide node takes one binary node and opens its largest-area inner child until it
+// has four children (or only leaves). Wide nodes are numbered level by level: level l
+// holds [meta[l], meta[l] + meta[kMaxLevels + 1 + l]) and bin[] maps them to binary nodes.
+
+__device__ __forceinline__ float box_area(float4 a, float4 b)
+{
+    const float x = b.x - a.x, y = b.y - a.y, z = b.z - a.z;
+    return x * y + y * z + z * x;
+}
+
+// power-of-two grid step covering [lo, hi] in 255 steps, as a biased exponent
+__device__ __forceinline__ uint32_t grid_exponent(float lo, float hi)
+{
+    const float t = (hi - lo) * (1.f / 255.f);
+    int e = (__float_as_int(t) >> 23) & 255;
+    if (t > __int_as_float(e << 23)) e++;
+    e = e < 1 ? 1 : e;
+    for (; e < 254; e++) // the decoded top of the grid must reach hi
+        if (wnode_decode(lo, 255u, 0, __int_as_float(e << 23)) >= hi) break;
+    return uint32_t(e);
+}
+
+__device__ __forceinline__ uint32_t quant_lo(float o, float step, float v)
+{
+    float f = floorf((v - o) / step);
+    uint32_t q = f <= 0.f ? 0u : (f >= 255.f ? 255u : uint32_t(f));
+    while (q > 0 && wnode_decode(o, q, 0, step) > v) q--;
+    return q;
+}
+
+__device__ __forceinline__ uint32_t quant_hi(float o, float step, float v)
+{
+    float f = ceilf((v - o) / step);
+    uint32_t q = f <= 0.f ? 0u : (f >= 255.f ? 255u : uint32_t(f));
+    while (q < 255 && wnode_decode(o, q, 0, step) < v) q++;
+    return q;
+}
+
+__global__ void k_wide_init(uint32_t* meta, uint32_t* bin)
+{
+    const int t = threadIdx.x;
+    if (t <= kMaxLevels) {
+        meta[t] = 0;
+        meta[kMaxLevels + 1 + t] = t == 0 ? 1u : 0u;
+    }
+    if (t == 0) bin[0] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, uint32_t* bin, uint32_t* meta,
+                                                    int level)
+{
+    uint32_t* count = meta + kMaxLevels + 1;
+    const uint32_t s = meta[level], n = count[level];
+    const uint32_t next = s + n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta[level + 1] = next;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t j = base + threadIdx.x;
+        int c[4] = {0, 0, 0, 0};
+        float4 lo[4], hi[4];
+        int nc = 0, ninner = 0;
+        if (j < n) {
+            const BNode& b = bn[bin[s + j]];
+            c[0] = b.c0;
+            c[1] = b.c1;
+            lo[0] = make_float4(b.lx0, b.ly0, b.lz0, 0.f);
+            hi[0] = make_float4(b.ux0, b.uy0, b.uz0, 0.f);
+            lo[1] = make_float4(b.lx1, b.ly1, b.lz1, 0.f);
+            hi[1] = make_float4(b.ux1, b.uy1, b.uz1, 0.f);
+            nc = 2;
+            while (nc < 4) {
+                int best = -1;
+                float ba = -1.f;
+                for (int k = 0; k < nc; k++)
+                    if (c[k] >= 0 && box_area(lo[k], hi[k]) > ba) {
+                        ba = box_area(lo[k], hi[k]);
+                        best = k;
+                    }
+                if (best < 0) break;
+                const BNode& x = bn[c[best]];
+                c[best] = x.c0;
+                lo[best] = make_float4(x.lx0, x.ly0, x.lz0, 0.f);
+                hi[best] = make_float4(x.ux0, x.uy0, x.uz0, 0.f);
+                c[nc] = x.c1;
+                lo[nc] = make_float4(x.lx1, x.ly1, x.lz1, 0.f);
+                hi[nc] = make_float4(x.ux1, x.uy1, x.uz1, 0.f);
+                nc++;
+            }
+            for (int k = 0; k < nc; k++) ninner += c[k] >= 0 ? 1 : 0;
+        }
+        // wave-aggregated allocation of the inner children at the next level
+        uint32_t pre = 0, tot = 0;
+        for (int bit = 0; bit < 3; bit++) {
+            const uint64_t m = __ballot((uint32_t(ninner) >> bit) & 1u);
+            pre += uint32_t(__popcll(m & lt)) << bit;
+            tot += uint32_t(__popcll(m)) << bit;
+        }
+        uint32_t at = 0;
+        if (lane == 0 && tot) at = atomicAdd(&count[level + 1], tot);
+        at = __shfl(at, 0) + next + pre;
+        if (j < n) {
+            float4 nlo = lo[0], nhi = hi[0];
+            for (int k = 1; k < nc; k++) {
+                nlo = make_float4(fminf(nlo.x, lo[k].x), fminf(nlo.y, lo[k].y), fminf(nlo.z, lo[k].z), 0.f);
+                nhi = make_float4(fmaxf(nhi.x, hi[k].x), fmaxf(nhi.y, hi[k].y), fmaxf(nhi.z, hi[k].z), 0.f);
+            }
+            const uint32_t ex = grid_exponent(nlo.x, nhi.x), ey = grid_exponent(nlo.y, nhi.y),
+                           ez = grid_exponent(nlo.z, nhi.z);
+            const float sx = __int_as_float(int(ex << 23)), sy = __int_as_float(int(ey << 23)),
+                        sz = __int_as_float(int(ez << 23));
+            WNode w;
+            w.ox = nlo.x;
+            w.oy = nlo.y;
+            w.oz = nlo.z;
+            w.meta = ex | (ey << 8) | (ez << 16) | (uint32_t(nc) << 24);
+            for (int k = 0; k < 6; k++) w.q[k] = 0;
+            for (int k = 0; k < 4; k++) w.child[k] = 0;
+            for (int k = 0; k < nc; k++) {
+                if (c[k] >= 0) {
+                    bin[at] = uint32_t(c[k]);
+                    w.child[k] = int(at++);
+                } else {
+                    w.child[k] = c[k];
+                }
+                const int sh = 8 * k;
+                w.q[0] |= quant_lo(w.ox, sx, lo[k].x) << sh;
+                w.q[1] |= quant_lo(w.oy, sy, lo[k].y) << sh;
+                w.q[2] |= quant_lo(w.oz, sz, lo[k].z) << sh;
+                w.q[3] |= quant_hi(w.ox, sx, hi[k].x) << sh;
+                w.q[4] |= quant_hi(w.oy, sy, hi[k].y) << sh;
+                w.q[5] |= quant_hi(w.oz, sz, hi[k].z) << sh;
+            }
+            w.pad0 = w.pad1 = 0;
+            wn[s + j] = w;
+        }
+    }
+}
+
+// one primitive: a wide root with a single leaf child
+__global__ void k_wide_single(const BNode* bn, WNode* wn)
+{
+    const BNode& b = bn[0];
+    WNode w;
+    w.ox = b.lx0;
+    w.oy = b.ly0;
+    w.oz = b.lz0;
+    const uint32_t ex = grid_exponent(b.lx0, b.ux0), ey = grid_exponent(b.ly0, b.uy0), ez = grid_exponent(b.lz0, b.uz0);
+    w.meta = ex | (ey << 8) | (ez << 16) | (1u << 24);
+    w.child[0] = ~0;
+    w.child[1] = w.child[2] = w.child[3] = 0;
+    w.q[0] = w.q[1] = w.q[2] = 0;
+    w.q[3] = quant_hi(b.lx0, __int_as_float(int(ex << 23)), b.ux0);
+    w.q[4] = quant_hi(b.ly0, __int_as_float(int(ey << 23)), b.uy0);
+    w.q[5] = quant_hi(b.lz0, __int_as_float(int(ez << 23)), b.uz0);
+    w.pad0 = w.pad1 = 0;
+    wn[0] = w;
+}
+
 __global__ void k_single(const float4* lo, const float4* hi, BNode* nodes, uint32_t* order)
 {
     BNode nd;
@@ -467,6 +628,43 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
     for (int l = levels - 1; l >= 0; l--)
         hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(tpb), 0, s, lo, hi, st.order, st.nodes, lvl, meta, l);
     return int(hipGetLastError());
+}
+
+int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s)
+{
+    w.n = n;
+    if (n == 0) return 0;
+    const uint32_t need = n > 1 ? n - 1 : 1;
+    if (w.cap < need) {
+        wbvh_free(w);
+        const uint32_t c = need + need / 4;
+        if (hipMalloc(&w.nodes, size_t(c) * sizeof(WNode)) != hipSuccess) return 1;
+        if (hipMalloc(&w.bin, size_t(c) * sizeof(uint32_t)) != hipSuccess) return 1;
+        w.cap = c;
+    }
+    if (n == 1) {
+        hipLaunchKernelGGL(k_wide_single, dim3(1), dim3(1), 0, s, bin.nodes, w.nodes);
+        return int(hipGetLastError());
+    }
+    if (sc.meta.ensure(2 * (kMaxLevels + 1) * 4)) return 1;
+    uint32_t* meta = sc.meta.as<uint32_t>();
+    int levels = 1;
+    while (levels < kMaxLevels && (uint64_t(1) << std::min(levels - 1, 62)) < n) levels++;
+    levels = std::min(kMaxLevels, levels + 31);
+    const uint32_t grid = std::min<uint32_t>(blocks_for(n, 256), 2048);
+    hipLaunchKernelGGL(k_wide_init, dim3(1), dim3(kMaxLevels + 1), 0, s, meta, w.bin);
+    for (int l = 0; l + 1 < levels; l++)
+        hipLaunchKernelGGL(k_wide_level, dim3(grid), dim3(256), 0, s, bin.nodes, w.nodes, w.bin, meta, l);
+    return int(hipGetLastError());
+}
+
+void wbvh_free(WBvhStore& w)
+{
+    if (w.nodes) (void)hipFree(w.nodes);
+    if (w.bin) (void)hipFree(w.bin);
+    w.nodes = nullptr;
+    w.bin = nullptr;
+    w.cap = 0;
 }
 
 void lbvh_free(BvhStore& st)

@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kMergeBlock = 256;
 constexpr int kMergeWaves = kMergeBlock / 64;
-constexpr uint32_t kCandCap = 192; // < 64 pending + up to 2 leaves per lane in one node step
+constexpr uint32_t kCandCap = 320; // < 64 pending + up to 4 leaves per lane in one node step
 constexpr uint32_t kPairCap = 128; // < 64 pending + at most one accept per lane in a test batch
 
 struct CandQueue {
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     // B-B1D walks query pieces; P-B2D (few candidates per query) walks whole segments
     const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : min(b.qcount[128 + bounce], b.seg_cap);
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
-    const Bvh& bvh = KIND == 0 ? b.maps.med : b.maps.beam;
+    const WBvh& bvh = KIND == 0 ? b.maps.wmed : b.maps.wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
 
     uint32_t cn = 0, qn = 0; // candidates / pairs queued (same value in every lane)
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     RayBox rb = make_raybox(mk(0.f), mk(1.f), 0.f, 0.f);
     int node = 0, sb = 0, sp = 0; // pending subtrees: stack[sb, sp), oldest (largest) at sb
     int stack[kBvhStack];
-    uint32_t n_nodes = 0, n_cands = 0, n_acc = 0, n_err = 0;
+    uint32_t n_nodes = 0, n_cands = 0, n_acc = 0, n_err = 0, n_trips = 0, n_cb = 0, n_pb = 0;
 
     for (;;) {
         if (!exhausted) { // lanes without work take the next query pieces
@@ -214,37 +214,46 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
         }
         const bool active = cur >= 0;
         const bool any_active = __ballot(active) != 0;
-        // one node of this lane's traversal; leaves hit become candidates
-        int l0 = -1, l1 = -1;
+        n_trips++;
+        // one 4-wide node of this lane's traversal; leaves hit become candidates
+        int lf[4] = {-1, -1, -1, -1};
         uint32_t qnow = 0;
         if (active) {
-            const BNode nd = bvh.nodes[node];
+            const WNode nd = bvh.nodes[node];
             n_nodes++;
-            float e0, e1;
-            const bool h0 = slab(rb, nd.lx0, nd.ly0, nd.lz0, nd.ux0, nd.uy0, nd.uz0, rb.tmax, e0);
-            const bool h1 = slab(rb, nd.lx1, nd.ly1, nd.lz1, nd.ux1, nd.uy1, nd.uz1, rb.tmax, e1);
+            const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
+            const int nc = int(nd.meta >> 24);
             int next = -1;
-            if (h0) {
-                if (nd.c0 < 0) l0 = ~nd.c0;
-                else next = nd.c0;
-            }
-            if (h1) {
-                if (nd.c1 < 0) {
-                    l1 = ~nd.c1;
-                } else if (next < 0) {
-                    next = nd.c1;
-                } else {
-                    if (sp == kBvhStack && sb > 0) { // slide the pending entries down
-                        for (int k = sb; k < sp; k++) stack[k - sb] = stack[k];
-                        sp -= sb;
-                        sb = 0;
-                    }
-                    if (sp < kBvhStack) {
-                        stack[sp++] = nd.c1;
-                    } else {
-                        n_err++; // traversal stack exhausted: the piece is abandoned and counted
-                        next = -1;
-                        sb = sp = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k < nc) {
+                    float e;
+                    const bool h = slab(rb, wnode_decode(nd.ox, nd.q[0], k, sx), wnode_decode(nd.oy, nd.q[1], k, sy),
+                                        wnode_decode(nd.oz, nd.q[2], k, sz), wnode_decode(nd.ox, nd.q[3], k, sx),
+                                        wnode_decode(nd.oy, nd.q[4], k, sy), wnode_decode(nd.oz, nd.q[5], k, sz),
+                                        rb.tmax, e);
+                    const int ch = nd.child[k];
+                    if (h) {
+                        if (ch < 0) {
+                            lf[k] = ~ch;
+                        } else if (next < 0) {
+                            next = ch;
+                        } else {
+                            if (sp == kBvhStack && sb > 0) { // slide the pending entries down
+                                for (int t = sb; t < sp; t++) stack[t - sb] = stack[t];
+                                sp -= sb;
+                                sb = 0;
+                            }
+                            if (sp < kBvhStack) {
+                                stack[sp++] = ch;
+                            } else {
+                                n_err++; // traversal stack exhausted: the piece is abandoned and counted
+                                sb = sp = 0;
+                                lf[0] = lf[1] = lf[2] = lf[3] = -1;
+                                next = -1;
+                                break;
+                            }
+                        }
                     }
                 }
             }
@@ -254,8 +263,8 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
             if (next < 0) cur = -1;
         }
         // all lanes take part in the queue appends (ballots over the whole wave)
-        cq_push(C, cn, l0 >= 0, qnow, uint32_t(l0), lo, hi);
-        cq_push(C, cn, l1 >= 0, qnow, uint32_t(l1), lo, hi);
+#pragma unroll
+        for (int k = 0; k < 4; k++) cq_push(C, cn, lf[k] >= 0, qnow, uint32_t(lf[k]), lo, hi);
         // Evaluate 64 accepted pairs at a time whenever 64 are queued (this keeps fewer than 64
         // pending before each candidate test), else test 64 candidates at a time. Once no lane
         // has work left, drain both queues: candidates first, then pairs.
@@ -265,6 +274,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
             if (!eval_p && !test_c) break;
             if (eval_p) {
                 const uint32_t m = qn < 64u ? qn : 64u;
+                n_pb++;
                 wave_lds_sync();
                 if (lane < m) {
                     const uint32_t e = qn - m + lane;
@@ -294,6 +304,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 qn -= m;
             } else {
                 const uint32_t n = cn < 64u ? cn : 64u;
+                n_cb++;
                 wave_lds_sync();
                 bool acc = false;
                 uint32_t qi = 0, slot = 0;
@@ -319,12 +330,16 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     }
     const uint32_t s_nodes = wave_sum(n_nodes), s_cands = wave_sum(n_cands), s_acc = wave_sum(n_acc),
                    s_err = wave_sum(n_err);
+    // loop trips and batches are the same in every lane: lane 0 reports them for the wave
     if (lane == 0) {
         stat_add(b.stats, STAT_SEG_NODES, s_nodes);
         stat_add(b.stats, STAT_SEG_CANDS, s_cands);
         stat_add(b.stats, STAT_ACCEPT, s_acc);
         stat_add(b.stats, KIND == 0 ? STAT_ACC_PB2D : STAT_ACC_BB1D, s_acc);
         stat_add(b.stats, STAT_ERRORS, s_err);
+        stat_add(b.stats, STAT_SEG_TRIPS, n_trips);
+        stat_add(b.stats, STAT_CAND_BATCHES, n_cb);
+        stat_add(b.stats, STAT_PAIR_BATCHES, n_pb);
     }
 }
 

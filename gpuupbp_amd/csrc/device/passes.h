@@ -329,6 +329,7 @@ UPBP_HD void subbeam_box(f3 o, f3 d, float len, float ta, float tb, float r, flo
 
 struct PhotonMaps {
     Bvh surf, med, beam;
+    WBvh wmed, wbeam; // 4-wide collapses of med and beam (segment merge kernels)
     const float4* surf_pts; // sorted: (hit.xyz, vertex slot)
     const float4* med_pts;
     const float4* beam_geo; // sorted: 3 per sub-beam: (origin.xyz, length), (dir.xyz, beam slot),
@@ -393,7 +394,7 @@ struct Bufs {
 
 enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_POINT_Q, STAT_SEG_Q, STAT_ACCEPT,
                   STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,
-                  STAT_SEG_PIECES, STAT_COUNT };
+                  STAT_SEG_PIECES, STAT_SEG_TRIPS, STAT_CAND_BATCHES, STAT_PAIR_BATCHES, STAT_COUNT };
 constexpr int kQcountWords = 512;
 
 // wave-aggregated counter increment

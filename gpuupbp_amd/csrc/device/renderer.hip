@@ -168,6 +168,7 @@ struct upbp_renderer {
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
     DevBuf mcolor, segq, pieces;
     BvhStore bvh_s, bvh_m, bvh_b;
+    WBvhStore wbvh_m, wbvh_b;
     float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
     upbp_tuning tune;
     hipEvent_t ev[4 * kMaxBounces + 8];
@@ -577,7 +578,8 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
     PhotonMaps& M = b.maps;
     if (lbvh_build(r->bvh_s, r->scratch, r->lo_s.as<float4>(), r->hi_s.as<float4>(), ns, s) ||
         lbvh_build(r->bvh_m, r->scratch, r->lo_m.as<float4>(), r->hi_m.as<float4>(), nm, s) ||
-        lbvh_build(r->bvh_b, r->scratch, r->lo_b.as<float4>(), r->hi_b.as<float4>(), nb, s))
+        lbvh_build(r->bvh_b, r->scratch, r->lo_b.as<float4>(), r->hi_b.as<float4>(), nb, s) ||
+        wbvh_build(r->wbvh_m, r->bvh_m, r->scratch, nm, s) || wbvh_build(r->wbvh_b, r->bvh_b, r->scratch, nb, s))
         return fail(r, "photon LBVH build failed", UPBP_E_HIP);
     if (ns)
         hipLaunchKernelGGL(k_gather_points, dim3(nblk(ns)), dim3(256), 0, s, r->bvh_s.order, r->list_s.as<uint32_t>(),
@@ -591,6 +593,8 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
     M.surf = {r->bvh_s.nodes, r->bvh_s.order, ns};
     M.med = {r->bvh_m.nodes, r->bvh_m.order, nm};
     M.beam = {r->bvh_b.nodes, r->bvh_b.order, nb};
+    M.wmed = {r->wbvh_m.nodes, nm};
+    M.wbeam = {r->wbvh_b.nodes, nb};
     M.surf_pts = r->pts_s.as<float4>();
     M.med_pts = r->pts_m.as<float4>();
     M.beam_geo = r->geo_b.as<float4>();
@@ -700,6 +704,9 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         stats->seg_nodes = st[STAT_SEG_NODES];
         stats->seg_candidates = st[STAT_SEG_CANDS];
         stats->seg_pieces = st[STAT_SEG_PIECES];
+        stats->seg_wave_trips = st[STAT_SEG_TRIPS];
+        stats->seg_cand_batches = st[STAT_CAND_BATCHES];
+        stats->seg_pair_batches = st[STAT_PAIR_BATCHES];
         stats->vertex_drops = st[STAT_VDROP];
         stats->beam_drops = st[STAT_BDROP];
         stats->path_errors = st[STAT_ERRORS];
@@ -789,6 +796,8 @@ extern "C" void upbp_destroy(upbp_renderer* r)
     lbvh_free(r->bvh_s);
     lbvh_free(r->bvh_m);
     lbvh_free(r->bvh_b);
+    wbvh_free(r->wbvh_m);
+    wbvh_free(r->wbvh_b);
     lbvh_scratch_free(r->scratch);
     for (auto& e : r->ev)
         if (e) (void)hipEventDestroy(e);

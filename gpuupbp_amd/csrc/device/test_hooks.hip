@@ -28,7 +28,7 @@ __global__ void k_point_boxes(const float* prims, uint32_t n, float r, float4* l
 // pass 0 counts accepted primitives per query, pass 1 writes them at offsets[q]
 // Beams (kinds 2, 3) enter the LBVH as sub-beams: leaf -> (beam, owned parameter range), and a
 // pair counts only for the sub-beam whose range holds the predicate's beam parameter.
-__global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const uint32_t* sub_prim,
+__global__ void k_range(int kind, int pass, Bvh bvh, WBvh wide, const float* prims, const uint32_t* sub_prim,
                         const float2* sub_rng, const float* queries, uint32_t nq, float r2, uint32_t* counts,
                         const uint32_t* offsets, uint32_t* out, uint32_t* stack_fail)
 {
@@ -64,7 +64,7 @@ __global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const u
         const float* q = queries + 8 * size_t(qi);
         const f3 O = mk(q[0], q[1], q[2]), D = mk(q[3], q[4], q[5]);
         const float t0 = q[6], t1 = q[7];
-        ok = bvh_ray_query(bvh, O, D, t0, t1, [&](uint32_t k, float&) {
+        auto leaf = [&](uint32_t k) {
             if (kind == 1) {
                 const uint32_t prim = bvh.order[k];
                 const float* x = prims + 3 * size_t(prim);
@@ -81,7 +81,9 @@ __global__ void k_range(int kind, int pass, Bvh bvh, const float* prims, const u
                     tb >= rg.x && tb < rg.y)
                     accept(prim);
             }
-        });
+        };
+        if (wide.nodes) ok = wbvh_ray_query(wide, O, D, t0, t1, leaf);
+        else ok = bvh_ray_query(bvh, O, D, t0, t1, [&](uint32_t k, float&) { leaf(k); });
     }
     if (!ok) atomicAdd(stack_fail, 1u);
     if (!pass) counts[qi] = c;
@@ -148,6 +150,15 @@ extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_pr
                                      uint32_t n_queries, float radius, float subbeam_len, uint32_t* offsets,
                                      uint32_t* indices, uint64_t max_indices, uint64_t* n_indices)
 {
+    return upbp_test_range_query_wide(kind, prims, n_prims, queries, n_queries, radius, subbeam_len, 0, offsets,
+                                      indices, max_indices, n_indices);
+}
+
+extern "C" int upbp_test_range_query_wide(int kind, const float* prims, uint32_t n_prims, const float* queries,
+                                          uint32_t n_queries, float radius, float subbeam_len, int wide,
+                                          uint32_t* offsets, uint32_t* indices, uint64_t max_indices,
+                                          uint64_t* n_indices)
+{
     if (kind < 0 || kind > 3 || !offsets || (n_prims && !prims) || (n_queries && !queries)) return UPBP_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
@@ -200,18 +211,22 @@ extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_pr
     }
     (void)hipMemset(fail_count, 0, 4);
     BvhStore st;
+    WBvhStore wst;
     BvhScratch sc;
     if (kind <= 1 && n_prims)
         hipLaunchKernelGGL(k_point_boxes, dim3(nb(n_prims)), dim3(256), 0, s, dp, n_prims, radius, lo, hi);
-    if (lbvh_build(st, sc, lo, hi, n_leaves, s)) {
+    const bool use_wide = wide && (kind == 1 || kind == 3);
+    if (lbvh_build(st, sc, lo, hi, n_leaves, s) || (use_wide && wbvh_build(wst, st, sc, n_leaves, s))) {
         lbvh_free(st);
+        wbvh_free(wst);
         lbvh_scratch_free(sc);
         return UPBP_E_HIP;
     }
     const Bvh bvh{st.nodes, st.order, n_leaves};
+    const WBvh wbvh{use_wide ? wst.nodes : nullptr, n_leaves};
     const float r2 = radius * radius;
     if (n_queries)
-        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 0, bvh, dp, dsp, dsr, dq, n_queries,
+        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 0, bvh, wbvh, dp, dsp, dsr, dq, n_queries,
                            r2, cnt, off, nullptr, fail_count);
     uint32_t* total = off + n_queries;
     int rc = scan_exclusive(cnt, off, n_queries, total, sc, s);
@@ -219,7 +234,7 @@ extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_pr
     if (!rc) rc = int(hipMemcpy(&host_total, total, 4, hipMemcpyDeviceToHost));
     uint32_t* dout = rc ? nullptr : t.alloc<uint32_t>(host_total);
     if (!rc && n_queries && dout)
-        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 1, bvh, dp, dsp, dsr, dq, n_queries,
+        hipLaunchKernelGGL(k_range, dim3(nb(n_queries)), dim3(256), 0, s, kind, 1, bvh, wbvh, dp, dsp, dsr, dq, n_queries,
                            r2, cnt, off, dout, fail_count);
     if (!rc) rc = int(hipDeviceSynchronize());
     if (!rc) rc = int(hipMemcpy(offsets, off, (size_t(n_queries) + 1) * 4, hipMemcpyDeviceToHost));
@@ -227,6 +242,7 @@ extern "C" int upbp_test_range_query(int kind, const float* prims, uint32_t n_pr
         rc = int(hipMemcpy(indices, dout, std::min<uint64_t>(host_total, max_indices) * 4, hipMemcpyDeviceToHost));
     if (!rc) rc = int(hipMemcpy(&fails, fail_count, 4, hipMemcpyDeviceToHost));
     lbvh_free(st);
+    wbvh_free(wst);
     lbvh_scratch_free(sc);
     if (n_queries == 0) offsets[0] = 0;
     if (n_indices) *n_indices = host_total;

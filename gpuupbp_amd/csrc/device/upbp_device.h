@@ -188,6 +188,30 @@ struct Bvh {
     uint32_t n;            // primitives
 };
 
+// 4-wide node collapsed from the binary LBVH (64 bytes). Child boxes are quantised to 8 bits
+// per plane on a power-of-two grid anchored at the node box's low corner; decoding is
+// origin + q * step in FP32 and the builder picks q so that the decoded box contains the
+// exact child box.
+struct __align__(16) WNode {
+    float ox, oy, oz;
+    uint32_t meta;   // bits 0-7 / 8-15 / 16-23: biased exponents of the x / y / z steps; 24-26: child count
+    int child[4];    // >= 0: wide node, < 0: ~sorted leaf position
+    uint32_t q[6];   // byte k of q[0..2] = child k low x, y, z; of q[3..5] = child k high x, y, z
+    uint32_t pad0, pad1;
+};
+
+struct WBvh {
+    const WNode* nodes;
+    uint32_t n; // primitives
+};
+
+UPBP_HD float wnode_step(uint32_t meta, int axis) { return __builtin_bit_cast(float, ((meta >> (8 * axis)) & 255u) << 23); }
+UPBP_HD float wnode_decode(float origin, uint32_t word, int k, float step)
+{
+#pragma clang fp contract(off)
+    return origin + float((word >> (8 * k)) & 255u) * step;
+}
+
 struct TriSet {
     const float4* tri;  // 3 float4 per sorted triangle: p0, e1 = p1 - p0, e2 = p2 - p0
     const float4* nrm;  // 4 float4 per sorted triangle: n0, n1, n2, face normal

@@ -78,9 +78,10 @@ class Renderer:
             pass
 
 
-def range_query(kind, prims, queries, radius, max_indices=None, subbeam_len=0.0):
+def range_query(kind, prims, queries, radius, max_indices=None, subbeam_len=0.0, wide=False):
     """Exact accepted index sets of an LBVH range query (CSR: offsets, indices). Beams (kinds 2, 3)
-    enter the LBVH as sub-beams of length about subbeam_len (0: whole beams)."""
+    enter the LBVH as sub-beams of length about subbeam_len (0: whole beams); wide=True walks the
+    4-wide collapse (segment kinds 1 and 3)."""
     prims = np.ascontiguousarray(prims, np.float32)
     queries = np.ascontiguousarray(queries, np.float32)
     nq = queries.shape[0]
@@ -88,8 +89,8 @@ def range_query(kind, prims, queries, radius, max_indices=None, subbeam_len=0.0)
     offsets = np.zeros(nq + 1, np.uint32)
     idx = np.zeros(cap, np.uint32)
     n = C.c_uint64()
-    rc = capi.lib().upbp_test_range_query(kind, _p(prims), prims.shape[0], _p(queries), nq, radius, subbeam_len,
-                                          _p(offsets), _p(idx), cap, C.byref(n))
+    rc = capi.lib().upbp_test_range_query_wide(kind, _p(prims), prims.shape[0], _p(queries), nq, radius,
+                                               subbeam_len, int(wide), _p(offsets), _p(idx), cap, C.byref(n))
     if rc != 0:
         raise RuntimeError("upbp_test_range_query failed: %d" % rc)
     return offsets, idx[: n.value]

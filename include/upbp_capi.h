@@ -152,6 +152,8 @@ typedef struct upbp_iter_stats {
     uint64_t merge_accepts, pb2d_accepts, bb1d_accepts;
     uint64_t seg_nodes, seg_candidates; /* LBVH nodes visited / leaves tested by segment queries */
     uint64_t seg_pieces;           /* pieces the segment queries were walked in */
+    uint64_t seg_wave_trips;       /* merge kernels: wave-level traversal loop trips */
+    uint64_t seg_cand_batches, seg_pair_batches; /* wave-level candidate-test / evaluation batches */
     uint64_t vertex_drops, beam_drops, path_errors;
     double algorithmic_bytes;      /* compulsory HBM bytes of this iteration */
     double dominant_bytes;         /* compulsory HBM bytes of the dominant kernel */
@@ -235,6 +237,11 @@ int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
 int upbp_test_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries,
                           uint32_t n_queries, float radius, float subbeam_len, uint32_t* offsets,
                           uint32_t* indices, uint64_t max_indices, uint64_t* n_indices);
+/* Same, with wide != 0 walking the 4-wide collapse of the LBVH (kinds 1 and 3, as the
+ * segment merge kernels do). */
+int upbp_test_range_query_wide(int kind, const float* prims, uint32_t n_prims, const float* queries,
+                               uint32_t n_queries, float radius, float subbeam_len, int wide, uint32_t* offsets,
+                               uint32_t* indices, uint64_t max_indices, uint64_t* n_indices);
 
 /* Closest-hit / all-hits ray queries against a triangle LBVH (replaces the optixTrace calls of
  * gpuupbp/kernel/intersector.cuh:120,214). rays: 8 floats each (origin, dir, tmin, tmax).

@@ -59,6 +59,18 @@ def test_range_query_matches_brute_force(kind, radius):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("kind,radius", [(1, 0.02), (3, 0.02)])
+def test_range_query_wide_tree_matches_brute_force(kind, radius):
+    # the 4-wide quantised collapse walked by the segment merge kernels loses no leaf
+    rng = np.random.default_rng(300 + kind)
+    prims, q = _cases(kind, rng, 20000 if kind == 1 else 6000, 1500, spread=3.0)
+    go, gi = gpu.range_query(kind, prims, q, radius, subbeam_len=0.05 if kind == 3 else 0.0, wide=True)
+    oo, oi = oracle_py.range_query(kind, prims, q, radius)
+    assert len(gi) == len(oi) and len(gi) > 100
+    for a, b in zip(_sets(go, gi), _sets(oo, oi)):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("kind,sub", [(2, 0.0), (2, 0.07), (3, 0.0), (3, 0.03), (3, 0.31)])
 def test_range_query_subbeams_report_each_beam_once(kind, sub):
     # beams cut into sub-beams (as the renderer's beam map does) give the brute-force index set
@@ -98,14 +110,15 @@ def test_range_query_near_parallel_beams():
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_range_query_edge_sizes(kind):
     rng = np.random.default_rng(7)
-    for n_prims in (0, 1, 2, 3):
+    for n_prims in (0, 1, 2, 3, 5):
         prims, q = _cases(kind, rng, max(n_prims, 1), 64, spread=0.05)
         prims = prims[:n_prims]
-        go, gi = gpu.range_query(kind, prims, q, 0.2)
         oo, oi = oracle_py.range_query(kind, prims, q, 0.2)
-        assert np.array_equal(go, oo)
-        for a, b in zip(_sets(go, gi), _sets(oo, oi)):
-            assert np.array_equal(a, b)
+        for wide in (False, True):
+            go, gi = gpu.range_query(kind, prims, q, 0.2, wide=wide)
+            assert np.array_equal(go, oo)
+            for a, b in zip(_sets(go, gi), _sets(oo, oi)):
+                assert np.array_equal(a, b)
 
 
 def test_range_query_closed_boundary():
