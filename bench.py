@@ -41,7 +41,7 @@ def cpu_baseline(scene_ptr, cfg):
     the full frame."""
     from oracle import oracle_py
     threads = min(16, os.cpu_count() or 1)
-    n_cam = 16384
+    n_cam = 65536
     t = oracle_py.render_sample_times(scene_ptr, cfg, n_cam, int(cfg.path_count_per_iter), threads)
     npx = cfg.res_x * cfg.res_y
     t_iter = t[0] + t[1] + t[2] * (npx / float(n_cam))
@@ -53,6 +53,19 @@ def cpu_baseline(scene_ptr, cfg):
         "sample": "one iteration: all %d light paths + photon maps (%.1f s + %.1f s), camera pass on %d of %d "
                   "pixels (%.1f s) scaled to the frame" % (int(cfg.path_count_per_iter), t[0], t[1], n_cam, npx, t[2]),
     }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or t.get("workload") != "blobs":
+        return None
+    return t["traffic_bytes_per_launch"]
 
 
 def main():
@@ -151,7 +164,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(st.dominant_kernel.decode()),
+            "traffic_source": "profiles/seg_merge_blobs_hbm_traffic.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; bytes)",
             "avg_launch_ms": dom_ms,
             "algorithmic_bytes_per_launch": st.dominant_bytes / launches,
             "measured_copy": copy,

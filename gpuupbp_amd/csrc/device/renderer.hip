@@ -756,7 +756,7 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
 {
     if (!t) return;
     memset(t, 0, sizeof(*t));
-    t->subbeam_factor = 16.f;
+    t->subbeam_factor = 12.f;
     t->query_piece_factor = 64.f;
     t->merge_blocks_per_cu = 0;
 }

@@ -210,7 +210,7 @@ int upbp_sync(upbp_renderer* r);
  * build had none). They change speed only, never which photons a query accepts. */
 typedef struct upbp_tuning {
     float subbeam_factor;     /* beams are cut into pieces of about subbeam_factor x the beam
-                                 radius for the beam LBVH; 0 keeps whole beams (default 16) */
+                                 radius for the beam LBVH; 0 keeps whole beams (default 12) */
     float query_piece_factor; /* camera-segment queries are walked in pieces of about this
                                  factor x the merge radius, at most 32 per ray plus one per
                                  segment; 0 keeps whole segments (default 64) */
