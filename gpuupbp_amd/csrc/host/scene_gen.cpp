@@ -178,7 +178,7 @@ void camera_aux(FILE* aux, P3 origin, P3 target, P3 up, double fov_deg)
     fprintf(aux, "CAMERA_TDIST 1.0\n");
 }
 
-int gen_box(const std::string& obj_path, uint32_t seed, bool furnace)
+int gen_box(const std::string& obj_path, uint32_t seed, bool furnace, bool fog = false)
 {
     const std::string mtl_path = strip_ext(obj_path) + ".mtl";
     FILE* obj = fopen(obj_path.c_str(), "w");
@@ -194,7 +194,11 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace)
             g = rng.uniform(0.5, 0.8);
             b = rng.uniform(0.5, 0.8);
         }
-        fprintf(mtl, "newmtl %s\nNs 0\nKd %.6f %.6f %.6f\nKs 0 0 0\nKe 0 0 0\n\n", walls[i], r, g, b);
+        if (fog && i == 2) // glossy back wall
+            fprintf(mtl, "newmtl %s\nNs 30\nKd %.6f %.6f %.6f\nKs 0.25 0.25 0.25\nKe 0 0 0\n\n", walls[i], 0.6 * r,
+                    0.6 * g, 0.6 * b);
+        else
+            fprintf(mtl, "newmtl %s\nNs 0\nKd %.6f %.6f %.6f\nKs 0 0 0\nKe 0 0 0\n\n", walls[i], r, g, b);
     }
     fprintf(mtl, "newmtl light\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 10 10 10\n\n");
     if (!furnace) fprintf(mtl, "newmtl sphere_boundary\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 0 0 0\n\n");
@@ -224,7 +228,7 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace)
     if (!furnace) {
         Mesh s = icosphere(3);
         w.usemtl("sphere_boundary");
-        w.mesh(s, {0, 0, 0}, 0.3, false);
+        w.mesh(s, {0, 0, 0}, fog ? 0.55 : 0.3, false);
     }
     fclose(obj);
     fclose(mtl);
@@ -234,7 +238,10 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace)
     else
         camera_aux(aux, {0, 0, 3.2}, {0, 0, 0}, {0, 1, 0}, 40.0);
     if (!furnace) {
-        fprintf(aux, "medium sphere_medium\nabsorption 0.1 0.1 0.1\nemission 0 0 0\nscattering 0.5 0.5 0.5\ng 0\n\n");
+        if (fog)
+            fprintf(aux, "medium sphere_medium\nabsorption 0.3 0.3 0.3\nemission 0 0 0\nscattering 2.5 2.2 2.0\ng 0.5\n\n");
+        else
+            fprintf(aux, "medium sphere_medium\nabsorption 0.1 0.1 0.1\nemission 0 0 0\nscattering 0.5 0.5 0.5\ng 0\n\n");
         fprintf(aux, "material sphere_boundary\ngeometryType imaginary\nmediumId sphere_medium\npriority 1\n");
     }
     fclose(aux);
@@ -326,6 +333,7 @@ extern "C" int upbp_generate_scene(const char* kind, uint32_t seed, const char* 
     const std::string k(kind);
     if (k == "box") return gen_box(obj_path, seed, false);
     if (k == "furnace") return gen_box(obj_path, seed, true);
+    if (k == "boxfog") return gen_box(obj_path, seed, false, true);
     if (k == "blobs") return gen_blobs(obj_path, seed);
     return UPBP_E_ARG;
 }

@@ -200,7 +200,8 @@ const char* upbp_last_error(const upbp_renderer* r);
 const char* upbp_global_error(void);
 
 /* Writes a seeded procedural scene as an OBJ + MTL + AUX triple (benchmark and test inputs; no
- * reference counterpart). kind: "box", "blobs" or "furnace". */
+ * reference counterpart). kind: "box", "boxfog" (denser anisotropic medium, glossy back wall),
+ * "blobs" or "furnace". */
 int upbp_generate_scene(const char* kind, uint32_t seed, const char* obj_path);
 
 /* Blocks until all work queued by r has finished. */

@@ -40,6 +40,8 @@ def lib():
         h.oracle_fresnel.argtypes = [C.c_float, C.c_float]
         h.oracle_fresnel.restype = C.c_float
         h.oracle_philox.argtypes = [vp, vp, vp]
+        h.oracle_mis_check.argtypes = [vp, vp, C.c_int, vp, C.c_uint32, C.c_uint32, C.c_int, vp, C.c_uint32,
+                                       C.POINTER(C.c_uint32)]
         _lib = h
     return _lib
 
@@ -117,3 +119,21 @@ def philox(ctr, key):
     o = np.zeros(4, np.uint32)
     lib().oracle_philox(_p(c), _p(k), _p(o))
     return o
+
+
+MIS_TECH = {0: "light_hit", 1: "nee", 2: "bpt", 3: "light_tracing", 4: "surf", 5: "pp3d"}
+
+
+def mis_check(scene_ptr, cfg, iteration, pixels, n_light_tracing_paths, threads=0, max_records=2000000):
+    """Records (technique, s, k, constant-time weight, FP64 balance-heuristic weight, the same
+    with the reference's next-event weight convention) of every contribution on the given
+    camera paths and the first light paths of one iteration."""
+    pixels = np.ascontiguousarray(pixels, np.uint32)
+    out = np.zeros((max_records, 6), np.float64)
+    n = C.c_uint32()
+    rc = lib().oracle_mis_check(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), iteration,
+                                _p(pixels), len(pixels), n_light_tracing_paths, threads, _p(out), max_records,
+                                C.byref(n))
+    if rc != 0:
+        raise RuntimeError("oracle_mis_check failed: %d" % rc)
+    return out[: min(n.value, max_records)]

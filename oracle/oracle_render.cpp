@@ -2,6 +2,7 @@
 // camera subpaths, the five photon-density estimators with constant-time MIS.
 // TEST INFRASTRUCTURE ONLY (see oracle.h). Citations: nolmoonen/gpuupbp.
 #include "oracle_core.h"
+#include "oracle_mis.h"
 
 #include <algorithm>
 #include <atomic>
@@ -245,6 +246,7 @@ struct SubpathState {
 struct LightVertex {
     V hit, throughput;
     uint32_t path_length;
+    uint32_t path = 0; // light path index
     BSDF bsdf;
     VertexWeights w;
     bool in_medium, behind_surf;
@@ -302,7 +304,32 @@ struct Ctx {
     std::vector<LightBeam> beams;
     Bvh bvh_surf, bvh_med, bvh_beam;
     std::atomic<uint64_t> errors{0};
+    // MIS logging (FP64 full-path oracle): emission points of the light paths, records per
+    // logged camera path (pixel -> slot) and per logged light path
+    bool mis_on = false;
+    std::vector<V> light_origin;
+    std::vector<int> light_pick;
+    std::vector<int32_t> mis_slot;
+    std::vector<std::vector<MisRec>>* mis_cam = nullptr;
+    uint32_t mis_lt_paths = 0;
+    std::vector<std::vector<MisRec>>* mis_lt = nullptr;
 };
+
+// camera-subpath vertex kept for the MIS log: its BSDF (incoming from the next vertex toward
+// the camera) and the distance pdfs of the edge it was reached by, in camera direction
+struct CamVert {
+    V pos;
+    BSDF bsdf;
+    float rsf_c, rsr_c;
+};
+
+struct MisLogger {
+    std::vector<MisRec>* out = nullptr;
+    std::vector<CamVert> cam;                  // current camera path, first hit first
+    const std::vector<LightVertex>* lverts = nullptr; // light tracing: vertices of the current light path
+    size_t lbegin = 0;
+};
+static thread_local MisLogger* g_mis = nullptr;
 
 static int priority_of(const Scene& sc, int mat) { return mat >= 0 ? sc.mats[size_t(mat)].priority : kGlobalPriority; }
 static bool real_geom(const Scene& sc, int mat) { return mat >= 0 && sc.mats[size_t(mat)].real; }
@@ -644,6 +671,119 @@ static int sample_scattering(const Ctx& c, bool& cont, const BSDF& b, V hit, con
     return OK;
 }
 
+// ---- MIS log --------------------------------------------------------------------
+// Primitive densities of the full path x_0 = light point, x_1..x_nl = light vertices lv,
+// then camera vertices cam[nc-1] .. cam[0], then the camera. With `merge` the last light
+// vertex and the last camera vertex are the same path vertex (the merge); otherwise the
+// edge between the two sides is a connection whose distance pdfs are conn_rsf (toward
+// the camera side) and conn_rsr (toward the light side).
+static void mis_record(const Ctx& c, int tech, float weight, V x0, int light_id, const LightVertex* lv, int nl,
+                       const CamVert* cam, int nc, bool merge, float conn_rsf, float conn_rsr)
+{
+    if (!g_mis || !g_mis->out) return;
+    const Scene& sc = *c.sc;
+    MisRec r;
+    r.tech = tech;
+    r.weight = weight;
+    const int k = nl + nc + 1 - (merge ? 1 : 0);
+    if (k < 2) return;
+    r.k = k;
+    r.s = merge ? nl : nl + 1;
+    if (tech == MT_LIGHT_HIT) r.s = 0;
+    const size_t nv = size_t(k) + 1;
+    r.type.assign(nv, MV_SURFACE);
+    r.fpdf.assign(nv, 0.0);
+    r.rpdf.assign(nv, 0.0);
+    r.cin.assign(nv, 1.0);
+    r.cout.assign(nv, 1.0);
+    r.sin_t.assign(nv, 0.0);
+    r.ratio.assign(nv, 0.0);
+    r.dist.assign(size_t(k), 0.0);
+    r.rsf.assign(size_t(k), 1.0);
+    r.rsr.assign(size_t(k), 1.0);
+    // positions: left (arriving edge) and right (leaving edge) differ only at a merge vertex
+    std::vector<V> pl(nv), pr(nv);
+    pl[0] = pr[0] = x0;
+    for (int j = 1; j <= nl; j++) pl[size_t(j)] = pr[size_t(j)] = lv[j - 1].hit;
+    auto cam_index = [&](int i) { return k - 1 - i; }; // path index of cam[i]
+    for (int i = 0; i < nc; i++) {
+        const int q = cam_index(i);
+        if (merge && q == nl) pr[size_t(q)] = cam[i].pos;
+        else pl[size_t(q)] = pr[size_t(q)] = cam[i].pos;
+    }
+    pl[size_t(k)] = pr[size_t(k)] = sc.cam.origin;
+    r.type[0] = MV_LIGHT;
+    r.type[size_t(k)] = MV_CAMERA;
+    // edges
+    for (int j = 0; j < k; j++) {
+        const V d = pl[size_t(j + 1)] - pr[size_t(j)];
+        r.dist[size_t(j)] = std::sqrt(double(d.x) * d.x + double(d.y) * d.y + double(d.z) * d.z);
+    }
+    for (int j = 1; j <= nl; j++) { // arriving edge of light vertex x_j
+        r.rsf[size_t(j - 1)] = 1.0 / double(lv[j - 1].w.for_inv);
+        r.rsr[size_t(j - 1)] = 1.0 / double(lv[j - 1].w.rev_inv);
+    }
+    for (int i = 0; i < nc; i++) { // arriving camera edge of cam[i]: x_q <- x_{q+1}
+        const int q = cam_index(i);
+        r.rsf[size_t(q)] = cam[i].rsr_c;
+        r.rsr[size_t(q)] = cam[i].rsf_c;
+    }
+    if (!merge) { // the connection edge x_nl -> x_{nl+1}
+        r.rsf[size_t(nl)] = conn_rsf;
+        r.rsr[size_t(nl)] = conn_rsr;
+    }
+    // interior vertices
+    auto norm_dir = [](V a, V b) { const V d = b - a; return d / std::sqrt(dot(d, d)); };
+    for (int q = 1; q < k; q++) {
+        float cosg, fp, rp, st;
+        const int li = q - 1;         // light vertex index when q <= nl
+        const int ci = k - 1 - q;     // camera vertex index when q > nl (or q == nl at a merge)
+        if (merge && q == nl) {
+            const BSDF& cb = cam[ci].bsdf;
+            const BSDF& lb = lv[li].bsdf;
+            bsdf_evaluate(cb, lb.dir_fix_world(), sc, cosg, &fp, &rp, &st);
+            r.rpdf[size_t(q)] = double(fp) * cb.cont_prob;
+            r.fpdf[size_t(q)] = double(rp) * lb.cont_prob;
+            r.cin[size_t(q)] = lb.d_factor();
+            r.cout[size_t(q)] = cb.d_factor();
+            r.sin_t[size_t(q)] = st;
+            r.type[size_t(q)] = lb.in_medium() ? MV_MEDIUM : MV_SURFACE;
+            if (lb.in_medium()) r.ratio[size_t(q)] = 1.0 / sc.media[size_t(lb.med)].min_pos_att;
+        } else if (q <= nl) {
+            const BSDF& b = lv[li].bsdf;
+            bsdf_evaluate(b, norm_dir(pr[size_t(q)], pl[size_t(q + 1)]), sc, cosg, &fp, &rp, &st);
+            r.fpdf[size_t(q)] = double(fp) * b.cont_prob;
+            r.rpdf[size_t(q)] = double(rp) * b.cont_prob;
+            r.cin[size_t(q)] = b.d_factor();
+            r.cout[size_t(q)] = std::fabs(cosg);
+            r.sin_t[size_t(q)] = st;
+            r.type[size_t(q)] = b.in_medium() ? MV_MEDIUM : MV_SURFACE;
+            if (b.in_medium()) r.ratio[size_t(q)] = 1.0 / sc.media[size_t(b.med)].min_pos_att;
+        } else {
+            const BSDF& b = cam[ci].bsdf;
+            bsdf_evaluate(b, norm_dir(pl[size_t(q)], pr[size_t(q - 1)]), sc, cosg, &fp, &rp, &st);
+            r.rpdf[size_t(q)] = double(fp) * b.cont_prob;
+            r.fpdf[size_t(q)] = double(rp) * b.cont_prob;
+            r.cin[size_t(q)] = std::fabs(cosg);
+            r.cout[size_t(q)] = b.d_factor();
+            r.sin_t[size_t(q)] = st;
+            r.type[size_t(q)] = b.in_medium() ? MV_MEDIUM : MV_SURFACE;
+            if (b.in_medium()) r.ratio[size_t(q)] = 1.0 / sc.media[size_t(b.med)].min_pos_att;
+        }
+    }
+    // light (area lights only) and camera
+    const Light& l = sc.lights[size_t(light_id)];
+    const V to1 = norm_dir(pr[0], pl[1]);
+    r.cos_light = std::fabs(double(dot(l.frame.z, to1)));
+    r.dpdf = double(l.inv_area) / double(sc.lights.size());
+    r.eps = r.cos_light / 3.14159265358979323846;
+    const V tok = norm_dir(pl[size_t(k)], pr[size_t(k - 1)]);
+    const double ca = double(dot(sc.cam.direction, tok));
+    const double ipd = double(sc.cam.image_plane_dist) / ca;
+    r.cam_pdf = ipd * ipd / ca;
+    g_mis->out->push_back(std::move(r));
+}
+
 // ---- light subpath ----------------------------------------------------------
 struct LightOut {
     std::vector<LightVertex> verts;
@@ -750,6 +890,13 @@ static int connect_to_camera(const Ctx& c, const SubpathState& s, V hit, const B
     const float w_light = (cam_pdf_a / float(c.p.n_light)) * rsr *
                           (f_s + bpt_s1 * wl.d_shared + (rpdf / wl.rev_inv) * wl.d_bpt);
     const float mis = 1.f / (w_light + 1.f);
+    if (g_mis && g_mis->lverts && g_mis->lverts->size() > g_mis->lbegin) {
+        const LightVertex* lv = g_mis->lverts->data() + g_mis->lbegin;
+        const int nl = int(g_mis->lverts->size() - g_mis->lbegin);
+        if (lv[nl - 1].path_length == uint32_t(nl)) // every vertex of the path was stored
+            mis_record(c, MT_LT, mis, c.light_origin[lv->path], c.light_pick[lv->path], lv, nl, nullptr, 0, false,
+                       segs.empty() ? 1.f : acc_for(segs), rsr);
+    }
     const float s2i = 1.f / i2s;
     const V contrib = mis * s.throughput * f * att / (float(c.p.n_light) * s2i);
     if (black_or_neg(contrib)) return OK;
@@ -758,11 +905,12 @@ static int connect_to_camera(const Ctx& c, const SubpathState& s, V hit, const B
 }
 
 // kernel/functs_light.cuh:86-156
-static int generate_light_sample(const Ctx& c, SubpathState& s, Stream& rng)
+static int generate_light_sample(const Ctx& c, SubpathState& s, Stream& rng, int* pick = nullptr)
 {
     const Scene& sc = *c.sc;
     const float lc = float(sc.lights.size());
     const int id = int(rng.next() * lc);
+    if (pick) *pick = id;
     const float d0 = rng.next(), d1 = rng.next(), p0 = rng.next(), p1 = rng.next();
     const Light& l = sc.lights[size_t(id)];
     float epdf, dpdf, cosl;
@@ -790,8 +938,20 @@ static void trace_light_path(const Ctx& c, uint32_t path_idx, LightOut& out)
     const bool store_beams = (p.do_bb1d && path_idx < p.bb1d_used) || p.do_bp2d;
     SubpathState s;
     Stream emit(kDomainLight, path_idx, p.iter_key, 0, P_EMIT);
-    int ret = generate_light_sample(c, s, emit);
+    int pick = 0;
+    int ret = generate_light_sample(c, s, emit, &pick);
     if (s.infinite && sc.media[sc.global_medium].has_attenuation) return;
+    MisLogger logger;
+    if (c.mis_on) {
+        const_cast<Ctx&>(c).light_origin[path_idx] = s.origin;
+        const_cast<Ctx&>(c).light_pick[path_idx] = pick;
+        if (path_idx < c.mis_lt_paths) {
+            logger.out = &(*c.mis_lt)[path_idx];
+            logger.lverts = &out.verts;
+            logger.lbegin = out.verts.size();
+            g_mis = &logger;
+        }
+    }
     bool origin_in_medium = false;
     std::vector<VolSeg> segs;
     std::vector<VolLite> lite;
@@ -847,6 +1007,7 @@ static void trace_light_path(const Ctx& c, uint32_t path_idx, LightOut& out)
             lv.hit = hitp;
             lv.throughput = s.throughput;
             lv.path_length = s.path_length;
+            lv.path = path_idx;
             lv.bsdf = b;
             lv.in_medium = origin_in_medium;
             lv.behind_surf = false;
@@ -870,6 +1031,7 @@ static void trace_light_path(const Ctx& c, uint32_t path_idx, LightOut& out)
         if (ret != OK) break;
         if (!cont) break;
     }
+    g_mis = nullptr;
     if (ret != OK) const_cast<Ctx&>(c).errors++;
 }
 
@@ -892,6 +1054,9 @@ static V light_radiance_mis(const Ctx& c, const Light& l, const SubpathState& s,
     const VertexWeights& w = s.sw.w;
     const float bpt_t1 = float(!w.prev_specular);
     const float w_cam = dpdf_a * bpt_t1 * w.d_shared + epdf_w * rsr * w.d_bpt;
+    if (g_mis && l.type == L_AREA)
+        mis_record(c, MT_LIGHT_HIT, 1.f / (1.f + w_cam), hit, int(&l - sc.lights.data()), nullptr, 0,
+                   g_mis->cam.data(), int(g_mis->cam.size()), false, rsr, rsf);
     return rad * (1.f / (1.f + w_cam));
 }
 
@@ -936,6 +1101,9 @@ static int direct_illumination(const Ctx& c, V& contrib, const SubpathState& s, 
     const float bt1 = float(!w.prev_specular);
     const float w_cam = (epdf_w * cos_to / (dpdf_w * cos_at)) * nrr * (ft + bt1 * w.d_shared + (rpdf / w.rev_inv) * w.d_bpt);
     const float mis = 1.f / (w_light + 1.f + w_cam);
+    if (g_mis && l.type == L_AREA)
+        mis_record(c, MT_NEE, mis, hit + dir * dist, id, nullptr, 0, g_mis->cam.data(), int(g_mis->cam.size()), false,
+                   nrr, nrf);
     contrib = ((cos_to / (pick * dpdf_w)) * (rad * natt * f)) * mis;
     if (black_or_neg(contrib)) contrib = v3(0.f);
     return OK;
@@ -991,6 +1159,13 @@ static int connect_vertices(const Ctx& c, V& contrib, const LightVertex& lv, con
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = l_pdf_a * rsr * (ft + bt1 * wc.d_shared + (crp / wc.rev_inv) * wc.d_bpt);
     const float mis = 1.f / (w_light + 1.f + w_cam);
+    if (g_mis) {
+        const LightVertex* first = c.verts.data() + c.path_begin[lv.path];
+        const int nl = int(lv.path_length);
+        if (first + (nl - 1) == &lv)
+            mis_record(c, MT_BPT, mis, c.light_origin[lv.path], c.light_pick[lv.path], first, nl, g_mis->cam.data(),
+                       int(g_mis->cam.size()), false, rsr, rsf);
+    }
     contrib = (g * fc * fl * att) * mis;
     if (black_or_neg(contrib)) contrib = v3(0.f);
     return OK;
@@ -1021,6 +1196,13 @@ static void eval_pp(const Ctx& c, V& acc, const SubpathState& s, const BSDF& bc,
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
     const float mis = 1.f / (w_light + w_local + w_cam);
+    if (g_mis) {
+        const LightVertex* first = c.verts.data() + c.path_begin[lv.path];
+        const int nl = int(lv.path_length);
+        if (first + (nl - 1) == &lv)
+            mis_record(c, medium ? MT_PP3D : MT_SURF, mis, c.light_origin[lv.path], c.light_pick[lv.path], first, nl,
+                       g_mis->cam.data(), int(g_mis->cam.size()), true, 1.f, 1.f);
+    }
     acc += mis * f * lv.throughput;
 }
 
@@ -1319,6 +1501,11 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
     SubpathState s;
     Stream jrng(kDomainCamera, pix, p.iter_key, 0, P_JITTER);
     int ret = generate_camera_sample(c, pix, s, sx, sy, jrng);
+    MisLogger logger;
+    if (c.mis_on && c.mis_slot[pix] >= 0) {
+        logger.out = &(*c.mis_cam)[size_t(c.mis_slot[pix])];
+        g_mis = &logger;
+    }
     V color = v3(0.f);
     bool origin_in_medium = false;
     bool only_spec_surf = (p.techniques & (UPBP_PREVIOUS | UPBP_COMPATIBLE)) != 0;
@@ -1394,6 +1581,7 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
             color += s.throughput * light_radiance_mis(c, sc.lights[size_t(is.light)], s, hitp, rsf, rsr);
             break;
         }
+        if (g_mis) g_mis->cam.push_back({hitp, b, rsf, rsr});
         if (s.path_length >= p.max_len) break;
         if (p.do_bpt && !b.delta() && s.path_length + 1 >= p.min_len && !sc.lights.empty() &&
             (b.in_medium() || !only_spec_surf)) {
@@ -1473,6 +1661,7 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
             }
         }
     }
+    g_mis = nullptr;
     if (ret != OK) const_cast<Ctx&>(c).errors++;
     return color;
 }
@@ -1605,6 +1794,10 @@ static void render_iteration(Ctx& c, const upbp_config& cfg, int iteration, int 
     c.med_idx.clear();
     c.path_begin.assign(p.n_light, 0);
     c.path_end.assign(p.n_light, 0);
+    if (c.mis_on) {
+        c.light_origin.assign(p.n_light, v3(0.f));
+        c.light_pick.assign(p.n_light, 0);
+    }
     if (p.trace_light) {
         const uint32_t nl = std::min(p.n_light, n_light_limit);
         std::vector<LightOut> outs(static_cast<size_t>(nl));
@@ -1711,6 +1904,72 @@ extern "C" int oracle_render(const upbp_scene* scene, const upbp_config* cfg, in
     }
     for (size_t k = 0; k < acc.size(); k++) out_rgb[k] = acc[k] / double(n_iterations);
     if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return UPBP_OK;
+}
+
+// Renders one iteration with the MIS log on for the given camera paths (pixels) and the
+// first n_lt_paths light paths; for every logged contribution writes (technique, s, k,
+// constant-time weight, FP64 balance-heuristic weight, the same with the reference's
+// next-event weight convention) to out (6 doubles per record).
+extern "C" int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg, int iteration, const uint32_t* pixels,
+                                uint32_t n_pixels, uint32_t n_lt_paths, int n_threads, double* out, uint32_t max_records,
+                                uint32_t* n_records)
+{
+    if (!scene || !cfg || (n_pixels && !pixels) || !n_records) return UPBP_E_ARG;
+    Scene sc;
+    prepare_scene(scene, cfg->res_x, cfg->res_y, sc);
+    Ctx c;
+    c.sc = &sc;
+    {
+        std::vector<Box> rb, ib;
+        for (auto& t : sc.real) rb.push_back(tri_box(t));
+        for (auto& t : sc.imag) ib.push_back(tri_box(t));
+        c.bvh_real.build(rb);
+        c.bvh_imag.build(ib);
+    }
+    const size_t npx = size_t(cfg->res_x) * cfg->res_y;
+    std::vector<std::vector<MisRec>> cam_recs(n_pixels), lt_recs(n_lt_paths);
+    c.mis_on = true;
+    c.mis_slot.assign(npx, -1);
+    for (uint32_t i = 0; i < n_pixels; i++)
+        if (pixels[i] < npx) c.mis_slot[pixels[i]] = int32_t(i);
+    c.mis_cam = &cam_recs;
+    c.mis_lt = &lt_recs;
+    c.mis_lt_paths = n_lt_paths;
+    const int threads = n_threads > 0 ? n_threads : int(std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<float> fb(3 * npx, 0.f);
+    render_iteration(c, *cfg, iteration, threads, fb, UINT32_MAX, UINT32_MAX, nullptr);
+    const Params& p = c.p;
+    MisParams mp;
+    mp.n_bpt = p.n_bpt;
+    mp.n_light = double(p.n_light);
+    mp.mf_surf = p.mf_surf;
+    mp.mf_pp3d = p.mf_pp3d;
+    mp.mf_pb2d = p.mf_pb2d;
+    mp.mf_bp2d = p.mf_bp2d;
+    mp.mf_bb1d = p.mf_bb1d;
+    mp.long_photon_beams = p.pbt == UPBP_BEAM_LONG;
+    mp.long_query_beams = p.qbt == UPBP_BEAM_LONG;
+    mp.connect_from_surf = p.connect_from_surf;
+    mp.reference_nee_weight = false;
+    MisParams mq = mp;
+    mq.reference_nee_weight = true;
+    uint32_t n = 0;
+    for (auto* set : {&cam_recs, &lt_recs})
+        for (auto& v : *set)
+            for (const MisRec& r : v) {
+                if (out && n < max_records) {
+                    double* o = out + 6 * size_t(n);
+                    o[0] = r.tech;
+                    o[1] = r.s;
+                    o[2] = r.k;
+                    o[3] = r.weight;
+                    o[4] = mis_explicit_weight(r, mp);
+                    o[5] = mis_explicit_weight(r, mq);
+                }
+                n++;
+            }
+    *n_records = n;
     return UPBP_OK;
 }
 
