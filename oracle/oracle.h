@@ -13,7 +13,8 @@
  * Parity pins (see tests/): range-query predicates against hand cases and
  * brute force; BSDF / phase / light sampling against closed forms; camera
  * mapping against its inverse; the renderer against an analytic direct-view
- * case and against the FP64 MIS oracle's sum-to-one invariant.
+ * case; every constant-time MIS weight against the FP64 full-path balance
+ * heuristic of oracle_mis.cpp (oracle_mis_check).
  */
 #ifndef UPBP_ORACLE_H
 #define UPBP_ORACLE_H
@@ -64,6 +65,14 @@ void oracle_hg_sample(float mean_cosine, float u0, float u1, float* cos_theta, f
 float oracle_fresnel(float cos_inc, float ior);
 /* Philox4x32-10 block (Salmon et al. SC'11) used by both oracle and product streams. */
 void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Renders one iteration with the MIS log on for the given camera paths and the first
+ * n_lt_paths light paths; writes 6 doubles per weighted contribution: technique (see
+ * oracle_mis.h), s, k, the constant-time weight, the FP64 balance-heuristic weight and the
+ * FP64 weight under the reference's conventions. */
+int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg, int iteration, const uint32_t* pixels,
+                     uint32_t n_pixels, uint32_t n_lt_paths, int n_threads, double* out, uint32_t max_records,
+                     uint32_t* n_records);
 
 #ifdef __cplusplus
 }

@@ -51,24 +51,30 @@ double mis_explicit_weight(const MisRec& r, const MisParams& p)
         for (int i = a; i < b; i++) v *= pr[size_t(i)];
         return v;
     };
-    auto merge_factor = [&](int m, double* surf, double* pp3d) {
+    // merge factors at x_m: total, and the share of r's own technique
+    auto merge_factor = [&](int m, double* own_share) {
         if (r.type[size_t(m)] == MV_SURFACE) {
-            *surf = p.mf_surf;
-            *pp3d = 0.0;
+            *own_share = r.tech == MT_SURF ? p.mf_surf : 0.0;
             return p.mf_surf;
         }
         const double fp = p.long_photon_beams ? 1.0 / r.rsf[size_t(m - 1)] : r.ratio[size_t(m)];
         const double rp = p.long_query_beams ? 1.0 / r.rsr[size_t(m)] : r.ratio[size_t(m)];
-        *surf = 0.0;
-        *pp3d = p.mf_pp3d;
-        return p.mf_pp3d + p.mf_pb2d * rp + p.mf_bp2d * fp + p.mf_bb1d * r.sin_t[size_t(m)] * fp * rp;
+        const double pp = p.mf_pp3d, pb = p.mf_pb2d * rp, bp = p.mf_bp2d * fp,
+                     bb = p.mf_bb1d * r.sin_t[size_t(m)] * fp * rp;
+        *own_share = r.tech == MT_PP3D ? pp : r.tech == MT_PB2D ? pb : r.tech == MT_BP2D ? bp : r.tech == MT_BB1D ? bb : 0;
+        return pp + pb + bp + bb;
     };
 
+    const bool beam_merge = r.tech == MT_BP2D || r.tech == MT_BB1D;
     double sum = 0.0, own = 0.0;
     // BPT, s = 0 .. k-1 (every vertex here is non-delta and every pair can be connected)
     for (int s = 0; s <= k - 1; s++) {
         if (s == 0 && k < 2) continue;
-        const double v = p.n_bpt * prod_f(0, s) * prod_r(s, k + 1);
+        double v = p.n_bpt * prod_f(0, s) * prod_r(s, k + 1);
+        if (p.reference_beam_distance && beam_merge && r.ref_light_dist > 0 && s == r.s) { // 1/P>_m as the reference has it
+            const double d = r.dist[size_t(s - 1)];
+            v *= (r.ref_light_dist * r.ref_light_dist) / (d * d);
+        }
         sum += v;
         if ((r.tech == MT_BPT || r.tech == MT_NEE || r.tech == MT_LIGHT_HIT) && r.s == s) own = v;
     }
@@ -80,12 +86,11 @@ double mis_explicit_weight(const MisRec& r, const MisParams& p)
     }
     // merges
     for (int m = 1; m <= k - 1; m++) {
-        double fs, fpp;
-        const double f = merge_factor(m, &fs, &fpp);
+        double share;
+        const double f = merge_factor(m, &share);
         const double base = prod_f(0, m + 1) * prod_r(m, k + 1);
         sum += base * f;
-        if (r.s == m && r.tech == MT_SURF) own = base * fs;
-        if (r.s == m && r.tech == MT_PP3D) own = base * fpp;
+        if (r.s == m && r.tech >= MT_SURF) own = base * share;
     }
     return sum > 0.0 ? own / sum : 0.0;
 }

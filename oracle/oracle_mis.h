@@ -24,7 +24,8 @@
 
 namespace orc {
 
-enum MisTech : int { MT_LIGHT_HIT = 0, MT_NEE = 1, MT_BPT = 2, MT_LT = 3, MT_SURF = 4, MT_PP3D = 5 };
+enum MisTech : int { MT_LIGHT_HIT = 0, MT_NEE = 1, MT_BPT = 2, MT_LT = 3, MT_SURF = 4, MT_PP3D = 5, MT_PB2D = 6,
+                     MT_BP2D = 7, MT_BB1D = 8 };
 enum MisVType : int { MV_LIGHT = 0, MV_SURFACE = 1, MV_MEDIUM = 2, MV_CAMERA = 3 };
 
 struct MisRec {
@@ -44,6 +45,10 @@ struct MisRec {
     std::vector<double> dist, rsf, rsr; // length, distance pdfs toward x_{j+1} (light) and toward x_j (camera)
     double dpdf = 0, eps = 0, cos_light = 0; // light: position pdf (area, pick included), emission pdf (solid angle)
     double cam_pdf = 0;                      // camera: solid-angle pdf of the direction to x_{k-1}
+    // beam merges on the light side: the length the reference squares into the light-side
+    // d_shared at the merge point (the distance along the beam from its segment start, or 1
+    // when it skips the factor); 0 when the true edge length is used
+    double ref_light_dist = 0;
 };
 
 struct MisParams {
@@ -53,6 +58,10 @@ struct MisParams {
     // camera-direction distance pdf of the shadow edge (gpuupbp/kernel/functs_camera.cuh:290,
     // w_light = bsdf pdf / (pick * direct pdf)); the light-hit estimator (:146-162) includes it.
     bool reference_nee_weight;
+    // Model the reference's light-side beam distance in B-P2D / B-B1D (measured from the
+    // beam's medium-segment start, functs_bp2d.cuh:138, and skipped for beams leaving a
+    // finite light, functs_bb1d.cuh:191) instead of the light vertex distance.
+    bool reference_beam_distance;
 };
 
 // Balance-heuristic weight of r's own technique among all techniques that sample its path.

@@ -121,7 +121,8 @@ def philox(ctr, key):
     return o
 
 
-MIS_TECH = {0: "light_hit", 1: "nee", 2: "bpt", 3: "light_tracing", 4: "surf", 5: "pp3d"}
+MIS_TECH = {0: "light_hit", 1: "nee", 2: "bpt", 3: "light_tracing", 4: "surf", 5: "pp3d", 6: "pb2d", 7: "bp2d",
+            8: "bb1d"}
 
 
 def mis_check(scene_ptr, cfg, iteration, pixels, n_light_tracing_paths, threads=0, max_records=2000000):

@@ -678,13 +678,14 @@ static int sample_scattering(const Ctx& c, bool& cont, const BSDF& b, V hit, con
 // edge between the two sides is a connection whose distance pdfs are conn_rsf (toward
 // the camera side) and conn_rsr (toward the light side).
 static void mis_record(const Ctx& c, int tech, float weight, V x0, int light_id, const LightVertex* lv, int nl,
-                       const CamVert* cam, int nc, bool merge, float conn_rsf, float conn_rsr)
+                       const CamVert* cam, int nc, bool merge, float conn_rsf, float conn_rsr, float ref_light_dist = 0.f)
 {
     if (!g_mis || !g_mis->out) return;
     const Scene& sc = *c.sc;
     MisRec r;
     r.tech = tech;
     r.weight = weight;
+    r.ref_light_dist = ref_light_dist;
     const int k = nl + nc + 1 - (merge ? 1 : 0);
     if (k < 2) return;
     r.k = k;
@@ -782,6 +783,45 @@ static void mis_record(const Ctx& c, int tech, float weight, V x0, int light_id,
     const double ipd = double(sc.cam.image_plane_dist) / ca;
     r.cam_pdf = ipd * ipd / ca;
     g_mis->out->push_back(std::move(r));
+}
+
+// medium "BSDF" (phase function) at a point of a ray with direction dir
+static BSDF medium_bsdf(const Ctx& c, V dir, int med, bool from_light)
+{
+    Isect is;
+    is.dist = 1.f;
+    is.mat = -1;
+    is.med = med;
+    BSDF b;
+    bsdf_setup(b, dir, is, *c.sc, from_light, -1.f);
+    return b;
+}
+
+// light vertices x_1 .. x_{n} of light path `path` (false unless every one was stored)
+static bool light_prefix(const Ctx& c, uint32_t path, uint32_t n, std::vector<LightVertex>& out)
+{
+    out.clear();
+    const uint32_t b = c.path_begin[path], e = c.path_end[path];
+    if (e - b < n) return false;
+    for (uint32_t j = 0; j < n; j++) {
+        if (c.verts[b + j].path_length != j + 1) return false;
+        out.push_back(c.verts[b + j]);
+    }
+    return true;
+}
+
+// a point on a photon beam as the light vertex of a beam merge
+static LightVertex beam_point(const Ctx& c, const LightBeam& lb, float t, float rsf, float rsr)
+{
+    LightVertex v;
+    v.hit = lb.origin + lb.dir * t;
+    v.path_length = lb.path_length;
+    v.path = lb.path_idx;
+    v.bsdf = medium_bsdf(c, lb.dir, lb.med, true);
+    v.w.for_inv = 1.f / rsf;
+    v.w.rev_inv = 1.f / rsr;
+    v.in_medium = true;
+    return v;
 }
 
 // ---- light subpath ----------------------------------------------------------
@@ -1271,6 +1311,15 @@ static void eval_pb2d(const Ctx& c, V& acc, const SegQuery& q, const LightVertex
     const float w_local = 1.f + val * (p.mf_bb1d * sin_theta * fpf * rpf + p.mf_bp2d * fpf + p.mf_pp3d);
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
+    if (g_mis) {
+        std::vector<LightVertex> lp;
+        if (light_prefix(c, lv.path, lv.path_length, lp)) {
+            std::vector<CamVert> cam = g_mis->cam;
+            cam.push_back({q.ro + q.rd * idist, medium_bsdf(c, q.rd, q.med_id, false), rsf, rsr});
+            mis_record(c, MT_PB2D, 1.f / (w_light + w_local + w_cam), c.light_origin[lv.path], c.light_pick[lv.path],
+                       lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f, 1.f);
+        }
+    }
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
 }
 
@@ -1352,6 +1401,18 @@ static void eval_bb1d(const Ctx& c, V& acc, const SegQuery& q, const LightBeam& 
     const float w_local = 1.f + val * (p.mf_bp2d * fpf + p.mf_pb2d * rpf + p.mf_pp3d);
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
+    if (g_mis) {
+        std::vector<LightVertex> lp;
+        if (light_prefix(c, lb.path_idx, lb.path_length - 1, lp)) {
+            lp.push_back(beam_point(c, lb, tp, rsfp, rsrp));
+            std::vector<CamVert> cam = g_mis->cam;
+            cam.push_back({q.ro + q.rd * tc, medium_bsdf(c, q.rd, q.med_id, false), rsfc, rsrc});
+            const float ref_d = (lb.path_length == 1 && !lb.infinite) ? 1.f : tp;
+            mis_record(c, MT_BB1D, 1.f / (w_light + w_local + w_cam), c.light_origin[lb.path_idx],
+                       c.light_pick[lb.path_idx], lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f,
+                       1.f, ref_d);
+        }
+    }
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
 }
 
@@ -1409,6 +1470,15 @@ static void eval_bp2d(const Ctx& c, V& acc, const SubpathState& s, const BSDF& b
     const float w_local = 1.f + val * (p.mf_bb1d * sin_theta * fpf * rpf + p.mf_pb2d * rpf + p.mf_pp3d);
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
+    if (g_mis) {
+        std::vector<LightVertex> lp;
+        if (light_prefix(c, lb.path_idx, lb.path_length - 1, lp)) {
+            lp.push_back(beam_point(c, lb, idist, rsf, rsr));
+            mis_record(c, MT_BP2D, 1.f / (w_light + w_local + w_cam), c.light_origin[lb.path_idx],
+                       c.light_pick[lb.path_idx], lp.data(), int(lp.size()), g_mis->cam.data(), int(g_mis->cam.size()),
+                       true, 1.f, 1.f, (lb.path_length == 1 && lb.infinite) ? 1.f : idist);
+        }
+    }
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
 }
 
@@ -1952,8 +2022,10 @@ extern "C" int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg,
     mp.long_query_beams = p.qbt == UPBP_BEAM_LONG;
     mp.connect_from_surf = p.connect_from_surf;
     mp.reference_nee_weight = false;
+    mp.reference_beam_distance = false;
     MisParams mq = mp;
     mq.reference_nee_weight = true;
+    mq.reference_beam_distance = true;
     uint32_t n = 0;
     for (auto* set : {&cam_recs, &lt_recs})
         for (auto& v : *set)
