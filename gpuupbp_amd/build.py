@@ -23,7 +23,8 @@ CLI = os.path.join(HERE, "gpuupbp")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host/config.cpp", "host/scene_loader.cpp", "host/exr_writer.cpp", "host/scene_gen.cpp"]
+HOST_SOURCES = ["host/config.cpp", "host/scene_loader.cpp", "host/exr_writer.cpp", "host/image_writers.cpp",
+                "host/scene_gen.cpp"]
 HIP_SOURCES = ["device/renderer.hip", "device/lbvh.hip", "device/light_pass.hip", "device/camera_pass.hip",
                "device/merge_pass.hip", "device/test_hooks.hip"]
 

@@ -110,6 +110,10 @@ _SIGS = [
     ("upbp_scene_get", C.POINTER(Scene), [C.c_void_p]),
     ("upbp_scene_free", None, [C.c_void_p]),
     ("upbp_save_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("upbp_save_bmp", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float]),
+    ("upbp_save_hdr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("upbp_save_image", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("upbp_memory_info", C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("upbp_generate_scene", C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p]),
     ("upbp_global_error", C.c_char_p, []),
     ("upbp_create", C.c_int, [C.POINTER(Scene), C.POINTER(Config), C.POINTER(C.c_void_p)]),

@@ -138,9 +138,12 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     PairQueue& Q = pq[threadIdx.x >> 6];
     Steal& S = stl[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
-    // B-B1D walks query pieces; P-B2D (few candidates per query) walks whole segments
+    // B-B1D walks query pieces from its XCD's region first (then helps the others);
+    // P-B2D (few candidates per query) walks whole segments
     const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : min(b.qcount[128 + bounce], b.seg_cap);
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
+    uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
+    uint32_t src = blockIdx.x % kXcds, tried = 0; // work range being consumed (same in every lane)
     const WBvh& bvh = KIND == 0 ? b.maps.wmed : b.maps.wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
 
@@ -159,13 +162,28 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
             const uint64_t m = __ballot(need);
             if (m) {
                 const int leader = __ffsll((unsigned long long)m) - 1;
-                uint32_t base = 0;
-                if (int(lane) == leader) base = atomicAdd(head, uint32_t(__popcll(m)));
-                base = __shfl(base, leader);
-                if (need) {
-                    const uint32_t wi = base + uint32_t(__popcll(m & lanes_below()));
-                    if (wi < total) {
-                        const uint2 pc = KIND == 1 ? b.pieces[wi] : make_uint2(wi, 1u << 16);
+                uint32_t first = 0, lim = 0;
+                bool got = false;
+                while (!got && !exhausted) {
+                    uint32_t base = 0;
+                    uint32_t* h = KIND == 1 ? &xw[208 + src] : head;
+                    const uint32_t rb0 = KIND == 1 ? xw[192 + src] : 0u, re0 = KIND == 1 ? xw[200 + src] : total;
+                    if (int(lane) == leader) base = atomicAdd(h, uint32_t(__popcll(m)));
+                    base = __shfl(base, leader);
+                    if (rb0 + base < re0) {
+                        got = true;
+                        first = rb0 + base;
+                        lim = re0;
+                    } else if (KIND == 1 && ++tried < uint32_t(kXcds)) {
+                        src = (src + 1) % kXcds;
+                    } else {
+                        exhausted = true;
+                    }
+                }
+                if (got && need) {
+                    const uint32_t wi = first + uint32_t(__popcll(m & lanes_below()));
+                    if (wi < lim) {
+                        const uint2 pc = KIND == 1 ? b.pieces_x[wi] : make_uint2(wi, 1u << 16);
                         const SegRec* r = b.segq + pc.x;
                         const float4 a = r->a, bb = r->b;
                         piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
@@ -175,7 +193,6 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                         cur = int(pc.x);
                     }
                 }
-                if (base + uint32_t(__popcll(m)) >= total) exhausted = true;
             }
         } else { // no pieces left: idle lanes take the oldest pending subtree of a busy lane
             const bool idle = cur < 0, donor = cur >= 0 && sp > sb;
@@ -343,6 +360,88 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     }
 }
 
+// ---- XCD-aware split of the B-B1D query pieces ------------------------------------
+// Each piece is put in one of 4x4x4 cells of the beam map's bounds by its midpoint; the
+// cells are numbered octant-major, so the eight cells of octant x (its pieces stored
+// contiguously) form the work range of XCD x. Workgroups are dispatched to the XCDs
+// round-robin, so block i works mostly on octant i % 8 and that XCD's L2 holds the
+// subtrees of one region instead of the whole tree.
+
+__device__ __forceinline__ uint32_t piece_cell(const Bufs& b, const WNode& root, uint2 pc)
+{
+    const SegRec* r = b.segq + pc.x;
+    const float4 a = r->a, bb = r->b;
+    float lo, hi, ta, tb;
+    piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
+    const float t = 0.5f * (ta + tb);
+    const float m[3] = {a.x + bb.x * t, a.y + bb.y * t, a.z + bb.z * t};
+    const float o[3] = {root.ox, root.oy, root.oz};
+    int c[3];
+    for (int k = 0; k < 3; k++) {
+        const float ext = 255.f * wnode_step(root.meta, k);
+        const float u = ext > 0.f ? (m[k] - o[k]) / ext : 0.f;
+        const int v = int(u * 4.f);
+        c[k] = v < 0 ? 0 : (v > 3 ? 3 : v);
+    }
+    const uint32_t octant = uint32_t((c[0] >> 1) | ((c[1] >> 1) << 1) | ((c[2] >> 1) << 2));
+    const uint32_t sub = uint32_t((c[0] & 1) | ((c[1] & 1) << 1) | ((c[2] & 1) << 2));
+    return octant * 8u + sub;
+}
+
+__global__ void __launch_bounds__(256) k_piece_cells(Bufs b, int bounce)
+{
+    __shared__ uint32_t hist[kCells];
+    if (threadIdx.x < kCells) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = min(b.qcount[320 + bounce], b.piece_cap);
+    uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
+    const WNode root = b.maps.wbeam.nodes[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = piece_cell(b, root, b.pieces[i]);
+        b.piece_cell[i] = uint8_t(c);
+        atomicAdd(&hist[c], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kCells && hist[threadIdx.x]) atomicAdd(&xw[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void k_piece_offsets(Bufs b, int bounce)
+{
+    uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
+    if (threadIdx.x != 0) return;
+    uint32_t run = 0;
+    for (int c = 0; c < kCells; c++) {
+        if (c % 8 == 0) xw[192 + c / 8] = run;
+        xw[64 + c] = run;
+        xw[128 + c] = run;
+        run += xw[c];
+        if (c % 8 == 7) xw[200 + c / 8] = run;
+    }
+    for (int x = 0; x < kXcds; x++) xw[208 + x] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_piece_scatter(Bufs b, int bounce)
+{
+    __shared__ uint32_t cnt[kCells], base[kCells];
+    const uint32_t n = min(b.qcount[320 + bounce], b.piece_cap);
+    uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t c0 = blockIdx.x * chunk, c1 = min(n, c0 + chunk);
+    if (threadIdx.x < kCells) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) atomicAdd(&cnt[b.piece_cell[i]], 1u);
+    __syncthreads();
+    if (threadIdx.x < kCells) {
+        base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&xw[128 + threadIdx.x], cnt[threadIdx.x]) : 0u;
+        cnt[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+        const uint32_t c = b.piece_cell[i];
+        b.pieces_x[base[c] + atomicAdd(&cnt[c], 1u)] = b.pieces[i];
+    }
+}
+
 } // namespace
 
 // blocks of the persistent merge grid: CUs x resident blocks per CU (at most the occupancy
@@ -367,8 +466,12 @@ void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int 
     const int blocks = merge_grid_blocks(p.merge_bpc);
     if (kind == 0 && p.build_pb2d && b.maps.med.n > 0)
         hipLaunchKernelGGL(k_seg_merge<0>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
-    if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0)
+    if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0) {
+        hipLaunchKernelGGL(k_piece_cells, dim3(1024), dim3(256), 0, s, b, bounce);
+        hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
+        hipLaunchKernelGGL(k_piece_scatter, dim3(1024), dim3(256), 0, s, b, bounce);
         hipLaunchKernelGGL(k_seg_merge<1>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+    }
 }
 
 } // namespace upbp

@@ -386,6 +386,9 @@ struct Bufs {
     float* mcolor;  // per camera path: P-B2D / B-B1D contributions of the merge kernels
     SegRec* segq;   // segment queries of the current camera bounce
     uint2* pieces;  // their pieces: (query, k | n << 16)
+    uint2* pieces_x; // the same, grouped by XCD region for the B-B1D merge kernel
+    uint8_t* piece_cell; // spatial cell of each piece
+    uint32_t* xwork; // per bounce: kXWork words of cell counts, offsets and per-XCD work ranges
     uint32_t seg_cap, piece_cap;
     double* accum;
     unsigned long long* stats; // see STAT_*
@@ -396,6 +399,10 @@ enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_P
                   STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,
                   STAT_SEG_PIECES, STAT_SEG_TRIPS, STAT_CAND_BATCHES, STAT_PAIR_BATCHES, STAT_COUNT };
 constexpr int kQcountWords = 512;
+// XCD-aware work split of the B-B1D merge: 4x4x4 spatial cells, 8 XCDs owning one octant
+// (2x2x2 cells) each. Per bounce: cell counts [0,64), cell offsets [64,128), fill cursors
+// [128,192), XCD range begin [192,200), end [200,208), heads [208,216).
+constexpr int kXcds = 8, kCells = 64, kXWork = 224;
 
 // wave-aggregated counter increment
 __device__ __forceinline__ void stat_add(unsigned long long* stats, int k, uint32_t v)

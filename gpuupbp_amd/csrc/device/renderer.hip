@@ -166,7 +166,7 @@ struct upbp_renderer {
     Bufs bufs;
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
-    DevBuf mcolor, segq, pieces;
+    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork;
     BvhStore bvh_s, bvh_m, bvh_b;
     WBvhStore wbvh_m, wbvh_b;
     float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
@@ -506,7 +506,10 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     const bool seg_merges = (c.algorithm_flags & (T_PB2D | T_BB1D)) != 0;
     const size_t seg_cap = seg_merges ? NP * kMaxSegments : 1;
     const size_t piece_cap = seg_merges ? NP * (kPiecesPerRay + kMaxSegments + 1) : 1;
-    if (ok) ok = !r->segq.ensure(seg_cap * sizeof(SegRec)) && !r->pieces.ensure(piece_cap * sizeof(uint2));
+    if (ok)
+        ok = !r->segq.ensure(seg_cap * sizeof(SegRec)) && !r->pieces.ensure(piece_cap * sizeof(uint2)) &&
+             !r->pieces_x.ensure(piece_cap * sizeof(uint2)) && !r->piece_cell.ensure(piece_cap) &&
+             !r->xwork.ensure(size_t(kMaxBounces) * kXWork * 4);
     if (!ok) {
         upbp_destroy(r);
         return fail(nullptr, "device allocation failed", UPBP_E_HIP);
@@ -531,6 +534,9 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     b.segq = r->segq.as<SegRec>();
     b.seg_cap = uint32_t(seg_cap);
     b.pieces = r->pieces.as<uint2>();
+    b.pieces_x = r->pieces_x.as<uint2>();
+    b.piece_cell = r->piece_cell.as<uint8_t>();
+    b.xwork = r->xwork.as<uint32_t>();
     b.piece_cap = uint32_t(piece_cap);
     b.accum = r->accum.as<double>();
     b.stats = r->stats.as<unsigned long long>();
@@ -622,6 +628,8 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     hipEvent_t* ev = r->ev;
     HIPCHK(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_fill, dim3(nblk(kQcountWords)), dim3(256), 0, s, b.qcount, 0u, size_t(kQcountWords));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(kMaxBounces) * kXWork)), dim3(256), 0, s, b.xwork, 0u,
+                       size_t(kMaxBounces) * kXWork);
     hipLaunchKernelGGL(k_fill, dim3(nblk(3 * size_t(r->n_pixels))), dim3(256), 0, s, reinterpret_cast<uint32_t*>(b.fb32),
                        0u, 3 * size_t(r->n_pixels));
     hipLaunchKernelGGL(k_fill, dim3(nblk(3 * size_t(r->n_pixels))), dim3(256), 0, s,
@@ -770,6 +778,16 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
     return UPBP_OK;
 }
 
+extern "C" int upbp_memory_info(uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return UPBP_E_ARG;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return UPBP_E_HIP;
+    *free_bytes = f;
+    *total_bytes = t;
+    return UPBP_OK;
+}
+
 extern "C" int upbp_sync(upbp_renderer* r)
 {
     if (!r) return fail(r, "null renderer", UPBP_E_ARG);
@@ -791,7 +809,8 @@ extern "C" void upbp_destroy(upbp_renderer* r)
     for (DevBuf* d : {&r->lstate, &r->lstack, &r->cstate, &r->cstack, &r->q0, &r->q1, &r->qcount, &r->verts, &r->nvert,
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
-                      &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces})
+                      &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
+                      &r->xwork})
         d->release();
     lbvh_free(r->bvh_s);
     lbvh_free(r->bvh_m);

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -54,13 +55,44 @@ int save_mean(upbp_renderer* r, const upbp_config& c, int iters, const std::stri
     if (e) return e;
     std::vector<float> rgb(n);
     for (size_t i = 0; i < n; i++) rgb[i] = float(acc[i]);
-    const std::string ext = path.size() >= 3 ? path.substr(path.size() - 3) : "";
-    if (ext != "exr") {
-        fprintf(stderr, "Error: output extension %s is not supported by this build (exr only)\n", ext.c_str());
-        return UPBP_E_ARG;
-    }
-    return upbp_save_exr(path.c_str(), rgb.data(), uint32_t(c.res_x), uint32_t(c.res_y));
+    e = upbp_save_image(path.c_str(), rgb.data(), uint32_t(c.res_x), uint32_t(c.res_y));
+    if (e == UPBP_E_ARG) fprintf(stderr, "Error: used unknown extension of %s\n", path.c_str());
+    return e;
 }
+
+// -log: one CSV line per iteration into <timestamp>_log.txt with the columns of the
+// reference's Logger (gpuupbp/misc/logger.cpp:31-86); times in seconds (HIP events, wall
+// clock), memory in MiB, the two utilisation columns 0 as in a non-debug reference build.
+struct CsvLog {
+    FILE* f = nullptr;
+    explicit CsvLog(bool on)
+    {
+        if (!on) return;
+        char name[64];
+        const time_t now = time(nullptr);
+        strftime(name, sizeof(name), "%Y-%m-%d-%H-%M-%S_log.txt", localtime(&now));
+        f = fopen(name, "w");
+        if (f)
+            fprintf(f, "%40s,%6s,%12s,%12s,%12s,%12s,%12s,%12s,%12s,%12s,%12s,%12s\n", "scene", "iter", "lighttime",
+                    "buildtime", "cameratime", "res_x", "res_y", "mem_used", "mem_free", "mem_total", "light_util",
+                    "camera_util");
+    }
+    ~CsvLog()
+    {
+        if (f) fclose(f);
+    }
+    void line(const upbp_config& c, int iter, const upbp_iter_stats& st)
+    {
+        if (!f) return;
+        uint64_t fr = 0, tot = 0;
+        upbp_memory_info(&fr, &tot);
+        const double mib = 1.0 / (1024.0 * 1024.0);
+        fprintf(f, "%40s,%6d,%12g,%12g,%12g,%12u,%12u,%12g,%12g,%12g,%12g,%12g\n", c.scene_obj_file, iter,
+                st.ms_light * 1e-3, st.ms_build * 1e-3, st.ms_camera * 1e-3, c.res_x, c.res_y, double(tot - fr) * mib,
+                double(fr) * mib, double(tot) * mib, 0.0, 0.0);
+        fflush(f);
+    }
+};
 
 } // namespace
 
@@ -97,13 +129,17 @@ int main(int argc, const char* argv[])
         }
     };
 
+    CsvLog log(cfg.do_log != 0);
+    upbp_iter_stats st;
+    upbp_iter_stats* stp = cfg.do_log ? &st : nullptr;
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     int iter = 0;
     if (cfg.max_time > 0.f) {
         const auto end = t0 + std::chrono::duration<double>(cfg.max_time);
         while (clk::now() < end) {
-            if ((e = upbp_run_iteration(r, iter, nullptr))) break;
+            if ((e = upbp_run_iteration(r, iter, stp))) break;
+            if (stp) log.line(cfg, iter, st);
             iter++;
             continuous(iter);
             if (cfg.continuous_output <= 0) upbp_sync(r); // keep the time check close to the device
@@ -111,7 +147,8 @@ int main(int argc, const char* argv[])
     } else {
         int percent = -1;
         for (iter = 0; iter < cfg.iteration_count; iter++) {
-            if ((e = upbp_run_iteration(r, iter, nullptr))) break;
+            if ((e = upbp_run_iteration(r, iter, stp))) break;
+            if (stp) log.line(cfg, iter, st);
             const int p = int((float(iter + 1) / float(cfg.iteration_count)) * 100.f);
             if (p != percent) {
                 percent = p;

@@ -115,6 +115,15 @@ def save_exr(path, rgb):
         raise RuntimeError("upbp_save_exr failed: %d" % rc)
 
 
+def save_image(path, rgb):
+    """EXR, BMP (gamma 2.2) or HDR by the file extension, like the reference's save()."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = rgb.shape
+    rc = capi.lib().upbp_save_image(path.encode(), _p(rgb), w, h)
+    if rc != 0:
+        raise RuntimeError("upbp_save_image failed: %d" % rc)
+
+
 def render(obj_path, argv, iterations=None, want_stats=False):
     """Parse argv like the CLI, load the OBJ scene, render, return the mean image."""
     rc, cfg = capi.parse_args(["-s", "-1", obj_path] + list(argv))

@@ -179,6 +179,17 @@ void upbp_scene_free(upbp_scene_handle* h);
  * uncompressed, HALF, channels B,G,R. rgb is W*H*3 floats, row-major from the top row. */
 int upbp_save_exr(const char* path, const float* rgb, uint32_t width, uint32_t height);
 
+/* Replaces save_bmp (gpuupbp/host/framebuffer.cpp:129-200): 24-bit uncompressed BMP, rows
+ * bottom-up, B,G,R, gamma-corrected and clamped (rows padded to 4 bytes). */
+int upbp_save_bmp(const char* path, const float* rgb, uint32_t width, uint32_t height, float gamma);
+
+/* Replaces save_hdr (framebuffer.cpp:202-233): Radiance RGBE, flat scanlines. */
+int upbp_save_hdr(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
+/* Replaces save (framebuffer.cpp:114-127): picks EXR, BMP (gamma 2.2) or HDR by the
+ * extension; UPBP_E_ARG for any other. */
+int upbp_save_image(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
 /* Replaces Renderer::init (gpuupbp/renderer/renderer_init.cpp:675-929). Copies the scene to the
  * GPU, builds the triangle LBVHs and sizes the SoA vertex/beam caches. The caller keeps
  * ownership of scene and cfg. */
@@ -206,6 +217,10 @@ int upbp_generate_scene(const char* kind, uint32_t seed, const char* obj_path);
 
 /* Blocks until all work queued by r has finished. */
 int upbp_sync(upbp_renderer* r);
+
+/* Device memory of the current GPU in bytes (replaces optix::report_memory,
+ * gpuupbp/misc/optix_helper.cpp, used by the -log CSV). */
+int upbp_memory_info(uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* Launch and data-structure knobs of one renderer (no reference counterpart: the OptiX
  * build had none). They change speed only, never which photons a query accepts. */

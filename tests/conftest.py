@@ -16,7 +16,7 @@ def pytest_configure(config):
 def scene_dir(tmp_path_factory):
     from gpuupbp_amd import capi
     d = tmp_path_factory.mktemp("scenes")
-    for kind in ("box", "furnace"):
+    for kind in ("box", "furnace", "boxfog"):
         os.makedirs(str(d / kind), exist_ok=True)
         capi.generate_scene(kind, 0, str(d / kind / (kind + ".obj")))
     return d
@@ -25,6 +25,11 @@ def scene_dir(tmp_path_factory):
 @pytest.fixture(scope="session")
 def box_obj(scene_dir):
     return str(scene_dir / "box" / "box.obj")
+
+
+@pytest.fixture(scope="session")
+def boxfog_obj(scene_dir):
+    return str(scene_dir / "boxfog" / "boxfog.obj")
 
 
 def box_args(res="64x64", alg="upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d", extra=()):

@@ -170,6 +170,18 @@ def test_render_path_parity_one_iteration(box_obj):
     assert abs(g.mean() - o.mean()) / o.mean() < 0.01
 
 
+@pytest.mark.parametrize("beams", [(), ("-qbt", "S", "-pbt", "S")])
+def test_render_path_parity_dense_medium(boxfog_obj, beams):
+    # denser anisotropic fog and a glossy wall: every merge estimator contributes; the GPU's
+    # MIS weights must follow the oracle's, which the FP64 MIS oracle pins
+    g, o, stats, _ = _render_pair(boxfog_obj, box_args("48x48", extra=("-r_initial", "0.03") + beams), iters=1)
+    assert stats[0].path_errors == 0 and stats[0].beam_drops == 0
+    assert stats[0].pb2d_accepts > 0 and stats[0].bb1d_accepts > 0
+    rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
+    assert np.mean(rel < 1e-3) > 0.95, np.mean(rel < 1e-3)
+    assert abs(g.mean() - o.mean()) / o.mean() < 0.01
+
+
 def test_render_converged_relmse(box_obj):
     g, o, stats, _ = _render_pair(box_obj, box_args("48x48"), iters=8)
     relmse = np.mean((g - o) ** 2 / (o ** 2 + 1e-2))

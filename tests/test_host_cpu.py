@@ -190,6 +190,44 @@ def test_exr_roundtrip(tmp_path):
     assert np.array_equal(back, ref)
 
 
+def test_bmp_writer(tmp_path):
+    # 24-bit BI_RGB, bottom-up rows padded to 4 bytes, B,G,R, gamma 2.2, clamped
+    # (framebuffer.cpp:129-200); width 5 needs one padding byte per row
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 1.5, (3, 5, 3)).astype(np.float32)
+    p = str(tmp_path / "o.bmp")
+    gpu.save_image(p, img)
+    b = open(p, "rb").read()
+    assert b[:2] == b"BM"
+    size, _, off, hs, w, h, planes, bpp, comp, data = struct.unpack("<IIIIiiHHII", b[2:38])
+    assert (size, off, hs, w, h, planes, bpp, comp, data) == (54 + 16 * 3, 54, 40, 5, 3, 1, 24, 0, 16 * 3)
+    ref = np.clip(np.power(img, 1 / 2.2) * 255.0, 0, 255).astype(np.uint8)
+    for y in range(3):
+        row = np.frombuffer(b[54 + 16 * y:54 + 16 * y + 15], np.uint8).reshape(5, 3)
+        assert np.array_equal(row[:, ::-1], ref[2 - y])  # bottom-up, B,G,R
+        assert b[54 + 16 * y + 15] == 0
+
+
+def test_hdr_writer(tmp_path):
+    # Radiance RGBE: shared exponent, mantissas in [128, 256) for the largest channel
+    rng = np.random.default_rng(4)
+    img = (rng.uniform(0, 8, (4, 6, 3)) * (rng.uniform(0, 1, (4, 6, 3)) > 0.1)).astype(np.float32)
+    img[0, 0] = 0.0
+    p = str(tmp_path / "o.hdr")
+    gpu.save_image(p, img)
+    b = open(p, "rb").read()
+    head = b"#?RADIANCE\n"
+    assert b.startswith(head) and b"FORMAT=32-bit_rle_rgbe\n\n-Y 4 +X 6\n" in b
+    px = np.frombuffer(b[b.index(b"+X 6\n") + 5:], np.uint8).reshape(4, 6, 4).astype(np.float64)
+    dec = px[..., :3] * np.ldexp(1.0, (px[..., 3:4] - 136).astype(int))
+    dec[px[..., 3] == 0] = 0.0
+    big = img.max(axis=2) > 1e-3
+    assert np.all(np.abs(dec - img)[big] <= img.max(axis=2)[big][:, None] / 128.0 + 1e-6)
+    assert np.all(px[0, 0] == 0)
+    with pytest.raises(RuntimeError):
+        gpu.save_image(str(tmp_path / "o.png"), img)
+
+
 # ---- C ABI ---------------------------------------------------------------------------------
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(ROOT, "include", "upbp_capi.h")).read()
