@@ -24,7 +24,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = "gfx950"
 
 HOST_SOURCES = ["host/config.cpp", "host/scene_loader.cpp", "host/exr_writer.cpp", "host/image_writers.cpp",
-                "host/scene_gen.cpp"]
+                "host/scene_gen.cpp", "host/scene_predef.cpp"]
 HIP_SOURCES = ["device/renderer.hip", "device/lbvh.hip", "device/light_pass.hip", "device/camera_pass.hip",
                "device/merge_pass.hip", "device/test_hooks.hip"]
 

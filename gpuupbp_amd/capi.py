@@ -22,6 +22,7 @@ UPBP_PREVIOUS = 1 << 9
 UPBP_COMPATIBLE = 1 << 10
 UPBP_SPECULAR_ONLY = 1 << 12
 UPBP_HELP = 1
+UPBP_LIGHT_AREA, UPBP_LIGHT_BACKGROUND, UPBP_LIGHT_DIRECTIONAL, UPBP_LIGHT_POINT = 0, 1, 2, 3
 
 
 class Config(C.Structure):
@@ -71,12 +72,17 @@ class Triangles(C.Structure):
                 ("mat_id", C.POINTER(C.c_int32)), ("light_id", C.POINTER(C.c_int32))]
 
 
+class Spheres(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("spheres", C.POINTER(C.c_float)), ("mat_id", C.POINTER(C.c_int32)),
+                ("light_id", C.POINTER(C.c_int32))]
+
+
 class Scene(C.Structure):
     _fields_ = [("real", Triangles), ("imag", Triangles), ("mat_count", C.c_uint32),
                 ("materials", C.POINTER(Material)), ("med_count", C.c_uint32), ("media", C.POINTER(Medium)),
                 ("light_count", C.c_uint32), ("lights", C.POINTER(Light)), ("global_medium_idx", C.c_uint32),
                 ("background_light_idx", C.c_int32), ("camera", Camera), ("scene_center", C.c_float * 3),
-                ("scene_radius", C.c_float)]
+                ("scene_radius", C.c_float), ("sph_real", Spheres), ("sph_imag", Spheres)]
 
 
 class Tuning(C.Structure):
@@ -107,6 +113,9 @@ _SIGS = [
     ("upbp_parse_commandline", C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Config)]),
     ("upbp_config_defaults", None, [C.POINTER(Config)]),
     ("upbp_scene_load_obj", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("upbp_scene_load_predefined", C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("upbp_predefined_scene_count", C.c_int32, []),
+    ("upbp_predefined_scene_name", C.c_char_p, [C.c_int32, C.c_int32]),
     ("upbp_scene_get", C.POINTER(Scene), [C.c_void_p]),
     ("upbp_scene_free", None, [C.c_void_p]),
     ("upbp_save_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
@@ -131,6 +140,7 @@ _SIGS = [
     ("upbp_tuning_defaults", None, [C.c_void_p]),
     ("upbp_set_tuning", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
 ]
 
 
@@ -163,11 +173,15 @@ def parse_args(argv):
 
 
 class SceneHandle:
-    """Owns a scene loaded by the C++ OBJ/MTL/AUX loader."""
+    """Owns a scene loaded by the C++ OBJ/MTL/AUX loader, or a predefined scene when path is an
+    int (the -s id)."""
 
     def __init__(self, path, res_x=256, res_y=256):
         h = C.c_void_p()
-        rc = lib().upbp_scene_load_obj(path.encode(), res_x, res_y, C.byref(h))
+        if isinstance(path, int):
+            rc = lib().upbp_scene_load_predefined(path, res_x, res_y, C.byref(h))
+        else:
+            rc = lib().upbp_scene_load_obj(path.encode(), res_x, res_y, C.byref(h))
         if rc != 0:
             raise RuntimeError("scene load failed (%d): %s" % (rc, lib().upbp_global_error().decode()))
         self.h = h
@@ -187,6 +201,13 @@ class SceneHandle:
             self.close()
         except Exception:
             pass
+
+
+def predefined_scenes():
+    """(short name, long name) of every predefined scene, indexed by -s id."""
+    n = lib().upbp_predefined_scene_count()
+    return [(lib().upbp_predefined_scene_name(i, 0).decode(), lib().upbp_predefined_scene_name(i, 1).decode())
+            for i in range(n)]
 
 
 def generate_scene(kind, seed, obj_path):

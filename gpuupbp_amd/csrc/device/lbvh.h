@@ -181,6 +181,53 @@ UPBP_HD bool tri_test(float4 p0, float4 e1, float4 e2, f3 o, f3 d, float tmin, f
     return true;
 }
 
+// Analytic sphere (centre s.xyz, radius s.w) against a ray: the roots of |o + t d - c| = r with
+// the near root refined by one more solve from the first estimate when the ray starts more
+// than a radius away, which removes the cancellation of b^2 - c for distant spheres
+// (gpuupbp/kernel/intersector.cu:35-95). Writes the roots in (tmin, tmax), near first, and the
+// outward normals at them; returns how many (0..2). A closest-hit caller takes the first, an
+// all-hits caller every one.
+UPBP_HD int sphere_test(float4 s, f3 ro, f3 rd, float tmin, float tmax, float* th, f3* nh)
+{
+#pragma clang fp contract(off)
+    const float radius = s.w;
+    const f3 o = ro - mk(s.x, s.y, s.z);
+    const float l = 1.f / sqrtf(dot(rd, rd));
+    const f3 d = rd * l;
+    float b = dot(o, d);
+    float c = dot(o, o) - radius * radius;
+    float disc = b * b - c;
+    if (!(disc > 0.f)) return 0;
+    float sdisc = sqrtf(disc);
+    const float root1 = -b - sdisc;
+    float root11 = 0.f;
+    const bool refine = fabsf(root1) > radius;
+    if (refine) {
+        const f3 o1 = o + d * root1;
+        b = dot(o1, d);
+        c = dot(o1, o1) - radius * radius;
+        disc = b * b - c;
+        if (disc > 0.f) {
+            sdisc = sqrtf(disc);
+            root11 = -b - sdisc;
+        }
+    }
+    int n = 0;
+    const float near = root1 + root11;
+    float t = near * l;
+    if (t > tmin && t < tmax) {
+        th[n] = t;
+        nh[n++] = (o + d * near) / radius;
+    }
+    const float root2 = (-b + sdisc) + (refine ? root1 : 0.f);
+    t = root2 * l;
+    if (t > tmin && t < tmax) {
+        th[n] = t;
+        nh[n++] = (o + d * root2) / radius;
+    }
+    return n;
+}
+
 } // namespace upbp
 
 // ---- host-side builder ---------------------------------------------------------

@@ -463,6 +463,29 @@ __device__ __forceinline__ bool trace_real(const DScene& sc, f3 o, f3 d, float t
             thi = t;
         }
     });
+    // analytic spheres (predefined scenes: a handful, tested exhaustively after the triangles)
+    int best_s = -1;
+    f3 sn{0.f, 0.f, 0.f};
+    for (uint32_t i = 0; i < sc.sph_real.n; i++) {
+        float th[2];
+        f3 nh[2];
+        if (sphere_test(sc.sph_real.sph[i], o, d, tmin, best >= 0 || best_s >= 0 ? bt : tmax, th, nh)) {
+            best_s = int(i);
+            bt = th[0];
+            sn = nh[0];
+        }
+    }
+    if (best_s >= 0) {
+        const int2 rec = sc.sph_real.rec[best_s];
+        out.dist = bt;
+        out.mat = rec.x;
+        out.med = sc.mats[rec.x].med_idx;
+        out.light = rec.y;
+        out.normal = sn;
+        out.shading = sn;
+        out.enter = dot(sn, d) < 0.f;
+        return true;
+    }
     if (best < 0) return false;
     const int2 rec = T.rec[best];
     const float4 fn = T.nrm[4 * best + 3];
@@ -489,17 +512,12 @@ __device__ __forceinline__ bool trace_imag(const DScene& sc, f3 o, f3 d, float t
     const TriSet& T = sc.imag;
     bool overflow = false;
     nh = 0;
-    stack_ok = bvh_ray_query(T.bvh, o, d, tmin, tmax, [&](uint32_t i, float&) {
-        float t, u, v;
-        if (overflow) return;
-        if (!tri_test(T.tri[3 * i], T.tri[3 * i + 1], T.tri[3 * i + 2], o, d, tmin, tmax, t, u, v)) return;
+    auto insert = [&](float t, int mat, f3 n) {
         if (nh == kMaxImaginary) {
             overflow = true;
             return;
         }
-        const int2 rec = T.rec[i];
-        const float4 fn = T.nrm[4 * i + 3];
-        LiteHit h{t, rec.x, sc.mats[rec.x].med_idx, dot(mk(fn.x, fn.y, fn.z), d) < 0.f ? 1 : 0};
+        LiteHit h{t, mat, sc.mats[mat].med_idx, dot(n, d) < 0.f ? 1 : 0};
         int pos = 0; // insertion after the last entry with distance <= t (pstack.cuh:70-95)
         for (int k = nh - 1; k >= 0; k--)
             if (hits[k].dist <= t) {
@@ -509,7 +527,21 @@ __device__ __forceinline__ bool trace_imag(const DScene& sc, f3 o, f3 d, float t
         for (int k = nh; k > pos; k--) hits[k] = hits[k - 1];
         hits[pos] = h;
         nh++;
+    };
+    stack_ok = bvh_ray_query(T.bvh, o, d, tmin, tmax, [&](uint32_t i, float&) {
+        float t, u, v;
+        if (overflow) return;
+        if (!tri_test(T.tri[3 * i], T.tri[3 * i + 1], T.tri[3 * i + 2], o, d, tmin, tmax, t, u, v)) return;
+        const float4 fn = T.nrm[4 * i + 3];
+        insert(t, T.rec[i].x, mk(fn.x, fn.y, fn.z));
     });
+    // imaginary spheres report both crossings (intersector.cu:78-93 under an any-hit program)
+    for (uint32_t i = 0; i < sc.sph_imag.n && !overflow; i++) {
+        float th[2];
+        f3 nrm[2];
+        const int k = sphere_test(sc.sph_imag.sph[i], o, d, tmin, tmax, th, nrm);
+        for (int j = 0; j < k && !overflow; j++) insert(th[j], sc.sph_imag.rec[i].x, nrm[j]);
+    }
     return !overflow;
 }
 

@@ -379,12 +379,40 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
 
 extern "C" const char* upbp_last_error(const upbp_renderer* r) { return r ? r->err.c_str() : g_err.c_str(); }
 
+// Every index the kernels dereference must be in range: a bad one would read outside a
+// device table.
+static const char* invalid_scene(const upbp_scene& s)
+{
+    const int64_t nm = s.mat_count, nd = s.med_count, nl = s.light_count;
+    if (nm == 0 || nd == 0) return "scene needs at least one material and one medium";
+    if (int64_t(s.global_medium_idx) >= nd) return "global medium index out of range";
+    if (s.background_light_idx >= 0 && s.background_light_idx >= nl) return "background light index out of range";
+    for (uint32_t i = 0; i < s.mat_count; i++)
+        if (s.materials[i].med_idx < -1 || s.materials[i].med_idx >= nd) return "material medium index out of range";
+    for (uint32_t i = 0; i < s.light_count; i++)
+        if (s.lights[i].mat_id < -1 || s.lights[i].mat_id >= nm || s.lights[i].med_id >= nd)
+            return "light material or medium index out of range";
+    if (s.camera.mat_id < -1 || s.camera.mat_id >= nm || s.camera.med_id >= nd)
+        return "camera material or medium index out of range";
+    for (const upbp_triangles* t : {&s.real, &s.imag})
+        for (uint32_t i = 0; i < t->count; i++)
+            if (t->mat_id[i] < 0 || t->mat_id[i] >= nm || t->light_id[i] < -1 || t->light_id[i] >= nl)
+                return "triangle material or light index out of range";
+    for (const upbp_spheres* t : {&s.sph_real, &s.sph_imag})
+        for (uint32_t i = 0; i < t->count; i++)
+            if (t->mat_id[i] < 0 || t->mat_id[i] >= nm || (t->light_id && (t->light_id[i] < -1 || t->light_id[i] >= nl)) ||
+                !(t->spheres[4 * i + 3] > 0.f))
+                return "sphere material, light index or radius out of range";
+    return nullptr;
+}
+
 extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp_renderer** out)
 {
     upbp_renderer* r = nullptr;
     if (!scene || !cfg || !out) return fail(r, "null argument", UPBP_E_ARG);
     *out = nullptr;
     if (scene->light_count == 0) return fail(r, "scene has no lights", UPBP_E_ARG);
+    if (const char* bad = invalid_scene(*scene)) return fail(r, bad, UPBP_E_ARG);
     if (cfg->path_count_per_iter < 1.f) return fail(r, "light path count must be positive", UPBP_E_ARG);
     if (cfg->max_path_length >= uint32_t(kMaxBounces)) return fail(r, "max path length must be below 64", UPBP_E_ARG);
     int ndev = 0;
@@ -445,11 +473,24 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
             o.position = F3(l.position);
         }
     }
-    r->owned.resize(3);
+    // analytic spheres (predefined scenes)
+    std::vector<float4> sph[2];
+    std::vector<int2> sph_rec[2];
+    for (int k = 0; k < 2; k++) {
+        const upbp_spheres& S = k == 0 ? scene->sph_real : scene->sph_imag;
+        for (uint32_t i = 0; i < S.count; i++) {
+            sph[k].push_back(make_float4(S.spheres[4 * i], S.spheres[4 * i + 1], S.spheres[4 * i + 2], S.spheres[4 * i + 3]));
+            sph_rec[k].push_back(make_int2(S.mat_id[i], S.light_id ? S.light_id[i] : -1));
+        }
+    }
+    r->owned.resize(7);
     sc.mats = upload(r, mats, r->owned[0]);
     sc.media = upload(r, media, r->owned[1]);
     sc.lights = upload(r, lights, r->owned[2]);
-    if (!sc.mats || !sc.media || !sc.lights) {
+    sc.sph_real = {upload(r, sph[0], r->owned[3]), upload(r, sph_rec[0], r->owned[4]), uint32_t(sph[0].size())};
+    sc.sph_imag = {upload(r, sph[1], r->owned[5]), upload(r, sph_rec[1], r->owned[6]), uint32_t(sph[1].size())};
+    if (!sc.mats || !sc.media || !sc.lights || !sc.sph_real.sph || !sc.sph_real.rec || !sc.sph_imag.sph ||
+        !sc.sph_imag.rec) {
         upbp_destroy(r);
         return fail(nullptr, "scene upload failed", UPBP_E_HIP);
     }
@@ -479,6 +520,9 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
         float mag = 0.f;
         for (const upbp_triangles* t : {&scene->real, &scene->imag})
             for (size_t i = 0; i < size_t(t->count) * 9; i++) mag = fmaxf(mag, fabsf(t->verts[i]));
+        for (int k = 0; k < 2; k++)
+            for (const float4& q : sph[k])
+                mag = fmaxf(mag, fmaxf(fabsf(q.x), fmaxf(fabsf(q.y), fabsf(q.z))) + q.w);
         for (int a = 0; a < 3; a++) mag = fmaxf(mag, fabsf(scene->camera.origin[a]));
         r->scene_mag = fmaxf(mag, 1e-3f);
     }

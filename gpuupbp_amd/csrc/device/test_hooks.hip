@@ -314,6 +314,44 @@ extern "C" int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const 
     return fails ? UPBP_E_CAPACITY : UPBP_OK;
 }
 
+// one thread per (ray, sphere): root count and the roots (INFINITY where absent)
+__global__ void k_sphere_pairs(const float4* sph, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= uint64_t(n_sph) * n_rays) return;
+    const uint32_t r = uint32_t(i / n_sph), k = uint32_t(i % n_sph);
+    const float* R = rays + 8 * size_t(r);
+    float th[2] = {INFINITY, INFINITY};
+    f3 nh[2];
+    const int n = sphere_test(sph[k], mk(R[0], R[1], R[2]), mk(R[3], R[4], R[5]), R[6], R[7], th, nh);
+    out[3 * i] = float(n);
+    out[3 * i + 1] = n > 0 ? th[0] : INFINITY;
+    out[3 * i + 2] = n > 1 ? th[1] : INFINITY;
+}
+
+extern "C" int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays,
+                                     float* out)
+{
+    if ((n_sph && !spheres) || (n_rays && !rays) || (uint64_t(n_sph) * n_rays && !out)) return UPBP_E_ARG;
+    const uint64_t pairs = uint64_t(n_sph) * n_rays;
+    if (pairs > (1ull << 30)) return UPBP_E_ARG;
+    if (pairs == 0) return UPBP_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
+    Tmp t;
+    float4* ds = t.alloc<float4>(n_sph);
+    float* dr = t.alloc<float>(8 * size_t(n_rays));
+    float* dout = t.alloc<float>(3 * pairs);
+    if (!ds || !dr || !dout) return UPBP_E_HIP;
+    (void)hipMemcpy(ds, spheres, size_t(n_sph) * 16, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dr, rays, 8 * size_t(n_rays) * 4, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_sphere_pairs, dim3(uint32_t((pairs + 255) / 256)), dim3(256), 0, 0, ds, n_sph, dr, n_rays,
+                       dout);
+    int rc = int(hipDeviceSynchronize());
+    if (!rc) rc = int(hipMemcpy(out, dout, 3 * pairs * 4, hipMemcpyDeviceToHost));
+    return rc ? UPBP_E_HIP : UPBP_OK;
+}
+
 extern "C" int upbp_test_copy(uint64_t bytes, int reps, double* ms_per_copy)
 {
     if (bytes < 16 || reps < 1 || !ms_per_copy) return UPBP_E_ARG;

@@ -219,6 +219,13 @@ struct TriSet {
     Bvh bvh;
 };
 
+// analytic spheres: centre and radius, (material, light) per sphere
+struct SphSet {
+    const float4* sph;
+    const int2* rec;
+    uint32_t n;
+};
+
 struct DScene {
     const DMaterial* mats;
     const DMedium* media;
@@ -229,6 +236,7 @@ struct DScene {
     f3 center;
     float radius, inv_radius_sqr;
     TriSet real, imag;
+    SphSet sph_real, sph_imag;
     int use_shading_normal;
 };
 

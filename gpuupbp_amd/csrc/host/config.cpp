@@ -82,6 +82,9 @@ void short_help(const char* prog)
     printf("           -t <time> | -i <iteration> | -o <output_name> ]\n\n");
     printf("    -s  Selects the scene (default 0):\n");
     printf("        -1    user must supply additional argument which contains path to an obj scene file\n");
+    for (int i = 0; i < 5; i++) printf("          %-2d    %s\n", i, upbp_predefined_scene_name(i, 1));
+    printf("          5..%d other predefined simple scenes, for complete list, please see full help (-hf)\n",
+           upbp_predefined_scene_count() - 1);
     printf("    -a  Selects the rendering algorithm (default upbp_all):\n");
     printf("        upbp_<tech>[+<tech>]*  custom selection of techniques from upbp "
            "(tech=bpt|surf|pp3d|pb2d|bb1d|bp2d)\n");
@@ -97,6 +100,9 @@ void short_help(const char* prog)
 void full_help(const char* prog)
 {
     short_help(prog);
+    printf("\n    Predefined scenes (-s):\n");
+    for (int i = 0; i < upbp_predefined_scene_count(); i++)
+        printf("        %-2d    %s\n", i, upbp_predefined_scene_name(i, 1));
     printf("\n    Radius options:\n");
     printf("    -r_alpha <alpha>  -r_alpha_{surf,pp3d,pb2d,bb1d,bp2d} <alpha>\n");
     printf("    -r_initial <init> -r_initial_{surf,pp3d,pb2d,bb1d,bp2d} <init>"
@@ -168,10 +174,12 @@ extern "C" int upbp_parse_commandline(int argc, const char** argv, upbp_config* 
         const std::string arg(argv[i]);
         if (arg == "-h" || arg == "--help" || arg == "/?") {
             short_help(argv[0]);
+            fflush(stdout);
             return UPBP_HELP;
         }
         if (arg == "-hf" || arg == "--help_full" || arg == "/??") {
             full_help(argv[0]);
+            fflush(stdout);
             return UPBP_HELP;
         }
         if (arg.empty() || arg[0] != '-') continue;
@@ -190,8 +198,7 @@ extern "C" int upbp_parse_commandline(int argc, const char** argv, upbp_config* 
             if (!v) return UPBP_E_ARG;
             std::istringstream iss(v);
             iss >> c.scene_id;
-            // this build has no predefined scene table yet: only -1 (OBJ) is accepted
-            if (iss.fail() || c.scene_id >= 41) return fail("invalid argument of -s option, please see help (-h)");
+            if (iss.fail() || c.scene_id >= upbp_predefined_scene_count()) return fail("invalid argument of -s option, please see help (-h)");
             if (c.scene_id == 3 || c.scene_id == 4 || c.scene_id == 9 || c.scene_id == 10)
                 return fail("scenes 3, 4, 9, and 10 cannot be rendered with UPBP, please see help (-h)");
             if (c.scene_id == -1) {

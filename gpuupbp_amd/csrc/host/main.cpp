@@ -102,7 +102,9 @@ int main(int argc, const char* argv[])
     // the reference exits with 0 both on parse errors and after printing help
     if (upbp_parse_commandline(argc, argv, &cfg) != UPBP_OK) return 0;
     upbp_scene_handle* sh = nullptr;
-    int e = upbp_scene_load_obj(cfg.scene_obj_file, uint32_t(cfg.res_x), uint32_t(cfg.res_y), &sh);
+    int e = cfg.scene_id > -1
+                ? upbp_scene_load_predefined(cfg.scene_id, uint32_t(cfg.res_x), uint32_t(cfg.res_y), &sh)
+                : upbp_scene_load_obj(cfg.scene_obj_file, uint32_t(cfg.res_x), uint32_t(cfg.res_y), &sh);
     if (e) {
         fprintf(stderr, "Error: %s\n", upbp_global_error());
         return 2;

@@ -13,6 +13,7 @@
 //  * the scene sphere is the half-diagonal of the triangle bounding box (scene_loader.hpp:65-71).
 // Errors return UPBP_E_IO / UPBP_E_PARSE instead of exit(2).
 #include "../../../include/upbp_capi.h"
+#include "scene_handle.h"
 
 #include <cmath>
 #include <cstdio>
@@ -452,16 +453,9 @@ bool read_aux(Loader& L, const std::string& path)
 
 } // namespace
 
-struct upbp_scene_handle {
-    upbp_scene scene;
-    std::vector<float> rv, rn, iv, in;
-    std::vector<int32_t> rm, rl, im, il;
-    std::vector<upbp_material> mats;
-    std::vector<upbp_medium> media;
-    std::vector<upbp_light> lights;
-};
-
 static thread_local std::string g_loader_error;
+
+void upbp_set_loader_error(const std::string& msg) { g_loader_error = msg; }
 
 extern "C" const char* upbp_global_error(void) { return g_loader_error.c_str(); }
 
@@ -644,14 +638,7 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
     s.scene_center[2] = (bmax.z + bmin.z) * .5f;
     s.scene_radius = std::sqrt(r2) * .5f;
 
-    s.real = {static_cast<uint32_t>(h->rm.size()), h->rv.data(), h->rn.data(), h->rm.data(), h->rl.data()};
-    s.imag = {static_cast<uint32_t>(h->im.size()), h->iv.data(), h->in.data(), h->im.data(), h->il.data()};
-    s.mat_count = static_cast<uint32_t>(h->mats.size());
-    s.materials = h->mats.data();
-    s.med_count = static_cast<uint32_t>(h->media.size());
-    s.media = h->media.data();
-    s.light_count = static_cast<uint32_t>(h->lights.size());
-    s.lights = h->lights.data();
+    h->bind();
     *out = h;
     return UPBP_OK;
 }

@@ -96,6 +96,18 @@ def range_query(kind, prims, queries, radius, max_indices=None, subbeam_len=0.0,
     return offsets, idx[: n.value]
 
 
+def sphere_hits(spheres, rays):
+    """GPU analytic sphere test of every (ray, sphere) pair: (n_rays, n_spheres, 3) of root count,
+    near root, far root (inf if absent)."""
+    spheres = np.ascontiguousarray(spheres, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros((rays.shape[0], spheres.shape[0], 3), np.float32)
+    rc = capi.lib().upbp_test_sphere_hits(_p(spheres), spheres.shape[0], _p(rays), rays.shape[0], _p(out))
+    if rc != 0:
+        raise RuntimeError("upbp_test_sphere_hits failed: %d" % rc)
+    return out
+
+
 def closest_hit(verts, rays):
     verts = np.ascontiguousarray(verts, np.float32)
     rays = np.ascontiguousarray(rays, np.float32)

@@ -121,6 +121,15 @@ typedef struct upbp_triangles {
     const int32_t* light_id; /* per triangle, -1 if not emissive */
 } upbp_triangles;
 
+/* Analytic spheres (predefined scenes only, gpuupbp/shared/scene.h Spheres, intersected as in
+ * gpuupbp/kernel/intersector.cu:35-101). */
+typedef struct upbp_spheres {
+    uint32_t count;
+    const float* spheres;    /* 4 floats per sphere: centre x, y, z, radius */
+    const int32_t* mat_id;   /* per sphere */
+    const int32_t* light_id; /* per sphere, -1 (spheres are never emissive in the reference) */
+} upbp_spheres;
+
 typedef struct upbp_scene {
     upbp_triangles real;
     upbp_triangles imag;
@@ -135,6 +144,8 @@ typedef struct upbp_scene {
     upbp_camera camera;
     float scene_center[3];
     float scene_radius;
+    upbp_spheres sph_real; /* zero-initialised (count 0) for OBJ scenes */
+    upbp_spheres sph_imag;
 } upbp_scene;
 
 /* Per-iteration statistics (device counters plus HIP-event times). */
@@ -172,6 +183,15 @@ void upbp_config_defaults(upbp_config* cfg);
 /* Replaces SceneLoader::load_from_obj + load_scene (gpuupbp/misc/scene_loader_obj.cpp:926-1104,
  * gpuupbp/misc/scene_loader.hpp:44-76): reads file.obj, its mtllib and file.obj.aux. */
 int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_t res_y, upbp_scene_handle** out);
+
+/* Replaces SceneLoader::load_predefined + load_scene (gpuupbp/misc/scene_loader_predef.cpp:126-603,
+ * scene_loader.hpp:44-76) over the scene table of init_scene_configs (gpuupbp/misc/config.cpp:100-
+ * 518): Cornell-box variants with analytic spheres, ids 0 .. upbp_predefined_scene_count() - 1.
+ * UPBP_E_ARG for an id outside the table. */
+int upbp_scene_load_predefined(int32_t scene_id, uint32_t res_x, uint32_t res_y, upbp_scene_handle** out);
+int32_t upbp_predefined_scene_count(void);
+/* Short (file-name) or long (help-text) name of a predefined scene; NULL outside the table. */
+const char* upbp_predefined_scene_name(int32_t scene_id, int32_t long_name);
 const upbp_scene* upbp_scene_get(const upbp_scene_handle* h);
 void upbp_scene_free(upbp_scene_handle* h);
 
@@ -264,6 +284,12 @@ int upbp_test_range_query_wide(int kind, const float* prims, uint32_t n_prims, c
  * Closest: hit_t[i] (INFINITY on miss), hit_prim[i] (-1 on miss). */
 int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays,
                           float* hit_t, int32_t* hit_prim);
+
+/* Analytic sphere test of every (ray, sphere) pair (replaces the sphere intersection program,
+ * gpuupbp/kernel/intersector.cu:35-95). spheres: 4 floats each (centre, radius); rays: 8 floats
+ * each. out[3 * (ray * n_sph + sphere) + {0, 1, 2}] = root count in (tmin, tmax), near root,
+ * far root (INFINITY where absent). At most 2^30 pairs. */
+int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out);
 
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM

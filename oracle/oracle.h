@@ -52,6 +52,9 @@ int oracle_range_query(int kind, const float* prims, uint32_t n_prims, const flo
 int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays, float* hit_t,
                        int32_t* hit_prim);
 
+/* Analytic sphere test of every (ray, sphere) pair, same layout as upbp_test_sphere_hits. */
+int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out);
+
 /* Camera mapping checks: raster -> primary ray direction and world -> raster, using the
  * reference's matrix construction (gpuupbp/host/camera.hpp:33-76). */
 int oracle_camera_ray(const upbp_scene* scene, const upbp_config* cfg, const float* raster_xy, uint32_t n,

@@ -166,6 +166,13 @@ void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& 
     };
     tris(in->real, s.real);
     tris(in->imag, s.imag);
+    auto sphs = [](const upbp_spheres& t, std::vector<Sph>& out) {
+        out.resize(t.count);
+        for (uint32_t i = 0; i < t.count; i++)
+            out[i] = {fromp(t.spheres + 4 * i), t.spheres[4 * i + 3], t.mat_id[i], t.light_id ? t.light_id[i] : -1};
+    };
+    sphs(in->sph_real, s.sreal);
+    sphs(in->sph_imag, s.simag);
     s.mats.resize(in->mat_count);
     for (uint32_t i = 0; i < in->mat_count; i++) {
         const upbp_material& m = in->materials[i];
@@ -633,6 +640,46 @@ V light_radiance(const Light& l, const Scene& sc, V ray_dir, V hit, float* for_p
     }
     default: return v3(0.f);
     }
+}
+
+// The roots of |o + t d - c| = r in FP32: b = o.d, c = o.o - r^2, t = -b -/+ sqrt(b^2 - c). When
+// the near root lies more than a radius away, the solve is repeated from the point it gives and
+// the two steps are added, as the reference does (gpuupbp/kernel/intersector.cu:56-75); the far
+// root is then that point's far root plus the first step (:86). An all-hits caller takes both,
+// a closest-hit caller the first.
+int sphere_hit(const Sph& s, V ro, V rd, float tmin, float tmax, float* th, V* nh)
+{
+    const V o = ro - s.c;
+    const float l = 1.f / std::sqrt(dot(rd, rd));
+    const V d = rd * l;
+    float b = dot(o, d), c = dot(o, o) - s.r * s.r, disc = b * b - c;
+    if (!(disc > 0.f)) return 0;
+    float sq = std::sqrt(disc);
+    const float first = -b - sq;
+    float second = 0.f;
+    const bool again = std::fabs(first) > s.r;
+    if (again) {
+        const V p = o + first * d;
+        b = dot(p, d);
+        c = dot(p, p) - s.r * s.r;
+        disc = b * b - c;
+        if (disc > 0.f) {
+            sq = std::sqrt(disc);
+            second = -b - sq;
+        }
+    }
+    int n = 0;
+    const float tn = first + second;
+    if (tn * l > tmin && tn * l < tmax) {
+        th[n] = tn * l;
+        nh[n++] = (o + tn * d) / s.r;
+    }
+    const float tf = (-b + sq) + (again ? first : 0.f);
+    if (tf * l > tmin && tf * l < tmax) {
+        th[n] = tf * l;
+        nh[n++] = (o + tf * d) / s.r;
+    }
+    return n;
 }
 
 } // namespace orc

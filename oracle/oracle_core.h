@@ -181,8 +181,16 @@ struct Tri {
     int mat, light;
 };
 
+// analytic sphere (predefined scenes)
+struct Sph {
+    V c;
+    float r;
+    int mat, light;
+};
+
 struct Scene {
     std::vector<Tri> real, imag;
+    std::vector<Sph> sreal, simag;
     std::vector<Material> mats;
     std::vector<Medium> media;
     std::vector<Light> lights;
@@ -194,6 +202,9 @@ struct Scene {
 };
 
 void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& out);
+// Ray against an analytic sphere (gpuupbp/kernel/intersector.cu:35-95): roots in (tmin, tmax),
+// near first, with outward normals; returns their number (0..2).
+int sphere_hit(const Sph& s, V ro, V rd, float tmin, float tmax, float* th, V* nh);
 V transform_point(const Mat4& m, V p);
 
 // ---- sampling helpers: kernel/sample.cuh -----------------------------------

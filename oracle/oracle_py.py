@@ -32,6 +32,7 @@ def lib():
         h.oracle_range_query.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, C.c_float, vp, vp, C.c_uint64,
                                          C.POINTER(C.c_uint64)]
         h.oracle_closest_hit.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
+        h.oracle_sphere_hits.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
         h.oracle_camera_ray.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_world_to_raster.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_hg_pdf.argtypes = [C.c_float, C.c_float]
@@ -95,6 +96,15 @@ def closest_hit(verts, rays):
     prim = np.zeros(rays.shape[0], np.int32)
     lib().oracle_closest_hit(_p(verts), verts.shape[0], _p(rays), rays.shape[0], _p(t), _p(prim))
     return t, prim
+
+
+def sphere_hits(spheres, rays):
+    """(n_rays, n_spheres, 3): root count in (tmin, tmax), near root, far root (inf if absent)."""
+    spheres = np.ascontiguousarray(spheres, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros((rays.shape[0], spheres.shape[0], 3), np.float32)
+    lib().oracle_sphere_hits(_p(spheres), spheres.shape[0], _p(rays), rays.shape[0], _p(out))
+    return out
 
 
 def camera_rays(scene_ptr, cfg, raster_xy):

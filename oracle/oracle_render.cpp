@@ -356,6 +356,29 @@ static bool trace_real(const Ctx& c, V o, V d, float tmin, float tmax, Isect& ou
             }
         }
     });
+    // analytic spheres after the triangles, exhaustively (a handful per predefined scene)
+    int best_s = -1;
+    V sn{0.f, 0.f, 0.f};
+    for (size_t i = 0; i < sc.sreal.size(); i++) {
+        float th[2];
+        V nh[2];
+        if (sphere_hit(sc.sreal[i], o, d, tmin, best >= 0 || best_s >= 0 ? bt : tmax, th, nh)) {
+            best_s = int(i);
+            bt = th[0];
+            sn = nh[0];
+        }
+    }
+    if (best_s >= 0) {
+        const Sph& q = sc.sreal[size_t(best_s)];
+        out.dist = bt;
+        out.mat = q.mat;
+        out.med = sc.mats[size_t(q.mat)].med_idx;
+        out.light = q.light;
+        out.normal = sn;
+        out.shading = sn;
+        out.enter = dot(sn, d) < 0.f;
+        return true;
+    }
     if (best < 0) return false;
     const Tri& t = sc.real[size_t(best)];
     out.dist = bt;
@@ -373,16 +396,31 @@ static bool trace_imag(const Ctx& c, V o, V d, float tmin, float tmax, PStack<Li
 {
     const Scene& sc = *c.sc;
     bool overflow = false;
-    std::vector<std::pair<float, uint32_t>> hits;
+    struct Cand {
+        float t;
+        uint32_t order; // triangles by index, then sphere crossings in test order
+        int mat;
+        bool enter;
+        bool operator<(const Cand& o) const { return t < o.t || (t == o.t && order < o.order); }
+    };
+    std::vector<Cand> hits;
     c.bvh_imag.query_segment(o, d, tmin, tmax, [&](uint32_t i, float&) {
         float t, u, v;
-        if (tri_hit(sc.imag[i], o, d, tmin, tmax, t, u, v)) hits.push_back({t, i});
+        if (tri_hit(sc.imag[i], o, d, tmin, tmax, t, u, v))
+            hits.push_back({t, i, sc.imag[i].mat, dot(sc.imag[i].face, d) < 0.f});
     });
+    // imaginary spheres report both crossings (intersector.cu:78-93 under an any-hit program)
+    uint32_t order = uint32_t(sc.imag.size());
+    for (const Sph& q : sc.simag) {
+        float th[2];
+        V nh[2];
+        const int k = sphere_hit(q, o, d, tmin, tmax, th, nh);
+        for (int j = 0; j < k; j++) hits.push_back({th[j], order++, q.mat, dot(nh[j], d) < 0.f});
+    }
     std::sort(hits.begin(), hits.end());
     for (auto& h : hits) {
-        const Tri& t = sc.imag[h.second];
-        LiteIsect li{h.first, t.mat, sc.mats[size_t(t.mat)].med_idx, dot(t.face, d) < 0.f};
-        if (!st.push(li, h.first)) {
+        LiteIsect li{h.t, h.mat, sc.mats[size_t(h.mat)].med_idx, h.enter};
+        if (!st.push(li, h.t)) {
             overflow = true;
             break;
         }
@@ -2101,6 +2139,24 @@ extern "C" int oracle_range_query(int kind, const float* prims, uint32_t n_prims
     offsets[n_queries] = uint32_t(n);
     if (n_indices) *n_indices = n;
     return n <= max_indices ? UPBP_OK : UPBP_E_CAPACITY;
+}
+
+extern "C" int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out)
+{
+    for (uint32_t r = 0; r < n_rays; r++) {
+        const float* R = rays + 8 * size_t(r);
+        for (uint32_t k = 0; k < n_sph; k++) {
+            const Sph s{fromp(spheres + 4 * size_t(k)), spheres[4 * size_t(k) + 3], 0, -1};
+            float th[2];
+            V nh[2];
+            const int n = sphere_hit(s, fromp(R), fromp(R + 3), R[6], R[7], th, nh);
+            float* o = out + 3 * (size_t(r) * n_sph + k);
+            o[0] = float(n);
+            o[1] = n > 0 ? th[0] : INFINITY;
+            o[2] = n > 1 ? th[1] : INFINITY;
+        }
+    }
+    return UPBP_OK;
 }
 
 extern "C" int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays, float* hit_t,
