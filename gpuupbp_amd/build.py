@@ -24,7 +24,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = "gfx950"
 
 HOST_SOURCES = ["host/config.cpp", "host/scene_loader.cpp", "host/exr_writer.cpp", "host/image_writers.cpp",
-                "host/scene_gen.cpp", "host/scene_predef.cpp"]
+                "host/scene_gen.cpp", "host/scene_predef.cpp",
+                "host/exr_reader.cpp"]
 HIP_SOURCES = ["device/renderer.hip", "device/lbvh.hip", "device/light_pass.hip", "device/camera_pass.hip",
                "device/merge_pass.hip", "device/test_hooks.hip"]
 
@@ -66,7 +67,7 @@ def build(jobs=8, oracle=True):
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, 16))) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = LIB + ".tmp"
-    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"])
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread", "-lz"])
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "host", "main.cpp")
     if os.path.exists(main_src):

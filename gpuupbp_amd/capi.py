@@ -58,7 +58,7 @@ class Medium(C.Structure):
 class Light(C.Structure):
     _fields_ = [("type", C.c_int32), ("mat_id", C.c_int32), ("med_id", C.c_int32), ("p0", C.c_float * 3),
                 ("p1", C.c_float * 3), ("p2", C.c_float * 3), ("position", C.c_float * 3),
-                ("intensity", C.c_float * 3), ("scale", C.c_float)]
+                ("intensity", C.c_float * 3), ("scale", C.c_float), ("uses_env_map", C.c_int32)]
 
 
 class Camera(C.Structure):
@@ -77,12 +77,16 @@ class Spheres(C.Structure):
                 ("light_id", C.POINTER(C.c_int32))]
 
 
+class EnvMap(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_float))]
+
+
 class Scene(C.Structure):
     _fields_ = [("real", Triangles), ("imag", Triangles), ("mat_count", C.c_uint32),
                 ("materials", C.POINTER(Material)), ("med_count", C.c_uint32), ("media", C.POINTER(Medium)),
                 ("light_count", C.c_uint32), ("lights", C.POINTER(Light)), ("global_medium_idx", C.c_uint32),
                 ("background_light_idx", C.c_int32), ("camera", Camera), ("scene_center", C.c_float * 3),
-                ("scene_radius", C.c_float), ("sph_real", Spheres), ("sph_imag", Spheres)]
+                ("scene_radius", C.c_float), ("sph_real", Spheres), ("sph_imag", Spheres), ("env_map", EnvMap)]
 
 
 class Tuning(C.Structure):
@@ -141,6 +145,8 @@ _SIGS = [
     ("upbp_set_tuning", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_test_env_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_read_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 ]
 
 
@@ -201,6 +207,20 @@ class SceneHandle:
             self.close()
         except Exception:
             pass
+
+
+def read_exr(path):
+    """(H, W, 3) float32 image of an OpenEXR file (the environment-map reader)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    rc = lib().upbp_read_exr(path.encode(), None, 0, C.byref(w), C.byref(h))
+    if rc != 0:
+        raise RuntimeError("upbp_read_exr failed (%d)" % rc)
+    import numpy as np
+    img = np.zeros((h.value, w.value, 3), np.float32)
+    rc = lib().upbp_read_exr(path.encode(), img.ctypes.data, img.size, C.byref(w), C.byref(h))
+    if rc != 0:
+        raise RuntimeError("upbp_read_exr failed (%d)" % rc)
+    return img
 
 
 def predefined_scenes():

@@ -7,6 +7,7 @@
 // the consumer rebuilds the identical frame.
 #pragma once
 
+#include "env_map.h"
 #include "upbp_device.h"
 
 namespace upbp {
@@ -348,11 +349,18 @@ UPBP_HD f3 light_illuminate(const DLight& l, const DScene& sc, f3 recv, float u0
         return l.intensity;
     }
     if (l.type == L_BACKGROUND) { // :128-151
-        dir = sample_uniform_sphere_w(u0, u1, &for_pdf_w);
+        f3 rad;
+        if (l.uses_env && sc.env.img) {
+            dir = env_sample(sc.env, u0, u1, for_pdf_w, rad);
+            rad = rad * l.scale;
+        } else {
+            dir = sample_uniform_sphere_w(u0, u1, &for_pdf_w);
+            rad = l.intensity * l.scale;
+        }
         dist = kInf;
         if (emission_pdf_w) *emission_pdf_w = for_pdf_w * concentric_disc_pdf_a() * sc.inv_radius_sqr;
         if (cos_at_light) *cos_at_light = 1.f;
-        return l.intensity * l.scale;
+        return rad;
     }
     if (l.type == L_DIRECTIONAL) { // :230-248
         dir = -l.frame.z;
@@ -389,7 +397,14 @@ UPBP_HD f3 light_emit(const DLight& l, const DScene& sc, float d0, float d1, flo
     }
     if (l.type == L_BACKGROUND) { // :153-190
         float fp;
-        dir = sample_uniform_sphere_w(d0, d1, &fp);
+        f3 rad;
+        if (l.uses_env && sc.env.img) {
+            dir = -env_sample(sc.env, d0, d1, fp, rad);
+            rad = rad * l.scale;
+        } else {
+            dir = sample_uniform_sphere_w(d0, d1, &fp);
+            rad = l.intensity * l.scale;
+        }
         float x, y;
         sample_concentric_disc(p0, p1, x, y);
         Frame f;
@@ -398,7 +413,7 @@ UPBP_HD f3 light_emit(const DLight& l, const DScene& sc, float d0, float d1, flo
         emission_pdf_w = fp * concentric_disc_pdf_a() * sc.inv_radius_sqr;
         if (for_pdf_a) *for_pdf_a = fp;
         if (cos_light) *cos_light = 1.f;
-        return l.intensity * l.scale;
+        return rad;
     }
     if (l.type == L_DIRECTIONAL) { // :250-272
         float x, y;
@@ -427,10 +442,17 @@ UPBP_HD f3 light_radiance(const DLight& l, const DScene& sc, f3 ray_dir, float* 
         return l.intensity;
     }
     if (l.type == L_BACKGROUND) { // :192-217
-        const float fp = uniform_sphere_pdf_w();
+        float fp;
+        f3 rad;
+        if (l.uses_env && sc.env.img) {
+            rad = env_lookup(sc.env, ray_dir, &fp) * l.scale;
+        } else {
+            fp = uniform_sphere_pdf_w();
+            rad = l.intensity * l.scale;
+        }
         if (for_pdf_a) *for_pdf_a = fp;
         if (emission_pdf_w) *emission_pdf_w = fp * concentric_disc_pdf_a() * sc.inv_radius_sqr;
-        return l.intensity * l.scale;
+        return rad;
     }
     return mk(0.f);
 }

@@ -6,6 +6,7 @@
 // upbp_run_iteration replaces Renderer::run_iteration (gpuupbp/renderer/renderer.cpp:37-418):
 // radius schedule and MIS factors (:56-119), light wavefront, deterministic photon lists +
 // three photon LBVH builds, camera wavefront, FP64 accumulation on the device.
+#include "env_tables.h"
 #include "kernels.h"
 
 #include "../../../include/upbp_capi.h"
@@ -394,6 +395,9 @@ static const char* invalid_scene(const upbp_scene& s)
             return "light material or medium index out of range";
     if (s.camera.mat_id < -1 || s.camera.mat_id >= nm || s.camera.med_id >= nd)
         return "camera material or medium index out of range";
+    if (s.env_map.rgb && (s.env_map.height == 0 || s.env_map.width < 2 * s.env_map.height ||
+                          s.env_map.height > (1u << 15)))
+        return "environment map must be at least twice as wide as high";
     for (const upbp_triangles* t : {&s.real, &s.imag})
         for (uint32_t i = 0; i < t->count; i++)
             if (t->mat_id[i] < 0 || t->mat_id[i] >= nm || t->light_id[i] < -1 || t->light_id[i] >= nl)
@@ -460,6 +464,7 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
         o.med_id = l.med_id;
         o.intensity = F3(l.intensity);
         o.scale = l.scale;
+        o.uses_env = l.type == UPBP_LIGHT_BACKGROUND && l.uses_env_map && scene->env_map.rgb ? 1 : 0;
         if (l.type == UPBP_LIGHT_AREA) { // host/light.hpp:30-47
             o.p0 = F3(l.p0);
             o.e1 = F3(l.p1) - o.p0;
@@ -483,12 +488,35 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
             sph_rec[k].push_back(make_int2(S.mat_id[i], S.light_id ? S.light_id[i] : -1));
         }
     }
-    r->owned.resize(7);
+    // environment map: image and luminance distribution
+    EnvTables env;
+    if (scene->env_map.rgb && scene->env_map.width && scene->env_map.height)
+        build_env_tables(scene->env_map, env);
+    r->owned.resize(14);
     sc.mats = upload(r, mats, r->owned[0]);
     sc.media = upload(r, media, r->owned[1]);
     sc.lights = upload(r, lights, r->owned[2]);
     sc.sph_real = {upload(r, sph[0], r->owned[3]), upload(r, sph_rec[0], r->owned[4]), uint32_t(sph[0].size())};
     sc.sph_imag = {upload(r, sph[1], r->owned[5]), upload(r, sph_rec[1], r->owned[6]), uint32_t(sph[1].size())};
+    if (!env.img.empty()) {
+        DEnvMap& m = sc.env;
+        m.img = upload(r, env.img, r->owned[7]);
+        m.cond_func = upload(r, env.cond_func, r->owned[8]);
+        m.cond_cdf = upload(r, env.cond_cdf, r->owned[9]);
+        m.cond_int = upload(r, env.cond_int, r->owned[10]);
+        m.marg_func = upload(r, env.marg_func, r->owned[11]);
+        m.marg_cdf = upload(r, env.marg_cdf, r->owned[12]);
+        m.marg_int = env.marg_int;
+        m.width = int(scene->env_map.width);
+        m.height = int(scene->env_map.height);
+        m.nu = 2 * m.height;
+        m.nv = m.height;
+        m.norm = .5f * kInvPi * kInvPi;
+        if (!m.img || !m.cond_func || !m.cond_cdf || !m.cond_int || !m.marg_func || !m.marg_cdf) {
+            upbp_destroy(r);
+            return fail(nullptr, "environment map upload failed", UPBP_E_HIP);
+        }
+    }
     if (!sc.mats || !sc.media || !sc.lights || !sc.sph_real.sph || !sc.sph_real.rec || !sc.sph_imag.sph ||
         !sc.sph_imag.rec) {
         upbp_destroy(r);

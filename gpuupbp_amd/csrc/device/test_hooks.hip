@@ -2,6 +2,7 @@
 // primitives, returning exact index sets for comparison with brute force.
 // The acceptance predicates are those of nolmoonen/gpuupbp gpuupbp/kernel/intersection.cuh:33-173,
 // applied as in gpuupbp/kernel/pde/program_{surf,pp3d,pb2d,bb1d,bp2d}.cu.
+#include "env_tables.h"
 #include "kernels.h"
 
 #include "../../../include/upbp_capi.h"
@@ -349,6 +350,69 @@ extern "C" int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const
                        dout);
     int rc = int(hipDeviceSynchronize());
     if (!rc) rc = int(hipMemcpy(out, dout, 3 * pairs * 4, hipMemcpyDeviceToHost));
+    return rc ? UPBP_E_HIP : UPBP_OK;
+}
+
+// per sample: sampled direction (3), its pdf, radiance (3), then lookup of that direction:
+// pdf and radiance (3) -- 11 floats
+__global__ void k_env_samples(DEnvMap m, const float* u, uint32_t n, float* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pdf, lpdf;
+    f3 rad;
+    const f3 d = env_sample(m, u[2 * i], u[2 * i + 1], pdf, rad);
+    const f3 lrad = env_lookup(m, d, &lpdf);
+    float* o = out + 11 * size_t(i);
+    o[0] = d.x;
+    o[1] = d.y;
+    o[2] = d.z;
+    o[3] = pdf;
+    o[4] = rad.x;
+    o[5] = rad.y;
+    o[6] = rad.z;
+    o[7] = lpdf;
+    o[8] = lrad.x;
+    o[9] = lrad.y;
+    o[10] = lrad.z;
+}
+
+extern "C" int upbp_test_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out)
+{
+    if (!env || !env->rgb || env->height == 0 || env->width < 2 * env->height || (n && (!u || !out)))
+        return UPBP_E_ARG;
+    if (n == 0) return UPBP_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
+    EnvTables t;
+    build_env_tables(*env, t);
+    Tmp tmp;
+    auto up = [&](const std::vector<float>& v) {
+        float* d = tmp.alloc<float>(v.size());
+        if (d) (void)hipMemcpy(d, v.data(), v.size() * 4, hipMemcpyHostToDevice);
+        return d;
+    };
+    DEnvMap m;
+    m.img = up(t.img);
+    m.cond_func = up(t.cond_func);
+    m.cond_cdf = up(t.cond_cdf);
+    m.cond_int = up(t.cond_int);
+    m.marg_func = up(t.marg_func);
+    m.marg_cdf = up(t.marg_cdf);
+    m.marg_int = t.marg_int;
+    m.width = int(env->width);
+    m.height = int(env->height);
+    m.nu = 2 * m.height;
+    m.nv = m.height;
+    m.norm = .5f * kInvPi * kInvPi;
+    float* du = tmp.alloc<float>(2 * size_t(n));
+    float* dout = tmp.alloc<float>(11 * size_t(n));
+    if (!m.img || !m.cond_func || !m.cond_cdf || !m.cond_int || !m.marg_func || !m.marg_cdf || !du || !dout)
+        return UPBP_E_HIP;
+    (void)hipMemcpy(du, u, 2 * size_t(n) * 4, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_env_samples, dim3((n + 255) / 256), dim3(256), 0, 0, m, du, n, dout);
+    int rc = int(hipDeviceSynchronize());
+    if (!rc) rc = int(hipMemcpy(out, dout, 11 * size_t(n) * 4, hipMemcpyDeviceToHost));
     return rc ? UPBP_E_HIP : UPBP_OK;
 }
 

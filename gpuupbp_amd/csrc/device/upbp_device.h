@@ -163,6 +163,7 @@ struct DLight {
     float inv_area;
     f3 position;
     float scale;
+    int uses_env; // background light reading sc.env
 };
 
 // Pinhole camera. Rays are formed directly from the image-plane basis; this is the
@@ -226,6 +227,19 @@ struct SphSet {
     uint32_t n;
 };
 
+// environment map of the background light and its luminance distribution (env_map.h)
+struct DEnvMap {
+    const float* img;       // width * height * 3, rows from the top
+    const float* cond_func; // nv * nu
+    const float* cond_cdf;  // nv * (nu + 1)
+    const float* cond_int;  // nv
+    const float* marg_func; // nv
+    const float* marg_cdf;  // nv + 1
+    float marg_int;
+    int width, height, nu, nv; // nu = 2 * height columns, nv = height rows
+    float norm;                // 1 / (2 pi^2): (u, v) density to solid angle, before the 1 / sin(theta)
+};
+
 struct DScene {
     const DMaterial* mats;
     const DMedium* media;
@@ -237,6 +251,7 @@ struct DScene {
     float radius, inv_radius_sqr;
     TriSet real, imag;
     SphSet sph_real, sph_imag;
+    DEnvMap env; // img == nullptr: no environment map
     int use_shading_normal;
 };
 

@@ -16,6 +16,8 @@ struct upbp_scene_handle {
     std::vector<upbp_material> mats;
     std::vector<upbp_medium> media;
     std::vector<upbp_light> lights;
+    std::vector<float> env; // environment map RGB
+    uint32_t env_w = 0, env_h = 0;
 
     // point the scene's views at the vectors (after they stop growing)
     void bind()
@@ -30,10 +32,14 @@ struct upbp_scene_handle {
         scene.media = media.data();
         scene.light_count = static_cast<uint32_t>(lights.size());
         scene.lights = lights.data();
+        scene.env_map = {env_w, env_h, env.empty() ? nullptr : env.data()};
     }
 };
 
 // error text of the last failed scene load on this thread (upbp_global_error)
 void upbp_set_loader_error(const std::string& msg);
+
+// OpenEXR into RGB floats (rows from the top); empty string on success, else the error
+std::string upbp_read_exr_rgb(const std::string& path, std::vector<float>& rgb, uint32_t& width, uint32_t& height);
 
 #endif

@@ -504,11 +504,41 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
             l.type = UPBP_LIGHT_BACKGROUND;
             memcpy(l.intensity, &al.emission, sizeof(l.intensity));
             l.scale = 1.f;
-            if (s.background_light_idx < 0) s.background_light_idx = static_cast<int32_t>(h->lights.size());
-            if (!al.env_map.empty()) {
-                delete h;
-                g_loader_error = "environment-map background lights are not supported by this build";
-                return UPBP_E_PARSE;
+            // only the first background light counts, and only it loads its map
+            // (scene_loader_obj.cpp:994-1013)
+            if (s.background_light_idx < 0) {
+                s.background_light_idx = static_cast<int32_t>(h->lights.size());
+                if (!al.env_map.empty()) {
+                    const std::string file = al.env_map[0] == '/' ? al.env_map : dir_of(path) + al.env_map;
+                    std::vector<float> img;
+                    uint32_t w = 0, ht = 0;
+                    const std::string err = upbp_read_exr_rgb(file, img, w, ht);
+                    if (!err.empty()) {
+                        delete h;
+                        g_loader_error = "environment map loading failed: " + err;
+                        return UPBP_E_IO;
+                    }
+                    if (w < 2 * ht) { // the distribution reads 2 * height columns per row
+                        delete h;
+                        g_loader_error = "environment map must be at least twice as wide as high";
+                        return UPBP_E_PARSE;
+                    }
+                    // rotation about the vertical axis by whole columns, then the scale
+                    // (tinyexr_wrapper.cpp:117-130)
+                    h->env.assign(img.size(), 0.f);
+                    const int rot = static_cast<int>(al.env_rotate * static_cast<float>(w));
+                    for (uint32_t y = 0; y < ht; y++)
+                        for (uint32_t x = 0; x < w; x++) {
+                            int xs = static_cast<int>(x) + rot;
+                            if (xs >= static_cast<int>(w)) xs -= static_cast<int>(w);
+                            if (xs < 0 || xs >= static_cast<int>(w)) continue; // rotations outside [0, 2)
+                            for (int k = 0; k < 3; k++)
+                                h->env[3 * (size_t(y) * w + size_t(xs)) + k] = img[3 * (size_t(y) * w + x) + k] * al.env_scale;
+                        }
+                    h->env_w = w;
+                    h->env_h = ht;
+                    l.uses_env_map = 1;
+                }
             }
         }
         h->lights.push_back(l);

@@ -108,6 +108,19 @@ def sphere_hits(spheres, rays):
     return out
 
 
+def env_samples(rgb, u):
+    """GPU environment-map sampling and lookup of an (H, W, 3) map at u (n, 2): (n, 11) floats
+    (direction, pdf, radiance, lookup pdf, lookup radiance)."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    env = capi.EnvMap(rgb.shape[1], rgb.shape[0], rgb.ctypes.data_as(C.POINTER(C.c_float)))
+    out = np.zeros((u.shape[0], 11), np.float32)
+    rc = capi.lib().upbp_test_env_samples(C.byref(env), _p(u), u.shape[0], _p(out))
+    if rc != 0:
+        raise RuntimeError("upbp_test_env_samples failed: %d" % rc)
+    return out
+
+
 def closest_hit(verts, rays):
     verts = np.ascontiguousarray(verts, np.float32)
     rays = np.ascontiguousarray(rays, np.float32)

@@ -104,6 +104,7 @@ typedef struct upbp_light {
     float position[3];         /* point light position / directional light direction */
     float intensity[3];        /* area/point/directional emission, background colour */
     float scale;               /* background scale */
+    int32_t uses_env_map;      /* background: radiance from upbp_scene.env_map instead of the colour */
 } upbp_light;
 
 typedef struct upbp_camera {
@@ -130,6 +131,14 @@ typedef struct upbp_spheres {
     const int32_t* light_id; /* per sphere, -1 (spheres are never emissive in the reference) */
 } upbp_spheres;
 
+/* Latitude-longitude environment map of the background light (gpuupbp/shared/env_map.h Image):
+ * RGB floats, rows from the top, already rotated and scaled as the AUX line asks
+ * (gpuupbp/misc/tinyexr_wrapper.cpp:97-136). The luminance distribution is built from it. */
+typedef struct upbp_env_map {
+    uint32_t width, height; /* 0 when the scene has none */
+    const float* rgb;
+} upbp_env_map;
+
 typedef struct upbp_scene {
     upbp_triangles real;
     upbp_triangles imag;
@@ -146,6 +155,7 @@ typedef struct upbp_scene {
     float scene_radius;
     upbp_spheres sph_real; /* zero-initialised (count 0) for OBJ scenes */
     upbp_spheres sph_imag;
+    upbp_env_map env_map;
 } upbp_scene;
 
 /* Per-iteration statistics (device counters plus HIP-event times). */
@@ -198,6 +208,13 @@ void upbp_scene_free(upbp_scene_handle* h);
 /* Replaces tinyexr_wrapper::save_exr (gpuupbp/misc/tinyexr_wrapper.cpp:34-95): scanline,
  * uncompressed, HALF, channels B,G,R. rgb is W*H*3 floats, row-major from the top row. */
 int upbp_save_exr(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
+/* Replaces tinyexr's LoadEXR as tinyexr_wrapper::load_image calls it (gpuupbp/misc/
+ * tinyexr_wrapper.cpp:97-136): single-part scanline OpenEXR with NONE, RLE, ZIPS or ZIP
+ * compression and HALF / FLOAT / UINT channels; R, G, B (or one grey channel) into rgb as
+ * width * height * 3 floats from the top row. rgb may be NULL to query the size;
+ * UPBP_E_CAPACITY if capacity_floats is too small. */
+int upbp_read_exr(const char* path, float* rgb, uint64_t capacity_floats, uint32_t* width, uint32_t* height);
 
 /* Replaces save_bmp (gpuupbp/host/framebuffer.cpp:129-200): 24-bit uncompressed BMP, rows
  * bottom-up, B,G,R, gamma-corrected and clamped (rows padded to 4 bytes). */
@@ -290,6 +307,12 @@ int upbp_test_closest_hit(const float* verts, uint32_t n_tris, const float* rays
  * each. out[3 * (ray * n_sph + sphere) + {0, 1, 2}] = root count in (tmin, tmax), near root,
  * far root (INFINITY where absent). At most 2^30 pairs. */
 int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out);
+
+/* Environment-map sampling and lookup (replaces sample / lookup of gpuupbp/kernel/env_map.cuh:
+ * 105-137 over the distribution of gpuupbp/host/env_map.hpp:42-125). u: 2 floats per sample.
+ * out: 11 floats per sample: sampled direction (3), its solid-angle pdf, radiance (3), then the
+ * lookup of that direction: pdf, radiance (3). */
+int upbp_test_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out);
 
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM

@@ -55,6 +55,9 @@ int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, u
 /* Analytic sphere test of every (ray, sphere) pair, same layout as upbp_test_sphere_hits. */
 int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out);
 
+/* Environment-map sampling and lookup, same layout as upbp_test_env_samples. */
+int oracle_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out);
+
 /* Camera mapping checks: raster -> primary ray direction and world -> raster, using the
  * reference's matrix construction (gpuupbp/host/camera.hpp:33-76). */
 int oracle_camera_ray(const upbp_scene* scene, const upbp_config* cfg, const float* raster_xy, uint32_t n,

@@ -2,6 +2,8 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.h). Citations: nolmoonen/gpuupbp.
 #include "oracle_core.h"
 
+#include <algorithm>
+
 namespace orc {
 
 static Mat4 mul(const Mat4& a, const Mat4& b)
@@ -147,6 +149,105 @@ static void init_medium(Medium& m, V absorption, V emission, V scattering, float
     m.max_beam_length = m.has_attenuation ? -std::log(1e-9f) / maxc(m.attenuation) : kInf;
 }
 
+// ---- environment map: host/env_map.hpp:42-144, kernel/env_map.cuh:36-137, distribution.cuh ----
+static void init_dist(Dist1D& d, const float* f, int n)
+{
+    d.func.assign(f, f + n);
+    d.cdf.assign(size_t(n) + 1, 0.f);
+    for (int i = 1; i < n + 1; ++i) d.cdf[size_t(i)] = d.cdf[size_t(i - 1)] + d.func[size_t(i - 1)] / float(n);
+    d.integral = d.cdf[size_t(n)];
+    for (int i = 1; i < n + 1; ++i)
+        d.cdf[size_t(i)] = d.integral == 0.f ? float(i) / float(n) : d.cdf[size_t(i)] / d.integral;
+}
+
+void init_env_map(const upbp_env_map& in, EnvMap& m)
+{
+    m.width = int(in.width);
+    m.height = int(in.height);
+    m.img.resize(size_t(m.width) * size_t(m.height));
+    for (size_t i = 0; i < m.img.size(); i++) m.img[i] = fromp(in.rgb + 3 * i);
+    const int nv = m.height, nu = 2 * m.height; // height maps to pi, 2 * height to 2 pi
+    std::vector<float> row(static_cast<size_t>(nu)), marg(static_cast<size_t>(nv));
+    m.rows.resize(size_t(nv));
+    for (int r = 0; r < nv; ++r) {
+        const float v = (float(r) + .5f) / float(nv);
+        const float st = std::sin(kPi * v);
+        for (int c = 0; c < nu; ++c) {
+            const V p = m.img[size_t(r) * size_t(m.width) + size_t(c)];
+            row[size_t(c)] = st * (0.212671f * p.x + 0.715160f * p.y + 0.072169f * p.z);
+        }
+        init_dist(m.rows[size_t(r)], row.data(), nu);
+        marg[size_t(r)] = m.rows[size_t(r)].integral;
+    }
+    init_dist(m.marginal, marg.data(), nv);
+    m.norm = .5f * kInvPi * kInvPi;
+}
+
+static float sample_dist(const Dist1D& d, float u, float& pdf, int& off)
+{
+    const int n = int(d.func.size());
+    // last cdf entry <= u
+    off = int(std::upper_bound(d.cdf.begin(), d.cdf.end(), u) - d.cdf.begin()) - 1;
+    off = std::max(0, std::min(off, n - 1));
+    float du = u - d.cdf[size_t(off)];
+    if (d.cdf[size_t(off + 1)] - d.cdf[size_t(off)] > 0.f) du /= d.cdf[size_t(off + 1)] - d.cdf[size_t(off)];
+    pdf = d.integral > 0.f ? d.func[size_t(off)] / d.integral : 0.f;
+    return (float(off) + du) / float(n);
+}
+
+static V env_radiance(const EnvMap& m, float u, float v)
+{
+    const float xf = u * float(m.width), yf = v * float(m.height);
+    const int x1 = std::max(0, std::min(int(xf), m.width - 1)), y1 = std::max(0, std::min(int(yf), m.height - 1));
+    const int x2 = x1 == m.width - 1 ? x1 : x1 + 1, y2 = y1 == m.height - 1 ? y1 : y1 + 1;
+    const float tx = xf - float(x1), ty = yf - float(y1);
+    auto at = [&](int x, int y) { return m.img[size_t(y) * size_t(m.width) + size_t(x)]; };
+    return (1.f - ty) * ((1.f - tx) * at(x1, y1) + tx * at(x2, y1)) + ty * ((1.f - tx) * at(x1, y2) + tx * at(x2, y2));
+}
+
+static float env_sin_theta(float v, int height)
+{
+    const float h = float(height);
+    return v < 1.f ? std::sin(kPi * (float(int(v * h)) + .5f) / h) : std::sin(kPi * ((h - 1.f) + .5f) / h);
+}
+
+V env_sample(const EnvMap& m, float u0, float u1, float& pdf_w, V& radiance)
+{
+    float pv, pu;
+    int iv, iu;
+    const float v = sample_dist(m.marginal, u1, pv, iv);
+    const float u = sample_dist(m.rows[size_t(iv)], u0, pu, iu);
+    pdf_w = m.norm * (pu * pv) / env_sin_theta(v, m.height);
+    radiance = env_radiance(m, u, v);
+    const float phi = u * 2.f * kPi, theta = v * kPi, st = std::sin(theta);
+    return {-st * std::cos(phi), st * std::sin(phi), std::cos(theta)};
+}
+
+float env_pdf_uv(const EnvMap& m, float u, float v)
+{
+    const int nu = int(m.rows[0].func.size()), nv = int(m.marginal.func.size());
+    const int iu = std::max(0, std::min(int(u * float(nu)), nu - 1));
+    const int iv = std::max(0, std::min(int(v * float(nv)), nv - 1));
+    const Dist1D& row = m.rows[size_t(iv)];
+    if (row.integral * m.marginal.integral == 0.f) return 0.f;
+    return row.func[size_t(iu)] * m.marginal.func[size_t(iv)] / (row.integral * m.marginal.integral);
+}
+
+V env_lookup(const EnvMap& m, V direction, float* pdf_w)
+{
+    const V d = normalize(direction);
+    const float phi = (d.x != 0.f || d.y != 0.f) ? std::atan2(d.y, d.x) : 0.f;
+    const float theta = std::acos(d.z);
+    const float u = std::min(std::max(.5f - phi * .5f * kInvPi, 0.f), 1.f);
+    const float v = std::min(std::max(theta * kInvPi, 0.f), 1.f);
+    V rad = env_radiance(m, u, v);
+    if (pdf_w) {
+        *pdf_w = m.norm * env_pdf_uv(m, u, v) / env_sin_theta(v, m.height);
+        if (*pdf_w == 0.f) rad = v3(0.f);
+    }
+    return rad;
+}
+
 void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& s)
 {
     auto tris = [](const upbp_triangles& t, std::vector<Tri>& out) {
@@ -208,8 +309,10 @@ void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& 
         } else {
             o.bg_color = fromp(l.intensity);
             o.scale = l.scale;
+            o.uses_env = l.uses_env_map != 0 && in->env_map.rgb != nullptr;
         }
     }
+    if (in->env_map.rgb && in->env_map.width) init_env_map(in->env_map, s.env);
     s.global_medium = in->global_medium_idx;
     s.background_light = in->background_light_idx;
     const upbp_camera& c = in->camera;
@@ -540,9 +643,15 @@ V light_illuminate(const Light& l, const Scene& sc, V recv, float u0, float u1, 
         if (emission_pdf_w) *emission_pdf_w = l.inv_area * cn * kInvPi;
         return l.intensity;
     }
-    case L_BACKGROUND: { // :128-151 (constant colour only)
-        dir = sample_uniform_sphere_w(u0, u1, &for_pdf_w);
-        const V rad = l.bg_color * l.scale;
+    case L_BACKGROUND: { // :128-151
+        V rad;
+        if (l.uses_env) {
+            dir = env_sample(sc.env, u0, u1, for_pdf_w, rad);
+            rad = rad * l.scale;
+        } else {
+            dir = sample_uniform_sphere_w(u0, u1, &for_pdf_w);
+            rad = l.bg_color * l.scale;
+        }
         dist = kInf;
         if (emission_pdf_w) *emission_pdf_w = for_pdf_w * concentric_disc_pdf_a() * sc.inv_radius_sqr;
         if (cos_at_light) *cos_at_light = 1.f;
@@ -587,8 +696,14 @@ V light_emit(const Light& l, const Scene& sc, float d0, float d1, float p0, floa
     }
     case L_BACKGROUND: { // :153-190
         float fp;
-        dir = sample_uniform_sphere_w(d0, d1, &fp);
-        const V rad = l.bg_color * l.scale;
+        V rad;
+        if (l.uses_env) {
+            dir = -env_sample(sc.env, d0, d1, fp, rad);
+            rad = rad * l.scale;
+        } else {
+            dir = sample_uniform_sphere_w(d0, d1, &fp);
+            rad = l.bg_color * l.scale;
+        }
         float x, y;
         sample_concentric_disc(p0, p1, x, y);
         Frame f;
@@ -631,8 +746,14 @@ V light_radiance(const Light& l, const Scene& sc, V ray_dir, V hit, float* for_p
         return l.intensity;
     }
     case L_BACKGROUND: { // :192-217
-        const float fp = uniform_sphere_pdf_w();
-        const V rad = l.bg_color * l.scale;
+        float fp;
+        V rad;
+        if (l.uses_env) {
+            rad = l.scale * env_lookup(sc.env, ray_dir, &fp);
+        } else {
+            fp = uniform_sphere_pdf_w();
+            rad = l.bg_color * l.scale;
+        }
         const float pos_pdf = concentric_disc_pdf_a() * sc.inv_radius_sqr;
         if (for_pdf_a) *for_pdf_a = fp;
         if (emission_pdf_w) *emission_pdf_w = fp * pos_pdf;

@@ -160,6 +160,22 @@ struct Light {
     V position;
     V bg_color;
     float scale;
+    bool uses_env = false;
+};
+
+// piecewise-constant distribution over [0, 1) (host/env_map.hpp:42-68)
+struct Dist1D {
+    std::vector<float> func, cdf; // n, n + 1
+    float integral = 0.f;
+};
+
+// environment map: lat-long image and its luminance distribution (shared/env_map.h:34-71)
+struct EnvMap {
+    int width = 0, height = 0;
+    std::vector<V> img;          // rows from the top
+    std::vector<Dist1D> rows;    // conditional over u per image row
+    Dist1D marginal;             // over rows
+    float norm = 0.f;
 };
 
 struct Mat4 {
@@ -191,6 +207,7 @@ struct Sph {
 struct Scene {
     std::vector<Tri> real, imag;
     std::vector<Sph> sreal, simag;
+    EnvMap env; // width 0: none
     std::vector<Material> mats;
     std::vector<Medium> media;
     std::vector<Light> lights;
@@ -202,6 +219,11 @@ struct Scene {
 };
 
 void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& out);
+// environment map (host/env_map.hpp:42-144, kernel/env_map.cuh:36-137)
+void init_env_map(const upbp_env_map& in, EnvMap& m);
+V env_sample(const EnvMap& m, float u0, float u1, float& pdf_w, V& radiance);
+V env_lookup(const EnvMap& m, V direction, float* pdf_w);
+float env_pdf_uv(const EnvMap& m, float u, float v);
 // Ray against an analytic sphere (gpuupbp/kernel/intersector.cu:35-95): roots in (tmin, tmax),
 // near first, with outward normals; returns their number (0..2).
 int sphere_hit(const Sph& s, V ro, V rd, float tmin, float tmax, float* th, V* nh);

@@ -33,6 +33,7 @@ def lib():
                                          C.POINTER(C.c_uint64)]
         h.oracle_closest_hit.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
         h.oracle_sphere_hits.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
+        h.oracle_env_samples.argtypes = [vp, vp, C.c_uint32, vp]
         h.oracle_camera_ray.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_world_to_raster.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_hg_pdf.argtypes = [C.c_float, C.c_float]
@@ -104,6 +105,22 @@ def sphere_hits(spheres, rays):
     rays = np.ascontiguousarray(rays, np.float32)
     out = np.zeros((rays.shape[0], spheres.shape[0], 3), np.float32)
     lib().oracle_sphere_hits(_p(spheres), spheres.shape[0], _p(rays), rays.shape[0], _p(out))
+    return out
+
+
+class _EnvMap(C.Structure):  # upbp_env_map
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_float))]
+
+
+def env_samples(rgb, u):
+    """Environment map (H, W, 3) sampled at u (n, 2): (n, 11) as upbp_test_env_samples."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    env = _EnvMap(rgb.shape[1], rgb.shape[0], rgb.ctypes.data_as(C.POINTER(C.c_float)))
+    out = np.zeros((u.shape[0], 11), np.float32)
+    rc = lib().oracle_env_samples(C.byref(env), _p(u), u.shape[0], _p(out))
+    if rc != 0:
+        raise RuntimeError("oracle_env_samples failed: %d" % rc)
     return out
 
 

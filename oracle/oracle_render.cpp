@@ -2159,6 +2159,22 @@ extern "C" int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const fl
     return UPBP_OK;
 }
 
+extern "C" int oracle_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out)
+{
+    if (!env || !env->rgb || env->height == 0 || env->width < 2 * env->height) return UPBP_E_ARG;
+    EnvMap m;
+    init_env_map(*env, m);
+    for (uint32_t i = 0; i < n; i++) {
+        float pdf, lpdf;
+        V rad;
+        const V d = env_sample(m, u[2 * i], u[2 * i + 1], pdf, rad);
+        const V lrad = env_lookup(m, d, &lpdf);
+        const float vals[11] = {d.x, d.y, d.z, pdf, rad.x, rad.y, rad.z, lpdf, lrad.x, lrad.y, lrad.z};
+        memcpy(out + 11 * size_t(i), vals, sizeof(vals));
+    }
+    return UPBP_OK;
+}
+
 extern "C" int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, uint32_t n_rays, float* hit_t,
                                   int32_t* hit_prim)
 {

@@ -1,5 +1,8 @@
 import os
+import shutil
 import sys
+
+import numpy as np
 
 import pytest
 
@@ -35,3 +38,26 @@ def boxfog_obj(scene_dir):
 def box_args(res="64x64", alg="upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d", extra=()):
     """Command line of the box workload (box [-1,1]^3, medium sphere, l=8, r=0.01)."""
     return ["-r", res, "-l", "8", "-a", alg, "-r_initial", "0.01", "-r_alpha", "1"] + list(extra)
+
+
+def env_scene(box_obj, dst, env, scale=2.0, rotate=0.25):
+    """The box workload lit additionally by an environment map through its open front."""
+    os.makedirs(dst, exist_ok=True)
+    base = os.path.dirname(box_obj)
+    for f in os.listdir(base):
+        shutil.copy(os.path.join(base, f), dst)
+    obj = os.path.join(dst, os.path.basename(box_obj))
+    from gpuupbp_amd import renderer as gpu
+    gpu.save_exr(os.path.join(dst, "env.exr"), env)
+    with open(obj + ".aux", "a") as f:
+        f.write("\nlight_background_em %g %g env.exr\n" % (scale, rotate))
+    return obj
+
+
+def sky(h=16, w=32):
+    """Positive everywhere, a bright sun patch and a horizon gradient."""
+    v = (np.arange(h) + 0.5) / h
+    img = np.zeros((h, w, 3), np.float32)
+    img[...] = (0.2 + 0.6 * (1 - v))[:, None, None] * np.array([0.6, 0.8, 1.0], np.float32)
+    img[3:5, 20:23] = [40.0, 35.0, 25.0]
+    return img
