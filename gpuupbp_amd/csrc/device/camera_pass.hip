@@ -24,7 +24,7 @@ struct SegCount {
 template <int MODE>
 __device__ __forceinline__ SegCount segment_queries(const DScene& sc, const VolSeg* segs, const LiteSeg* lite, int n,
                                                     unsigned flags, uint32_t qbt, float piece_len, const Sub& s,
-                                                    uint32_t pix, SegRec* out, uint32_t rec_base, uint2* pout)
+                                                    uint32_t pix, SegRec* out, uint8_t* npl)
 {
     f3 att = mk(1.f);
     float rsf = 1.f, rsr = 1.f;
@@ -41,7 +41,7 @@ __device__ __forceinline__ SegCount segment_queries(const DScene& sc, const VolS
                 const SegQ q{s.origin, s.dir, dmin, dmax, mid, rsf, rsr, kEndInMedium | (i == 0 ? flags : 0u),
                              s.path_length};
                 store_segrec(out + c.recs, q, s.sw, s.throughput * att, pix);
-                for (uint32_t k = 0; k < np; k++) pout[c.pieces + k] = make_uint2(rec_base + c.recs, k | (np << 16));
+                npl[c.recs] = uint8_t(np);
             }
             c.recs++;
             c.pieces += np;
@@ -82,6 +82,20 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t c, uint32_t* counter)
     if (int(lane) == leader && tot) base = atomicAdd(counter, tot);
     base = __shfl(base, leader);
     return base + pre;
+}
+
+// Wave-aggregated allocation of one block for the whole wave (c < 2^BITS per lane): returns
+// the block's first slot and its size (the same in every lane).
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_alloc_block(uint32_t c, uint32_t* counter, uint32_t& tot)
+{
+    const uint32_t lane = __lane_id();
+    tot = 0;
+    for (int bit = 0; bit < BITS; bit++) tot += uint32_t(__popcll(__ballot((c >> bit) & 1u))) << bit;
+    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    uint32_t base = 0;
+    if (int(lane) == leader && tot) base = atomicAdd(counter, tot);
+    return __shfl(base, leader);
 }
 
 // kernel/functs_camera.cuh:41-103
@@ -162,19 +176,43 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         const VolSeg* sg = use_full ? segs : nullptr;
         const LiteSeg* lt = use_full ? nullptr : lite;
         const int ns = use_full ? nseg : nlite;
-        const SegCount c0 = segment_queries<0>(sc, sg, lt, ns, rf, p.qbt, 0.f, s, pix, nullptr, 0, nullptr);
+        const SegCount c0 = segment_queries<0>(sc, sg, lt, ns, rf, p.qbt, 0.f, s, pix, nullptr, nullptr);
         // piece length: the tuned length, stretched so that one ray makes at most kPiecesPerRay
         // pieces plus one per segment
         const float plen = p.piece_len > 0.f ? fmaxf(p.piece_len, c0.len / float(kPiecesPerRay)) : 0.f;
-        const SegCount c1 = segment_queries<1>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, nullptr, 0, nullptr);
+        const SegCount c1 = segment_queries<1>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, nullptr, nullptr);
         const uint32_t at = wave_alloc<4>(c1.recs, &b.qcount[128 + bounce]);
-        const uint32_t pat = wave_alloc<6>(c1.pieces, &b.qcount[320 + bounce]);
+        uint32_t ptot;
+        const uint32_t pbase = wave_alloc_block<6>(c1.pieces, &b.qcount[320 + bounce], ptot);
         if (c1.recs) {
-            if (at + c1.recs > b.seg_cap || pat + c1.pieces > b.piece_cap) {
+            // an overflowing wave writes no pieces at all (the merge kernels skip pieces whose
+            // record index is out of range)
+            if (__ballot(at + c1.recs > b.seg_cap) || pbase + ptot > b.piece_cap) {
                 ret = S_ERR_SEGMENTS;
                 return false;
             }
-            segment_queries<2>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, b.segq + at, at, b.pieces + pat);
+            uint8_t npl[16];
+            segment_queries<2>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, b.segq + at, npl);
+            // The wave's pieces go to its block piece-index-major: first every lane's first
+            // piece, then every lane's second, ... so that the merge kernel, which hands 64
+            // consecutive pieces to one wave, gives a wave the same stretch of 64 neighbouring
+            // rays instead of the whole length of one or two rays.
+            const uint32_t lane = __lane_id();
+            const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            uint32_t start = 0, rec = 0, k = 0;
+            for (uint32_t j = 0;; j++) {
+                const uint64_t mj = __ballot(c1.pieces > j);
+                if (!mj) break;
+                if (c1.pieces > j) {
+                    b.pieces[pbase + start + uint32_t(__popcll(mj & below))] =
+                        make_uint2(at + rec, k | (uint32_t(npl[rec]) << 16));
+                    if (++k == npl[rec]) {
+                        k = 0;
+                        rec++;
+                    }
+                }
+                start += uint32_t(__popcll(mj));
+            }
             cc.sq += c1.recs;
             cc.pieces += c1.pieces;
         }

@@ -28,16 +28,22 @@ constexpr int kMergeWaves = kMergeBlock / 64;
 constexpr uint32_t kCandCap = 320; // < 64 pending + up to 4 leaves per lane in one node step
 constexpr uint32_t kPairCap = 128; // < 64 pending + at most one accept per lane in a test batch
 
+// The piece of a candidate or stolen subtree travels as (k, n) packed in 16 bits (k | n << 6,
+// k < n <= 32); the camera range it owns is recomputed from the query record, which the
+// consumer loads anyway. Small entries keep the LDS of a 4-wave block under 32 KB, so that
+// five blocks fit a CU.
+__device__ __forceinline__ uint32_t pack_kn(uint32_t k, uint32_t n) { return k | (n << 6); }
+
 struct CandQueue {
     uint32_t q[kCandCap];    // segment query index
     uint32_t leaf[kCandCap]; // sorted LBVH leaf
-    float lo[kCandCap], hi[kCandCap]; // camera parameters the query piece owns
+    uint16_t kn[kCandCap];   // piece of the query
 };
 
 // subtrees handed from busy lanes to idle lanes of the same wave
 struct Steal {
     uint32_t q[64], node[64];
-    float lo[64], hi[64], ta[64], tb[64];
+    uint16_t kn[64];
 };
 
 struct PairQueue {
@@ -76,16 +82,14 @@ __device__ __forceinline__ void pq_push(PairQueue& Q, uint32_t& qn, bool keep, u
     qn += uint32_t(__popcll(m));
 }
 
-__device__ __forceinline__ void cq_push(CandQueue& C, uint32_t& cn, bool keep, uint32_t q, uint32_t leaf, float lo,
-                                        float hi)
+__device__ __forceinline__ void cq_push(CandQueue& C, uint32_t& cn, bool keep, uint32_t q, uint32_t leaf, uint32_t kn)
 {
     const uint64_t m = __ballot(keep);
     if (keep) {
         const uint32_t at = cn + uint32_t(__popcll(m & lanes_below()));
         C.q[at] = q;
         C.leaf[at] = leaf;
-        C.lo[at] = lo;
-        C.hi[at] = hi;
+        C.kn[at] = uint16_t(kn);
     }
     cn += uint32_t(__popcll(m));
 }
@@ -140,9 +144,11 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     const uint32_t lane = __lane_id();
     // B-B1D walks query pieces from its XCD's region first (then helps the others);
     // P-B2D (few candidates per query) walks whole segments
-    const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : min(b.qcount[128 + bounce], b.seg_cap);
+    const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
+    const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : n_recs;
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
     uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
+    const bool split = KIND == 1 && p.xcd_split;
     uint32_t src = blockIdx.x % kXcds, tried = 0; // work range being consumed (same in every lane)
     const WBvh& bvh = KIND == 0 ? b.maps.wmed : b.maps.wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
@@ -150,7 +156,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     uint32_t cn = 0, qn = 0; // candidates / pairs queued (same value in every lane)
     bool exhausted = false;  // no pieces left to take (same value in every lane)
     int cur = -1;            // query of this lane's current piece
-    float lo = 0.f, hi = 0.f, ta = 0.f, tb = 0.f;
+    uint32_t kn = 0;         // and which piece of it (pack_kn)
     RayBox rb = make_raybox(mk(0.f), mk(1.f), 0.f, 0.f);
     int node = 0, sb = 0, sp = 0; // pending subtrees: stack[sb, sp), oldest (largest) at sb
     int stack[kBvhStack];
@@ -166,15 +172,15 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 bool got = false;
                 while (!got && !exhausted) {
                     uint32_t base = 0;
-                    uint32_t* h = KIND == 1 ? &xw[208 + src] : head;
-                    const uint32_t rb0 = KIND == 1 ? xw[192 + src] : 0u, re0 = KIND == 1 ? xw[200 + src] : total;
+                    uint32_t* h = split ? &xw[208 + src] : head;
+                    const uint32_t rb0 = split ? xw[192 + src] : 0u, re0 = split ? xw[200 + src] : total;
                     if (int(lane) == leader) base = atomicAdd(h, uint32_t(__popcll(m)));
                     base = __shfl(base, leader);
                     if (rb0 + base < re0) {
                         got = true;
                         first = rb0 + base;
                         lim = re0;
-                    } else if (KIND == 1 && ++tried < uint32_t(kXcds)) {
+                    } else if (split && ++tried < uint32_t(kXcds)) {
                         src = (src + 1) % kXcds;
                     } else {
                         exhausted = true;
@@ -182,12 +188,16 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 }
                 if (got && need) {
                     const uint32_t wi = first + uint32_t(__popcll(m & lanes_below()));
-                    if (wi < lim) {
-                        const uint2 pc = KIND == 1 ? b.pieces_x[wi] : make_uint2(wi, 1u << 16);
+                    const uint2 pc = wi < lim ? (split ? b.pieces_x[wi] : KIND == 1 ? b.pieces[wi] : make_uint2(wi, 1u << 16))
+                                              : make_uint2(~0u, 0u);
+                    // a piece left unwritten by an overflowing camera wave is skipped
+                    if (pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16) && (pc.y >> 16) <= kPiecesPerRay) {
                         const SegRec* r = b.segq + pc.x;
                         const float4 a = r->a, bb = r->b;
+                        float lo, hi, ta, tb;
                         piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
                         rb = make_raybox(mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), ta, tb);
+                        kn = pack_kn(pc.y & 0xffffu, pc.y >> 16);
                         node = 0;
                         sb = sp = 0;
                         cur = int(pc.x);
@@ -204,10 +214,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                     if (k < t) {
                         S.q[k] = uint32_t(cur);
                         S.node[k] = uint32_t(stack[sb++]);
-                        S.lo[k] = lo;
-                        S.hi[k] = hi;
-                        S.ta[k] = ta;
-                        S.tb[k] = tb;
+                        S.kn[k] = uint16_t(kn);
                     }
                 }
                 wave_lds_sync();
@@ -216,12 +223,11 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                     if (k < t) {
                         cur = int(S.q[k]);
                         node = int(S.node[k]);
-                        lo = S.lo[k];
-                        hi = S.hi[k];
-                        ta = S.ta[k];
-                        tb = S.tb[k];
+                        kn = S.kn[k];
                         const SegRec* r = b.segq + cur;
                         const float4 a = r->a, bb = r->b;
+                        float lo, hi, ta, tb;
+                        piece_range(a.w, bb.w, kn & 63u, kn >> 6, lo, hi, ta, tb);
                         rb = make_raybox(mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), ta, tb);
                         sb = sp = 0;
                     }
@@ -281,7 +287,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
         }
         // all lanes take part in the queue appends (ballots over the whole wave)
 #pragma unroll
-        for (int k = 0; k < 4; k++) cq_push(C, cn, lf[k] >= 0, qnow, uint32_t(lf[k]), lo, hi);
+        for (int k = 0; k < 4; k++) cq_push(C, cn, lf[k] >= 0, qnow, uint32_t(lf[k]), kn);
         // Evaluate 64 accepted pairs at a time whenever 64 are queued (this keeps fewer than 64
         // pending before each candidate test), else test 64 candidates at a time. Once no lane
         // has work left, drain both queues: candidates first, then pairs.
@@ -329,10 +335,11 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 if (lane < n) {
                     const uint32_t e = cn - n + lane;
                     qi = C.q[e];
-                    const uint32_t leaf = C.leaf[e];
-                    const float clo = C.lo[e], chi = C.hi[e];
+                    const uint32_t leaf = C.leaf[e], ckn = C.kn[e];
                     const SegRec* r = b.segq + qi;
                     const float4 a = r->a, bb = r->b, d = r->d;
+                    float clo, chi, cta, ctb;
+                    piece_range(a.w, bb.w, ckn & 63u, ckn >> 6, clo, chi, cta, ctb);
                     acc = leaf_accept<KIND>(p, b.maps, leaf, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), a.w, bb.w, clo,
                                             chi, __float_as_int(d.z), slot, x, y, z, w);
                 }
@@ -396,8 +403,10 @@ __global__ void __launch_bounds__(256) k_piece_cells(Bufs b, int bounce)
     const uint32_t n = min(b.qcount[320 + bounce], b.piece_cap);
     uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
     const WNode root = b.maps.wbeam.nodes[0];
+    const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t c = piece_cell(b, root, b.pieces[i]);
+        const uint2 pc = b.pieces[i];
+        const uint32_t c = pc.x < n_recs ? piece_cell(b, root, pc) : 0u;
         b.piece_cell[i] = uint8_t(c);
         atomicAdd(&hist[c], 1u);
     }
@@ -467,9 +476,11 @@ void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int 
     if (kind == 0 && p.build_pb2d && b.maps.med.n > 0)
         hipLaunchKernelGGL(k_seg_merge<0>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
     if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0) {
-        hipLaunchKernelGGL(k_piece_cells, dim3(1024), dim3(256), 0, s, b, bounce);
-        hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
-        hipLaunchKernelGGL(k_piece_scatter, dim3(1024), dim3(256), 0, s, b, bounce);
+        if (p.xcd_split) {
+            hipLaunchKernelGGL(k_piece_cells, dim3(1024), dim3(256), 0, s, b, bounce);
+            hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
+            hipLaunchKernelGGL(k_piece_scatter, dim3(1024), dim3(256), 0, s, b, bounce);
+        }
         hipLaunchKernelGGL(k_seg_merge<1>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
     }
 }

@@ -363,6 +363,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.sub_len = r->tune.subbeam_factor > 0.f ? r->tune.subbeam_factor * p.rad_beam : 0.f;
     p.box_slack = 2e-4f * r->scene_mag;
     p.merge_bpc = r->tune.merge_blocks_per_cu;
+    p.xcd_split = r->tune.xcd_split;
     {
         float rq = 0.f;
         if (p.do_pb2d) rq = fmaxf(rq, r_pb2d);
@@ -844,7 +845,8 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
 extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
-    if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0)
+    if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
+        t->xcd_split < 0 || t->xcd_split > 1)
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     return UPBP_OK;

@@ -23,7 +23,10 @@ def test_env_sampling_matches_oracle():
     same_cell = np.abs(o[:, 7] - o[:, 3]) <= 1e-4 * o[:, 3]
     assert same_cell.mean() > 0.97
     assert np.allclose(g[same_cell, 7], o[same_cell, 7], rtol=1e-5)
-    assert np.allclose(g[same_cell, 8:], o[same_cell, 8:], rtol=1e-5, atol=1e-6)
+    # the lookup re-derives (u, v) through acos / atan2 of a direction from sin / cos: an ulp in
+    # (u, v) moves the bilinear weights, and next to the sun patch (40 vs 0.5 per pixel) that is
+    # a relative change of a few 1e-4
+    assert np.allclose(g[same_cell, 8:], o[same_cell, 8:], rtol=1e-3, atol=1e-4)
 
 
 def test_env_lit_scene_path_parity(box_obj, tmp_path):
