@@ -91,7 +91,7 @@ class Scene(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("subbeam_factor", C.c_float), ("query_piece_factor", C.c_float),
-                ("merge_blocks_per_cu", C.c_int32), ("xcd_split", C.c_int32)]
+                ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32)]
 
 
 class IterStats(C.Structure):

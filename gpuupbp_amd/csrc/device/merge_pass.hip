@@ -148,7 +148,8 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : n_recs;
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
     uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
-    const bool split = KIND == 1 && p.xcd_split;
+    const bool split = KIND == 1 && p.piece_order == 1;
+    const uint2* plist = KIND == 1 && p.piece_order != 0 ? b.pieces_x : b.pieces;
     uint32_t src = blockIdx.x % kXcds, tried = 0; // work range being consumed (same in every lane)
     const WBvh& bvh = KIND == 0 ? b.maps.wmed : b.maps.wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
@@ -188,8 +189,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 }
                 if (got && need) {
                     const uint32_t wi = first + uint32_t(__popcll(m & lanes_below()));
-                    const uint2 pc = wi < lim ? (split ? b.pieces_x[wi] : KIND == 1 ? b.pieces[wi] : make_uint2(wi, 1u << 16))
-                                              : make_uint2(~0u, 0u);
+                    const uint2 pc = wi < lim ? (KIND == 1 ? plist[wi] : make_uint2(wi, 1u << 16)) : make_uint2(~0u, 0u);
                     // a piece left unwritten by an overflowing camera wave is skipped
                     if (pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16) && (pc.y >> 16) <= kPiecesPerRay) {
                         const SegRec* r = b.segq + pc.x;
@@ -470,13 +470,68 @@ int merge_grid_blocks(int bpc)
     return cus * (bpc > 0 ? std::min(bpc, per) : per);
 }
 
+// ---- Morton order of the B-B1D query pieces --------------------------------------------
+__device__ __forceinline__ uint32_t spread_bits10(uint32_t v)
+{
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_piece_keys(Bufs b, int bounce, uint32_t n, uint32_t* keys, uint32_t* vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
+    const uint2 pc = b.pieces[i];
+    uint32_t key = 0;
+    if (pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16)) {
+        const WNode root = b.maps.wbeam.nodes[0];
+        const SegRec* r = b.segq + pc.x;
+        const float4 a = r->a, bb = r->b;
+        float lo, hi, ta, tb;
+        piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
+        const float t = 0.5f * (ta + tb);
+        const float m[3] = {a.x + bb.x * t, a.y + bb.y * t, a.z + bb.z * t};
+        const float o[3] = {root.ox, root.oy, root.oz};
+        uint32_t q[3];
+        for (int k = 0; k < 3; k++) {
+            const float ext = 256.f * wnode_step(root.meta, k);
+            const float u = ext > 0.f ? (m[k] - o[k]) / ext : 0.f;
+            q[k] = uint32_t(fminf(fmaxf(u * 1024.f, 0.f), 1023.f));
+        }
+        key = (spread_bits10(q[0]) << 2) | (spread_bits10(q[1]) << 1) | spread_bits10(q[2]);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+__global__ void __launch_bounds__(256) k_piece_gather(Bufs b, uint32_t n, const uint32_t* vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b.pieces_x[i] = b.pieces[vals[i]];
+}
+
+void launch_piece_keys(const Bufs& b, int bounce, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s)
+{
+    if (n) hipLaunchKernelGGL(k_piece_keys, dim3((n + 255) / 256), dim3(256), 0, s, b, bounce, n, keys, vals);
+}
+
+void launch_piece_gather(const Bufs& b, uint32_t n, const uint32_t* vals, hipStream_t s)
+{
+    if (n) hipLaunchKernelGGL(k_piece_gather, dim3((n + 255) / 256), dim3(256), 0, s, b, n, vals);
+}
+
 void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int kind, int bounce, hipStream_t s)
 {
     const int blocks = merge_grid_blocks(p.merge_bpc);
     if (kind == 0 && p.build_pb2d && b.maps.med.n > 0)
         hipLaunchKernelGGL(k_seg_merge<0>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
     if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0) {
-        if (p.xcd_split) {
+        if (p.piece_order == 1) {
             hipLaunchKernelGGL(k_piece_cells, dim3(1024), dim3(256), 0, s, b, bounce);
             hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
             hipLaunchKernelGGL(k_piece_scatter, dim3(1024), dim3(256), 0, s, b, bounce);

@@ -31,7 +31,7 @@ struct IterParams {
     float piece_len; // target length of one camera-segment query piece
     float box_slack; // extra beam-box inflation covering FP32 error of the beam predicates
     int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
-    int xcd_split;   // B-B1D: region-bucketed pieces, one region per XCD first
+    int piece_order; // B-B1D work order: 0 as written, 1 XCD regions, 2 Morton-sorted (pieces_x)
     uint64_t iter_key;
 };
 

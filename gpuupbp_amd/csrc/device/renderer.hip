@@ -167,7 +167,7 @@ struct upbp_renderer {
     Bufs bufs;
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
-    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork;
+    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals;
     BvhStore bvh_s, bvh_m, bvh_b;
     WBvhStore wbvh_m, wbvh_b;
     float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
@@ -363,7 +363,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.sub_len = r->tune.subbeam_factor > 0.f ? r->tune.subbeam_factor * p.rad_beam : 0.f;
     p.box_slack = 2e-4f * r->scene_mag;
     p.merge_bpc = r->tune.merge_blocks_per_cu;
-    p.xcd_split = r->tune.xcd_split;
+    p.piece_order = r->tune.piece_order;
     {
         float rq = 0.f;
         if (p.do_pb2d) rq = fmaxf(rq, r_pb2d);
@@ -735,6 +735,19 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
             if (timed) HIPCHK(hipEventRecord(e[1], s));
             launch_seg_merge(r->sc, p, b, 0, k, s);
             if (timed) HIPCHK(hipEventRecord(e[2], s));
+            if (p.piece_order == 2 && p.build_bb1d && b.maps.beam.n > 0) {
+                // Morton order of the pieces: count to the host, sort (key, index), gather
+                uint32_t cnt = 0;
+                HIPCHK(hipMemcpyAsync(&cnt, b.qcount + 320 + k, 4, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                const uint32_t n = std::min(cnt, b.piece_cap);
+                if (r->pkeys.ensure(size_t(b.piece_cap) * 4) || r->pvals.ensure(size_t(b.piece_cap) * 4))
+                    return fail(r, "device allocation failed", UPBP_E_HIP);
+                launch_piece_keys(b, k, n, r->pkeys.as<uint32_t>(), r->pvals.as<uint32_t>(), s);
+                if (radix_sort_pairs(r->pkeys.as<uint32_t>(), r->pvals.as<uint32_t>(), n, 30, r->scratch, s))
+                    return fail(r, "piece sort failed", UPBP_E_HIP);
+                launch_piece_gather(b, n, r->pvals.as<uint32_t>(), s);
+            }
             launch_seg_merge(r->sc, p, b, 1, k, s);
             if (timed) HIPCHK(hipEventRecord(e[3], s));
         }
@@ -846,7 +859,7 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
     if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
-        t->xcd_split < 0 || t->xcd_split > 1)
+        t->piece_order < 0 || t->piece_order > 2)
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     return UPBP_OK;

@@ -269,8 +269,10 @@ typedef struct upbp_tuning {
                                  segment; 0 keeps whole segments (default 64) */
     int32_t merge_blocks_per_cu; /* resident blocks per CU of the segment merge kernels; 0 = as
                                     many as fit (default) */
-    int32_t xcd_split;           /* 1: B-B1D query pieces are bucketed by region and each XCD
-                                    starts on its own region; 0: one global work list (default) */
+    int32_t piece_order;         /* order in which the B-B1D merge takes query pieces: 0 as the
+                                    camera pass wrote them (default); 1 bucketed by region, each
+                                    XCD starting on its own; 2 sorted by the Morton code of the
+                                    piece midpoint (one host synchronisation per bounce) */
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
