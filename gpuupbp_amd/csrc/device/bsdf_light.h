@@ -8,6 +8,7 @@
 #pragma once
 
 #include "env_map.h"
+#include "guide.h"
 #include "upbp_device.h"
 
 namespace upbp {
@@ -24,8 +25,9 @@ struct Bsdf {
     f3 geom_normal; // local
     float cos_fix_geom;
     float diff_prob, phong_prob, refl_prob, refr_prob, reflect_coeff, ior;
-    // medium: incoming ray direction (world_dir_fix = -wdir, frame built from wdir)
-    f3 wdir;
+    // medium: incoming ray direction (world_dir_fix = -wdir, frame built from wdir) and the
+    // scattering position (for the learned proposal's record lookup)
+    f3 wdir, pos;
 
     UPBP_HD bool valid() const { return flags & VALID; }
     UPBP_HD bool delta() const { return flags & DELTA; }
@@ -84,7 +86,7 @@ UPBP_HD void bsdf_setup(Bsdf& b, f3 ray_dir, const Hit& h, const DScene& sc, boo
     b.diff_prob = b.phong_prob = b.refl_prob = b.refr_prob = b.reflect_coeff = 0.f;
     b.cos_fix_geom = 0.f;
     b.ior = -1.f;
-    b.local_dir_fix = b.geom_normal = b.wdir = mk(0.f);
+    b.local_dir_fix = b.geom_normal = b.wdir = b.pos = mk(0.f);
     if (h.on_surface()) {
         const DMaterial& m = sc.mats[h.mat];
         b.id = h.mat;
@@ -140,6 +142,7 @@ UPBP_HD void bsdf_restore_medium(Bsdf& b, f3 ray_dir, int med, float cont_prob, 
     b.diff_prob = b.phong_prob = b.refl_prob = b.refr_prob = b.reflect_coeff = 0.f;
     b.ior = -1.f;
     b.cont_prob = cont_prob;
+    b.pos = mk(0.f);
 }
 
 // kernel/bsdf.cuh:227-284
@@ -215,8 +218,16 @@ UPBP_HD f3 bsdf_evaluate(const Bsdf& b, f3 dir_gen, const DScene& sc, float& cos
         return mk(0.f);
     }
     const float p = hg_pdf(-b.wdir, dir_gen, m.mean_cosine, sin_theta);
-    if (for_pdf) *for_pdf = p;
-    if (rev_pdf) *rev_pdf = p;
+    // the camera technique samples the learned mixture: its pdf is the forward one at a camera
+    // vertex and the reverse one at a light vertex (arriving along -dir_gen, leaving along -wdir)
+    float fp = p, rp = p;
+    if (sc.guide.alpha > 0.f) {
+        const float a = sc.guide.alpha;
+        if (!b.from_light()) fp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, b.wdir), dir_gen) + (1.f - a) * p;
+        else rp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), -b.wdir) + (1.f - a) * p;
+    }
+    if (for_pdf) *for_pdf = fp;
+    if (rev_pdf) *rev_pdf = rp;
     return m.scattering * p;
 }
 
@@ -234,7 +245,10 @@ UPBP_HD float bsdf_pdf_rev(const Bsdf& b, f3 dir_gen, const DScene& sc)
         pdf_phong(b, sc.mats[b.id], lg, &fp, &rp);
         return rp;
     }
-    return hg_pdf(-b.wdir, dir_gen, sc.media[b.id].mean_cosine, nullptr);
+    const float p = hg_pdf(-b.wdir, dir_gen, sc.media[b.id].mean_cosine, nullptr);
+    if (!(sc.guide.alpha > 0.f) || !b.from_light()) return p; // light technique: the phase function
+    const float a = sc.guide.alpha;
+    return a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), -b.wdir) + (1.f - a) * p;
 }
 
 // kernel/bsdf.cuh:567-660 with the component samplers of :455-559
@@ -322,8 +336,23 @@ UPBP_HD f3 bsdf_sample(const Bsdf& b, float u0, float u1, float u2, f3& dir_gen,
         sin_theta = 0.f;
         return mk(0.f);
     }
-    dir_gen = hg_sample(-b.wdir, m.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
-    return m.scattering * pdf;
+    const DGuide& g = sc.guide;
+    if (!(g.alpha > 0.f) || b.from_light()) {
+        dir_gen = hg_sample(-b.wdir, m.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
+        return m.scattering * pdf;
+    }
+    // camera path: one-sample mixture of the learned lobes (probability alpha, chosen by the
+    // otherwise unused component variate u2) and the phase function
+    const float4* rec = guide_record(g, b.pos, b.wdir);
+    float hg;
+    if (u2 < g.alpha) {
+        dir_gen = vmf_mix_sample(rec, u2 / g.alpha, u0, u1);
+        hg = hg_pdf(-b.wdir, dir_gen, m.mean_cosine, &sin_theta);
+    } else {
+        dir_gen = hg_sample(-b.wdir, m.mean_cosine, u0, u1, b.frame, hg, &sin_theta);
+    }
+    pdf = g.alpha * vmf_mix_pdf(rec, dir_gen) + (1.f - g.alpha) * hg;
+    return m.scattering * hg;
 }
 
 // ---- lights: kernel/light.cuh ----------------------------------------------------

@@ -249,6 +249,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     float for_ratio = 0.f, rev_ratio = 0.f;
     if (bs.in_medium()) for_ratio = rev_ratio = 1.f / sc.media[bs.id].min_pos_att;
     const f3 hitp = s.origin + s.dir * is.dist;
+    bs.pos = hitp;
     pflags = is.in_medium() ? (pflags | PF_ORIGIN_IN_MEDIUM) : (pflags & ~PF_ORIGIN_IN_MEDIUM);
     {
         const float d = bs.d_factor();

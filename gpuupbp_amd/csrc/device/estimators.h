@@ -189,9 +189,12 @@ __device__ __forceinline__ void eval_pb2d(const DScene& sc, const IterParams& p,
     }
     const f3 pdir = lv.bsdf.world_dir_fix();
     float sin_theta;
-    float fpdf = hg_pdf(-q.rd, pdir, m.mean_cosine, &sin_theta);
-    float rpdf = fpdf;
-    const f3 f = mk(fpdf);
+    const float hg = hg_pdf(-q.rd, pdir, m.mean_cosine, &sin_theta);
+    // camera technique (learned mixture when enabled) toward the light side; light technique
+    // (the phase function) toward the camera side
+    float fpdf = cam_medium_pdf(sc, m, q.ro + q.rd * idist, q.rd, pdir, nullptr);
+    float rpdf = hg;
+    const f3 f = mk(hg);
     if (black_or_neg(f)) return;
     fpdf *= m.cont_prob;
     rpdf *= lv.bsdf.cont_prob;
@@ -258,9 +261,10 @@ __device__ __forceinline__ void eval_bb1d(const DScene& sc, const IterParams& p,
         wl.d_pde = lb.sw.d_pde_a * fi + lb.sw.d_pde_b;
         wl.d_bpt = lb.sw.d_bpt_a * fi + lb.sw.d_bpt_b;
     }
-    float fpdf = hg_pdf(-q.rd, -lb.dir, m.mean_cosine, nullptr);
-    float rpdf = fpdf;
-    const f3 f = mk(fpdf);
+    const float hg = hg_pdf(-q.rd, -lb.dir, m.mean_cosine, nullptr);
+    float fpdf = cam_medium_pdf(sc, m, q.ro + q.rd * tc, q.rd, -lb.dir, nullptr);
+    float rpdf = hg;
+    const f3 f = mk(hg);
     if (black_or_neg(f)) return;
     fpdf *= m.cont_prob;
     rpdf *= m.cont_prob;
@@ -321,9 +325,10 @@ __device__ __forceinline__ void eval_bp2d(const DScene& sc, const IterParams& p,
     }
     const f3 cdir = bc.world_dir_fix();
     float sin_theta;
-    float fpdf = hg_pdf(cdir, -lb.dir, m.mean_cosine, &sin_theta);
-    float rpdf = fpdf;
-    const f3 f = mk(fpdf);
+    const float hg = hg_pdf(cdir, -lb.dir, m.mean_cosine, &sin_theta);
+    float fpdf = cam_medium_pdf(sc, m, bc.pos, -cdir, -lb.dir, nullptr);
+    float rpdf = hg;
+    const f3 f = mk(hg);
     if (black_or_neg(f)) return;
     fpdf *= bc.cont_prob;
     rpdf *= m.cont_prob;

@@ -201,6 +201,7 @@ __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p
     float for_ratio = 0.f, rev_ratio = 0.f;
     if (bs.in_medium()) for_ratio = rev_ratio = 1.f / sc.media[bs.id].min_pos_att;
     const f3 hitp = s.origin + s.dir * is.dist;
+    bs.pos = hitp;
     const bool now_in_medium = is.in_medium();
     pflags = now_in_medium ? (pflags | PF_ORIGIN_IN_MEDIUM) : (pflags & ~PF_ORIGIN_IN_MEDIUM);
     {

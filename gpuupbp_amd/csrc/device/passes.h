@@ -211,6 +211,7 @@ __device__ __forceinline__ void load_vertex(const VertexRec* v, LVertex& x)
     const int id = __float_as_int(g.y);
     if (fl & Bsdf::IN_MEDIUM) bsdf_restore_medium(x.bsdf, src, id, d.w, fl & 15);
     else bsdf_restore_surface(x.bsdf, src, mk(c.y, c.z, c.w), mk(d.x, d.y, d.z), id, e.x, e.y, d.w, fl & 15);
+    x.bsdf.pos = x.hit;
     x.w.d_shared = e.z;
     x.w.d_bpt = e.w;
     x.w.d_pde = f.x;

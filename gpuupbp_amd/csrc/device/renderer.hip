@@ -168,6 +168,7 @@ struct upbp_renderer {
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
     DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals;
+    DevBuf guide; // learned proposal records
     BvhStore bvh_s, bvh_m, bvh_b;
     WBvhStore wbvh_m, wbvh_b;
     float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
@@ -853,6 +854,33 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     t->subbeam_factor = 12.f;
     t->query_piece_factor = 64.f;
     t->merge_blocks_per_cu = 0;
+}
+
+extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
+                                    float alpha)
+{
+    if (!r) return UPBP_E_ARG;
+    if (!(alpha >= 0.f && alpha <= 1.f)) return fail(r, "guide alpha must lie in [0, 1]", UPBP_E_ARG);
+    if (alpha == 0.f) {
+        r->sc.guide.alpha = 0.f;
+        return UPBP_OK;
+    }
+    if (!lobes || !lo || !hi || res < 1 || res > 256) return fail(r, "invalid guide grid", UPBP_E_ARG);
+    for (int a = 0; a < 3; a++)
+        if (!(hi[a] > lo[a])) return fail(r, "empty guide bounds", UPBP_E_ARG);
+    const size_t n = size_t(res) * res * res * kGuideBins;
+    std::vector<float4> recs(n * kGuideRec);
+    for (size_t i = 0; i < n; i++) pack_guide_record(lobes + i * kGuideLobes * 5, &recs[i * kGuideRec]);
+    if (r->guide.ensure(recs.size() * sizeof(float4)) ||
+        hipMemcpy(r->guide.p, recs.data(), recs.size() * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(r, "guide upload failed", UPBP_E_HIP);
+    DGuide& g = r->sc.guide;
+    g.recs = r->guide.as<float4>();
+    g.res = res;
+    g.lo = F3(lo);
+    g.inv_cell = mk(float(res) / (hi[0] - lo[0]), float(res) / (hi[1] - lo[1]), float(res) / (hi[2] - lo[2]));
+    g.alpha = alpha;
+    return UPBP_OK;
 }
 
 extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)

@@ -416,6 +416,38 @@ extern "C" int upbp_test_env_samples(const upbp_env_map* env, const float* u, ui
     return rc ? UPBP_E_HIP : UPBP_OK;
 }
 
+__global__ void k_guide_samples(const float4* rec, const float* u, uint32_t n, float* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 d = vmf_mix_sample(rec, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+    out[4 * i] = d.x;
+    out[4 * i + 1] = d.y;
+    out[4 * i + 2] = d.z;
+    out[4 * i + 3] = vmf_mix_pdf(rec, d);
+}
+
+extern "C" int upbp_test_guide_samples(const float* lobes, const float* u, uint32_t n, float* out)
+{
+    if (!lobes || (n && (!u || !out))) return UPBP_E_ARG;
+    if (n == 0) return UPBP_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
+    float4 rec[kGuideRec];
+    pack_guide_record(lobes, rec);
+    Tmp t;
+    float4* drec = t.alloc<float4>(kGuideRec);
+    float* du = t.alloc<float>(3 * size_t(n));
+    float* dout = t.alloc<float>(4 * size_t(n));
+    if (!drec || !du || !dout) return UPBP_E_HIP;
+    (void)hipMemcpy(drec, rec, sizeof(rec), hipMemcpyHostToDevice);
+    (void)hipMemcpy(du, u, 3 * size_t(n) * 4, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_guide_samples, dim3((n + 255) / 256), dim3(256), 0, 0, drec, du, n, dout);
+    int rc = int(hipDeviceSynchronize());
+    if (!rc) rc = int(hipMemcpy(out, dout, 4 * size_t(n) * 4, hipMemcpyDeviceToHost));
+    return rc ? UPBP_E_HIP : UPBP_OK;
+}
+
 extern "C" int upbp_test_copy(uint64_t bytes, int reps, double* ms_per_copy)
 {
     if (bytes < 16 || reps < 1 || !ms_per_copy) return UPBP_E_ARG;

@@ -240,6 +240,16 @@ struct DEnvMap {
     float norm;                // 1 / (2 pi^2): (u, v) density to solid angle, before the 1 / sin(theta)
 };
 
+// learned proposal of camera paths in media (guide.h): per (grid cell, arrival bin) record
+constexpr int kGuideLobes = 8, kGuideBins = 6, kGuideRec = 12;
+constexpr float kGuideKappaMin = 1e-2f, kGuideKappaMax = 1e4f; // lobe concentration range
+struct DGuide {
+    const float4* recs; // res^3 * kGuideBins records of kGuideRec float4
+    f3 lo, inv_cell;    // grid origin, cells per unit length per axis
+    int res;            // cells per axis
+    float alpha;        // probability of sampling the mixture at camera medium vertices (0: off)
+};
+
 struct DScene {
     const DMaterial* mats;
     const DMedium* media;
@@ -252,6 +262,7 @@ struct DScene {
     TriSet real, imag;
     SphSet sph_real, sph_imag;
     DEnvMap env; // img == nullptr: no environment map
+    DGuide guide;
     int use_shading_normal;
 };
 

@@ -34,6 +34,15 @@ class Renderer:
         self.iterations += 1
         return st
 
+    def set_guide_lobes(self, lobes, res, lo, hi, alpha):
+        """Install explicit learned-proposal lobes ((res^3 * 6, 8, 5): mu.xyz, kappa, weight)."""
+        lobes = np.ascontiguousarray(lobes, np.float32)
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        rc = capi.lib().upbp_set_guide_lobes(self.h, _p(lobes), res, _p(lo), _p(hi), alpha)
+        if rc != 0:
+            raise RuntimeError("upbp_set_guide_lobes failed: %s" % capi.lib().upbp_last_error(self.h).decode())
+
     def set_tuning(self, **kw):
         """Speed knobs of the C ABI's upbp_tuning (subbeam_factor, query_piece_factor,
         merge_blocks_per_cu, piece_order); unnamed fields keep their defaults."""
@@ -118,6 +127,17 @@ def env_samples(rgb, u):
     rc = capi.lib().upbp_test_env_samples(C.byref(env), _p(u), u.shape[0], _p(out))
     if rc != 0:
         raise RuntimeError("upbp_test_env_samples failed: %d" % rc)
+    return out
+
+
+def guide_samples(lobes, u):
+    """GPU sampling of one learned-proposal mixture (8, 5) at u (n, 3): (n, 4) direction, pdf."""
+    lobes = np.ascontiguousarray(lobes, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.zeros((u.shape[0], 4), np.float32)
+    rc = capi.lib().upbp_test_guide_samples(_p(lobes), _p(u), u.shape[0], _p(out))
+    if rc != 0:
+        raise RuntimeError("upbp_test_guide_samples failed: %d" % rc)
     return out
 
 

@@ -277,6 +277,18 @@ typedef struct upbp_tuning {
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
 
+/* ---- learned scattering proposal (no counterpart in the reference) -------
+ * Camera paths in media sample, with probability alpha, a mixture of eight von Mises-Fisher
+ * lobes looked up on a res^3 grid over [lo, hi] with six bins of the arrival direction (the
+ * dominant axis and its sign, bin = 2 * axis + (component < 0)), else the phase function. The
+ * mixture pdf enters every camera-technique pdf (forward at camera vertices, reverse at light
+ * vertices, connections and merges), so the estimator stays unbiased and MIS consistent.
+ * upbp_set_guide_lobes installs explicit lobes: res^3 * 6 records, record ((z * res + y) * res
+ * + x) * 6 + bin, of 8 lobes x (mu.xyz, kappa, weight); mu is normalised, kappa clamped to
+ * [1e-2, 1e4], weights normalised. alpha 0 switches the proposal off (the plain renderer). */
+int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
+                         float alpha);
+
 /* ---- test hooks (parity tests and autotuning only) ----------------------
  * Range queries over a device LBVH, returning the accepted index set of every
  * query in CSR form (offsets[n_queries+1], indices). They replace the OptiX
@@ -317,6 +329,11 @@ int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const float* ray
  * out: 11 floats per sample: sampled direction (3), its solid-angle pdf, radiance (3), then the
  * lookup of that direction: pdf, radiance (3). */
 int upbp_test_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out);
+
+/* One learned-proposal mixture (8 x (mu.xyz, kappa, weight), packed as upbp_set_guide_lobes
+ * packs it) sampled with u (3 per sample: lobe variate, two direction variates): out 4 per
+ * sample, the direction and the mixture pdf there. */
+int upbp_test_guide_samples(const float* lobes, const float* u, uint32_t n, float* out);
 
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM

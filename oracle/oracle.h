@@ -55,6 +55,14 @@ int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, u
 /* Analytic sphere test of every (ray, sphere) pair, same layout as upbp_test_sphere_hits. */
 int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out);
 
+/* One vMF mixture (8 x (mu.xyz, kappa, weight)) sampled with u (3 per sample: lobe, two for
+ * the direction): out 4 per sample (direction, mixture pdf). */
+int oracle_guide_samples(const float* lobes, const float* u, uint32_t n, float* out);
+
+/* Learned proposal for every later oracle render (same arguments as upbp_set_guide_lobes;
+ * alpha 0 switches it off). */
+int oracle_set_guide_lobes(const float* lobes, int32_t res, const float* lo, const float* hi, float alpha);
+
 /* Environment-map sampling and lookup, same layout as upbp_test_env_samples. */
 int oracle_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out);
 

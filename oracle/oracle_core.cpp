@@ -248,8 +248,11 @@ V env_lookup(const EnvMap& m, V direction, float* pdf_w)
     return rad;
 }
 
+Guide g_oracle_guide; // installed by oracle_set_guide_lobes, copied into every prepared scene
+
 void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& s)
 {
+    s.guide = g_oracle_guide;
     auto tris = [](const upbp_triangles& t, std::vector<Tri>& out) {
         out.resize(t.count);
         for (uint32_t i = 0; i < t.count; i++) {
@@ -462,6 +465,80 @@ static void pdf_phong(const BSDF& b, const Material& m, V lg, float* fp, float* 
     if (rp) *rp += pdf;
 }
 
+// ---- learned proposal (no reference counterpart) ------------------------------------------
+void init_guide(Guide& g, const float* lobes, int res, const float* lo, const float* hi, float alpha)
+{
+    g = Guide();
+    g.alpha = alpha;
+    if (!(alpha > 0.f)) return;
+    g.res = res;
+    g.lo = fromp(lo);
+    g.inv_cell = {float(res) / (hi[0] - lo[0]), float(res) / (hi[1] - lo[1]), float(res) / (hi[2] - lo[2])};
+    const size_t n = size_t(res) * res * res * 6;
+    g.lobes.resize(n * 8);
+    for (size_t r = 0; r < n; r++) {
+        const float* in = lobes + r * 40;
+        float wsum = 0.f;
+        for (int i = 0; i < 8; i++) wsum += std::fmax(in[5 * i + 4], 0.f);
+        for (int i = 0; i < 8; i++) {
+            const float* l = in + 5 * i;
+            GuideLobe& o = g.lobes[r * 8 + size_t(i)];
+            const float len = std::sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+            o.mu = len > 0.f ? V{l[0] / len, l[1] / len, l[2] / len} : V{0.f, 0.f, 1.f};
+            o.kappa = std::fmin(std::fmax(l[3], 1e-2f), 1e4f);
+            o.w = wsum > 0.f ? std::fmax(l[4], 0.f) / wsum : 1.f / 8.f;
+            o.c = o.w * o.kappa / (2.f * kPi * -std::expm1(-2.f * o.kappa));
+        }
+    }
+}
+
+static int axis_cell(float f, int r) { return f < 0.f ? 0 : (f >= float(r) ? r - 1 : int(f)); }
+
+const GuideLobe* guide_record(const Guide& g, V p, V d)
+{
+    const int r = g.res;
+    const int ix = axis_cell((p.x - g.lo.x) * g.inv_cell.x, r), iy = axis_cell((p.y - g.lo.y) * g.inv_cell.y, r),
+              iz = axis_cell((p.z - g.lo.z) * g.inv_cell.z, r);
+    const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    const int bin = (ax >= ay && ax >= az) ? (d.x < 0.f ? 1 : 0) : (ay >= az ? (d.y < 0.f ? 3 : 2) : (d.z < 0.f ? 5 : 4));
+    return &g.lobes[(size_t((iz * r + iy) * r + ix) * 6 + size_t(bin)) * 8];
+}
+
+float vmf_mix_pdf(const GuideLobe* rec, V w)
+{
+    float pdf = 0.f;
+    for (int i = 0; i < 8; i++) pdf += rec[i].c * std::exp(rec[i].kappa * (dot(rec[i].mu, w) - 1.f));
+    return pdf;
+}
+
+// Jakob 2012, "Numerically stable sampling of the von Mises Fisher distribution on S2"
+V vmf_mix_sample(const GuideLobe* rec, float us, float u0, float u1)
+{
+    int pick = 7;
+    float acc = 0.f;
+    for (int i = 0; i < 8; i++) {
+        acc += rec[i].w;
+        if (us < acc) {
+            pick = i;
+            break;
+        }
+    }
+    const GuideLobe& l = rec[pick];
+    const float ct = 1.f + std::log(u0 + (1.f - u0) * std::exp(-2.f * l.kappa)) / l.kappa;
+    const float st = std::sqrt(std::fmax(0.f, 1.f - ct * ct));
+    const float phi = 2.f * kPi * u1;
+    Frame f;
+    f.set_from_z(l.mu);
+    return f.to_world({std::cos(phi) * st, std::sin(phi) * st, ct});
+}
+
+float cam_medium_pdf(const Scene& sc, float mean_cosine, V p, V arrive, V gen)
+{
+    const float hg = hg_pdf(-arrive, gen, mean_cosine, nullptr);
+    if (!(sc.guide.alpha > 0.f)) return hg;
+    return sc.guide.alpha * vmf_mix_pdf(guide_record(sc.guide, p, arrive), gen) + (1.f - sc.guide.alpha) * hg;
+}
+
 // kernel/bsdf.cuh:334-396
 V bsdf_evaluate(const BSDF& b, V dir_gen, const Scene& sc, float& cos_gen, float* for_pdf, float* rev_pdf,
                 float* sin_theta)
@@ -490,8 +567,14 @@ V bsdf_evaluate(const BSDF& b, V dir_gen, const Scene& sc, float& cos_gen, float
         return v3(0.f);
     }
     const float p = hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, sin_theta);
-    if (for_pdf) *for_pdf = p;
-    if (rev_pdf) *rev_pdf = p;
+    float fp = p, rp = p; // the camera technique's side follows the learned mixture
+    const float a = sc.guide.alpha;
+    if (a > 0.f) {
+        if (!b.from_light()) fp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -b.world_dir_fix), dir_gen) + (1.f - a) * p;
+        else rp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), b.world_dir_fix) + (1.f - a) * p;
+    }
+    if (for_pdf) *for_pdf = fp;
+    if (rev_pdf) *rev_pdf = rp;
     return b.scatter_coef * p;
 }
 
@@ -510,7 +593,14 @@ float bsdf_pdf(const BSDF& b, V dir_gen, const Scene& sc, bool reverse)
         pdf_phong(b, m, lg, &fp, &rp);
         return reverse ? rp : fp;
     }
-    return hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, nullptr);
+    const float p = hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, nullptr);
+    const float a = sc.guide.alpha;
+    // a camera vertex's forward pdf and a light vertex's reverse pdf belong to the camera
+    // technique (the learned mixture); the other two to the phase function
+    const bool camera_technique = b.from_light() ? reverse : !reverse;
+    if (!(a > 0.f) || !camera_technique) return p;
+    if (!b.from_light()) return a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -b.world_dir_fix), dir_gen) + (1.f - a) * p;
+    return a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), b.world_dir_fix) + (1.f - a) * p;
 }
 
 // kernel/bsdf.cuh:455-559 (component samplers)
@@ -619,8 +709,22 @@ V bsdf_sample(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pd
         sin_theta = 0.f;
         return v3(0.f);
     }
-    dir_gen = hg_sample(b.world_dir_fix, b.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
-    return b.scatter_coef * pdf;
+    const float a = sc.guide.alpha;
+    if (!(a > 0.f) || b.from_light()) {
+        dir_gen = hg_sample(b.world_dir_fix, b.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
+        return b.scatter_coef * pdf;
+    }
+    // camera path: the learned mixture with probability alpha (variate u2), else the phase function
+    const GuideLobe* rec = guide_record(sc.guide, b.pos, -b.world_dir_fix);
+    float hg;
+    if (u2 < a) {
+        dir_gen = vmf_mix_sample(rec, u2 / a, u0, u1);
+        hg = hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, &sin_theta);
+    } else {
+        dir_gen = hg_sample(b.world_dir_fix, b.mean_cosine, u0, u1, b.frame, hg, &sin_theta);
+    }
+    pdf = a * vmf_mix_pdf(rec, dir_gen) + (1.f - a) * hg;
+    return b.scatter_coef * hg;
 }
 
 // ---- lights: kernel/light.cuh:41-543 ---------------------------------------

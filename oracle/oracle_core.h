@@ -204,8 +204,22 @@ struct Sph {
     int mat, light;
 };
 
+// Learned proposal of camera paths in media: res^3 cells x 6 arrival-direction bins, each a
+// mixture of 8 vMF lobes (no counterpart in the reference). alpha 0: off.
+struct GuideLobe {
+    V mu;
+    float kappa, w, c; // concentration, normalised weight, w kappa / (2 pi (1 - exp(-2 kappa)))
+};
+struct Guide {
+    float alpha = 0.f;
+    int res = 0;
+    V lo{0, 0, 0}, inv_cell{0, 0, 0};
+    std::vector<GuideLobe> lobes; // 8 per record, record ((z * res + y) * res + x) * 6 + bin
+};
+
 struct Scene {
     std::vector<Tri> real, imag;
+    Guide guide;
     std::vector<Sph> sreal, simag;
     EnvMap env; // width 0: none
     std::vector<Material> mats;
@@ -219,6 +233,14 @@ struct Scene {
 };
 
 void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& out);
+// learned proposal: records from (mu, kappa, weight) x 8 per record, and the camera technique's
+// pdf at a medium point (arriving along `arrive`, leaving along `gen`)
+void init_guide(Guide& g, const float* lobes, int res, const float* lo, const float* hi, float alpha);
+extern Guide g_oracle_guide;
+const GuideLobe* guide_record(const Guide& g, V p, V arrive);
+float vmf_mix_pdf(const GuideLobe* rec, V w);
+V vmf_mix_sample(const GuideLobe* rec, float us, float u0, float u1);
+float cam_medium_pdf(const Scene& sc, float mean_cosine, V p, V arrive, V gen);
 // environment map (host/env_map.hpp:42-144, kernel/env_map.cuh:36-137)
 void init_env_map(const upbp_env_map& in, EnvMap& m);
 V env_sample(const EnvMap& m, float u0, float u1, float& pdf_w, V& radiance);
@@ -378,6 +400,7 @@ struct BSDF {
     V world_dir_fix{0, 0, 0};
     V scatter_coef{0, 0, 0};
     bool scatter_pos = false;
+    V pos{0, 0, 0}; // scattering position (learned proposal lookup)
 
     bool valid() const { return flags & VALID; }
     bool delta() const { return flags & DELTA; }

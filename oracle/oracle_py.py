@@ -34,6 +34,8 @@ def lib():
         h.oracle_closest_hit.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
         h.oracle_sphere_hits.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
         h.oracle_env_samples.argtypes = [vp, vp, C.c_uint32, vp]
+        h.oracle_set_guide_lobes.argtypes = [vp, C.c_int32, vp, vp, C.c_float]
+        h.oracle_guide_samples.argtypes = [vp, vp, C.c_uint32, vp]
         h.oracle_camera_ray.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_world_to_raster.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_hg_pdf.argtypes = [C.c_float, C.c_float]
@@ -106,6 +108,28 @@ def sphere_hits(spheres, rays):
     out = np.zeros((rays.shape[0], spheres.shape[0], 3), np.float32)
     lib().oracle_sphere_hits(_p(spheres), spheres.shape[0], _p(rays), rays.shape[0], _p(out))
     return out
+
+
+def guide_samples(lobes, u):
+    """One vMF mixture (8, 5) sampled at u (n, 3): (n, 4) direction and mixture pdf."""
+    lobes = np.ascontiguousarray(lobes, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.zeros((u.shape[0], 4), np.float32)
+    lib().oracle_guide_samples(_p(lobes), _p(u), u.shape[0], _p(out))
+    return out
+
+
+def set_guide(lobes, res, lo, hi, alpha):
+    """Install the learned proposal for later oracle renders (alpha 0: off)."""
+    if alpha == 0:
+        lib().oracle_set_guide_lobes(None, 0, None, None, 0.0)
+        return
+    lobes = np.ascontiguousarray(lobes, np.float32)
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    rc = lib().oracle_set_guide_lobes(_p(lobes), res, _p(lo), _p(hi), alpha)
+    if rc != 0:
+        raise RuntimeError("oracle_set_guide_lobes failed: %d" % rc)
 
 
 class _EnvMap(C.Structure):  # upbp_env_map

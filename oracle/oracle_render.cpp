@@ -824,7 +824,7 @@ static void mis_record(const Ctx& c, int tech, float weight, V x0, int light_id,
 }
 
 // medium "BSDF" (phase function) at a point of a ray with direction dir
-static BSDF medium_bsdf(const Ctx& c, V dir, int med, bool from_light)
+static BSDF medium_bsdf(const Ctx& c, V dir, int med, bool from_light, V pos)
 {
     Isect is;
     is.dist = 1.f;
@@ -832,6 +832,7 @@ static BSDF medium_bsdf(const Ctx& c, V dir, int med, bool from_light)
     is.med = med;
     BSDF b;
     bsdf_setup(b, dir, is, *c.sc, from_light, -1.f);
+    b.pos = pos;
     return b;
 }
 
@@ -855,7 +856,7 @@ static LightVertex beam_point(const Ctx& c, const LightBeam& lb, float t, float 
     v.hit = lb.origin + lb.dir * t;
     v.path_length = lb.path_length;
     v.path = lb.path_idx;
-    v.bsdf = medium_bsdf(c, lb.dir, lb.med, true);
+    v.bsdf = medium_bsdf(c, lb.dir, lb.med, true, v.hit);
     v.w.for_inv = 1.f / rsf;
     v.w.rev_inv = 1.f / rsr;
     v.in_medium = true;
@@ -1068,6 +1069,7 @@ static void trace_light_path(const Ctx& c, uint32_t path_idx, LightOut& out)
         float for_ratio = 0.f, rev_ratio = 0.f;
         if (b.in_medium()) for_ratio = rev_ratio = 1.f / sc.media[size_t(b.med)].min_pos_att;
         const V hitp = s.origin + s.dir * is.dist;
+        b.pos = hitp;
         origin_in_medium = is.in_medium();
         {
             const float d = b.d_factor();
@@ -1322,9 +1324,10 @@ static void eval_pb2d(const Ctx& c, V& acc, const SegQuery& q, const LightVertex
     }
     const V pdir = lv.bsdf.dir_fix_world();
     float sin_theta;
-    float fpdf = hg_pdf(-q.rd, pdir, m.mean_cosine, &sin_theta);
-    float rpdf = fpdf;
-    const V f = v3(fpdf);
+    const float hg = hg_pdf(-q.rd, pdir, m.mean_cosine, &sin_theta);
+    float fpdf = cam_medium_pdf(*c.sc, m.mean_cosine, q.ro + q.rd * idist, q.rd, pdir); // camera technique
+    float rpdf = hg;                                                      // light technique
+    const V f = v3(hg);
     if (black_or_neg(f)) return;
     fpdf *= m.cont_prob;
     rpdf *= lv.bsdf.cont_prob;
@@ -1353,7 +1356,7 @@ static void eval_pb2d(const Ctx& c, V& acc, const SegQuery& q, const LightVertex
         std::vector<LightVertex> lp;
         if (light_prefix(c, lv.path, lv.path_length, lp)) {
             std::vector<CamVert> cam = g_mis->cam;
-            cam.push_back({q.ro + q.rd * idist, medium_bsdf(c, q.rd, q.med_id, false), rsf, rsr});
+            cam.push_back({q.ro + q.rd * idist, medium_bsdf(c, q.rd, q.med_id, false, q.ro + q.rd * idist), rsf, rsr});
             mis_record(c, MT_PB2D, 1.f / (w_light + w_local + w_cam), c.light_origin[lv.path], c.light_pick[lv.path],
                        lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f, 1.f);
         }
@@ -1403,9 +1406,10 @@ static void eval_bb1d(const Ctx& c, V& acc, const SegQuery& q, const LightBeam& 
         wl.d_pde = lb.sw.d_pde_a * fi + lb.sw.d_pde_b;
         wl.d_bpt = lb.sw.d_bpt_a * fi + lb.sw.d_bpt_b;
     }
-    float fpdf = hg_pdf(-q.rd, -lb.dir, m.mean_cosine, nullptr);
-    float rpdf = fpdf;
-    const V f = v3(fpdf);
+    const float hg = hg_pdf(-q.rd, -lb.dir, m.mean_cosine, nullptr);
+    float fpdf = cam_medium_pdf(*c.sc, m.mean_cosine, q.ro + q.rd * tc, q.rd, -lb.dir);
+    float rpdf = hg;
+    const V f = v3(hg);
     if (black_or_neg(f)) return;
     fpdf *= m.cont_prob;
     rpdf *= m.cont_prob;
@@ -1444,7 +1448,7 @@ static void eval_bb1d(const Ctx& c, V& acc, const SegQuery& q, const LightBeam& 
         if (light_prefix(c, lb.path_idx, lb.path_length - 1, lp)) {
             lp.push_back(beam_point(c, lb, tp, rsfp, rsrp));
             std::vector<CamVert> cam = g_mis->cam;
-            cam.push_back({q.ro + q.rd * tc, medium_bsdf(c, q.rd, q.med_id, false), rsfc, rsrc});
+            cam.push_back({q.ro + q.rd * tc, medium_bsdf(c, q.rd, q.med_id, false, q.ro + q.rd * tc), rsfc, rsrc});
             const float ref_d = (lb.path_length == 1 && !lb.infinite) ? 1.f : tp;
             mis_record(c, MT_BB1D, 1.f / (w_light + w_local + w_cam), c.light_origin[lb.path_idx],
                        c.light_pick[lb.path_idx], lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f,
@@ -1481,9 +1485,10 @@ static void eval_bp2d(const Ctx& c, V& acc, const SubpathState& s, const BSDF& b
     }
     const V cdir = bc.dir_fix_world();
     float sin_theta;
-    float fpdf = hg_pdf(cdir, -lb.dir, m.mean_cosine, &sin_theta);
-    float rpdf = fpdf;
-    const V f = v3(fpdf);
+    const float hg = hg_pdf(cdir, -lb.dir, m.mean_cosine, &sin_theta);
+    float fpdf = cam_medium_pdf(*c.sc, m.mean_cosine, hit, -cdir, -lb.dir);
+    float rpdf = hg;
+    const V f = v3(hg);
     if (black_or_neg(f)) return;
     fpdf *= bc.cont_prob;
     rpdf *= m.cont_prob;
@@ -1672,6 +1677,7 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
         float for_ratio = 0.f, rev_ratio = 0.f;
         if (b.in_medium()) for_ratio = rev_ratio = 1.f / sc.media[size_t(b.med)].min_pos_att;
         const V hitp = s.origin + s.dir * is.dist;
+        b.pos = hitp;
         origin_in_medium = is.in_medium();
         {
             const float d = b.d_factor();
@@ -2156,6 +2162,32 @@ extern "C" int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const fl
             o[2] = n > 1 ? th[1] : INFINITY;
         }
     }
+    return UPBP_OK;
+}
+
+extern "C" int oracle_guide_samples(const float* lobes, const float* u, uint32_t n, float* out)
+{
+    Guide g;
+    const float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
+    std::vector<float> rec(size_t(6) * 40);
+    for (int b = 0; b < 6; b++) memcpy(&rec[size_t(b) * 40], lobes, 40 * sizeof(float));
+    init_guide(g, rec.data(), 1, lo, hi, 1.f);
+    const GuideLobe* r = g.lobes.data();
+    for (uint32_t i = 0; i < n; i++) {
+        const V d = vmf_mix_sample(r, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+        out[4 * i] = d.x;
+        out[4 * i + 1] = d.y;
+        out[4 * i + 2] = d.z;
+        out[4 * i + 3] = vmf_mix_pdf(r, d);
+    }
+    return UPBP_OK;
+}
+
+extern "C" int oracle_set_guide_lobes(const float* lobes, int32_t res, const float* lo, const float* hi, float alpha)
+{
+    if (!(alpha >= 0.f && alpha <= 1.f)) return UPBP_E_ARG;
+    if (alpha > 0.f && (!lobes || !lo || !hi || res < 1 || res > 256)) return UPBP_E_ARG;
+    init_guide(g_oracle_guide, lobes, res, lo, hi, alpha);
     return UPBP_OK;
 }
 

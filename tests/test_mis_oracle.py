@@ -29,21 +29,35 @@ from oracle import oracle_py
 ALL = "upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d"
 
 
-def _records(scene_dir, kind, beams=()):
+def _records(scene_dir, kind, beams=(), guide_alpha=0.0):
     import os
     obj = capi.generate_scene(kind, 0, os.path.join(scene_dir, kind + ".obj"))
     args = ["-s", "-1", obj, "-r", "64x64", "-l", "7", "-a", ALL, "-r_initial", "0.03", "-r_alpha", "1"] + list(beams)
     rc, cfg = capi.parse_args(args)
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
-    rec = oracle_py.mis_check(sh.ptr, cfg, 0, np.arange(0, 64 * 64, 3), 2000)
+    if guide_alpha > 0:  # learned proposal with random lobes on a 3^3 grid
+        rng = np.random.default_rng(5)
+        n = 27 * 6 * 8
+        mu = rng.normal(size=(n, 3))
+        lobes = np.hstack([mu / np.linalg.norm(mu, axis=1, keepdims=True), np.exp(rng.uniform(-2, 4, (n, 1))),
+                           rng.uniform(0.05, 1, (n, 1))]).astype(np.float32)
+        c, r = np.array(list(sh.scene.scene_center)), sh.scene.scene_radius
+        oracle_py.set_guide(lobes, 3, c - r, c + r, guide_alpha)
+    try:
+        rec = oracle_py.mis_check(sh.ptr, cfg, 0, np.arange(0, 64 * 64, 3), 2000)
+    finally:
+        oracle_py.set_guide(None, 0, None, None, 0.0)
     sh.close()
     return rec
 
 
-@pytest.mark.parametrize("kind,beams", [("box", ()), ("boxfog", ()), ("boxfog", ("-qbt", "S", "-pbt", "S")),
-                                        ("boxfog", ("-qbt", "S", "-pbt", "L"))])
-def test_constant_time_weights_match_fp64_balance_heuristic(scene_dir, kind, beams):
-    rec = _records(scene_dir, kind, beams)
+@pytest.mark.parametrize("kind,beams,guide", [("box", (), 0.0), ("boxfog", (), 0.0),
+                                              ("boxfog", ("-qbt", "S", "-pbt", "S"), 0.0),
+                                              ("boxfog", ("-qbt", "S", "-pbt", "L"), 0.0), ("boxfog", (), 0.7)])
+def test_constant_time_weights_match_fp64_balance_heuristic(scene_dir, kind, beams, guide):
+    # guide > 0: camera paths sample a learned vMF mixture in media; its pdf must enter the
+    # forward recursion at camera vertices and the reverse one at light vertices
+    rec = _records(scene_dir, kind, beams, guide)
     tech, w_ct, w_ref = rec[:, 0].astype(int), rec[:, 3], rec[:, 5]
     rel = np.abs(w_ct - w_ref) / np.maximum(w_ref, 1e-12)
     counts = {oracle_py.MIS_TECH[t]: int(np.sum(tech == t)) for t in oracle_py.MIS_TECH}
