@@ -368,7 +368,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     return true;
 }
 
-__global__ void __launch_bounds__(256) k_camera_bounce(DScene sc, IterParams p, Bufs b, int bounce)
+__global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams p, Bufs b, int bounce)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.qcount[64 + bounce];

@@ -46,7 +46,9 @@ UPBP_HD const float4* guide_record(const DGuide& g, f3 p, f3 arrive)
 UPBP_HD float lane4(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
 
 // sum_i c_i exp(kappa_i (mu_i . w - 1))
-UPBP_HD float vmf_mix_pdf(const float4* rec, f3 w)
+// out of line: keeps the mixture's twelve record loads and eight exponentials out of the
+// register allocation of the bounce and merge kernels, which call it only when alpha > 0
+inline __host__ __device__ __noinline__ float vmf_mix_pdf(const float4* rec, f3 w)
 {
     float pdf = 0.f;
 #pragma unroll
@@ -60,7 +62,7 @@ UPBP_HD float vmf_mix_pdf(const float4* rec, f3 w)
 
 // lobe chosen by weight with us in [0, 1), direction by the inversion of Jakob ("Numerically
 // stable sampling of the von Mises Fisher distribution on S2", 2012) with u0, u1
-UPBP_HD f3 vmf_mix_sample(const float4* rec, float us, float u0, float u1)
+inline __host__ __device__ __noinline__ f3 vmf_mix_sample(const float4* rec, float us, float u0, float u1)
 {
     int pick = -1;
     float acc = 0.f;
