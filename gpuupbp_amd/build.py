@@ -27,7 +27,7 @@ HOST_SOURCES = ["host/config.cpp", "host/scene_loader.cpp", "host/exr_writer.cpp
                 "host/scene_gen.cpp", "host/scene_predef.cpp",
                 "host/exr_reader.cpp"]
 HIP_SOURCES = ["device/renderer.hip", "device/lbvh.hip", "device/light_pass.hip", "device/camera_pass.hip",
-               "device/merge_pass.hip", "device/test_hooks.hip"]
+               "device/merge_pass.hip", "device/test_hooks.hip", "device/guide_mlp.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include")]
 HIP_FLAGS = COMMON + ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

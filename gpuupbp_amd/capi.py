@@ -147,6 +147,14 @@ _SIGS = [
     ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_test_env_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_test_guide_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_mlp_param_count", C.c_uint32, []),
+    ("upbp_mlp_create", C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("upbp_mlp_destroy", None, [C.c_void_p]),
+    ("upbp_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("upbp_mlp_get_params", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("upbp_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_mlp_loss_grad", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_void_p]),
+    ("upbp_mlp_train_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     ("upbp_set_guide_lobes", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_float]),
     ("upbp_read_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 ]

@@ -92,7 +92,7 @@ UPBP_HD float cam_medium_pdf(const DScene& sc, const DMedium& m, f3 p, f3 arrive
 }
 
 // (mu, kappa, w) x 8 -> the device record: lobes, normalised weights, lobe constants
-inline void pack_guide_record(const float* in, float4* out)
+UPBP_HD void pack_guide_record(const float* in, float4* out)
 {
     float wsum = 0.f;
     for (int i = 0; i < kGuideLobes; i++) wsum += fmaxf(in[5 * i + 4], 0.f);

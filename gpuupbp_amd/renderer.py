@@ -130,6 +130,61 @@ def env_samples(rgb, u):
     return out
 
 
+class Mlp:
+    """The learned proposal's network on the GPU (BF16 MFMA forward / backward / Adam)."""
+
+    def __init__(self, seed=3):
+        self.h = C.c_void_p()
+        rc = capi.lib().upbp_mlp_create(seed, C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("upbp_mlp_create failed: %d" % rc)
+        self.n = int(capi.lib().upbp_mlp_param_count())
+
+    def close(self):
+        if self.h:
+            capi.lib().upbp_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.n, np.float32)
+        if capi.lib().upbp_mlp_get_params(self.h, _p(p)) != 0:
+            raise RuntimeError("upbp_mlp_get_params failed")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if capi.lib().upbp_mlp_set_params(self.h, _p(p)) != 0:
+            raise RuntimeError("upbp_mlp_set_params failed")
+
+    def forward(self, samples):
+        samples = np.ascontiguousarray(samples, np.float32)
+        out = np.zeros((samples.shape[0], 40), np.float32)
+        if capi.lib().upbp_mlp_forward(self.h, _p(samples), samples.shape[0], _p(out)) != 0:
+            raise RuntimeError("upbp_mlp_forward failed")
+        return out
+
+    def loss_grad(self, samples):
+        samples = np.ascontiguousarray(samples, np.float32)
+        loss = C.c_float()
+        g = np.zeros(self.n, np.float32)
+        if capi.lib().upbp_mlp_loss_grad(self.h, _p(samples), samples.shape[0], C.byref(loss), _p(g)) != 0:
+            raise RuntimeError("upbp_mlp_loss_grad failed")
+        return loss.value, g
+
+    def train_step(self, samples, lr=1e-3):
+        samples = np.ascontiguousarray(samples, np.float32)
+        loss = C.c_float()
+        if capi.lib().upbp_mlp_train_step(self.h, _p(samples), samples.shape[0], lr, C.byref(loss)) != 0:
+            raise RuntimeError("upbp_mlp_train_step failed")
+        return loss.value
+
+
 def guide_samples(lobes, u):
     """GPU sampling of one learned-proposal mixture (8, 5) at u (n, 3): (n, 4) direction, pdf."""
     lobes = np.ascontiguousarray(lobes, np.float32)

@@ -289,6 +289,23 @@ int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
 int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
                          float alpha);
 
+/* The proposal's network (BF16 MFMA forward, backward and Adam with FP32 accumulation and an
+ * FP32 master copy): 51 inputs (48 Fourier features of the position in [0,1]^3 and the arrival
+ * direction), 3 x 64 ReLU, 40 outputs (8 lobes x raw mean direction, raw concentration, logit;
+ * kappa = softplus + 0.01, weights = softmax). Parameters flat: W1[64][51], b1, W2[64][64], b2,
+ * W3[64][64], b3, W4[40][64], b4. Samples: forward n x 6 (position, arrival direction), training
+ * n x 10 (position, arrival direction, scattered direction, weight W); the loss is the batch
+ * mean of W * -log q(scattered). */
+typedef struct upbp_mlp upbp_mlp;
+uint32_t upbp_mlp_param_count(void);
+int upbp_mlp_create(uint32_t seed, upbp_mlp** out); /* He-initialised */
+void upbp_mlp_destroy(upbp_mlp* m);
+int upbp_mlp_set_params(upbp_mlp* m, const float* params);
+int upbp_mlp_get_params(upbp_mlp* m, float* params);
+int upbp_mlp_forward(upbp_mlp* m, const float* samples, uint32_t n, float* out); /* n x 40 raw outputs */
+int upbp_mlp_loss_grad(upbp_mlp* m, const float* samples, uint32_t n, float* loss, float* grads);
+int upbp_mlp_train_step(upbp_mlp* m, const float* samples, uint32_t n, float lr, float* loss); /* one Adam step */
+
 /* ---- test hooks (parity tests and autotuning only) ----------------------
  * Range queries over a device LBVH, returning the accepted index set of every
  * query in CSR form (offsets[n_queries+1], indices). They replace the OptiX

@@ -55,6 +55,15 @@ int oracle_closest_hit(const float* verts, uint32_t n_tris, const float* rays, u
 /* Analytic sphere test of every (ray, sphere) pair, same layout as upbp_test_sphere_hits. */
 int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const float* rays, uint32_t n_rays, float* out);
 
+/* FP64 network of the learned proposal (oracle_mlp.cpp): parameter count, input encoding,
+ * forward pass (samples n x 6: position in [0,1]^3, arrival direction -> n x 40 raw outputs)
+ * and the weighted mixture NLL with its analytic gradient (samples n x 10: position, arrival
+ * direction, scattered direction, weight). */
+uint32_t oracle_mlp_param_count(void);
+void oracle_mlp_encode(const float* x, const float* d, float* out);
+int oracle_mlp_forward(const double* params, const float* samples, uint32_t n, double* out);
+int oracle_mlp_loss_grad(const double* params, const float* samples, uint32_t n, double* loss, double* grads);
+
 /* One vMF mixture (8 x (mu.xyz, kappa, weight)) sampled with u (3 per sample: lobe, two for
  * the direction): out 4 per sample (direction, mixture pdf). */
 int oracle_guide_samples(const float* lobes, const float* u, uint32_t n, float* out);

@@ -36,6 +36,10 @@ def lib():
         h.oracle_env_samples.argtypes = [vp, vp, C.c_uint32, vp]
         h.oracle_set_guide_lobes.argtypes = [vp, C.c_int32, vp, vp, C.c_float]
         h.oracle_guide_samples.argtypes = [vp, vp, C.c_uint32, vp]
+        h.oracle_mlp_param_count.restype = C.c_uint32
+        h.oracle_mlp_encode.argtypes = [vp, vp, vp]
+        h.oracle_mlp_forward.argtypes = [vp, vp, C.c_uint32, vp]
+        h.oracle_mlp_loss_grad.argtypes = [vp, vp, C.c_uint32, vp, vp]
         h.oracle_camera_ray.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_world_to_raster.argtypes = [vp, vp, vp, C.c_uint32, vp]
         h.oracle_hg_pdf.argtypes = [C.c_float, C.c_float]
@@ -117,6 +121,35 @@ def guide_samples(lobes, u):
     out = np.zeros((u.shape[0], 4), np.float32)
     lib().oracle_guide_samples(_p(lobes), _p(u), u.shape[0], _p(out))
     return out
+
+
+def mlp_param_count():
+    return int(lib().oracle_mlp_param_count())
+
+
+def mlp_encode(x, d):
+    out = np.zeros(51, np.float32)
+    lib().oracle_mlp_encode(_p(np.ascontiguousarray(x, np.float32)), _p(np.ascontiguousarray(d, np.float32)), _p(out))
+    return out
+
+
+def mlp_forward(params, samples):
+    """FP64 forward: samples (n, 6) -> raw outputs (n, 40)."""
+    params = np.ascontiguousarray(params, np.float64)
+    samples = np.ascontiguousarray(samples, np.float32)
+    out = np.zeros((samples.shape[0], 40), np.float64)
+    lib().oracle_mlp_forward(_p(params), _p(samples), samples.shape[0], _p(out))
+    return out
+
+
+def mlp_loss_grad(params, samples, want_grad=True):
+    """FP64 weighted mixture NLL of samples (n, 10) and its gradient."""
+    params = np.ascontiguousarray(params, np.float64)
+    samples = np.ascontiguousarray(samples, np.float32)
+    loss = C.c_double()
+    g = np.zeros(params.shape[0], np.float64) if want_grad else None
+    lib().oracle_mlp_loss_grad(_p(params), _p(samples), samples.shape[0], C.byref(loss), _p(g) if want_grad else None)
+    return loss.value, g
 
 
 def set_guide(lobes, res, lo, hi, alpha):
