@@ -107,7 +107,14 @@ class IterStats(C.Structure):
                 ("seg_nodes", C.c_uint64), ("seg_candidates", C.c_uint64), ("seg_pieces", C.c_uint64),
                 ("seg_wave_trips", C.c_uint64), ("seg_cand_batches", C.c_uint64), ("seg_pair_batches", C.c_uint64),
                 ("vertex_drops", C.c_uint64), ("beam_drops", C.c_uint64), ("path_errors", C.c_uint64),
-                ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double)]
+                ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double),
+                ("guide_samples", C.c_uint64), ("guide_loss", C.c_double), ("ms_guide", C.c_double)]
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("grid_res", C.c_int32), ("train", C.c_int32),
+                ("steps_per_iteration", C.c_int32), ("batch", C.c_int32), ("learning_rate", C.c_float),
+                ("seed", C.c_uint32), ("max_samples", C.c_uint32)]
 
 
 _lib = None
@@ -155,6 +162,8 @@ _SIGS = [
     ("upbp_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_mlp_loss_grad", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_void_p]),
     ("upbp_mlp_train_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
+    ("upbp_guide_defaults", None, [C.c_void_p]),
+    ("upbp_set_guide", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_set_guide_lobes", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_float]),
     ("upbp_read_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 ]

@@ -155,7 +155,7 @@ struct CamCounters {
 
 // One segment of kernel/trace_camera.cu:79-382. Returns whether the path continues.
 __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b, uint32_t pix, int bounce, Sub& s,
-                            uint32_t& pflags, f3& color, int& ret, CamCounters& cc)
+                            uint32_t& pflags, f3& color, int& ret, CamCounters& cc, float& pre_scatter_lum)
 {
     const bool origin_in_medium = (pflags & PF_ORIGIN_IN_MEDIUM) != 0;
     bool only_spec_surf = (pflags & PF_ONLY_SPEC_SURF) != 0;
@@ -353,6 +353,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     }
     bool cont;
     Rng srng(kDomainCamera, pix, p.iter_key, seg, P_SCATTER);
+    pre_scatter_lum = luminance(s.throughput);
     ret = sample_scattering(sc, p, cont, bs, hitp, is, s, srng);
     if (ret != S_OK) return false;
     if (!cont) return false;
@@ -366,6 +367,55 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         }
     }
     return true;
+}
+
+// Training data of the learned proposal. What a camera path collects at a vertex (emission,
+// next-event estimation, connections, point and B-P2D merges) is credited to its previous
+// medium scattering: the sample (position, arrival, scattered direction) gets the weight
+// lum(collected) / lum(throughput before that scattering), an estimate of incident radiance
+// times phase over the sampling pdf, which is what the KL gradient of the proposal needs. The
+// segment merges of the bounce are not seen here and do not contribute.
+__device__ __noinline__ void guide_record_sample(const DScene& sc, const Bufs& b, uint32_t pix, f3 collected,
+                                                 bool cont, f3 arrive, const Sub& s, float pre_lum)
+{
+    float* pend = b.gpend + size_t(pix) * 10;
+    bool emit = false;
+    float rec[10];
+    if (pend[9] > 0.f) {
+        const float w = luminance(collected) / pend[9];
+        if (w > 0.f && isfinite(w)) {
+            for (int k = 0; k < 9; k++) rec[k] = pend[k];
+            rec[9] = w;
+            emit = true;
+        }
+        pend[9] = 0.f;
+    }
+    const uint64_t m = __ballot(emit);
+    if (m) { // wave-aggregated append
+        const uint32_t lane = __lane_id();
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (int(lane) == leader) base = atomicAdd(b.gcount, uint32_t(__popcll(m)));
+        base = __shfl(base, leader);
+        const uint32_t at = base + uint32_t(__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane)))));
+        if (emit && at < b.gcap)
+            for (int k = 0; k < 10; k++) b.gsamp[size_t(at) * 10 + k] = rec[k];
+    }
+    // a medium scattering that continues becomes the pending record
+    if (cont && s.sw.w.prev_in_medium && pre_lum > 0.f) {
+        const DGuide& g = sc.guide;
+        const float inv_res = 1.f / float(g.res);
+        pend[0] = (s.origin.x - g.lo.x) * g.inv_cell.x * inv_res;
+        pend[1] = (s.origin.y - g.lo.y) * g.inv_cell.y * inv_res;
+        pend[2] = (s.origin.z - g.lo.z) * g.inv_cell.z * inv_res;
+        pend[3] = arrive.x;
+        pend[4] = arrive.y;
+        pend[5] = arrive.z;
+        pend[6] = s.dir.x;
+        pend[7] = s.dir.y;
+        pend[8] = s.dir.z;
+        pend[9] = pre_lum;
+    }
 }
 
 __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams p, Bufs b, int bounce)
@@ -382,11 +432,14 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams 
     f3 color = st.color;
     int ret = S_OK;
     CamCounters cc;
-    bool cont = camera_step(sc, p, b, pix, bounce, s, pflags, color, ret, cc);
+    const f3 color_before = color, arrive = s.dir;
+    float pre_lum = 0.f;
+    bool cont = camera_step(sc, p, b, pix, bounce, s, pflags, color, ret, cc, pre_lum);
     if (ret != S_OK) {
         cont = false;
         stat_add(b.stats, STAT_ERRORS, 1);
     }
+    if (b.gsamp) guide_record_sample(sc, b, pix, color - color_before, cont, arrive, s, pre_lum);
     stat_add(b.stats, STAT_BPT_CONN, cc.conn);
     stat_add(b.stats, STAT_POINT_Q, cc.pq);
     stat_add(b.stats, STAT_SEG_Q, cc.sq);

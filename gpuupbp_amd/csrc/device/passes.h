@@ -392,6 +392,14 @@ struct Bufs {
     uint8_t* piece_cell; // spatial cell of each piece
     uint32_t* xwork; // per bounce: kXWork words of cell counts, offsets and per-XCD work ranges
     uint32_t seg_cap, piece_cap;
+    // learned-proposal training (null when off): per camera path the pending record of its
+    // last medium scattering (position in [0,1]^3, arrival, scattered direction, luminance of
+    // the throughput before it; 10 floats, weight 0 = none), the sample buffer (10 floats each:
+    // position, arrival, scattered, weight) and its counter
+    float* gpend;
+    float* gsamp;
+    uint32_t* gcount;
+    uint32_t gcap;
     double* accum;
     unsigned long long* stats; // see STAT_*
     PhotonMaps maps;

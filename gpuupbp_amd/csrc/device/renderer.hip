@@ -7,6 +7,7 @@
 // radius schedule and MIS factors (:56-119), light wavefront, deterministic photon lists +
 // three photon LBVH builds, camera wavefront, FP64 accumulation on the device.
 #include "env_tables.h"
+#include "guide_mlp.h"
 #include "kernels.h"
 
 #include "../../../include/upbp_capi.h"
@@ -169,6 +170,14 @@ struct upbp_renderer {
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
     DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals;
     DevBuf guide; // learned proposal records
+    // online training of the proposal (upbp_set_guide)
+    upbp_guide_config gcfg{};
+    bool gtrain = false;
+    GuideNet* gnet = nullptr;
+    DevBuf gpend, gsamp, gcount;
+    uint32_t gcap = 0;
+    double gloss = 0.0, gms = 0.0;
+    uint64_t gsamples = 0;
     BvhStore bvh_s, bvh_m, bvh_b;
     WBvhStore wbvh_m, wbvh_b;
     float scene_mag = 1.f;       // largest coordinate magnitude of the scene and camera
@@ -690,6 +699,43 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 
 } // namespace
 
+// Adam steps of the proposal on the samples the camera pass collected, then new records; the
+// next iteration samples from them. One host synchronisation (the sample count).
+static int train_guide(upbp_renderer* r)
+{
+    hipStream_t s = r->stream;
+    uint32_t n = 0;
+    HIPCHK(hipEventRecord(r->ev[4], s));
+    HIPCHK(hipMemcpyAsync(&n, r->gcount.p, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    n = std::min(n, r->gcap);
+    r->gsamples = n;
+    r->gloss = 0.0;
+    r->gms = 0.0;
+    if (n < 256) return UPBP_OK;
+    const upbp_guide_config& g = r->gcfg;
+    const uint32_t batch = std::min<uint32_t>(uint32_t(g.batch), n);
+    double loss = 0.0;
+    for (int k = 0; k < g.steps_per_iteration; k++) {
+        const uint32_t off = uint32_t((uint64_t(k) * batch) % uint64_t(n - batch + 1));
+        if (r->gnet->gradient(r->gsamp.as<float>() + size_t(off) * 10, batch, s) || r->gnet->adam_step(g.learning_rate, s))
+            return fail(r, "guide training failed", UPBP_E_HIP);
+        float l = 0.f;
+        HIPCHK(hipMemcpyAsync(&l, r->gnet->loss.p, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        loss += double(l) / double(batch);
+    }
+    if (g.steps_per_iteration > 0) {
+        if (r->gnet->records(r->sc.guide.res, r->guide.as<float4>(), s)) return fail(r, "guide refresh failed", UPBP_E_HIP);
+        r->sc.guide.alpha = g.alpha;
+        r->gloss = loss / double(g.steps_per_iteration);
+    }
+    HIPCHK(hipEventRecord(r->ev[5], s));
+    HIPCHK(hipEventSynchronize(r->ev[5]));
+    r->gms = elapsed(r->ev[4], r->ev[5]);
+    return UPBP_OK;
+}
+
 extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
 {
     if (!r) return fail(r, "null renderer", UPBP_E_ARG);
@@ -727,6 +773,16 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         memset(&b.maps, 0, sizeof(b.maps));
     }
     HIPCHK(hipEventRecord(ev[2], s));
+    const bool collect = r->gtrain && p.trace_camera;
+    b.gpend = collect ? r->gpend.as<float>() : nullptr;
+    b.gsamp = collect ? r->gsamp.as<float>() : nullptr;
+    b.gcount = collect ? r->gcount.as<uint32_t>() : nullptr;
+    b.gcap = collect ? r->gcap : 0u;
+    if (collect) {
+        hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(r->n_pixels) * 10)), dim3(256), 0, s, r->gpend.as<uint32_t>(), 0u,
+                           size_t(r->n_pixels) * 10);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->gcount.as<uint32_t>(), 0u, size_t(4));
+    }
     if (p.trace_camera) {
         launch_camera_generate(r->sc, p, b, s);
         for (int k = 0; k < L; k++) {
@@ -756,6 +812,10 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     launch_camera_finish(p, b, float(r->cfg.res_x), float(r->cfg.res_y), s);
     HIPCHK(hipEventRecord(ev[3], s));
     HIPCHK(hipGetLastError());
+    if (collect) { // train the proposal on this iteration's samples, then refresh its records
+        const int e = train_guide(r);
+        if (e) return e;
+    }
     if (stats) {
         HIPCHK(hipEventSynchronize(ev[3]));
         memset(stats, 0, sizeof(*stats));
@@ -829,6 +889,9 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         stats->dominant_bytes = g[dom].bytes;
         stats->dominant_launches = p.trace_camera ? uint64_t(L) : 0;
         snprintf(stats->dominant_kernel, sizeof(stats->dominant_kernel), "%s", g[dom].name);
+        stats->guide_samples = r->gsamples;
+        stats->guide_loss = r->gloss;
+        stats->ms_guide = r->gms;
         stats->algorithmic_bytes = b_bounce + b_pb2d + b_bb1d + 320.0 * double(stats->light_extend_steps) +
                                    112.0 * double(counts[0] + counts[1]) + 128.0 * double(stats->beams) +
                                    88.0 * double(counts[0] + counts[1] + counts[2]) + 60.0 * double(r->n_pixels);
@@ -883,6 +946,58 @@ extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_
     return UPBP_OK;
 }
 
+extern "C" void upbp_guide_defaults(upbp_guide_config* g)
+{
+    if (!g) return;
+    memset(g, 0, sizeof(*g));
+    g->alpha = 0.5f;
+    g->grid_res = 16;
+    g->train = 1;
+    g->steps_per_iteration = 4;
+    g->batch = 65536;
+    g->learning_rate = 1e-3f;
+    g->seed = 3;
+    g->max_samples = 1u << 22;
+}
+
+extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
+{
+    if (!r || !g) return UPBP_E_ARG;
+    if (!(g->alpha >= 0.f && g->alpha <= 1.f) || g->grid_res < 1 || g->grid_res > 64 || g->steps_per_iteration < 0 ||
+        g->batch < 16 || g->batch > (1 << 22) || !(g->learning_rate > 0.f) || g->max_samples < 16)
+        return fail(r, "invalid guide configuration", UPBP_E_ARG);
+    r->gcfg = *g;
+    delete r->gnet;
+    r->gnet = nullptr;
+    r->gtrain = false;
+    r->sc.guide.alpha = 0.f;
+    if (g->alpha == 0.f) return UPBP_OK;
+    // grid over the bounding cube of the scene sphere
+    const f3 c = r->sc.center;
+    const float rad = r->sc.radius;
+    const int res = g->grid_res;
+    const size_t nrec = size_t(res) * res * res * kGuideBins;
+    if (r->guide.ensure(nrec * kGuideRec * sizeof(float4))) return fail(r, "guide allocation failed", UPBP_E_HIP);
+    DGuide& dg = r->sc.guide;
+    dg.recs = r->guide.as<float4>();
+    dg.res = res;
+    dg.lo = c - mk(rad);
+    dg.inv_cell = mk(float(res) / (2.f * rad));
+    r->gnet = new GuideNet();
+    if (r->gnet->init(g->seed) || r->gnet->records(res, r->guide.as<float4>(), r->stream))
+        return fail(r, "guide network setup failed", UPBP_E_HIP);
+    r->gcap = g->max_samples;
+    if (r->gpend.ensure(size_t(r->n_pixels) * 10 * 4) || r->gsamp.ensure(size_t(r->gcap) * 10 * 4) ||
+        r->gcount.ensure(16))
+        return fail(r, "guide allocation failed", UPBP_E_HIP);
+    r->gtrain = g->train != 0;
+    // an untrained network proposes nearly uniform lobes: the first iteration samples the phase
+    // function only and collects training data; the mixture switches on after the first steps
+    if (!r->gtrain) dg.alpha = g->alpha;
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return UPBP_OK;
+}
+
 extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
@@ -925,8 +1040,10 @@ extern "C" void upbp_destroy(upbp_renderer* r)
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
                       &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
-                      &r->xwork})
+                      &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->gsamp, &r->gcount})
         d->release();
+    delete r->gnet;
+    r->gnet = nullptr;
     lbvh_free(r->bvh_s);
     lbvh_free(r->bvh_m);
     lbvh_free(r->bvh_b);

@@ -9,6 +9,8 @@
 //             each, 1.31M in total) on a floor, three overlapping imaginary media
 //             spheres with priorities 1..3, two quad lights and a point light.
 //   "furnace": closed box with albedo-1 diffuse walls lit by a quad light (tests only).
+//   "slabroom": room of ~196k triangles with a dense forward-scattering medium slab, lit
+//             only through a 5% aperture in one wall ("slabroom_small": coarse walls).
 #include "../../../include/upbp_capi.h"
 
 #include <cmath>
@@ -325,6 +327,86 @@ int gen_blobs(const std::string& obj_path, uint32_t seed)
     return UPBP_OK;
 }
 
+// n x n grid of quads spanning origin + u * [0,1] + v * [0,1] (normal along u x v), shared
+// vertices; cells whose centre satisfies skip(u, v) are left out
+template <typename Skip>
+void grid(Writer& w, P3 origin, P3 eu, P3 ev, int n, Skip skip)
+{
+    for (int j = 0; j <= n; j++)
+        for (int i = 0; i <= n; i++) {
+            const P3 p = origin + eu * (double(i) / n) + ev * (double(j) / n);
+            fprintf(w.obj, "v %.6f %.6f %.6f\n", p.x, p.y, p.z);
+        }
+    auto id = [&](int i, int j) { return w.vbase + uint32_t(j * (n + 1) + i) + 1; };
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            if (skip((i + .5) / n, (j + .5) / n)) continue;
+            fprintf(w.obj, "f %u %u %u\n", id(i, j), id(i + 1, j), id(i + 1, j + 1));
+            fprintf(w.obj, "f %u %u %u\n", id(i, j), id(i + 1, j + 1), id(i, j + 1));
+        }
+    w.vbase += uint32_t((n + 1) * (n + 1));
+}
+
+// Room x in [-2, 2], y in [0, 3], z in [-3, 3] of walls tessellated n x n per face (n = 128:
+// about 196k triangles), with a square aperture of 5% of the x = +2 wall's area centred at
+// y = 2, z = 0; the only light is a quad outside that wall facing the aperture. A dense
+// forward-scattering medium slab (imaginary box x in [-1.5, 1.5], y in [0.5, 2.5],
+// z in [-0.5, 0.5]; sigma_t ~ U[2, 10], albedo 0.95, g 0.9) sits between the camera and the
+// far wall, so most of what the camera sees arrives through the aperture after several
+// scattering events in the slab.
+int gen_slabroom(const std::string& obj_path, uint32_t seed, int n)
+{
+    const std::string mtl_path = strip_ext(obj_path) + ".mtl";
+    FILE* obj = fopen(obj_path.c_str(), "w");
+    FILE* mtl = fopen(mtl_path.c_str(), "w");
+    FILE* aux = fopen((obj_path + ".aux").c_str(), "w");
+    if (!obj || !mtl || !aux) return UPBP_E_IO;
+    Rng rng(seed);
+    fprintf(mtl, "newmtl walls\nNs 0\nKd 0.6 0.6 0.6\nKs 0 0 0\nKe 0 0 0\n\n");
+    fprintf(mtl, "newmtl floor\nNs 0\nKd 0.45 0.4 0.35\nKs 0 0 0\nKe 0 0 0\n\n");
+    fprintf(mtl, "newmtl light\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 400 380 340\n\n");
+    fprintf(mtl, "newmtl slab_boundary\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 0 0 0\n\n");
+    Writer w;
+    w.obj = obj;
+    fprintf(obj, "mtllib %s\n", base_name(mtl_path).c_str());
+    auto none = [](double, double) { return false; };
+    // inward-facing walls
+    w.usemtl("floor");
+    grid(w, {-2, 0, -3}, {0, 0, 6}, {4, 0, 0}, n, none); // y = 0, normal +y
+    w.usemtl("walls");
+    grid(w, {-2, 3, -3}, {4, 0, 0}, {0, 0, 6}, n, none);  // ceiling, normal -y
+    grid(w, {-2, 0, -3}, {4, 0, 0}, {0, 3, 0}, n, none);  // z = -3, normal +z
+    grid(w, {-2, 0, 3}, {0, 3, 0}, {4, 0, 0}, n, none);   // z = +3, normal -z
+    grid(w, {-2, 0, -3}, {0, 3, 0}, {0, 0, 6}, n, none);  // x = -2, normal +x
+    // x = +2 (normal -x), with the aperture: 5% of 3 x 6 is 0.9, a square of side 0.9487
+    const double half = 0.5 * std::sqrt(0.05 * 18.0);
+    grid(w, {2, 0, -3}, {0, 0, 6}, {0, 3, 0}, n, [&](double u, double v) {
+        const double z = -3 + 6 * u, y = 3 * v;
+        return std::fabs(z) < half && std::fabs(y - 2.0) < half;
+    });
+    // the light outside, facing the aperture (normal -x)
+    w.usemtl("light");
+    w.quad({3.2, 1.25, -0.75}, {3.2, 1.25, 0.75}, {3.2, 2.75, 0.75}, {3.2, 2.75, -0.75});
+    // the slab, outward normals
+    const P3 a{-1.5, 0.5, -0.5}, b{1.5, 2.5, 0.5};
+    w.usemtl("slab_boundary");
+    w.quad({a.x, a.y, a.z}, {b.x, a.y, a.z}, {b.x, a.y, b.z}, {a.x, a.y, b.z}); // -y
+    w.quad({a.x, b.y, a.z}, {a.x, b.y, b.z}, {b.x, b.y, b.z}, {b.x, b.y, a.z}); // +y
+    w.quad({a.x, a.y, a.z}, {a.x, b.y, a.z}, {b.x, b.y, a.z}, {b.x, a.y, a.z}); // -z
+    w.quad({a.x, a.y, b.z}, {b.x, a.y, b.z}, {b.x, b.y, b.z}, {a.x, b.y, b.z}); // +z
+    w.quad({a.x, a.y, a.z}, {a.x, a.y, b.z}, {a.x, b.y, b.z}, {a.x, b.y, a.z}); // -x
+    w.quad({b.x, a.y, a.z}, {b.x, b.y, a.z}, {b.x, b.y, b.z}, {b.x, a.y, b.z}); // +x
+    fclose(obj);
+    fclose(mtl);
+    camera_aux(aux, {0.0, 1.5, -2.9}, {0.0, 1.5, 0.0}, {0, 1, 0}, 60.0);
+    const double st = rng.uniform(2.0, 10.0);
+    fprintf(aux, "medium slab\nabsorption %.5f %.5f %.5f\nemission 0 0 0\nscattering %.5f %.5f %.5f\ng 0.9\n\n",
+            0.05 * st, 0.05 * st, 0.05 * st, 0.95 * st, 0.95 * st, 0.95 * st);
+    fprintf(aux, "material slab_boundary\ngeometryType imaginary\nmediumId slab\npriority 1\n");
+    fclose(aux);
+    return UPBP_OK;
+}
+
 } // namespace
 
 extern "C" int upbp_generate_scene(const char* kind, uint32_t seed, const char* obj_path)
@@ -335,5 +417,7 @@ extern "C" int upbp_generate_scene(const char* kind, uint32_t seed, const char* 
     if (k == "furnace") return gen_box(obj_path, seed, true);
     if (k == "boxfog") return gen_box(obj_path, seed, false, true);
     if (k == "blobs") return gen_blobs(obj_path, seed);
+    if (k == "slabroom") return gen_slabroom(obj_path, seed, 128);
+    if (k == "slabroom_small") return gen_slabroom(obj_path, seed, 8); // tests: same room, coarse walls
     return UPBP_E_ARG;
 }

@@ -34,6 +34,16 @@ class Renderer:
         self.iterations += 1
         return st
 
+    def set_guide(self, **kw):
+        """Learned proposal trained online (fields of upbp_guide_config; unnamed ones default)."""
+        g = capi.GuideConfig()
+        capi.lib().upbp_guide_defaults(C.byref(g))
+        for k, v in kw.items():
+            setattr(g, k, v)
+        rc = capi.lib().upbp_set_guide(self.h, C.byref(g))
+        if rc != 0:
+            raise RuntimeError("upbp_set_guide failed: %s" % capi.lib().upbp_last_error(self.h).decode())
+
     def set_guide_lobes(self, lobes, res, lo, hi, alpha):
         """Install explicit learned-proposal lobes ((res^3 * 6, 8, 5): mu.xyz, kappa, weight)."""
         lobes = np.ascontiguousarray(lobes, np.float32)

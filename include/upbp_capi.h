@@ -178,6 +178,9 @@ typedef struct upbp_iter_stats {
     uint64_t vertex_drops, beam_drops, path_errors;
     double algorithmic_bytes;      /* compulsory HBM bytes of this iteration */
     double dominant_bytes;         /* compulsory HBM bytes of the dominant kernel */
+    uint64_t guide_samples;        /* learned proposal: training samples collected */
+    double guide_loss;             /* mean training loss of the iteration's Adam steps */
+    double ms_guide;               /* training and record refresh time */
 } upbp_iter_stats;
 
 typedef struct upbp_scene_handle upbp_scene_handle;
@@ -288,6 +291,26 @@ int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
  * [1e-2, 1e4], weights normalised. alpha 0 switches the proposal off (the plain renderer). */
 int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
                          float alpha);
+
+/* The learned proposal trained online: camera paths credit what they collect at a vertex
+ * (emission, next-event estimation, connections, point and B-P2D merges) to their previous
+ * medium scattering; after each iteration the network takes steps_per_iteration Adam steps
+ * on batches of those samples and the grid records are re-evaluated from it. The grid covers
+ * the bounding cube of the scene sphere. The first iteration samples the phase function only
+ * (the untrained network is nearly uniform); alpha applies from the first refresh on. alpha
+ * 0 switches the proposal off. */
+typedef struct upbp_guide_config {
+    float alpha;                 /* mixture probability (default 0.5) */
+    int32_t grid_res;            /* record grid cells per axis (default 16, at most 64) */
+    int32_t train;               /* 1: train online (default); 0: keep the network as initialised */
+    int32_t steps_per_iteration; /* Adam steps after each iteration (default 4) */
+    int32_t batch;               /* samples per step (default 65536) */
+    float learning_rate;         /* Adam step size (default 1e-3) */
+    uint32_t seed;               /* He initialisation (default 3) */
+    uint32_t max_samples;        /* training samples kept per iteration (default 4M) */
+} upbp_guide_config;
+void upbp_guide_defaults(upbp_guide_config* g);
+int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g);
 
 /* The proposal's network (BF16 MFMA forward, backward and Adam with FP32 accumulation and an
  * FP32 master copy): 51 inputs (48 Fourier features of the position in [0,1]^3 and the arrival

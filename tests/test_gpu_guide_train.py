@@ -1,0 +1,46 @@
+"""Online training of the learned proposal inside the renderer (GPU): samples are collected,
+the loss falls, and the guided renderer converges to the plain one."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import box_args
+from gpuupbp_amd import capi
+from gpuupbp_amd import renderer as gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(obj, args, iters, guide=None, first=0):
+    rc, cfg = capi.parse_args(["-s", "-1", obj] + args)
+    assert rc == 0
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+    with gpu.Renderer(sh.ptr, cfg) as r:
+        if guide is not None:
+            r.set_guide(**guide)
+        stats = [r.run_iteration(first + i) for i in range(iters)]
+        img = r.read_accum(iters)
+    return img, stats
+
+
+def test_training_collects_samples_and_lowers_the_loss(tmp_path):
+    obj = capi.generate_scene("slabroom_small", 3, str(tmp_path / "slabroom.obj"))
+    args = ["-r", "128x72", "-l", "12", "-r_initial", "0.01"]
+    img, stats = _run(obj, args, 16, guide=dict(alpha=0.5, grid_res=8, steps_per_iteration=8, batch=16384,
+                                                learning_rate=3e-3))
+    assert all(s.path_errors == 0 for s in stats) and np.isfinite(img).all()
+    assert stats[0].guide_samples > 2000, stats[0].guide_samples
+    losses = [s.guide_loss for s in stats]
+    assert np.mean(losses[-4:]) < np.mean(losses[:2]) - 0.1, losses
+
+
+def test_guided_renderer_converges_to_the_plain_one(boxfog_obj):
+    args = box_args("48x48", extra=("-r_initial", "0.03", "-l", "6"))
+    plain, _ = _run(boxfog_obj, args, 48)
+    guided, st = _run(boxfog_obj, args, 48, guide=dict(alpha=0.5, grid_res=4, steps_per_iteration=4, batch=8192))
+    assert all(s.path_errors == 0 for s in st)
+    assert abs(guided.mean() - plain.mean()) / plain.mean() < 0.02, (guided.mean(), plain.mean())
+    gb = guided.reshape(6, 8, 6, 8, 3).mean(axis=(1, 3))
+    pb = plain.reshape(6, 8, 6, 8, 3).mean(axis=(1, 3))
+    assert np.median(np.abs(gb - pb) / (pb + 1e-2)) < 0.05
