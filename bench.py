@@ -55,6 +55,59 @@ def cpu_baseline(scene_ptr, cfg):
     }
 
 
+LEARNED_ARGS = ["-l", "16", "-r_initial", "0.01", "-a", "upbp_all"]
+
+
+def equal_time(seconds, ref_seconds, workdir, res, guide):
+    """Learned proposal vs plain renderer at equal wall-clock time on the slab-room workload
+    (196k triangles, dense g=0.9 slab lit through a 5% aperture): relMSE of each against a
+    plain render of ref_seconds with other random streams. The learned run's time includes its
+    online training. Returns the result dictionary."""
+    import numpy as np
+    from gpuupbp_amd import capi
+    from gpuupbp_amd.renderer import Renderer
+    obj = os.path.join(workdir, "slabroom.obj")
+    if not os.path.exists(obj + ".aux"):
+        capi.generate_scene("slabroom", 3, obj)
+    rc, cfg = capi.parse_args(["-s", "-1", obj, "-r", "%dx%d" % res] + LEARNED_ARGS)
+    assert rc == 0
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+
+    def run(sec, g, first, label):
+        r = Renderer(sh.ptr, cfg)
+        if g is not None:
+            r.set_guide(**g)
+        r.sync()
+        t0 = time.perf_counter()
+        n, last = 0, t0
+        while True:
+            r.run_iteration(first + n, want_stats=False)
+            r.sync()
+            n += 1
+            now = time.perf_counter()
+            if now - last > 20:
+                print("bench: %s %d iterations, %.0f s" % (label, n, now - t0), file=sys.stderr, flush=True)
+                last = now
+            if now - t0 >= sec:
+                break
+        img = r.read_accum(n)
+        r.close()
+        return img, n, time.perf_counter() - t0
+
+    ref, n_ref, _ = run(ref_seconds, None, 1 << 20, "reference")
+    plain, n_plain, t_plain = run(seconds, None, 0, "plain")
+    learned, n_learned, t_learned = run(seconds, guide, 0, "learned")
+    rel = lambda im: float(np.mean((im - ref) ** 2 / (ref ** 2 + 1e-2)))
+    rp, rl = rel(plain), rel(learned)
+    return {"workload": "slabroom: 194,970 tris, slab sigma_t 7.23 (U[2,10], seed 3), albedo 0.95, g 0.9, "
+                        "5%% aperture, %dx%d, l=16, r=0.01, upbp_all" % res,
+            "seconds": seconds, "reference_seconds": ref_seconds, "reference_iterations": n_ref,
+            "iterations_plain": n_plain, "iterations_learned": n_learned,
+            "wall_plain_s": round(t_plain, 2), "wall_learned_s": round(t_learned, 2),
+            "relmse_plain": rp, "relmse_learned": rl, "relmse_ratio_plain_over_learned": rp / rl if rl > 0 else None,
+            "guide": guide}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (scaled by the copy-kernel calibration), or None."""
@@ -79,6 +132,14 @@ def main():
     ap.add_argument("--piece", type=float, default=None, help="query piece length in merge radii")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel run once for bandwidth and counter calibration (0: skip)")
+    ap.add_argument("--learned", type=float, default=None,
+                    help="seconds per renderer of the learned-vs-plain equal-time comparison (slab room)")
+    ap.add_argument("--ref-seconds", type=float, default=None, help="reference render time (default 5 x --learned)")
+    ap.add_argument("--learned-res", default="1280x720")
+    ap.add_argument("--alpha", type=float, default=0.5)
+    ap.add_argument("--guide-res", type=int, default=16)
+    ap.add_argument("--guide-steps", type=int, default=4)
+    ap.add_argument("--guide-lr", type=float, default=1e-3)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the iteration-sharded multi-GPU path is not built", file=sys.stderr)
@@ -89,6 +150,13 @@ def main():
 
     workdir = a.workdir or tempfile.mkdtemp(prefix="upbp_bench_")
     os.makedirs(workdir, exist_ok=True)
+    if a.learned is not None:
+        w, h = (int(v) for v in a.learned_res.split("x"))
+        guide = dict(alpha=a.alpha, grid_res=a.guide_res, steps_per_iteration=a.guide_steps,
+                     learning_rate=a.guide_lr, batch=65536)
+        res = equal_time(a.learned, a.ref_seconds or 5 * a.learned, workdir, (w, h), guide)
+        print(json.dumps({"metric": "equal-time relMSE, learned vs plain, 1 MI355X", **res}))
+        return
     obj = make_scene(workdir)
     rc, cfg = capi.parse_args(["-s", "-1", obj] + ARGS)
     assert rc == 0

@@ -153,7 +153,7 @@ _SIGS = [
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_test_env_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
-    ("upbp_test_guide_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_test_guide_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_mlp_param_count", C.c_uint32, []),
     ("upbp_mlp_create", C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
     ("upbp_mlp_destroy", None, [C.c_void_p]),

@@ -223,8 +223,8 @@ UPBP_HD f3 bsdf_evaluate(const Bsdf& b, f3 dir_gen, const DScene& sc, float& cos
     float fp = p, rp = p;
     if (sc.guide.alpha > 0.f) {
         const float a = sc.guide.alpha;
-        if (!b.from_light()) fp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, b.wdir), dir_gen) + (1.f - a) * p;
-        else rp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), -b.wdir) + (1.f - a) * p;
+        if (!b.from_light()) fp = a * guide_pdf(sc.guide, m.mean_cosine, b.pos, b.wdir, dir_gen) + (1.f - a) * p;
+        else rp = a * guide_pdf(sc.guide, m.mean_cosine, b.pos, -dir_gen, -b.wdir) + (1.f - a) * p;
     }
     if (for_pdf) *for_pdf = fp;
     if (rev_pdf) *rev_pdf = rp;
@@ -245,10 +245,11 @@ UPBP_HD float bsdf_pdf_rev(const Bsdf& b, f3 dir_gen, const DScene& sc)
         pdf_phong(b, sc.mats[b.id], lg, &fp, &rp);
         return rp;
     }
-    const float p = hg_pdf(-b.wdir, dir_gen, sc.media[b.id].mean_cosine, nullptr);
+    const float g = sc.media[b.id].mean_cosine;
+    const float p = hg_pdf(-b.wdir, dir_gen, g, nullptr);
     if (!(sc.guide.alpha > 0.f) || !b.from_light()) return p; // light technique: the phase function
     const float a = sc.guide.alpha;
-    return a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), -b.wdir) + (1.f - a) * p;
+    return a * guide_pdf(sc.guide, g, b.pos, -dir_gen, -b.wdir) + (1.f - a) * p;
 }
 
 // kernel/bsdf.cuh:567-660 with the component samplers of :455-559
@@ -341,17 +342,18 @@ UPBP_HD f3 bsdf_sample(const Bsdf& b, float u0, float u1, float u2, f3& dir_gen,
         dir_gen = hg_sample(-b.wdir, m.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
         return m.scattering * pdf;
     }
-    // camera path: one-sample mixture of the learned lobes (probability alpha, chosen by the
+    // camera path: one-sample mixture of the learned product (probability alpha, chosen by the
     // otherwise unused component variate u2) and the phase function
-    const float4* rec = guide_record(g, b.pos, b.wdir);
+    const float4* rec = guide_record(g, b.pos);
+    const PhaseLobes h = phase_lobes(b.wdir, m.mean_cosine);
     float hg;
     if (u2 < g.alpha) {
-        dir_gen = vmf_mix_sample(rec, u2 / g.alpha, u0, u1);
+        dir_gen = vmf_prod_sample(rec, h, u2 / g.alpha, u0, u1);
         hg = hg_pdf(-b.wdir, dir_gen, m.mean_cosine, &sin_theta);
     } else {
         dir_gen = hg_sample(-b.wdir, m.mean_cosine, u0, u1, b.frame, hg, &sin_theta);
     }
-    pdf = g.alpha * vmf_mix_pdf(rec, dir_gen) + (1.f - g.alpha) * hg;
+    pdf = g.alpha * vmf_prod_pdf(rec, h, dir_gen) + (1.f - g.alpha) * hg;
     return m.scattering * hg;
 }
 

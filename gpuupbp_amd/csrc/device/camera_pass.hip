@@ -155,7 +155,7 @@ struct CamCounters {
 
 // One segment of kernel/trace_camera.cu:79-382. Returns whether the path continues.
 __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b, uint32_t pix, int bounce, Sub& s,
-                            uint32_t& pflags, f3& color, int& ret, CamCounters& cc, float& pre_scatter_lum)
+                            uint32_t& pflags, f3& color, int& ret, CamCounters& cc, float& guide_den)
 {
     const bool origin_in_medium = (pflags & PF_ORIGIN_IN_MEDIUM) != 0;
     bool only_spec_surf = (pflags & PF_ONLY_SPEC_SURF) != 0;
@@ -353,8 +353,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     }
     bool cont;
     Rng srng(kDomainCamera, pix, p.iter_key, seg, P_SCATTER);
-    pre_scatter_lum = luminance(s.throughput);
-    ret = sample_scattering(sc, p, cont, bs, hitp, is, s, srng);
+    ret = sample_scattering(sc, p, cont, bs, hitp, is, s, srng, b.gsamp ? &guide_den : nullptr);
     if (ret != S_OK) return false;
     if (!cont) return false;
     if (bs.on_surface()) {
@@ -369,25 +368,31 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     return true;
 }
 
-// Training data of the learned proposal. What a camera path collects at a vertex (emission,
-// next-event estimation, connections, point and B-P2D merges) is credited to its previous
-// medium scattering: the sample (position, arrival, scattered direction) gets the weight
-// lum(collected) / lum(throughput before that scattering), an estimate of incident radiance
-// times phase over the sampling pdf, which is what the KL gradient of the proposal needs. The
-// segment merges of the bounce are not seen here and do not contribute.
-__device__ __noinline__ void guide_record_sample(const DScene& sc, const Bufs& b, uint32_t pix, f3 collected,
-                                                 bool cont, f3 arrive, const Sub& s, float pre_lum)
+// Training data of the learned proposal. A medium scattering of a camera path that continues
+// is credited with everything the path collects after it: emission, next-event estimation,
+// connections and point merges at later vertices (the path colour) and the P-B2D / B-B1D
+// merges along its later segments (the merge colour). With throughput T before the scattering,
+// scattering function f, continuation probability cp and sampling pdf p, the path carries
+// T f / (p cp) onward, so lum(collected after) cp / lum(T f) estimates L_i / p: the incident
+// radiance along the sampled direction over its pdf, the weight of the KL gradient that fits
+// the proposal's radiance mixture to L_i. The sample is (position, scattered direction) with
+// the arrival inputs zero: the mixture depends on the position only.
+//
+// The scattering at vertex k waits in the path's pending slot until the next bounce, where the
+// merges along the segment it sampled have run; it then enters the pool with the snapshot
+// lum(path colour after vertex k) + lum(merge colour before segment k + 1's merges).
+// k_guide_finalise turns snapshots into weights once the iteration's colours are final.
+__device__ __noinline__ void guide_record_sample(const DScene& sc, const Bufs& b, uint32_t pix, f3 color,
+                                                 bool cont, const Sub& s, float den)
 {
-    float* pend = b.gpend + size_t(pix) * 10;
+    float* pend = b.gpend + size_t(pix) * kGuidePend;
     bool emit = false;
-    float rec[10];
+    float rec[kGuideRaw];
     if (pend[9] > 0.f) {
-        const float w = luminance(collected) / pend[9];
-        if (w > 0.f && isfinite(w)) {
-            for (int k = 0; k < 9; k++) rec[k] = pend[k];
-            rec[9] = w;
-            emit = true;
-        }
+        for (int k = 0; k < 10; k++) rec[k] = pend[k];
+        rec[10] = pend[10] + luminance(mk(b.mcolor[3 * pix], b.mcolor[3 * pix + 1], b.mcolor[3 * pix + 2]));
+        rec[11] = __uint_as_float(pix);
+        emit = true;
         pend[9] = 0.f;
     }
     const uint64_t m = __ballot(emit);
@@ -399,22 +404,49 @@ __device__ __noinline__ void guide_record_sample(const DScene& sc, const Bufs& b
         base = __shfl(base, leader);
         const uint32_t at = base + uint32_t(__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane)))));
         if (emit && at < b.gcap)
-            for (int k = 0; k < 10; k++) b.gsamp[size_t(at) * 10 + k] = rec[k];
+            for (int k = 0; k < kGuideRaw; k++) b.graw[size_t(at) * kGuideRaw + k] = rec[k];
     }
     // a medium scattering that continues becomes the pending record
-    if (cont && s.sw.w.prev_in_medium && pre_lum > 0.f) {
+    if (cont && s.sw.w.prev_in_medium && den > 0.f) {
         const DGuide& g = sc.guide;
         const float inv_res = 1.f / float(g.res);
         pend[0] = (s.origin.x - g.lo.x) * g.inv_cell.x * inv_res;
         pend[1] = (s.origin.y - g.lo.y) * g.inv_cell.y * inv_res;
         pend[2] = (s.origin.z - g.lo.z) * g.inv_cell.z * inv_res;
-        pend[3] = arrive.x;
-        pend[4] = arrive.y;
-        pend[5] = arrive.z;
+        pend[3] = pend[4] = pend[5] = 0.f;
         pend[6] = s.dir.x;
         pend[7] = s.dir.y;
         pend[8] = s.dir.z;
-        pend[9] = pre_lum;
+        pend[9] = den;
+        pend[10] = luminance(color);
+    }
+}
+
+// pool of raw records -> training samples (10 floats: position, arrival, scattered, weight) with
+// a positive weight, compacted; after the splat, when path and merge colours are final
+__global__ void __launch_bounds__(256) k_guide_finalise(Bufs b)
+{
+    const uint32_t n = min(b.gcount[0], b.gcap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* r = b.graw + size_t(i) * kGuideRaw;
+        const uint32_t pix = __float_as_uint(r[11]);
+        const float fin = luminance(b.cstate[pix].color) +
+                          luminance(mk(b.mcolor[3 * pix], b.mcolor[3 * pix + 1], b.mcolor[3 * pix + 2]));
+        const float w = (fin - r[10]) / r[9];
+        const bool keep = w > 0.f && isfinite(w);
+        const uint64_t m = __ballot(keep);
+        if (!m) continue;
+        const uint32_t lane = __lane_id();
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (int(lane) == leader) base = atomicAdd(b.gcount + 1, uint32_t(__popcll(m)));
+        base = __shfl(base, leader);
+        const uint32_t at = base + uint32_t(__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane)))));
+        if (keep) {
+            float* o = b.gsamp + size_t(at) * 10;
+            for (int k = 0; k < 9; k++) o[k] = r[k];
+            o[9] = w;
+        }
     }
 }
 
@@ -432,14 +464,13 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams 
     f3 color = st.color;
     int ret = S_OK;
     CamCounters cc;
-    const f3 color_before = color, arrive = s.dir;
-    float pre_lum = 0.f;
-    bool cont = camera_step(sc, p, b, pix, bounce, s, pflags, color, ret, cc, pre_lum);
+    float guide_den = 0.f;
+    bool cont = camera_step(sc, p, b, pix, bounce, s, pflags, color, ret, cc, guide_den);
     if (ret != S_OK) {
         cont = false;
         stat_add(b.stats, STAT_ERRORS, 1);
     }
-    if (b.gsamp) guide_record_sample(sc, b, pix, color - color_before, cont, arrive, s, pre_lum);
+    if (b.gsamp) guide_record_sample(sc, b, pix, color, cont, s, guide_den);
     stat_add(b.stats, STAT_BPT_CONN, cc.conn);
     stat_add(b.stats, STAT_POINT_Q, cc.pq);
     stat_add(b.stats, STAT_SEG_Q, cc.sq);
@@ -492,6 +523,7 @@ void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float
     hipLaunchKernelGGL(k_camera_splat, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, p, b, res_x, res_y);
     const uint32_t n = 3 * p.n_pixels;
     hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, b.fb32, b.accum, n);
+    if (b.gsamp) hipLaunchKernelGGL(k_guide_finalise, dim3(1024), dim3(256), 0, s, b);
 }
 
 } // namespace upbp

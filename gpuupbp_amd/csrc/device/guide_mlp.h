@@ -14,12 +14,13 @@ struct GuideNet {
     upbp_host::DevBuf x0, h1, h2, h3, out;        // activations (BF16) and raw outputs (FP32, rows of 48)
     upbp_host::DevBuf d_o, d3, d2, d1;            // deltas (BF16)
     upbp_host::DevBuf grid_in;                    // record-grid inputs
+    upbp_host::DevBuf batch;                      // training batch drawn from the sample pool
     int step = 0;
 
     ~GuideNet()
     {
         for (upbp_host::DevBuf* b : {&prm, &adam_m, &adam_v, &grad, &wq, &bq, &gW, &gb, &loss, &x0, &h1, &h2, &h3,
-                                     &out, &d_o, &d3, &d2, &d1, &grid_in})
+                                     &out, &d_o, &d3, &d2, &d1, &grid_in, &batch})
             b->release();
     }
     static uint32_t param_count();
@@ -33,8 +34,10 @@ struct GuideNet {
     int forward(const float* samples, int stride, uint32_t n, hipStream_t s);
     // loss sum and flat gradient of n device training samples (10 floats each)
     int gradient(const float* samples, uint32_t n, hipStream_t s);
+    // the same for n samples drawn at random (seeded by draw) from a device pool of n_pool
+    int gradient_drawn(const float* pool, uint32_t n_pool, uint32_t n, uint32_t draw, hipStream_t s);
     int adam_step(float lr, hipStream_t s);
-    // proposal records of a res^3 x 6 grid over [0,1]^3 into recs (device)
+    // proposal records of a res^3 grid over [0,1]^3 into recs (device)
     int records(int res, float4* recs, hipStream_t s);
 };
 

@@ -271,20 +271,37 @@ __global__ void __launch_bounds__(256) k_pack_records(const float* O, uint32_t n
     for (int k = 0; k < kGuideRec; k++) recs[size_t(i) * kGuideRec + k] = rec[k];
 }
 
-// inputs of the record grid: cell centres in [0,1]^3 and the six bin axes
+// inputs of the record grid: cell centres in [0,1]^3; the direction inputs stay zero (the
+// proposal's radiance mixture depends on the position only)
 __global__ void __launch_bounds__(256) k_grid_inputs(int res, float* samples)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = uint32_t(res) * res * res * kGuideBins;
+    const uint32_t n = uint32_t(res) * res * res;
     if (i >= n) return;
-    const uint32_t bin = i % kGuideBins, cell = i / kGuideBins;
-    const uint32_t x = cell % res, y = (cell / res) % res, z = cell / (uint32_t(res) * res);
+    const uint32_t x = i % res, y = (i / res) % res, z = i / (uint32_t(res) * res);
     float* s = samples + size_t(i) * 6;
     s[0] = (float(x) + .5f) / float(res);
     s[1] = (float(y) + .5f) / float(res);
     s[2] = (float(z) + .5f) / float(res);
     s[3] = s[4] = s[5] = 0.f;
-    s[3 + bin / 2] = (bin & 1) ? -1.f : 1.f;
+}
+
+// batch of n training samples drawn uniformly, with replacement, from a pool of n_pool (10
+// floats each); the pool is in append order (pixel and bounce order), so contiguous slices
+// would train on one image region only
+__global__ void __launch_bounds__(256) k_draw_batch(const float* pool, uint32_t n_pool, uint32_t n, uint32_t draw,
+                                                    float* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t z = (uint64_t(draw) << 32 | i) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const uint32_t j = uint32_t(((z >> 32) * uint64_t(n_pool)) >> 32);
+    const float* src = pool + size_t(j) * 10;
+    float* dst = out + size_t(i) * 10;
+    for (int k = 0; k < 10; k++) dst[k] = src[k];
 }
 
 __global__ void __launch_bounds__(256) k_zero(uint32_t* p, size_t n)
@@ -418,6 +435,13 @@ int GuideNet::gradient(const float* samples, uint32_t n, hipStream_t s)
     return hipGetLastError() == hipSuccess ? UPBP_OK : UPBP_E_HIP;
 }
 
+int GuideNet::gradient_drawn(const float* pool, uint32_t n_pool, uint32_t n, uint32_t draw, hipStream_t s)
+{
+    if (n_pool == 0 || batch.ensure(size_t(n) * 40)) return UPBP_E_HIP;
+    hipLaunchKernelGGL(k_draw_batch, dim3((n + 255) / 256), dim3(256), 0, s, pool, n_pool, n, draw, batch.as<float>());
+    return gradient(batch.as<float>(), n, s);
+}
+
 int GuideNet::adam_step(float lr, hipStream_t s)
 {
     step++;
@@ -427,10 +451,10 @@ int GuideNet::adam_step(float lr, hipStream_t s)
     return quantise(s);
 }
 
-// records of a res^3 x 6 grid: the network at cell centres and bin axes, packed
+// records of a res^3 grid: the network at cell centres, packed
 int GuideNet::records(int res, float4* recs, hipStream_t s)
 {
-    const uint32_t n = uint32_t(res) * res * res * kGuideBins;
+    const uint32_t n = uint32_t(res) * res * res;
     if (grid_in.ensure(size_t(n) * 6 * 4)) return UPBP_E_HIP;
     hipLaunchKernelGGL(k_grid_inputs, dim3((n + 255) / 256), dim3(256), 0, s, res, grid_in.as<float>());
     const int e = forward(grid_in.as<float>(), 6, n, s);

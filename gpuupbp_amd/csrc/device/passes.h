@@ -393,10 +393,13 @@ struct Bufs {
     uint32_t* xwork; // per bounce: kXWork words of cell counts, offsets and per-XCD work ranges
     uint32_t seg_cap, piece_cap;
     // learned-proposal training (null when off): per camera path the pending record of its
-    // last medium scattering (position in [0,1]^3, arrival, scattered direction, luminance of
-    // the throughput before it; 10 floats, weight 0 = none), the sample buffer (10 floats each:
-    // position, arrival, scattered, weight) and its counter
+    // last medium scattering (kGuidePend floats: position in [0,1]^3, zero arrival inputs,
+    // scattered direction, training-weight denominator (0 = none), luminance of the path colour
+    // after it), the raw pool (kGuideRaw floats: the pending record with the colour snapshot
+    // completed by the merge colour, then the pixel), the training samples finalised from it
+    // (10 floats: position, arrival, scattered, weight) and the two counters
     float* gpend;
+    float* graw;
     float* gsamp;
     uint32_t* gcount;
     uint32_t gcap;
@@ -787,9 +790,11 @@ struct Sub { // subpath state held in registers within a bounce kernel
     StateWeights sw;
 };
 
-// kernel/functs_shared.cuh:37-154
+// kernel/functs_shared.cuh:37-154. guide_den (optional, camera paths): lum(throughput times
+// the scattering function) / continuation probability of a continuing scattering, the
+// denominator of the learned proposal's training weight.
 __device__ __forceinline__ int sample_scattering(const DScene& sc, const IterParams& p, bool& cont, const Bsdf& b,
-                                                 f3 hitp, const Hit& is, Sub& s, Rng& rng)
+                                                 f3 hitp, const Hit& is, Sub& s, Rng& rng, float* guide_den = nullptr)
 {
     const float r0 = rng.next(), r1 = rng.next(), r2 = rng.next(), r3 = rng.next();
     float for_pdf, cos_out, sin_theta;
@@ -810,6 +815,7 @@ __device__ __forceinline__ int sample_scattering(const DScene& sc, const IterPar
     }
     for_pdf *= cp;
     rev_pdf *= cp;
+    if (guide_den) *guide_den = luminance(s.throughput * f) / cp;
     Weights& w = s.sw.w;
     const float bpt_i1 = float(!w.prev_specular && !specular);
     if (specular) {

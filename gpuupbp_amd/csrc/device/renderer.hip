@@ -174,7 +174,7 @@ struct upbp_renderer {
     upbp_guide_config gcfg{};
     bool gtrain = false;
     GuideNet* gnet = nullptr;
-    DevBuf gpend, gsamp, gcount;
+    DevBuf gpend, graw, gsamp, gcount;
     uint32_t gcap = 0;
     double gloss = 0.0, gms = 0.0;
     uint64_t gsamples = 0;
@@ -706,7 +706,7 @@ static int train_guide(upbp_renderer* r)
     hipStream_t s = r->stream;
     uint32_t n = 0;
     HIPCHK(hipEventRecord(r->ev[4], s));
-    HIPCHK(hipMemcpyAsync(&n, r->gcount.p, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&n, r->gcount.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     n = std::min(n, r->gcap);
     r->gsamples = n;
@@ -717,8 +717,8 @@ static int train_guide(upbp_renderer* r)
     const uint32_t batch = std::min<uint32_t>(uint32_t(g.batch), n);
     double loss = 0.0;
     for (int k = 0; k < g.steps_per_iteration; k++) {
-        const uint32_t off = uint32_t((uint64_t(k) * batch) % uint64_t(n - batch + 1));
-        if (r->gnet->gradient(r->gsamp.as<float>() + size_t(off) * 10, batch, s) || r->gnet->adam_step(g.learning_rate, s))
+        if (r->gnet->gradient_drawn(r->gsamp.as<float>(), n, batch, uint32_t(r->gnet->step), s) ||
+            r->gnet->adam_step(g.learning_rate, s))
             return fail(r, "guide training failed", UPBP_E_HIP);
         float l = 0.f;
         HIPCHK(hipMemcpyAsync(&l, r->gnet->loss.p, 4, hipMemcpyDeviceToHost, s));
@@ -775,12 +775,13 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     HIPCHK(hipEventRecord(ev[2], s));
     const bool collect = r->gtrain && p.trace_camera;
     b.gpend = collect ? r->gpend.as<float>() : nullptr;
+    b.graw = collect ? r->graw.as<float>() : nullptr;
     b.gsamp = collect ? r->gsamp.as<float>() : nullptr;
     b.gcount = collect ? r->gcount.as<uint32_t>() : nullptr;
     b.gcap = collect ? r->gcap : 0u;
     if (collect) {
-        hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(r->n_pixels) * 10)), dim3(256), 0, s, r->gpend.as<uint32_t>(), 0u,
-                           size_t(r->n_pixels) * 10);
+        hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(r->n_pixels) * kGuidePend)), dim3(256), 0, s,
+                           r->gpend.as<uint32_t>(), 0u, size_t(r->n_pixels) * kGuidePend);
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->gcount.as<uint32_t>(), 0u, size_t(4));
     }
     if (p.trace_camera) {
@@ -931,7 +932,7 @@ extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_
     if (!lobes || !lo || !hi || res < 1 || res > 256) return fail(r, "invalid guide grid", UPBP_E_ARG);
     for (int a = 0; a < 3; a++)
         if (!(hi[a] > lo[a])) return fail(r, "empty guide bounds", UPBP_E_ARG);
-    const size_t n = size_t(res) * res * res * kGuideBins;
+    const size_t n = size_t(res) * res * res;
     std::vector<float4> recs(n * kGuideRec);
     for (size_t i = 0; i < n; i++) pack_guide_record(lobes + i * kGuideLobes * 5, &recs[i * kGuideRec]);
     if (r->guide.ensure(recs.size() * sizeof(float4)) ||
@@ -957,7 +958,7 @@ extern "C" void upbp_guide_defaults(upbp_guide_config* g)
     g->batch = 65536;
     g->learning_rate = 1e-3f;
     g->seed = 3;
-    g->max_samples = 1u << 22;
+    g->max_samples = 1u << 24;
 }
 
 extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
@@ -976,7 +977,7 @@ extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
     const f3 c = r->sc.center;
     const float rad = r->sc.radius;
     const int res = g->grid_res;
-    const size_t nrec = size_t(res) * res * res * kGuideBins;
+    const size_t nrec = size_t(res) * res * res;
     if (r->guide.ensure(nrec * kGuideRec * sizeof(float4))) return fail(r, "guide allocation failed", UPBP_E_HIP);
     DGuide& dg = r->sc.guide;
     dg.recs = r->guide.as<float4>();
@@ -987,8 +988,8 @@ extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
     if (r->gnet->init(g->seed) || r->gnet->records(res, r->guide.as<float4>(), r->stream))
         return fail(r, "guide network setup failed", UPBP_E_HIP);
     r->gcap = g->max_samples;
-    if (r->gpend.ensure(size_t(r->n_pixels) * 10 * 4) || r->gsamp.ensure(size_t(r->gcap) * 10 * 4) ||
-        r->gcount.ensure(16))
+    if (r->gpend.ensure(size_t(r->n_pixels) * kGuidePend * 4) || r->graw.ensure(size_t(r->gcap) * kGuideRaw * 4) ||
+        r->gsamp.ensure(size_t(r->gcap) * 10 * 4) || r->gcount.ensure(16))
         return fail(r, "guide allocation failed", UPBP_E_HIP);
     r->gtrain = g->train != 0;
     // an untrained network proposes nearly uniform lobes: the first iteration samples the phase
@@ -1040,7 +1041,7 @@ extern "C" void upbp_destroy(upbp_renderer* r)
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
                       &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
-                      &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->gsamp, &r->gcount})
+                      &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount})
         d->release();
     delete r->gnet;
     r->gnet = nullptr;

@@ -416,20 +416,22 @@ extern "C" int upbp_test_env_samples(const upbp_env_map* env, const float* u, ui
     return rc ? UPBP_E_HIP : UPBP_OK;
 }
 
-__global__ void k_guide_samples(const float4* rec, const float* u, uint32_t n, float* out)
+__global__ void k_guide_samples(const float4* rec, f3 arrive, float g, const float* u, uint32_t n, float* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const f3 d = vmf_mix_sample(rec, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+    const PhaseLobes h = phase_lobes(arrive, g);
+    const f3 d = vmf_prod_sample(rec, h, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
     out[4 * i] = d.x;
     out[4 * i + 1] = d.y;
     out[4 * i + 2] = d.z;
-    out[4 * i + 3] = vmf_mix_pdf(rec, d);
+    out[4 * i + 3] = vmf_prod_pdf(rec, h, d);
 }
 
-extern "C" int upbp_test_guide_samples(const float* lobes, const float* u, uint32_t n, float* out)
+extern "C" int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n,
+                                       float* out)
 {
-    if (!lobes || (n && (!u || !out))) return UPBP_E_ARG;
+    if (!lobes || !arrive || (n && (!u || !out))) return UPBP_E_ARG;
     if (n == 0) return UPBP_OK;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPBP_E_NODEVICE;
@@ -442,7 +444,8 @@ extern "C" int upbp_test_guide_samples(const float* lobes, const float* u, uint3
     if (!drec || !du || !dout) return UPBP_E_HIP;
     (void)hipMemcpy(drec, rec, sizeof(rec), hipMemcpyHostToDevice);
     (void)hipMemcpy(du, u, 3 * size_t(n) * 4, hipMemcpyHostToDevice);
-    hipLaunchKernelGGL(k_guide_samples, dim3((n + 255) / 256), dim3(256), 0, 0, drec, du, n, dout);
+    hipLaunchKernelGGL(k_guide_samples, dim3((n + 255) / 256), dim3(256), 0, 0, drec, mk(arrive[0], arrive[1], arrive[2]),
+                       g, du, n, dout);
     int rc = int(hipDeviceSynchronize());
     if (!rc) rc = int(hipMemcpy(out, dout, 4 * size_t(n) * 4, hipMemcpyDeviceToHost));
     return rc ? UPBP_E_HIP : UPBP_OK;

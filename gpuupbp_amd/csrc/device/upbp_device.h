@@ -241,10 +241,11 @@ struct DEnvMap {
 };
 
 // learned proposal of camera paths in media (guide.h): per (grid cell, arrival bin) record
-constexpr int kGuideLobes = 8, kGuideBins = 6, kGuideRec = 12;
+constexpr int kGuideLobes = 8, kGuideRec = 12;
+constexpr int kGuidePend = 12, kGuideRaw = 12; // floats per pending / raw training record
 constexpr float kGuideKappaMin = 1e-2f, kGuideKappaMax = 1e4f; // lobe concentration range
 struct DGuide {
-    const float4* recs; // res^3 * kGuideBins records of kGuideRec float4
+    const float4* recs; // res^3 records of kGuideRec float4, cell (z * res + y) * res + x
     f3 lo, inv_cell;    // grid origin, cells per unit length per axis
     int res;            // cells per axis
     float alpha;        // probability of sampling the mixture at camera medium vertices (0: off)

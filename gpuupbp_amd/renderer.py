@@ -45,7 +45,7 @@ class Renderer:
             raise RuntimeError("upbp_set_guide failed: %s" % capi.lib().upbp_last_error(self.h).decode())
 
     def set_guide_lobes(self, lobes, res, lo, hi, alpha):
-        """Install explicit learned-proposal lobes ((res^3 * 6, 8, 5): mu.xyz, kappa, weight)."""
+        """Install explicit learned-proposal lobes ((res^3, 8, 5): mu.xyz, kappa, weight)."""
         lobes = np.ascontiguousarray(lobes, np.float32)
         lo = np.ascontiguousarray(lo, np.float32)
         hi = np.ascontiguousarray(hi, np.float32)
@@ -195,12 +195,14 @@ class Mlp:
         return loss.value
 
 
-def guide_samples(lobes, u):
-    """GPU sampling of one learned-proposal mixture (8, 5) at u (n, 3): (n, 4) direction, pdf."""
+def guide_samples(lobes, u, arrive=(0.0, 0.0, 1.0), g=0.0):
+    """GPU sampling of one learned-proposal record (8, 5) times the phase lobe of a path arriving
+    along `arrive` in a medium of mean cosine g, at u (n, 3): (n, 4) direction, pdf."""
     lobes = np.ascontiguousarray(lobes, np.float32)
+    arrive = np.ascontiguousarray(arrive, np.float32)
     u = np.ascontiguousarray(u, np.float32)
     out = np.zeros((u.shape[0], 4), np.float32)
-    rc = capi.lib().upbp_test_guide_samples(_p(lobes), _p(u), u.shape[0], _p(out))
+    rc = capi.lib().upbp_test_guide_samples(_p(lobes), _p(arrive), g, _p(u), u.shape[0], _p(out))
     if rc != 0:
         raise RuntimeError("upbp_test_guide_samples failed: %d" % rc)
     return out

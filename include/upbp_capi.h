@@ -281,14 +281,16 @@ void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
 
 /* ---- learned scattering proposal (no counterpart in the reference) -------
- * Camera paths in media sample, with probability alpha, a mixture of eight von Mises-Fisher
- * lobes looked up on a res^3 grid over [lo, hi] with six bins of the arrival direction (the
- * dominant axis and its sign, bin = 2 * axis + (component < 0)), else the phase function. The
- * mixture pdf enters every camera-technique pdf (forward at camera vertices, reverse at light
- * vertices, connections and merges), so the estimator stays unbiased and MIS consistent.
- * upbp_set_guide_lobes installs explicit lobes: res^3 * 6 records, record ((z * res + y) * res
- * + x) * 6 + bin, of 8 lobes x (mu.xyz, kappa, weight); mu is normalised, kappa clamped to
- * [1e-2, 1e4], weights normalised. alpha 0 switches the proposal off (the plain renderer). */
+ * Camera paths in media sample, with probability alpha, the product of a learned mixture of
+ * eight von Mises-Fisher lobes (incident radiance, looked up on a res^3 grid over [lo, hi]) and
+ * the vMF lobe matching the medium's phase function (mean direction along the arrival for
+ * g >= 0, against it for g < 0, concentration r (3 - r^2) / (1 - r^2) with r = min(|g|, 0.999)),
+ * else the phase function. The product's pdf enters every camera-technique pdf (forward at
+ * camera vertices, reverse at light vertices, connections and merges), so the estimator stays
+ * unbiased and MIS consistent.
+ * upbp_set_guide_lobes installs explicit lobes: res^3 records, record (z * res + y) * res + x,
+ * of 8 lobes x (mu.xyz, kappa, weight); mu is normalised, kappa clamped to [1e-2, 1e4],
+ * weights normalised. alpha 0 switches the proposal off (the plain renderer). */
 int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
                          float alpha);
 
@@ -307,7 +309,7 @@ typedef struct upbp_guide_config {
     int32_t batch;               /* samples per step (default 65536) */
     float learning_rate;         /* Adam step size (default 1e-3) */
     uint32_t seed;               /* He initialisation (default 3) */
-    uint32_t max_samples;        /* training samples kept per iteration (default 4M) */
+    uint32_t max_samples;        /* training samples kept per iteration (default 16M) */
 } upbp_guide_config;
 void upbp_guide_defaults(upbp_guide_config* g);
 int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g);
@@ -370,10 +372,12 @@ int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const float* ray
  * lookup of that direction: pdf, radiance (3). */
 int upbp_test_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out);
 
-/* One learned-proposal mixture (8 x (mu.xyz, kappa, weight), packed as upbp_set_guide_lobes
- * packs it) sampled with u (3 per sample: lobe variate, two direction variates): out 4 per
- * sample, the direction and the mixture pdf there. */
-int upbp_test_guide_samples(const float* lobes, const float* u, uint32_t n, float* out);
+/* One learned-proposal record (8 x (mu.xyz, kappa, weight), packed as upbp_set_guide_lobes
+ * packs it) times the phase lobe of a path arriving along `arrive` in a medium of mean cosine
+ * g, sampled with u (3 per sample: lobe variate, two direction variates): out 4 per sample, the
+ * direction and the product's pdf there. */
+int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n,
+                            float* out);
 
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM

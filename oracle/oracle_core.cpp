@@ -474,7 +474,7 @@ void init_guide(Guide& g, const float* lobes, int res, const float* lo, const fl
     g.res = res;
     g.lo = fromp(lo);
     g.inv_cell = {float(res) / (hi[0] - lo[0]), float(res) / (hi[1] - lo[1]), float(res) / (hi[2] - lo[2])};
-    const size_t n = size_t(res) * res * res * 6;
+    const size_t n = size_t(res) * res * res;
     g.lobes.resize(n * 8);
     for (size_t r = 0; r < n; r++) {
         const float* in = lobes + r * 40;
@@ -494,49 +494,101 @@ void init_guide(Guide& g, const float* lobes, int res, const float* lo, const fl
 
 static int axis_cell(float f, int r) { return f < 0.f ? 0 : (f >= float(r) ? r - 1 : int(f)); }
 
-const GuideLobe* guide_record(const Guide& g, V p, V d)
+const GuideLobe* guide_record(const Guide& g, V p)
 {
     const int r = g.res;
     const int ix = axis_cell((p.x - g.lo.x) * g.inv_cell.x, r), iy = axis_cell((p.y - g.lo.y) * g.inv_cell.y, r),
               iz = axis_cell((p.z - g.lo.z) * g.inv_cell.z, r);
-    const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
-    const int bin = (ax >= ay && ax >= az) ? (d.x < 0.f ? 1 : 0) : (ay >= az ? (d.y < 0.f ? 3 : 2) : (d.z < 0.f ? 5 : 4));
-    return &g.lobes[(size_t((iz * r + iy) * r + ix) * 6 + size_t(bin)) * 8];
+    return &g.lobes[size_t((iz * r + iy) * r + ix) * 8];
 }
 
-float vmf_mix_pdf(const GuideLobe* rec, V w)
-{
-    float pdf = 0.f;
-    for (int i = 0; i < 8; i++) pdf += rec[i].c * std::exp(rec[i].kappa * (dot(rec[i].mu, w) - 1.f));
-    return pdf;
-}
+// the phase function of mean cosine g as three vMF lobes about the arrival direction (against
+// it for g < 0): the fitted table's rows interpolated linearly in min(|g|, 0.99); a_j is the
+// lobe weight over the lobe's sphere integral
+static float sphere_integral(float k) { return k > 1e-6f ? 2.f * kPi * -std::expm1(-2.f * k) / k : 4.f * kPi; }
 
-// Jakob 2012, "Numerically stable sampling of the von Mises Fisher distribution on S2"
-V vmf_mix_sample(const GuideLobe* rec, float us, float u0, float u1)
+PhaseFit phase_fit(V arrive, float mean_cosine)
 {
-    int pick = 7;
-    float acc = 0.f;
-    for (int i = 0; i < 8; i++) {
-        acc += rec[i].w;
-        if (us < acc) {
-            pick = i;
-            break;
-        }
+    PhaseFit h;
+    h.mu = mean_cosine < 0.f ? -arrive : arrive;
+    const float t = std::fmin(std::fabs(mean_cosine), 0.99f) / kPhaseFitStep;
+    const int row = std::min(int(t), kPhaseFitN - 2);
+    const float f = t - float(row);
+    for (int j = 0; j < 3; j++) {
+        h.k[j] = (1.f - f) * kPhaseFit[row][j] + f * kPhaseFit[row + 1][j];
+        h.a[j] = ((1.f - f) * kPhaseFit[row][3 + j] + f * kPhaseFit[row + 1][3 + j]) / sphere_integral(h.k[j]);
     }
-    const GuideLobe& l = rec[pick];
-    const float ct = 1.f + std::log(u0 + (1.f - u0) * std::exp(-2.f * l.kappa)) / l.kappa;
+    return h;
+}
+
+// lobe (i, j) of the product: concentration |kappa_i mu_i + k_j mu_h| and its unnormalised mass
+static float prod_mass(const GuideLobe& l, const PhaseFit& h, int j, float* kp_out)
+{
+    const float lh = dot(l.mu, h.mu);
+    const float kp = std::sqrt(std::fmax(0.f, l.kappa * l.kappa + h.k[j] * h.k[j] + 2.f * l.kappa * h.k[j] * lh));
+    if (kp_out) *kp_out = kp;
+    return l.c * h.a[j] * std::exp(kp - l.kappa - h.k[j]) * sphere_integral(kp);
+}
+
+float guide_prod_pdf(const GuideLobe* rec, const PhaseFit& h, V w)
+{
+    float num = 0.f, z = 0.f;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 3; j++) {
+            num += rec[i].c * h.a[j] *
+                   std::exp(rec[i].kappa * dot(rec[i].mu, w) + h.k[j] * dot(h.mu, w) - rec[i].kappa - h.k[j]);
+            z += prod_mass(rec[i], h, j, nullptr);
+        }
+    return z > 0.f ? num / z : 0.f;
+}
+
+// product lobe by mass with us; its direction by Jakob 2012 ("Numerically stable sampling of
+// the von Mises Fisher distribution on S2") as cos = 1 + log1p((1 - u0) expm1(-2 k)) / k
+V guide_prod_sample(const GuideLobe* rec, const PhaseFit& h, float us, float u0, float u1)
+{
+    float tot = 0.f;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 3; j++) tot += prod_mass(rec[i], h, j, nullptr);
+    int pi = 7, pj = 2;
+    float acc = 0.f;
+    bool found = false;
+    for (int i = 0; i < 8 && !found; i++)
+        for (int j = 0; j < 3; j++) {
+            acc += prod_mass(rec[i], h, j, nullptr);
+            if (us * tot < acc) {
+                pi = i;
+                pj = j;
+                found = true;
+                break;
+            }
+        }
+    const GuideLobe& l = rec[pi];
+    const V v = l.mu * l.kappa + h.mu * h.k[pj];
+    const float kv = std::sqrt(dot(v, v));
+    V mu{0.f, 0.f, 1.f};
+    float k = 1e-6f;
+    if (kv > 1e-6f) {
+        k = kv;
+        mu = v / kv;
+    }
+    const float ct = std::fmax(-1.f, 1.f + std::log1p((1.f - u0) * std::expm1(-2.f * k)) / k);
     const float st = std::sqrt(std::fmax(0.f, 1.f - ct * ct));
     const float phi = 2.f * kPi * u1;
     Frame f;
-    f.set_from_z(l.mu);
+    f.set_from_z(mu);
     return f.to_world({std::cos(phi) * st, std::sin(phi) * st, ct});
+}
+
+float guide_pdf(const Guide& g, float mean_cosine, V p, V arrive, V gen)
+{
+    return guide_prod_pdf(guide_record(g, p), phase_fit(arrive, mean_cosine), gen);
 }
 
 float cam_medium_pdf(const Scene& sc, float mean_cosine, V p, V arrive, V gen)
 {
     const float hg = hg_pdf(-arrive, gen, mean_cosine, nullptr);
     if (!(sc.guide.alpha > 0.f)) return hg;
-    return sc.guide.alpha * vmf_mix_pdf(guide_record(sc.guide, p, arrive), gen) + (1.f - sc.guide.alpha) * hg;
+    return sc.guide.alpha * guide_pdf(sc.guide, mean_cosine, p, arrive, gen) + (1.f - sc.guide.alpha) * hg;
 }
 
 // kernel/bsdf.cuh:334-396
@@ -570,8 +622,8 @@ V bsdf_evaluate(const BSDF& b, V dir_gen, const Scene& sc, float& cos_gen, float
     float fp = p, rp = p; // the camera technique's side follows the learned mixture
     const float a = sc.guide.alpha;
     if (a > 0.f) {
-        if (!b.from_light()) fp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -b.world_dir_fix), dir_gen) + (1.f - a) * p;
-        else rp = a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), b.world_dir_fix) + (1.f - a) * p;
+        if (!b.from_light()) fp = a * guide_pdf(sc.guide, b.mean_cosine, b.pos, -b.world_dir_fix, dir_gen) + (1.f - a) * p;
+        else rp = a * guide_pdf(sc.guide, b.mean_cosine, b.pos, -dir_gen, b.world_dir_fix) + (1.f - a) * p;
     }
     if (for_pdf) *for_pdf = fp;
     if (rev_pdf) *rev_pdf = rp;
@@ -599,8 +651,8 @@ float bsdf_pdf(const BSDF& b, V dir_gen, const Scene& sc, bool reverse)
     // technique (the learned mixture); the other two to the phase function
     const bool camera_technique = b.from_light() ? reverse : !reverse;
     if (!(a > 0.f) || !camera_technique) return p;
-    if (!b.from_light()) return a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -b.world_dir_fix), dir_gen) + (1.f - a) * p;
-    return a * vmf_mix_pdf(guide_record(sc.guide, b.pos, -dir_gen), b.world_dir_fix) + (1.f - a) * p;
+    if (!b.from_light()) return a * guide_pdf(sc.guide, b.mean_cosine, b.pos, -b.world_dir_fix, dir_gen) + (1.f - a) * p;
+    return a * guide_pdf(sc.guide, b.mean_cosine, b.pos, -dir_gen, b.world_dir_fix) + (1.f - a) * p;
 }
 
 // kernel/bsdf.cuh:455-559 (component samplers)
@@ -714,16 +766,17 @@ V bsdf_sample(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pd
         dir_gen = hg_sample(b.world_dir_fix, b.mean_cosine, u0, u1, b.frame, pdf, &sin_theta);
         return b.scatter_coef * pdf;
     }
-    // camera path: the learned mixture with probability alpha (variate u2), else the phase function
-    const GuideLobe* rec = guide_record(sc.guide, b.pos, -b.world_dir_fix);
+    // camera path: the learned product with probability alpha (variate u2), else the phase function
+    const GuideLobe* rec = guide_record(sc.guide, b.pos);
+    const PhaseFit h = phase_fit(-b.world_dir_fix, b.mean_cosine);
     float hg;
     if (u2 < a) {
-        dir_gen = vmf_mix_sample(rec, u2 / a, u0, u1);
+        dir_gen = guide_prod_sample(rec, h, u2 / a, u0, u1);
         hg = hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, &sin_theta);
     } else {
         dir_gen = hg_sample(b.world_dir_fix, b.mean_cosine, u0, u1, b.frame, hg, &sin_theta);
     }
-    pdf = a * vmf_mix_pdf(rec, dir_gen) + (1.f - a) * hg;
+    pdf = a * guide_prod_pdf(rec, h, dir_gen) + (1.f - a) * hg;
     return b.scatter_coef * hg;
 }
 

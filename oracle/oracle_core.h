@@ -8,6 +8,8 @@
 
 #include "oracle.h"
 
+#include "../gpuupbp_amd/csrc/device/phase_lobes.h"
+
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -204,8 +206,9 @@ struct Sph {
     int mat, light;
 };
 
-// Learned proposal of camera paths in media: res^3 cells x 6 arrival-direction bins, each a
-// mixture of 8 vMF lobes (no counterpart in the reference). alpha 0: off.
+// Learned proposal of camera paths in media: res^3 cells, each a mixture of 8 vMF lobes of
+// incident radiance, sampled as its product with the vMF lobe fitted to the phase function
+// (no counterpart in the reference). alpha 0: off.
 struct GuideLobe {
     V mu;
     float kappa, w, c; // concentration, normalised weight, w kappa / (2 pi (1 - exp(-2 kappa)))
@@ -214,7 +217,7 @@ struct Guide {
     float alpha = 0.f;
     int res = 0;
     V lo{0, 0, 0}, inv_cell{0, 0, 0};
-    std::vector<GuideLobe> lobes; // 8 per record, record ((z * res + y) * res + x) * 6 + bin
+    std::vector<GuideLobe> lobes; // 8 per record, record (z * res + y) * res + x
 };
 
 struct Scene {
@@ -233,13 +236,21 @@ struct Scene {
 };
 
 void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& out);
-// learned proposal: records from (mu, kappa, weight) x 8 per record, and the camera technique's
-// pdf at a medium point (arriving along `arrive`, leaving along `gen`)
+// learned proposal: records from (mu, kappa, weight) x 8 per record; the product of a record
+// with the phase function's three fitted vMF lobes (the table of
+// gpuupbp_amd/csrc/device/phase_lobes.h, shared data); the camera technique's pdf at a medium
+// point (arriving along `arrive`, leaving along `gen`)
 void init_guide(Guide& g, const float* lobes, int res, const float* lo, const float* hi, float alpha);
 extern Guide g_oracle_guide;
-const GuideLobe* guide_record(const Guide& g, V p, V arrive);
-float vmf_mix_pdf(const GuideLobe* rec, V w);
-V vmf_mix_sample(const GuideLobe* rec, float us, float u0, float u1);
+const GuideLobe* guide_record(const Guide& g, V p);
+struct PhaseFit { // h(w) = sum_j a_j exp(k_j (mu . w - 1))
+    V mu;
+    float k[3], a[3];
+};
+PhaseFit phase_fit(V arrive, float mean_cosine);
+float guide_prod_pdf(const GuideLobe* rec, const PhaseFit& h, V w);
+V guide_prod_sample(const GuideLobe* rec, const PhaseFit& h, float us, float u0, float u1);
+float guide_pdf(const Guide& g, float mean_cosine, V p, V arrive, V gen);
 float cam_medium_pdf(const Scene& sc, float mean_cosine, V p, V arrive, V gen);
 // environment map (host/env_map.hpp:42-144, kernel/env_map.cuh:36-137)
 void init_env_map(const upbp_env_map& in, EnvMap& m);

@@ -2165,20 +2165,20 @@ extern "C" int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const fl
     return UPBP_OK;
 }
 
-extern "C" int oracle_guide_samples(const float* lobes, const float* u, uint32_t n, float* out)
+extern "C" int oracle_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n,
+                                    float* out)
 {
-    Guide g;
+    Guide gd;
     const float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
-    std::vector<float> rec(size_t(6) * 40);
-    for (int b = 0; b < 6; b++) memcpy(&rec[size_t(b) * 40], lobes, 40 * sizeof(float));
-    init_guide(g, rec.data(), 1, lo, hi, 1.f);
-    const GuideLobe* r = g.lobes.data();
+    init_guide(gd, lobes, 1, lo, hi, 1.f);
+    const GuideLobe* r = gd.lobes.data();
+    const PhaseFit h = phase_fit({arrive[0], arrive[1], arrive[2]}, g);
     for (uint32_t i = 0; i < n; i++) {
-        const V d = vmf_mix_sample(r, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+        const V d = guide_prod_sample(r, h, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
         out[4 * i] = d.x;
         out[4 * i + 1] = d.y;
         out[4 * i + 2] = d.z;
-        out[4 * i + 3] = vmf_mix_pdf(r, d);
+        out[4 * i + 3] = guide_prod_pdf(r, h, d);
     }
     return UPBP_OK;
 }

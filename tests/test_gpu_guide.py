@@ -1,4 +1,4 @@
-"""GPU parity of the learned scattering proposal: the vMF mixture sampler against the oracle's,
+"""GPU parity of the learned scattering proposal: the vMF product sampler against the oracle's,
 and renders with a proposal installed, path for path against the oracle."""
 import numpy as np
 import pytest
@@ -19,12 +19,15 @@ def _lobes(rng, n, kappa_hi=200.0):
     return np.hstack([mu, kappa, w]).astype(np.float32).reshape(n, 8, 5)
 
 
-def test_mixture_sampler_matches_oracle():
+@pytest.mark.parametrize("mean_cosine", [0.0, 0.9, -0.5])
+def test_product_sampler_matches_oracle(mean_cosine):
     rng = np.random.default_rng(2)
     for lobes in _lobes(rng, 4):
         u = rng.uniform(0, 1, (20000, 3)).astype(np.float32)
-        g = gpu.guide_samples(lobes, u)
-        o = oracle_py.guide_samples(lobes, u)
+        arrive = rng.normal(size=3)
+        arrive /= np.linalg.norm(arrive)
+        g = gpu.guide_samples(lobes, u, arrive, mean_cosine)
+        o = oracle_py.guide_samples(lobes, u, arrive, mean_cosine)
         # same FP32 operations; expf / logf / sinf / cosf may differ in an ulp, which the
         # inversion 1 + log(u + (1 - u) exp(-2 kappa)) / kappa amplifies for small kappa
         dd = np.max(np.abs(g[:, :3] - o[:, :3]), axis=1)
@@ -54,7 +57,7 @@ def test_guided_render_path_parity(boxfog_obj, alpha):
     rc, cfg = capi.parse_args(["-s", "-1", boxfog_obj])
     sh = capi.SceneHandle(boxfog_obj, 64, 64)
     c, r = np.array(list(sh.scene.scene_center), np.float32), sh.scene.scene_radius
-    lobes = _lobes(np.random.default_rng(11), 3 ** 3 * 6, kappa_hi=60.0)
+    lobes = _lobes(np.random.default_rng(11), 3 ** 3, kappa_hi=60.0)
     g, o, st, ost = _render(boxfog_obj, box_args("48x48", extra=("-r_initial", "0.03")), lobes, 3, c - r, c + r,
                             alpha)
     assert st.path_errors == 0 and st.surface_vertices + st.medium_vertices == ost.light_vertices

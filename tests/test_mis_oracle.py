@@ -37,7 +37,7 @@ def _records(scene_dir, kind, beams=(), guide_alpha=0.0):
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
     if guide_alpha > 0:  # learned proposal with random lobes on a 3^3 grid
         rng = np.random.default_rng(5)
-        n = 27 * 6 * 8
+        n = 27 * 8
         mu = rng.normal(size=(n, 3))
         lobes = np.hstack([mu / np.linalg.norm(mu, axis=1, keepdims=True), np.exp(rng.uniform(-2, 4, (n, 1))),
                            rng.uniform(0.05, 1, (n, 1))]).astype(np.float32)
@@ -55,7 +55,7 @@ def _records(scene_dir, kind, beams=(), guide_alpha=0.0):
                                               ("boxfog", ("-qbt", "S", "-pbt", "S"), 0.0),
                                               ("boxfog", ("-qbt", "S", "-pbt", "L"), 0.0), ("boxfog", (), 0.7)])
 def test_constant_time_weights_match_fp64_balance_heuristic(scene_dir, kind, beams, guide):
-    # guide > 0: camera paths sample a learned vMF mixture in media; its pdf must enter the
+    # guide > 0: camera paths sample a learned vMF product in media; its pdf must enter the
     # forward recursion at camera vertices and the reverse one at light vertices
     rec = _records(scene_dir, kind, beams, guide)
     tech, w_ct, w_ref = rec[:, 0].astype(int), rec[:, 3], rec[:, 5]
