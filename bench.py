@@ -61,8 +61,10 @@ LEARNED_ARGS = ["-l", "16", "-r_initial", "0.01", "-a", "upbp_all"]
 def equal_time(seconds, ref_seconds, workdir, res, guide):
     """Learned proposal vs plain renderer at equal wall-clock time on the slab-room workload
     (196k triangles, dense g=0.9 slab lit through a 5% aperture): relMSE of each against a
-    plain render of ref_seconds with other random streams. The learned run's time includes its
-    online training. Returns the result dictionary."""
+    plain reference render of ref_seconds (two halves on other random streams). The learned
+    run's time includes its online training. The reference's own error, estimated from its
+    halves as relMSE(half_a, half_b) / 4, is reported and also taken off both figures.
+    Returns the result dictionary."""
     import numpy as np
     from gpuupbp_amd import capi
     from gpuupbp_amd.renderer import Renderer
@@ -94,17 +96,24 @@ def equal_time(seconds, ref_seconds, workdir, res, guide):
         r.close()
         return img, n, time.perf_counter() - t0
 
-    ref, n_ref, _ = run(ref_seconds, None, 1 << 20, "reference")
+    ref_a, n_a, _ = run(ref_seconds / 2, None, 1 << 20, "reference a")
+    ref_b, n_b, _ = run(ref_seconds / 2, None, 1 << 21, "reference b")
+    ref = (ref_a * n_a + ref_b * n_b) / (n_a + n_b)
+    rel = lambda im, r: float(np.mean((im - r) ** 2 / (r ** 2 + 1e-2)))
+    ref_err = rel(ref_a, ref_b) / 4
     plain, n_plain, t_plain = run(seconds, None, 0, "plain")
     learned, n_learned, t_learned = run(seconds, guide, 0, "learned")
-    rel = lambda im: float(np.mean((im - ref) ** 2 / (ref ** 2 + 1e-2)))
-    rp, rl = rel(plain), rel(learned)
+    rp, rl = rel(plain, ref), rel(learned, ref)
+    cp, cl = rp - ref_err, rl - ref_err
     return {"workload": "slabroom: 194,970 tris, slab sigma_t 7.23 (U[2,10], seed 3), albedo 0.95, g 0.9, "
                         "5%% aperture, %dx%d, l=16, r=0.01, upbp_all" % res,
-            "seconds": seconds, "reference_seconds": ref_seconds, "reference_iterations": n_ref,
+            "seconds": seconds, "reference_seconds": ref_seconds, "reference_iterations": n_a + n_b,
             "iterations_plain": n_plain, "iterations_learned": n_learned,
             "wall_plain_s": round(t_plain, 2), "wall_learned_s": round(t_learned, 2),
             "relmse_plain": rp, "relmse_learned": rl, "relmse_ratio_plain_over_learned": rp / rl if rl > 0 else None,
+            "reference_relmse_estimate": ref_err,
+            "relmse_plain_less_reference": cp, "relmse_learned_less_reference": cl,
+            "relmse_ratio_less_reference": cp / cl if cl > 0 else None,
             "guide": guide}
 
 
@@ -136,10 +145,10 @@ def main():
                     help="seconds per renderer of the learned-vs-plain equal-time comparison (slab room)")
     ap.add_argument("--ref-seconds", type=float, default=None, help="reference render time (default 5 x --learned)")
     ap.add_argument("--learned-res", default="1280x720")
-    ap.add_argument("--alpha", type=float, default=0.5)
-    ap.add_argument("--guide-res", type=int, default=16)
-    ap.add_argument("--guide-steps", type=int, default=4)
-    ap.add_argument("--guide-lr", type=float, default=1e-3)
+    ap.add_argument("--alpha", type=float, default=None, help="learned-proposal mixture weight (library default)")
+    ap.add_argument("--guide-res", type=int, default=None)
+    ap.add_argument("--guide-steps", type=int, default=None)
+    ap.add_argument("--guide-lr", type=float, default=None)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the iteration-sharded multi-GPU path is not built", file=sys.stderr)
@@ -152,8 +161,8 @@ def main():
     os.makedirs(workdir, exist_ok=True)
     if a.learned is not None:
         w, h = (int(v) for v in a.learned_res.split("x"))
-        guide = dict(alpha=a.alpha, grid_res=a.guide_res, steps_per_iteration=a.guide_steps,
-                     learning_rate=a.guide_lr, batch=65536)
+        guide = {k: v for k, v in dict(alpha=a.alpha, grid_res=a.guide_res, steps_per_iteration=a.guide_steps,
+                                       learning_rate=a.guide_lr).items() if v is not None}
         res = equal_time(a.learned, a.ref_seconds or 5 * a.learned, workdir, (w, h), guide)
         print(json.dumps({"metric": "equal-time relMSE, learned vs plain, 1 MI355X", **res}))
         return

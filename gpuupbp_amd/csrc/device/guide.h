@@ -18,15 +18,16 @@
 // so the forward and reverse MIS recursions, the connections and every merge evaluate the same
 // exact pdf the camera paths sample from.
 //
-// Record layout (kGuideRec float4): lobes i = 0..7 as (mu.xyz, kappa); the lobe weights w_i
-// (two float4); the lobe constants c_i = w_i kappa_i / (2 pi (1 - exp(-2 kappa_i))) (two
-// float4), so that the radiance mixture is q(w) = sum_i c_i exp(kappa_i (mu_i . w - 1)).
+// Record layout (kGuideRec float4, 160 bytes): lobes i = 0..7 as (mu.xyz, kappa), then the lobe
+// constants c_i = w_i kappa_i / (2 pi (1 - exp(-2 kappa_i))) (two float4) with the normalised
+// lobe weights w_i, so that the radiance mixture is q(w) = sum_i c_i exp(kappa_i (mu_i . w - 1)).
 //
 // Phase lobes j = 0..2 about mu_h: h(w) = sum_j a_j exp(k_j (mu_h . w - 1)). Lobe (i, j) of the
 // product has v_ij = kappa_i mu_i + k_j mu_h, and the normalised product has the pdf
-//   p(w) = sum_ij c_i a_j exp(v_ij . w - kappa_i - k_j) / Z,
-//   Z = sum_ij c_i a_j exp(|v_ij| - kappa_i - k_j) 2 pi (1 - exp(-2 |v_ij|)) / |v_ij|,
-// its lobe (i, j) chosen with probability (term ij of Z) / Z.
+//   p(w) = q(w) h(w) / Z,
+//   Z = sum_ij c_i a_j 2 pi (exp(|v_ij| - kappa_i - k_j) - exp(-|v_ij| - kappa_i - k_j)) / |v_ij|,
+// its lobe (i, j) chosen with probability (term ij of Z) / Z. The second exponential of a term
+// is below 1e-7 of the first once |v_ij| > 8 and is then left out.
 #pragma once
 
 #include "phase_lobes.h"
@@ -51,6 +52,13 @@ UPBP_HD float lane4(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.
 
 // 2 pi (1 - exp(-2 k)) / k, the integral of exp(k (mu . w - 1)) over the sphere
 UPBP_HD float vmf_mass(float k) { return k > 1e-6f ? 2.f * kPi * -expm1f(-2.f * k) / k : 4.f * kPi; }
+
+// exp(kp - s) vmf_mass(kp): the sphere integral of exp(kp (mu . w - 1) + kp - s)
+UPBP_HD float prod_mass(float kp, float s)
+{
+    if (kp > 8.f) return 2.f * kPi * expf(kp - s) / kp;
+    return expf(kp - s) * vmf_mass(kp);
+}
 
 // the phase function of mean cosine g, for a path travelling along `arrive`, as three vMF lobes
 // about mu (along the arrival for g >= 0, against it for g < 0); the table's rows are
@@ -79,23 +87,26 @@ UPBP_HD PhaseLobes phase_lobes(f3 arrive, float g)
 // allocation of the bounce and merge kernels, which call them only when alpha > 0
 inline __host__ __device__ __noinline__ float vmf_prod_pdf(const float4* rec, PhaseLobes h, f3 w)
 {
-    float num = 0.f, z = 0.f;
+    float q = 0.f, z = 0.f, hv = 0.f;
     const float hw = dot(h.mu, w);
+#pragma unroll
+    for (int j = 0; j < kPhaseLobes; j++) hv += h.a[j] * expf(h.k[j] * (hw - 1.f));
 #pragma unroll
     for (int i = 0; i < kGuideLobes; i++) {
         const float4 l = rec[i];
-        const float c = lane4(rec[kGuideLobes + 2 + i / 4], i & 3);
+        const float c = lane4(rec[kGuideLobes + i / 4], i & 3);
         const float lw = l.x * w.x + l.y * w.y + l.z * w.z, lh = l.x * h.mu.x + l.y * h.mu.y + l.z * h.mu.z;
+        q += c * expf(l.w * (lw - 1.f));
+        float zi = 0.f;
 #pragma unroll
         for (int j = 0; j < kPhaseLobes; j++) {
             // |v|^2 = kappa^2 + k^2 + 2 kappa k (mu_i . mu_h)
             const float kp = sqrtf(fmaxf(0.f, l.w * l.w + h.k[j] * h.k[j] + 2.f * l.w * h.k[j] * lh));
-            const float ca = c * h.a[j];
-            num += ca * expf(l.w * lw + h.k[j] * hw - l.w - h.k[j]);
-            z += ca * expf(kp - l.w - h.k[j]) * vmf_mass(kp);
+            zi += h.a[j] * prod_mass(kp, l.w + h.k[j]);
         }
+        z += c * zi;
     }
-    return z > 0.f ? num / z : 0.f;
+    return z > 0.f ? q * hv / z : 0.f;
 }
 
 // direction of the vMF lobe (mu, kappa) by the inversion of Jakob, "Numerically stable sampling
@@ -119,12 +130,12 @@ inline __host__ __device__ __noinline__ f3 vmf_prod_sample(const float4* rec, Ph
 #pragma unroll
     for (int i = 0; i < kGuideLobes; i++) {
         const float4 l = rec[i];
-        const float c = lane4(rec[kGuideLobes + 2 + i / 4], i & 3);
+        const float c = lane4(rec[kGuideLobes + i / 4], i & 3);
         const float lh = l.x * h.mu.x + l.y * h.mu.y + l.z * h.mu.z;
 #pragma unroll
         for (int j = 0; j < kPhaseLobes; j++) {
             const float kp = sqrtf(fmaxf(0.f, l.w * l.w + h.k[j] * h.k[j] + 2.f * l.w * h.k[j] * lh));
-            tot += c * h.a[j] * expf(kp - l.w - h.k[j]) * vmf_mass(kp);
+            tot += c * h.a[j] * prod_mass(kp, l.w + h.k[j]);
         }
     }
     const float t = us * tot;
@@ -132,11 +143,11 @@ inline __host__ __device__ __noinline__ f3 vmf_prod_sample(const float4* rec, Ph
     float acc = 0.f;
     for (int i = 0; i < kGuideLobes; i++) {
         const float4 l = rec[i];
-        const float c = lane4(rec[kGuideLobes + 2 + i / 4], i & 3);
+        const float c = lane4(rec[kGuideLobes + i / 4], i & 3);
         const float lh = l.x * h.mu.x + l.y * h.mu.y + l.z * h.mu.z;
         for (int j = 0; j < kPhaseLobes; j++) {
             const float kp = sqrtf(fmaxf(0.f, l.w * l.w + h.k[j] * h.k[j] + 2.f * l.w * h.k[j] * lh));
-            acc += c * h.a[j] * expf(kp - l.w - h.k[j]) * vmf_mass(kp);
+            acc += c * h.a[j] * prod_mass(kp, l.w + h.k[j]);
             if (t < acc && pick == kGuideLobes * kPhaseLobes - 1) pick = i * kPhaseLobes + j;
         }
     }
@@ -179,10 +190,8 @@ UPBP_HD void pack_guide_record(const float* in, float4* out)
         w[i] = wsum > 0.f ? fmaxf(l[4], 0.f) / wsum : 1.f / float(kGuideLobes);
         c[i] = w[i] * kappa / (2.f * kPi * -expm1f(-2.f * kappa));
     }
-    out[8] = make_float4(w[0], w[1], w[2], w[3]);
-    out[9] = make_float4(w[4], w[5], w[6], w[7]);
-    out[10] = make_float4(c[0], c[1], c[2], c[3]);
-    out[11] = make_float4(c[4], c[5], c[6], c[7]);
+    out[8] = make_float4(c[0], c[1], c[2], c[3]);
+    out[9] = make_float4(c[4], c[5], c[6], c[7]);
 }
 
 } // namespace upbp

@@ -951,12 +951,12 @@ extern "C" void upbp_guide_defaults(upbp_guide_config* g)
 {
     if (!g) return;
     memset(g, 0, sizeof(*g));
-    g->alpha = 0.5f;
-    g->grid_res = 16;
+    g->alpha = 0.7f;
+    g->grid_res = 32;
     g->train = 1;
-    g->steps_per_iteration = 4;
+    g->steps_per_iteration = 16;
     g->batch = 65536;
-    g->learning_rate = 1e-3f;
+    g->learning_rate = 3e-3f;
     g->seed = 3;
     g->max_samples = 1u << 24;
 }

@@ -241,7 +241,7 @@ struct DEnvMap {
 };
 
 // learned proposal of camera paths in media (guide.h): per (grid cell, arrival bin) record
-constexpr int kGuideLobes = 8, kGuideRec = 12;
+constexpr int kGuideLobes = 8, kGuideRec = 10;
 constexpr int kGuidePend = 12, kGuideRaw = 12; // floats per pending / raw training record
 constexpr float kGuideKappaMin = 1e-2f, kGuideKappaMax = 1e4f; // lobe concentration range
 struct DGuide {

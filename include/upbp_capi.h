@@ -302,12 +302,12 @@ int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, cons
  * (the untrained network is nearly uniform); alpha applies from the first refresh on. alpha
  * 0 switches the proposal off. */
 typedef struct upbp_guide_config {
-    float alpha;                 /* mixture probability (default 0.5) */
-    int32_t grid_res;            /* record grid cells per axis (default 16, at most 64) */
+    float alpha;                 /* mixture probability (default 0.7) */
+    int32_t grid_res;            /* record grid cells per axis (default 32, at most 64) */
     int32_t train;               /* 1: train online (default); 0: keep the network as initialised */
-    int32_t steps_per_iteration; /* Adam steps after each iteration (default 4) */
+    int32_t steps_per_iteration; /* Adam steps after each iteration (default 16) */
     int32_t batch;               /* samples per step (default 65536) */
-    float learning_rate;         /* Adam step size (default 1e-3) */
+    float learning_rate;         /* Adam step size (default 3e-3) */
     uint32_t seed;               /* He initialisation (default 3) */
     uint32_t max_samples;        /* training samples kept per iteration (default 16M) */
 } upbp_guide_config;

@@ -522,24 +522,27 @@ PhaseFit phase_fit(V arrive, float mean_cosine)
 }
 
 // lobe (i, j) of the product: concentration |kappa_i mu_i + k_j mu_h| and its unnormalised mass
-static float prod_mass(const GuideLobe& l, const PhaseFit& h, int j, float* kp_out)
+// c_i a_j 2 pi (exp(kp - kappa_i - k_j) - exp(-kp - kappa_i - k_j)) / kp, the second term dropped
+// above kp = 8 where it is below 1e-7 of the first
+static float prod_mass(const GuideLobe& l, const PhaseFit& h, int j)
 {
     const float lh = dot(l.mu, h.mu);
     const float kp = std::sqrt(std::fmax(0.f, l.kappa * l.kappa + h.k[j] * h.k[j] + 2.f * l.kappa * h.k[j] * lh));
-    if (kp_out) *kp_out = kp;
-    return l.c * h.a[j] * std::exp(kp - l.kappa - h.k[j]) * sphere_integral(kp);
+    const float s = l.kappa + h.k[j];
+    const float m = kp > 8.f ? 2.f * kPi * std::exp(kp - s) / kp : std::exp(kp - s) * sphere_integral(kp);
+    return l.c * h.a[j] * m;
 }
 
+// q(w) h(w) / Z
 float guide_prod_pdf(const GuideLobe* rec, const PhaseFit& h, V w)
 {
-    float num = 0.f, z = 0.f;
-    for (int i = 0; i < 8; i++)
-        for (int j = 0; j < 3; j++) {
-            num += rec[i].c * h.a[j] *
-                   std::exp(rec[i].kappa * dot(rec[i].mu, w) + h.k[j] * dot(h.mu, w) - rec[i].kappa - h.k[j]);
-            z += prod_mass(rec[i], h, j, nullptr);
-        }
-    return z > 0.f ? num / z : 0.f;
+    float q = 0.f, hv = 0.f, z = 0.f;
+    for (int j = 0; j < 3; j++) hv += h.a[j] * std::exp(h.k[j] * (dot(h.mu, w) - 1.f));
+    for (int i = 0; i < 8; i++) {
+        q += rec[i].c * std::exp(rec[i].kappa * (dot(rec[i].mu, w) - 1.f));
+        for (int j = 0; j < 3; j++) z += prod_mass(rec[i], h, j);
+    }
+    return z > 0.f ? q * hv / z : 0.f;
 }
 
 // product lobe by mass with us; its direction by Jakob 2012 ("Numerically stable sampling of
@@ -548,13 +551,13 @@ V guide_prod_sample(const GuideLobe* rec, const PhaseFit& h, float us, float u0,
 {
     float tot = 0.f;
     for (int i = 0; i < 8; i++)
-        for (int j = 0; j < 3; j++) tot += prod_mass(rec[i], h, j, nullptr);
+        for (int j = 0; j < 3; j++) tot += prod_mass(rec[i], h, j);
     int pi = 7, pj = 2;
     float acc = 0.f;
     bool found = false;
     for (int i = 0; i < 8 && !found; i++)
         for (int j = 0; j < 3; j++) {
-            acc += prod_mass(rec[i], h, j, nullptr);
+            acc += prod_mass(rec[i], h, j);
             if (us * tot < acc) {
                 pi = i;
                 pj = j;

@@ -1,5 +1,6 @@
 """Online training of the learned proposal inside the renderer (GPU): samples are collected,
-the loss falls, and the guided renderer converges to the plain one."""
+the trained proposal lowers the error per iteration, and the guided renderer converges to the
+plain one."""
 import os
 
 import numpy as np
@@ -24,15 +25,29 @@ def _run(obj, args, iters, guide=None, first=0):
     return img, stats
 
 
-def test_training_collects_samples_and_lowers_the_loss(tmp_path):
+def test_training_collects_samples(tmp_path):
     obj = capi.generate_scene("slabroom_small", 3, str(tmp_path / "slabroom.obj"))
     args = ["-r", "128x72", "-l", "12", "-r_initial", "0.01"]
-    img, stats = _run(obj, args, 16, guide=dict(alpha=0.5, grid_res=8, steps_per_iteration=8, batch=16384,
-                                                learning_rate=3e-3))
+    img, stats = _run(obj, args, 8, guide=dict(grid_res=8, steps_per_iteration=8, batch=16384))
     assert all(s.path_errors == 0 for s in stats) and np.isfinite(img).all()
     assert stats[0].guide_samples > 2000, stats[0].guide_samples
-    losses = [s.guide_loss for s in stats]
-    assert np.mean(losses[-4:]) < np.mean(losses[:2]) - 0.1, losses
+    assert all(np.isfinite(s.guide_loss) and s.guide_loss != 0.0 for s in stats), [s.guide_loss for s in stats]
+
+
+def test_trained_proposal_lowers_the_error_per_iteration(tmp_path):
+    # What the trained network gains over the method without it, at equal iterations: relMSE
+    # against a long plain render on other random streams, same light paths and camera rays for
+    # plain and learned. The slab room's camera paths must turn towards the aperture's light
+    # against a g = 0.9 phase function, which the product proposal learns online.
+    obj = capi.generate_scene("slabroom_small", 3, str(tmp_path / "slabroom.obj"))
+    args = ["-r", "160x90", "-l", "16", "-r_initial", "0.01"]
+    ref, _ = _run(obj, args, 600, first=1 << 20)
+    plain, _ = _run(obj, args, 64)
+    learned, st = _run(obj, args, 64, guide=dict(grid_res=16))
+    rel = lambda im: float(np.mean((im - ref) ** 2 / (ref ** 2 + 1e-2)))
+    print("relMSE plain %.5f learned %.5f" % (rel(plain), rel(learned)))
+    assert all(s.path_errors == 0 for s in st)
+    assert rel(learned) < 0.9 * rel(plain), (rel(plain), rel(learned))
 
 
 def test_guided_renderer_converges_to_the_plain_one(boxfog_obj):
