@@ -382,8 +382,10 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
 // merges along the segment it sampled have run; it then enters the pool with the snapshot
 // lum(path colour after vertex k) + lum(merge colour before segment k + 1's merges).
 // k_guide_finalise turns snapshots into weights once the iteration's colours are final.
-__device__ __noinline__ void guide_record_sample(const DScene& sc, const Bufs& b, uint32_t pix, f3 color,
-                                                 bool cont, const Sub& s, float den)
+// Inlined: a call taking the kernel's scene and buffer arguments by reference would copy them
+// to scratch memory for the whole bounce kernel.
+__device__ __forceinline__ void guide_record_sample(const DScene& sc, const Bufs& b, uint32_t pix, f3 color,
+                                                 bool scattered_in_medium, f3 origin, f3 dir, float den)
 {
     float* pend = b.gpend + size_t(pix) * kGuidePend;
     bool emit = false;
@@ -407,16 +409,16 @@ __device__ __noinline__ void guide_record_sample(const DScene& sc, const Bufs& b
             for (int k = 0; k < kGuideRaw; k++) b.graw[size_t(at) * kGuideRaw + k] = rec[k];
     }
     // a medium scattering that continues becomes the pending record
-    if (cont && s.sw.w.prev_in_medium && den > 0.f) {
+    if (scattered_in_medium && den > 0.f) {
         const DGuide& g = sc.guide;
         const float inv_res = 1.f / float(g.res);
-        pend[0] = (s.origin.x - g.lo.x) * g.inv_cell.x * inv_res;
-        pend[1] = (s.origin.y - g.lo.y) * g.inv_cell.y * inv_res;
-        pend[2] = (s.origin.z - g.lo.z) * g.inv_cell.z * inv_res;
+        pend[0] = (origin.x - g.lo.x) * g.inv_cell.x * inv_res;
+        pend[1] = (origin.y - g.lo.y) * g.inv_cell.y * inv_res;
+        pend[2] = (origin.z - g.lo.z) * g.inv_cell.z * inv_res;
         pend[3] = pend[4] = pend[5] = 0.f;
-        pend[6] = s.dir.x;
-        pend[7] = s.dir.y;
-        pend[8] = s.dir.z;
+        pend[6] = dir.x;
+        pend[7] = dir.y;
+        pend[8] = dir.z;
         pend[9] = den;
         pend[10] = luminance(color);
     }
@@ -470,7 +472,7 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams 
         cont = false;
         stat_add(b.stats, STAT_ERRORS, 1);
     }
-    if (b.gsamp) guide_record_sample(sc, b, pix, color, cont, s, guide_den);
+    if (b.gsamp) guide_record_sample(sc, b, pix, color, cont && s.sw.w.prev_in_medium, s.origin, s.dir, guide_den);
     stat_add(b.stats, STAT_BPT_CONN, cc.conn);
     stat_add(b.stats, STAT_POINT_Q, cc.pq);
     stat_add(b.stats, STAT_SEG_Q, cc.sq);
