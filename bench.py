@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--subbeam", type=float, default=None, help="sub-beam length in beam radii")
     ap.add_argument("--piece", type=float, default=None, help="query piece length in merge radii")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the library's autotuner (upbp_autotune) before the warm-up and use its choice")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel run once for bandwidth and counter calibration (0: skip)")
     ap.add_argument("--learned", type=float, default=None,
@@ -180,6 +182,13 @@ def main():
             copy = {"bytes": nbytes, "ms": cms.value, "GB_per_s": 2.0 * nbytes / (cms.value * 1e-3) / 1e9}
     r = Renderer(sh.ptr, cfg)
     tune = r.set_tuning(subbeam_factor=a.subbeam, query_piece_factor=a.piece)
+    tuning = {f: getattr(tune, f) for f, _ in type(tune)._fields_}
+    autotune = None
+    if a.autotune:
+        ta = time.perf_counter()
+        tuning, tms = r.autotune(iteration=0, repeats=1)
+        autotune = {"seconds": round(time.perf_counter() - ta, 2), "ms_per_iteration": round(tms, 2)}
+        print("bench: autotuned %s (%.1f ms per iteration)" % (tuning, tms), file=sys.stderr, flush=True)
     print("bench: scene loaded, %d warm-up + %d timed iterations" % (a.warmup, a.steps), file=sys.stderr, flush=True)
     for i in range(a.warmup):
         r.run_iteration(i, want_stats=False)
@@ -218,7 +227,8 @@ def main():
             "max_path_length": 12,
             "light_paths": int(cfg.path_count_per_iter),
             "parallelism": "single GPU",
-            "tuning": {"subbeam_factor": tune.subbeam_factor, "query_piece_factor": tune.query_piece_factor},
+            "tuning": tuning,
+            "autotune": autotune,
         },
         "paths_per_s": (npx + cfg.path_count_per_iter) / (ms * 1e-3),
         "queries_per_s": (st.point_queries + st.segment_queries) / (ms * 1e-3),

@@ -13,6 +13,7 @@
 #include "../../../include/upbp_capi.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1006,6 +1007,76 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
         t->piece_order < 0 || t->piece_order > 2)
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
+    return UPBP_OK;
+}
+
+// Coordinate search over the tuning grid: each knob in turn takes the value that renders
+// `iteration` fastest with the others held, starting from the current tuning. A candidate is
+// timed by host wall clock over `repeats` iterations after one untimed run (which also grows
+// buffers a new setting needs). The knobs never change which pairs a query accepts, so the
+// trial iterations render the same image; the accumulator is saved before and restored after
+// them, and online training of the learned proposal is paused meanwhile.
+extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeats, upbp_tuning* chosen,
+                             float* ms_per_iteration)
+{
+    if (!r) return fail(r, "null renderer", UPBP_E_ARG);
+    if (repeats < 1 || repeats > 64 || iteration < 0) return fail(r, "invalid autotune arguments", UPBP_E_ARG);
+    const size_t acc_bytes = size_t(r->n_pixels) * 3 * sizeof(double);
+    DevBuf saved;
+    struct Release {
+        DevBuf& b;
+        ~Release() { b.release(); }
+    } release_saved{saved};
+    if (saved.ensure(acc_bytes)) return fail(r, "autotune allocation failed", UPBP_E_HIP);
+    HIPCHK(hipMemcpyAsync(saved.p, r->accum.p, acc_bytes, hipMemcpyDeviceToDevice, r->stream));
+    const bool train = r->gtrain;
+    r->gtrain = false;
+    const upbp_tuning start = r->tune;
+    auto timed = [&](const upbp_tuning& t, double& ms) -> int {
+        r->tune = t;
+        for (int k = 0; k <= repeats; k++) {
+            HIPCHK(hipStreamSynchronize(r->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            const int e = upbp_run_iteration(r, iteration, nullptr);
+            if (e) return e;
+            HIPCHK(hipStreamSynchronize(r->stream));
+            const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (k == 0) ms = 0.0;
+            else ms += dt / double(repeats);
+        }
+        return UPBP_OK;
+    };
+    upbp_tuning best = start;
+    double best_ms = 0.0;
+    int e = timed(best, best_ms);
+    const float subbeam[] = {8.f, 12.f, 16.f, 24.f}, piece[] = {32.f, 64.f, 128.f};
+    const int blocks[] = {0, 1, 2}, order[] = {0, 2};
+    for (int knob = 0; knob < 4 && !e; knob++) {
+        const int n = knob == 0 ? 4 : (knob == 1 ? 3 : (knob == 2 ? 3 : 2));
+        const upbp_tuning base = best;
+        for (int v = 0; v < n && !e; v++) {
+            upbp_tuning t = base;
+            if (knob == 0) t.subbeam_factor = subbeam[v];
+            else if (knob == 1) t.query_piece_factor = piece[v];
+            else if (knob == 2) t.merge_blocks_per_cu = blocks[v];
+            else t.piece_order = order[v];
+            if (memcmp(&t, &base, sizeof(t)) == 0) continue;
+            double ms = 0.0;
+            e = timed(t, ms);
+            if (!e && ms < best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        }
+    }
+    r->tune = e ? start : best;
+    r->gtrain = train;
+    if (hipMemcpyAsync(r->accum.p, saved.p, acc_bytes, hipMemcpyDeviceToDevice, r->stream) != hipSuccess ||
+        hipStreamSynchronize(r->stream) != hipSuccess)
+        return fail(r, "autotune restore failed", UPBP_E_HIP);
+    if (e) return e;
+    if (chosen) *chosen = best;
+    if (ms_per_iteration) *ms_per_iteration = float(best_ms);
     return UPBP_OK;
 }
 

@@ -53,6 +53,15 @@ class Renderer:
         if rc != 0:
             raise RuntimeError("upbp_set_guide_lobes failed: %s" % capi.lib().upbp_last_error(self.h).decode())
 
+    def autotune(self, iteration=0, repeats=1):
+        """Runs upbp_autotune: returns the chosen tuning (dict) and its ms per iteration."""
+        t = capi.Tuning()
+        ms = C.c_float()
+        rc = capi.lib().upbp_autotune(self.h, iteration, repeats, C.byref(t), C.byref(ms))
+        if rc != 0:
+            raise RuntimeError("upbp_autotune failed: %s" % capi.lib().upbp_last_error(self.h).decode())
+        return {f: getattr(t, f) for f, _ in capi.Tuning._fields_}, ms.value
+
     def set_tuning(self, **kw):
         """Speed knobs of the C ABI's upbp_tuning (subbeam_factor, query_piece_factor,
         merge_blocks_per_cu, piece_order); unnamed fields keep their defaults."""

@@ -279,6 +279,13 @@ typedef struct upbp_tuning {
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
+/* Autotuner (no reference counterpart): a coordinate search over subbeam_factor {8, 12, 16,
+ * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2} and piece_order {0, 2},
+ * from the current tuning, timing `repeats` (1..64) renders of `iteration` per candidate after
+ * one untimed render. The fastest tuning stays set and is returned in *chosen with its time in
+ * *ms_per_iteration (either may be NULL). The accumulated image is restored afterwards and
+ * online training is paused meanwhile, so the renderer's output is as if it had not run. */
+int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeats, upbp_tuning* chosen, float* ms_per_iteration);
 
 /* ---- learned scattering proposal (no counterpart in the reference) -------
  * Camera paths in media sample, with probability alpha, the product of a learned mixture of
