@@ -114,7 +114,7 @@ class IterStats(C.Structure):
 class GuideConfig(C.Structure):
     _fields_ = [("alpha", C.c_float), ("grid_res", C.c_int32), ("train", C.c_int32),
                 ("steps_per_iteration", C.c_int32), ("batch", C.c_int32), ("learning_rate", C.c_float),
-                ("seed", C.c_uint32), ("max_samples", C.c_uint32)]
+                ("seed", C.c_uint32), ("max_samples", C.c_uint32), ("surface_alpha", C.c_float)]
 
 
 _lib = None
@@ -154,7 +154,8 @@ _SIGS = [
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_test_env_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
-    ("upbp_test_guide_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_test_guide_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int32, C.c_void_p, C.c_uint32,
+                                          C.c_void_p]),
     ("upbp_mlp_param_count", C.c_uint32, []),
     ("upbp_mlp_create", C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
     ("upbp_mlp_destroy", None, [C.c_void_p]),

@@ -189,6 +189,9 @@ UPBP_HD void pdf_phong(const Bsdf& b, const DMaterial& m, f3 lg, float* fp, floa
     if (rp) *rp += pdf;
 }
 
+// world shading normal on the side of local cosine c (the side a camera path leaves to)
+UPBP_HD f3 surface_side(const Bsdf& b, float c) { return c < 0.f ? -b.frame.z : b.frame.z; }
+
 // kernel/bsdf.cuh:334-396
 UPBP_HD f3 bsdf_evaluate(const Bsdf& b, f3 dir_gen, const DScene& sc, float& cos_gen, float* for_pdf, float* rev_pdf,
                          float* sin_theta)
@@ -207,6 +210,17 @@ UPBP_HD f3 bsdf_evaluate(const Bsdf& b, f3 dir_gen, const DScene& sc, float& cos
         const DMaterial& m = sc.mats[b.id];
         res += eval_diffuse(b, m, lg, for_pdf, rev_pdf);
         res += eval_phong(b, m, lg, for_pdf, rev_pdf);
+        // the camera technique samples the learned product at guidable surfaces: its pdf is the
+        // forward one at a camera vertex and the reverse one at a light vertex
+        const float a = sc.guide.alpha_s;
+        if (a > 0.f && surface_guidable(m)) {
+            if (!b.from_light() && for_pdf)
+                *for_pdf = a * guide_surface_pdf(sc.guide, b.pos, surface_side(b, b.local_dir_fix.z), dir_gen) +
+                           (1.f - a) * *for_pdf;
+            else if (b.from_light() && rev_pdf)
+                *rev_pdf = a * guide_surface_pdf(sc.guide, b.pos, surface_side(b, cg), b.world_dir_fix()) +
+                           (1.f - a) * *rev_pdf;
+        }
         return res;
     }
     cos_gen = 1.f;
@@ -241,8 +255,12 @@ UPBP_HD float bsdf_pdf_rev(const Bsdf& b, f3 dir_gen, const DScene& sc)
         if (fabsf(cg) < kEpsCosine || fabsf(cgg) < kEpsCosine) return 0.f;
         if (cg * b.local_dir_fix.z < 0.f || cgg * b.cos_fix_geom < 0.f) return 0.f;
         float fp = 0.f, rp = 0.f;
+        const DMaterial& m = sc.mats[b.id];
         pdf_diffuse(b, lg, &fp, &rp);
-        pdf_phong(b, sc.mats[b.id], lg, &fp, &rp);
+        pdf_phong(b, m, lg, &fp, &rp);
+        const float a = sc.guide.alpha_s;
+        if (a > 0.f && b.from_light() && surface_guidable(m)) // the camera technique's side
+            rp = a * guide_surface_pdf(sc.guide, b.pos, surface_side(b, cg), b.world_dir_fix()) + (1.f - a) * rp;
         return rp;
     }
     const float g = sc.media[b.id].mean_cosine;
@@ -252,11 +270,11 @@ UPBP_HD float bsdf_pdf_rev(const Bsdf& b, f3 dir_gen, const DScene& sc)
     return a * guide_pdf(sc.guide, g, b.pos, -dir_gen, -b.wdir) + (1.f - a) * p;
 }
 
-// kernel/bsdf.cuh:567-660 with the component samplers of :455-559
-UPBP_HD f3 bsdf_sample(const Bsdf& b, float u0, float u1, float u2, f3& dir_gen, float& pdf, float& cos_gen,
-                       const DScene& sc, unsigned& event, float& sin_theta)
+// kernel/bsdf.cuh:567-660 with the component samplers of :455-559, surface part
+UPBP_HD f3 bsdf_sample_surface(const Bsdf& b, float u0, float u1, float u2, f3& dir_gen, float& pdf, float& cos_gen,
+                               const DScene& sc, unsigned& event, float& sin_theta)
 {
-    if (b.on_surface()) {
+    {
         unsigned ev;
         if (u2 < b.diff_prob) ev = Bsdf::DIFFUSE;
         else if (u2 < b.diff_prob + b.phong_prob) ev = Bsdf::PHONG;
@@ -327,6 +345,32 @@ UPBP_HD f3 bsdf_sample(const Bsdf& b, float u0, float u1, float u2, f3& dir_gen,
             (refr && cgg * b.cos_fix_geom > 0.f) || (!refr && cgg * b.cos_fix_geom < 0.f))
             return mk(0.f);
         return res;
+    }
+}
+
+// kernel/bsdf.cuh:567-660
+UPBP_HD f3 bsdf_sample(const Bsdf& b, float u0, float u1, float u2, f3& dir_gen, float& pdf, float& cos_gen,
+                       const DScene& sc, unsigned& event, float& sin_theta)
+{
+    if (b.on_surface()) {
+        const float a = sc.guide.alpha_s;
+        if (!(a > 0.f) || b.from_light() || !surface_guidable(sc.mats[b.id]))
+            return bsdf_sample_surface(b, u0, u1, u2, dir_gen, pdf, cos_gen, sc, event, sin_theta);
+        // camera path at a guidable surface: the learned product with the cosine lobe
+        // (probability alpha_s, chosen by the component variate u2, which is then rescaled for
+        // the BSDF's own component choice), else the BSDF; the pdf is the mixture's either way
+        const f3 n = surface_side(b, b.local_dir_fix.z);
+        if (u2 < a) {
+            dir_gen = vmf_prod_sample(guide_record(sc.guide, b.pos), cosine_lobes(n), u2 / a, u0, u1);
+            event = Bsdf::DIFFUSE;
+            sin_theta = 0.f;
+            pdf = 0.f;
+            return bsdf_evaluate(b, dir_gen, sc, cos_gen, &pdf, nullptr, nullptr);
+        }
+        const f3 f = bsdf_sample_surface(b, u0, u1, (u2 - a) / (1.f - a), dir_gen, pdf, cos_gen, sc, event, sin_theta);
+        if (black_or_neg(f)) return f;
+        pdf = a * guide_surface_pdf(sc.guide, b.pos, n, dir_gen) + (1.f - a) * pdf;
+        return f;
     }
     cos_gen = 1.f;
     event = Bsdf::SCATTER;

@@ -368,12 +368,13 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     return true;
 }
 
-// Training data of the learned proposal. A medium scattering of a camera path that continues
-// is credited with everything the path collects after it: emission, next-event estimation,
-// connections and point merges at later vertices (the path colour) and the P-B2D / B-B1D
-// merges along its later segments (the merge colour). With throughput T before the scattering,
-// scattering function f, continuation probability cp and sampling pdf p, the path carries
-// T f / (p cp) onward, so lum(collected after) cp / lum(T f) estimates L_i / p: the incident
+// Training data of the learned proposal. A scattering of a camera path in a medium or at a
+// guidable surface that continues is credited with everything the path collects after it:
+// emission, next-event estimation, connections and point merges at later vertices (the path
+// colour) and the P-B2D / B-B1D merges along its later segments (the merge colour). With
+// throughput T before the scattering, scattering function f, cosine cos, continuation
+// probability cp and sampling pdf p, the path carries T f cos / (p cp) onward, so
+// lum(collected after) cp / lum(T f cos) estimates L_i / p: the incident
 // radiance along the sampled direction over its pdf, the weight of the KL gradient that fits
 // the proposal's radiance mixture to L_i. The sample is (position, scattered direction) with
 // the arrival inputs zero: the mixture depends on the position only.
@@ -385,7 +386,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
 // Inlined: a call taking the kernel's scene and buffer arguments by reference would copy them
 // to scratch memory for the whole bounce kernel.
 __device__ __forceinline__ void guide_record_sample(const DScene& sc, const Bufs& b, uint32_t pix, f3 color,
-                                                 bool scattered_in_medium, f3 origin, f3 dir, float den)
+                                                 bool scattered, f3 origin, f3 dir, float den)
 {
     float* pend = b.gpend + size_t(pix) * kGuidePend;
     bool emit = false;
@@ -408,8 +409,8 @@ __device__ __forceinline__ void guide_record_sample(const DScene& sc, const Bufs
         if (emit && at < b.gcap)
             for (int k = 0; k < kGuideRaw; k++) b.graw[size_t(at) * kGuideRaw + k] = rec[k];
     }
-    // a medium scattering that continues becomes the pending record
-    if (scattered_in_medium && den > 0.f) {
+    // a scattering in a medium or at a guidable surface that continues becomes the pending record
+    if (scattered && den > 0.f) {
         const DGuide& g = sc.guide;
         const float inv_res = 1.f / float(g.res);
         pend[0] = (origin.x - g.lo.x) * g.inv_cell.x * inv_res;
@@ -472,7 +473,7 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams 
         cont = false;
         stat_add(b.stats, STAT_ERRORS, 1);
     }
-    if (b.gsamp) guide_record_sample(sc, b, pix, color, cont && s.sw.w.prev_in_medium, s.origin, s.dir, guide_den);
+    if (b.gsamp) guide_record_sample(sc, b, pix, color, cont, s.origin, s.dir, guide_den);
     stat_add(b.stats, STAT_BPT_CONN, cc.conn);
     stat_add(b.stats, STAT_POINT_Q, cc.pq);
     stat_add(b.stats, STAT_SEG_Q, cc.sq);

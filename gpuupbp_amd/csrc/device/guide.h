@@ -65,6 +65,7 @@ UPBP_HD float prod_mass(float kp, float s)
 // interpolated linearly in |g| (clamped to 0.99)
 struct PhaseLobes {
     f3 mu;
+    int n; // lobes in use
     float k[kPhaseLobes], a[kPhaseLobes];
 };
 
@@ -72,6 +73,7 @@ UPBP_HD PhaseLobes phase_lobes(f3 arrive, float g)
 {
     PhaseLobes h;
     h.mu = g < 0.f ? -arrive : arrive;
+    h.n = kPhaseLobes;
     const float t = fminf(fabsf(g), 0.99f) / kPhaseFitStep;
     const int i = t >= float(kPhaseFitN - 1) ? kPhaseFitN - 2 : int(t);
     const float f = t - float(i);
@@ -83,14 +85,25 @@ UPBP_HD PhaseLobes phase_lobes(f3 arrive, float g)
     return h;
 }
 
+// the clamped cosine about the surface normal n as its one fitted vMF lobe (phase_lobes.h)
+UPBP_HD PhaseLobes cosine_lobes(f3 n)
+{
+    PhaseLobes h;
+    h.mu = n;
+    h.n = 1;
+    h.k[0] = kCosineKappa;
+    h.a[0] = 1.f / vmf_mass(kCosineKappa);
+    for (int j = 1; j < kPhaseLobes; j++) h.k[j] = h.a[j] = 0.f;
+    return h;
+}
+
 // out of line: keeps the product's record loads and exponentials out of the register
 // allocation of the bounce and merge kernels, which call them only when alpha > 0
 inline __host__ __device__ __noinline__ float vmf_prod_pdf(const float4* rec, PhaseLobes h, f3 w)
 {
     float q = 0.f, z = 0.f, hv = 0.f;
     const float hw = dot(h.mu, w);
-#pragma unroll
-    for (int j = 0; j < kPhaseLobes; j++) hv += h.a[j] * expf(h.k[j] * (hw - 1.f));
+    for (int j = 0; j < h.n; j++) hv += h.a[j] * expf(h.k[j] * (hw - 1.f));
 #pragma unroll
     for (int i = 0; i < kGuideLobes; i++) {
         const float4 l = rec[i];
@@ -98,8 +111,7 @@ inline __host__ __device__ __noinline__ float vmf_prod_pdf(const float4* rec, Ph
         const float lw = l.x * w.x + l.y * w.y + l.z * w.z, lh = l.x * h.mu.x + l.y * h.mu.y + l.z * h.mu.z;
         q += c * expf(l.w * (lw - 1.f));
         float zi = 0.f;
-#pragma unroll
-        for (int j = 0; j < kPhaseLobes; j++) {
+        for (int j = 0; j < h.n; j++) {
             // |v|^2 = kappa^2 + k^2 + 2 kappa k (mu_i . mu_h)
             const float kp = sqrtf(fmaxf(0.f, l.w * l.w + h.k[j] * h.k[j] + 2.f * l.w * h.k[j] * lh));
             zi += h.a[j] * prod_mass(kp, l.w + h.k[j]);
@@ -132,25 +144,25 @@ inline __host__ __device__ __noinline__ f3 vmf_prod_sample(const float4* rec, Ph
         const float4 l = rec[i];
         const float c = lane4(rec[kGuideLobes + i / 4], i & 3);
         const float lh = l.x * h.mu.x + l.y * h.mu.y + l.z * h.mu.z;
-#pragma unroll
-        for (int j = 0; j < kPhaseLobes; j++) {
+        for (int j = 0; j < h.n; j++) {
             const float kp = sqrtf(fmaxf(0.f, l.w * l.w + h.k[j] * h.k[j] + 2.f * l.w * h.k[j] * lh));
             tot += c * h.a[j] * prod_mass(kp, l.w + h.k[j]);
         }
     }
     const float t = us * tot;
-    int pick = kGuideLobes * kPhaseLobes - 1;
+    int pick = -1;
     float acc = 0.f;
     for (int i = 0; i < kGuideLobes; i++) {
         const float4 l = rec[i];
         const float c = lane4(rec[kGuideLobes + i / 4], i & 3);
         const float lh = l.x * h.mu.x + l.y * h.mu.y + l.z * h.mu.z;
-        for (int j = 0; j < kPhaseLobes; j++) {
+        for (int j = 0; j < h.n; j++) {
             const float kp = sqrtf(fmaxf(0.f, l.w * l.w + h.k[j] * h.k[j] + 2.f * l.w * h.k[j] * lh));
             acc += c * h.a[j] * prod_mass(kp, l.w + h.k[j]);
-            if (t < acc && pick == kGuideLobes * kPhaseLobes - 1) pick = i * kPhaseLobes + j;
+            if (t < acc && pick < 0) pick = i * kPhaseLobes + j;
         }
     }
+    if (pick < 0) pick = (kGuideLobes - 1) * kPhaseLobes + h.n - 1;
     const int pi = pick / kPhaseLobes, pj = pick % kPhaseLobes;
     const float4 l = rec[pi];
     const f3 v = mk(l.x * l.w + h.mu.x * h.k[pj], l.y * l.w + h.mu.y * h.k[pj], l.z * l.w + h.mu.z * h.k[pj]);
@@ -164,6 +176,18 @@ inline __host__ __device__ __noinline__ f3 vmf_prod_sample(const float4* rec, Ph
 UPBP_HD float guide_pdf(const DGuide& gd, float g, f3 p, f3 arrive, f3 gen)
 {
     return vmf_prod_pdf(guide_record(gd, p), phase_lobes(arrive, g), gen);
+}
+
+// Surfaces that take the learned proposal: no mirror and no refraction component, so that the
+// camera path's BSDF sample is always a non-specular reflection (a material property: the
+// same at camera vertices and at stored light vertices)
+UPBP_HD bool surface_guidable(const DMaterial& m) { return luminance(m.mirror) == 0.f && !(m.ior > 0.f); }
+
+// pdf of the learned part of the proposal at surface point p with the normal n on the side the
+// camera path leaves to, along gen
+UPBP_HD float guide_surface_pdf(const DGuide& gd, f3 p, f3 n, f3 gen)
+{
+    return vmf_prod_pdf(guide_record(gd, p), cosine_lobes(n), gen);
 }
 
 // Camera technique's solid-angle pdf of leaving p along gen after arriving along arrive, in

@@ -791,7 +791,8 @@ struct Sub { // subpath state held in registers within a bounce kernel
 };
 
 // kernel/functs_shared.cuh:37-154. guide_den (optional, camera paths): lum(throughput times
-// the scattering function) / continuation probability of a continuing scattering, the
+// the scattering function times the cosine) / continuation probability of a continuing
+// scattering in a medium or at a guidable surface whose proposal is on (0 elsewhere), the
 // denominator of the learned proposal's training weight.
 __device__ __forceinline__ int sample_scattering(const DScene& sc, const IterParams& p, bool& cont, const Bsdf& b,
                                                  f3 hitp, const Hit& is, Sub& s, Rng& rng, float* guide_den = nullptr)
@@ -815,7 +816,11 @@ __device__ __forceinline__ int sample_scattering(const DScene& sc, const IterPar
     }
     for_pdf *= cp;
     rev_pdf *= cp;
-    if (guide_den) *guide_den = luminance(s.throughput * f) / cp;
+    if (guide_den) {
+        const bool trains = b.in_medium() ? sc.guide.train_media != 0
+                                          : (sc.guide.train_surfaces != 0 && surface_guidable(sc.mats[b.id]));
+        *guide_den = trains ? luminance(s.throughput * f) * cos_out / cp : 0.f;
+    }
     Weights& w = s.sw.w;
     const float bpt_i1 = float(!w.prev_specular && !specular);
     if (specular) {

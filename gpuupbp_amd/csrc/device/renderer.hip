@@ -729,6 +729,7 @@ static int train_guide(upbp_renderer* r)
     if (g.steps_per_iteration > 0) {
         if (r->gnet->records(r->sc.guide.res, r->guide.as<float4>(), s)) return fail(r, "guide refresh failed", UPBP_E_HIP);
         r->sc.guide.alpha = g.alpha;
+        r->sc.guide.alpha_s = g.surface_alpha;
         r->gloss = loss / double(g.steps_per_iteration);
     }
     HIPCHK(hipEventRecord(r->ev[5], s));
@@ -927,7 +928,7 @@ extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_
     if (!r) return UPBP_E_ARG;
     if (!(alpha >= 0.f && alpha <= 1.f)) return fail(r, "guide alpha must lie in [0, 1]", UPBP_E_ARG);
     if (alpha == 0.f) {
-        r->sc.guide.alpha = 0.f;
+        r->sc.guide.alpha = r->sc.guide.alpha_s = 0.f;
         return UPBP_OK;
     }
     if (!lobes || !lo || !hi || res < 1 || res > 256) return fail(r, "invalid guide grid", UPBP_E_ARG);
@@ -944,7 +945,7 @@ extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_
     g.res = res;
     g.lo = F3(lo);
     g.inv_cell = mk(float(res) / (hi[0] - lo[0]), float(res) / (hi[1] - lo[1]), float(res) / (hi[2] - lo[2]));
-    g.alpha = alpha;
+    g.alpha = g.alpha_s = alpha;
     return UPBP_OK;
 }
 
@@ -952,28 +953,30 @@ extern "C" void upbp_guide_defaults(upbp_guide_config* g)
 {
     if (!g) return;
     memset(g, 0, sizeof(*g));
-    g->alpha = 0.7f;
-    g->grid_res = 32;
+    g->alpha = 0.f;
+    g->grid_res = 64;
     g->train = 1;
-    g->steps_per_iteration = 16;
+    g->steps_per_iteration = 10;
     g->batch = 65536;
-    g->learning_rate = 3e-3f;
+    g->learning_rate = 8e-3f;
     g->seed = 3;
     g->max_samples = 1u << 24;
+    g->surface_alpha = 0.65f;
 }
 
 extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
 {
     if (!r || !g) return UPBP_E_ARG;
-    if (!(g->alpha >= 0.f && g->alpha <= 1.f) || g->grid_res < 1 || g->grid_res > 64 || g->steps_per_iteration < 0 ||
+    if (!(g->alpha >= 0.f && g->alpha <= 1.f) || !(g->surface_alpha >= 0.f && g->surface_alpha <= 1.f) ||
+        g->grid_res < 1 || g->grid_res > 64 || g->steps_per_iteration < 0 ||
         g->batch < 16 || g->batch > (1 << 22) || !(g->learning_rate > 0.f) || g->max_samples < 16)
         return fail(r, "invalid guide configuration", UPBP_E_ARG);
     r->gcfg = *g;
     delete r->gnet;
     r->gnet = nullptr;
     r->gtrain = false;
-    r->sc.guide.alpha = 0.f;
-    if (g->alpha == 0.f) return UPBP_OK;
+    r->sc.guide.alpha = r->sc.guide.alpha_s = 0.f;
+    if (g->alpha == 0.f && g->surface_alpha == 0.f) return UPBP_OK;
     // grid over the bounding cube of the scene sphere
     const f3 c = r->sc.center;
     const float rad = r->sc.radius;
@@ -993,9 +996,15 @@ extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
         r->gsamp.ensure(size_t(r->gcap) * 10 * 4) || r->gcount.ensure(16))
         return fail(r, "guide allocation failed", UPBP_E_HIP);
     r->gtrain = g->train != 0;
+    // samples only from the scatterings whose proposal is on
+    dg.train_media = g->alpha > 0.f;
+    dg.train_surfaces = g->surface_alpha > 0.f;
     // an untrained network proposes nearly uniform lobes: the first iteration samples the phase
     // function only and collects training data; the mixture switches on after the first steps
-    if (!r->gtrain) dg.alpha = g->alpha;
+    if (!r->gtrain) {
+        dg.alpha = g->alpha;
+        dg.alpha_s = g->surface_alpha;
+    }
     HIPCHK(hipStreamSynchronize(r->stream));
     return UPBP_OK;
 }

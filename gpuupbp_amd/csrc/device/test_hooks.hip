@@ -416,11 +416,12 @@ extern "C" int upbp_test_env_samples(const upbp_env_map* env, const float* u, ui
     return rc ? UPBP_E_HIP : UPBP_OK;
 }
 
-__global__ void k_guide_samples(const float4* rec, f3 arrive, float g, const float* u, uint32_t n, float* out)
+__global__ void k_guide_samples(const float4* rec, f3 arrive, float g, int surface, const float* u, uint32_t n,
+                                float* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const PhaseLobes h = phase_lobes(arrive, g);
+    const PhaseLobes h = surface ? cosine_lobes(arrive) : phase_lobes(arrive, g);
     const f3 d = vmf_prod_sample(rec, h, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
     out[4 * i] = d.x;
     out[4 * i + 1] = d.y;
@@ -428,8 +429,8 @@ __global__ void k_guide_samples(const float4* rec, f3 arrive, float g, const flo
     out[4 * i + 3] = vmf_prod_pdf(rec, h, d);
 }
 
-extern "C" int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n,
-                                       float* out)
+extern "C" int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, int32_t surface,
+                                       const float* u, uint32_t n, float* out)
 {
     if (!lobes || !arrive || (n && (!u || !out))) return UPBP_E_ARG;
     if (n == 0) return UPBP_OK;
@@ -445,7 +446,7 @@ extern "C" int upbp_test_guide_samples(const float* lobes, const float* arrive, 
     (void)hipMemcpy(drec, rec, sizeof(rec), hipMemcpyHostToDevice);
     (void)hipMemcpy(du, u, 3 * size_t(n) * 4, hipMemcpyHostToDevice);
     hipLaunchKernelGGL(k_guide_samples, dim3((n + 255) / 256), dim3(256), 0, 0, drec, mk(arrive[0], arrive[1], arrive[2]),
-                       g, du, n, dout);
+                       g, int(surface), du, n, dout);
     int rc = int(hipDeviceSynchronize());
     if (!rc) rc = int(hipMemcpy(out, dout, 4 * size_t(n) * 4, hipMemcpyDeviceToHost));
     return rc ? UPBP_E_HIP : UPBP_OK;

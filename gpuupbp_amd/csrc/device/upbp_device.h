@@ -248,7 +248,9 @@ struct DGuide {
     const float4* recs; // res^3 records of kGuideRec float4, cell (z * res + y) * res + x
     f3 lo, inv_cell;    // grid origin, cells per unit length per axis
     int res;            // cells per axis
-    float alpha;        // probability of sampling the mixture at camera medium vertices (0: off)
+    float alpha;        // probability of sampling the product at camera medium vertices (0: off)
+    float alpha_s;      // the same at non-specular camera surface vertices (0: off)
+    int train_media, train_surfaces; // which scatterings feed the training samples
 };
 
 struct DScene {

@@ -204,14 +204,15 @@ class Mlp:
         return loss.value
 
 
-def guide_samples(lobes, u, arrive=(0.0, 0.0, 1.0), g=0.0):
-    """GPU sampling of one learned-proposal record (8, 5) times the phase lobe of a path arriving
-    along `arrive` in a medium of mean cosine g, at u (n, 3): (n, 4) direction, pdf."""
+def guide_samples(lobes, u, arrive=(0.0, 0.0, 1.0), g=0.0, surface=False):
+    """GPU sampling of one learned-proposal record (8, 5) times the phase lobes of a path
+    arriving along `arrive` in a medium of mean cosine g (or, with surface, times the cosine
+    lobe about the normal `arrive`), at u (n, 3): (n, 4) direction, pdf."""
     lobes = np.ascontiguousarray(lobes, np.float32)
     arrive = np.ascontiguousarray(arrive, np.float32)
     u = np.ascontiguousarray(u, np.float32)
     out = np.zeros((u.shape[0], 4), np.float32)
-    rc = capi.lib().upbp_test_guide_samples(_p(lobes), _p(arrive), g, _p(u), u.shape[0], _p(out))
+    rc = capi.lib().upbp_test_guide_samples(_p(lobes), _p(arrive), g, int(surface), _p(u), u.shape[0], _p(out))
     if rc != 0:
         raise RuntimeError("upbp_test_guide_samples failed: %d" % rc)
     return out

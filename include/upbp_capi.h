@@ -290,33 +290,41 @@ int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeats, upbp_tun
 /* ---- learned scattering proposal (no counterpart in the reference) -------
  * Camera paths in media sample, with probability alpha, the product of a learned mixture of
  * eight von Mises-Fisher lobes (incident radiance, looked up on a res^3 grid over [lo, hi]) and
- * the vMF lobe matching the medium's phase function (mean direction along the arrival for
- * g >= 0, against it for g < 0, concentration r (3 - r^2) / (1 - r^2) with r = min(|g|, 0.999)),
- * else the phase function. The product's pdf enters every camera-technique pdf (forward at
- * camera vertices, reverse at light vertices, connections and merges), so the estimator stays
- * unbiased and MIS consistent.
+ * three vMF lobes fitted to the medium's Henyey-Greenstein phase function (mean direction along
+ * the arrival for g >= 0, against it for g < 0; the table of phase_lobes.h), else the phase
+ * function. At surfaces without mirror or refraction components camera paths sample, with
+ * probability surface_alpha, the product of the same mixture with the vMF lobe fitted to the
+ * clamped cosine about the shading normal, else the BSDF. The proposal's pdf enters every
+ * camera-technique pdf (forward at camera vertices, reverse at light vertices, connections
+ * and merges), so the estimator stays unbiased and MIS consistent.
  * upbp_set_guide_lobes installs explicit lobes: res^3 records, record (z * res + y) * res + x,
  * of 8 lobes x (mu.xyz, kappa, weight); mu is normalised, kappa clamped to [1e-2, 1e4],
- * weights normalised. alpha 0 switches the proposal off (the plain renderer). */
+ * weights normalised; alpha applies in media and at surfaces. alpha 0 switches the proposal
+ * off (the plain renderer). */
 int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
                          float alpha);
 
-/* The learned proposal trained online: camera paths credit what they collect at a vertex
- * (emission, next-event estimation, connections, point and B-P2D merges) to their previous
- * medium scattering; after each iteration the network takes steps_per_iteration Adam steps
- * on batches of those samples and the grid records are re-evaluated from it. The grid covers
- * the bounding cube of the scene sphere. The first iteration samples the phase function only
- * (the untrained network is nearly uniform); alpha applies from the first refresh on. alpha
- * 0 switches the proposal off. */
+/* The learned proposal trained online: each scattering of a camera path in a medium or at a
+ * guidable surface is credited with everything the path collects after it (emission,
+ * next-event estimation, connections, point merges and the segment merges of its later
+ * segments), weighted as an estimate of incident radiance over the sampling pdf; after each
+ * iteration the network takes steps_per_iteration Adam steps on random batches of those
+ * samples and the grid records are re-evaluated from it. The grid covers the bounding cube of
+ * the scene sphere. The first iteration samples the phase function and BSDFs only (the
+ * untrained network is nearly uniform); the alphas apply from the first refresh on. Both
+ * alphas 0 switch the proposal off. */
 typedef struct upbp_guide_config {
-    float alpha;                 /* mixture probability (default 0.7) */
-    int32_t grid_res;            /* record grid cells per axis (default 32, at most 64) */
+    float alpha;                 /* mixture probability in media (default 0: on the slab-room
+                                    workload guided medium paths attract more merge work than
+                                    they save; 0.7 lowers the error per iteration) */
+    int32_t grid_res;            /* record grid cells per axis (default 64, at most 64) */
     int32_t train;               /* 1: train online (default); 0: keep the network as initialised */
-    int32_t steps_per_iteration; /* Adam steps after each iteration (default 16) */
+    int32_t steps_per_iteration; /* Adam steps after each iteration (default 10) */
     int32_t batch;               /* samples per step (default 65536) */
-    float learning_rate;         /* Adam step size (default 3e-3) */
+    float learning_rate;         /* Adam step size (default 8e-3) */
     uint32_t seed;               /* He initialisation (default 3) */
     uint32_t max_samples;        /* training samples kept per iteration (default 16M) */
+    float surface_alpha;         /* mixture probability at non-specular surfaces (default 0.65) */
 } upbp_guide_config;
 void upbp_guide_defaults(upbp_guide_config* g);
 int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g);
@@ -380,11 +388,12 @@ int upbp_test_sphere_hits(const float* spheres, uint32_t n_sph, const float* ray
 int upbp_test_env_samples(const upbp_env_map* env, const float* u, uint32_t n, float* out);
 
 /* One learned-proposal record (8 x (mu.xyz, kappa, weight), packed as upbp_set_guide_lobes
- * packs it) times the phase lobe of a path arriving along `arrive` in a medium of mean cosine
- * g, sampled with u (3 per sample: lobe variate, two direction variates): out 4 per sample, the
- * direction and the product's pdf there. */
-int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n,
-                            float* out);
+ * packs it) times the phase lobes of a path arriving along `arrive` in a medium of mean cosine
+ * g (surface 0), or times the cosine lobe about the normal `arrive` (surface 1), sampled with u
+ * (3 per sample: lobe variate, two direction variates): out 4 per sample, the direction and
+ * the product's pdf there. */
+int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, int32_t surface, const float* u,
+                            uint32_t n, float* out);
 
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM

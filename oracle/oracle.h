@@ -64,10 +64,12 @@ void oracle_mlp_encode(const float* x, const float* d, float* out);
 int oracle_mlp_forward(const double* params, const float* samples, uint32_t n, double* out);
 int oracle_mlp_loss_grad(const double* params, const float* samples, uint32_t n, double* loss, double* grads);
 
-/* One learned-proposal record (8 x (mu.xyz, kappa, weight)) times the phase lobe of a path
- * arriving along `arrive` in a medium of mean cosine g, sampled with u (3 per sample: lobe, two
- * for the direction): out 4 per sample (direction, product pdf). */
-int oracle_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n, float* out);
+/* One learned-proposal record (8 x (mu.xyz, kappa, weight)) times the phase lobes of a path
+ * arriving along `arrive` in a medium of mean cosine g (surface 0) or the cosine lobe about the
+ * normal `arrive` (surface 1), sampled with u (3 per sample: lobe, two for the direction): out
+ * 4 per sample (direction, product pdf). */
+int oracle_guide_samples(const float* lobes, const float* arrive, float g, int32_t surface, const float* u, uint32_t n,
+                         float* out);
 
 /* Learned proposal for every later oracle render (same arguments as upbp_set_guide_lobes;
  * alpha 0 switches it off). */

@@ -469,7 +469,7 @@ static void pdf_phong(const BSDF& b, const Material& m, V lg, float* fp, float* 
 void init_guide(Guide& g, const float* lobes, int res, const float* lo, const float* hi, float alpha)
 {
     g = Guide();
-    g.alpha = alpha;
+    g.alpha = g.alpha_s = alpha;
     if (!(alpha > 0.f)) return;
     g.res = res;
     g.lo = fromp(lo);
@@ -511,6 +511,7 @@ PhaseFit phase_fit(V arrive, float mean_cosine)
 {
     PhaseFit h;
     h.mu = mean_cosine < 0.f ? -arrive : arrive;
+    h.n = 3;
     const float t = std::fmin(std::fabs(mean_cosine), 0.99f) / kPhaseFitStep;
     const int row = std::min(int(t), kPhaseFitN - 2);
     const float f = t - float(row);
@@ -520,6 +521,21 @@ PhaseFit phase_fit(V arrive, float mean_cosine)
     }
     return h;
 }
+
+// the clamped cosine about a surface normal as its one fitted lobe
+PhaseFit cosine_fit(V normal)
+{
+    PhaseFit h;
+    h.mu = normal;
+    h.n = 1;
+    h.k[0] = kCosineKappa;
+    h.a[0] = 1.f / sphere_integral(kCosineKappa);
+    h.k[1] = h.k[2] = h.a[1] = h.a[2] = 0.f;
+    return h;
+}
+
+// no mirror and no refraction: the BSDF's own samples are non-specular reflections
+bool surface_guidable(const Material& m) { return luminance(m.mirror) == 0.f && !(m.ior > 0.f); }
 
 // lobe (i, j) of the product: concentration |kappa_i mu_i + k_j mu_h| and its unnormalised mass
 // c_i a_j 2 pi (exp(kp - kappa_i - k_j) - exp(-kp - kappa_i - k_j)) / kp, the second term dropped
@@ -537,10 +553,10 @@ static float prod_mass(const GuideLobe& l, const PhaseFit& h, int j)
 float guide_prod_pdf(const GuideLobe* rec, const PhaseFit& h, V w)
 {
     float q = 0.f, hv = 0.f, z = 0.f;
-    for (int j = 0; j < 3; j++) hv += h.a[j] * std::exp(h.k[j] * (dot(h.mu, w) - 1.f));
+    for (int j = 0; j < h.n; j++) hv += h.a[j] * std::exp(h.k[j] * (dot(h.mu, w) - 1.f));
     for (int i = 0; i < 8; i++) {
         q += rec[i].c * std::exp(rec[i].kappa * (dot(rec[i].mu, w) - 1.f));
-        for (int j = 0; j < 3; j++) z += prod_mass(rec[i], h, j);
+        for (int j = 0; j < h.n; j++) z += prod_mass(rec[i], h, j);
     }
     return z > 0.f ? q * hv / z : 0.f;
 }
@@ -551,12 +567,12 @@ V guide_prod_sample(const GuideLobe* rec, const PhaseFit& h, float us, float u0,
 {
     float tot = 0.f;
     for (int i = 0; i < 8; i++)
-        for (int j = 0; j < 3; j++) tot += prod_mass(rec[i], h, j);
-    int pi = 7, pj = 2;
+        for (int j = 0; j < h.n; j++) tot += prod_mass(rec[i], h, j);
+    int pi = 7, pj = h.n - 1;
     float acc = 0.f;
     bool found = false;
     for (int i = 0; i < 8 && !found; i++)
-        for (int j = 0; j < 3; j++) {
+        for (int j = 0; j < h.n; j++) {
             acc += prod_mass(rec[i], h, j);
             if (us * tot < acc) {
                 pi = i;
@@ -612,6 +628,21 @@ V bsdf_evaluate(const BSDF& b, V dir_gen, const Scene& sc, float& cos_gen, float
         const Material& m = sc.mats[size_t(b.mat)];
         res += eval_diffuse(b, m, lg, for_pdf, rev_pdf);
         res += eval_phong(b, m, lg, for_pdf, rev_pdf);
+        // the camera technique's side follows the learned product at guidable surfaces: the
+        // normal on the side the camera path leaves to (wo at a camera vertex, dir_gen at a
+        // light vertex)
+        const float a = sc.guide.alpha_s;
+        if (a > 0.f && surface_guidable(m)) {
+            const V nz = b.frame.z;
+            if (!b.from_light() && for_pdf) {
+                const V n = b.local_dir_fix.z < 0.f ? -nz : nz;
+                *for_pdf = a * guide_prod_pdf(guide_record(sc.guide, b.pos), cosine_fit(n), dir_gen) + (1.f - a) * *for_pdf;
+            } else if (b.from_light() && rev_pdf) {
+                const V n = cg < 0.f ? -nz : nz;
+                *rev_pdf = a * guide_prod_pdf(guide_record(sc.guide, b.pos), cosine_fit(n), b.dir_fix_world()) +
+                           (1.f - a) * *rev_pdf;
+            }
+        }
         return res;
     }
     cos_gen = 1.f;
@@ -646,6 +677,17 @@ float bsdf_pdf(const BSDF& b, V dir_gen, const Scene& sc, bool reverse)
         const Material& m = sc.mats[size_t(b.mat)];
         pdf_diffuse(b, lg, &fp, &rp);
         pdf_phong(b, m, lg, &fp, &rp);
+        const float a = sc.guide.alpha_s;
+        const bool camera_technique = b.from_light() ? reverse : !reverse;
+        if (a > 0.f && camera_technique && surface_guidable(m)) {
+            const V nz = b.frame.z;
+            if (!b.from_light())
+                fp = a * guide_prod_pdf(guide_record(sc.guide, b.pos), cosine_fit(b.local_dir_fix.z < 0.f ? -nz : nz),
+                                        dir_gen) + (1.f - a) * fp;
+            else
+                rp = a * guide_prod_pdf(guide_record(sc.guide, b.pos), cosine_fit(cg < 0.f ? -nz : nz), b.dir_fix_world()) +
+                     (1.f - a) * rp;
+        }
         return reverse ? rp : fp;
     }
     const float p = hg_pdf(b.world_dir_fix, dir_gen, b.mean_cosine, nullptr);
@@ -713,11 +755,11 @@ static V sample_refract(const BSDF& b, V& lg, float& pdf)
     return v3(0.f);
 }
 
-// kernel/bsdf.cuh:567-660
-V bsdf_sample(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pdf, float& cos_gen,
-              const Scene& sc, unsigned& event, float& sin_theta)
+// kernel/bsdf.cuh:567-660, surface part
+static V bsdf_sample_surface(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pdf, float& cos_gen,
+                             const Scene& sc, unsigned& event, float& sin_theta)
 {
-    if (b.on_surface()) {
+    {
         unsigned ev;
         if (u2 < b.diff_prob) ev = BSDF::DIFFUSE;
         else if (u2 < b.diff_prob + b.phong_prob) ev = BSDF::PHONG;
@@ -755,6 +797,32 @@ V bsdf_sample(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pd
             (refr && cgg * b.cos_fix_geom > 0.f) || (!refr && cgg * b.cos_fix_geom < 0.f))
             return v3(0.f);
         return res;
+    }
+}
+
+// kernel/bsdf.cuh:567-660
+V bsdf_sample(const BSDF& b, float u0, float u1, float u2, V& dir_gen, float& pdf, float& cos_gen,
+              const Scene& sc, unsigned& event, float& sin_theta)
+{
+    if (b.on_surface()) {
+        const float a = sc.guide.alpha_s;
+        if (!(a > 0.f) || b.from_light() || !surface_guidable(sc.mats[size_t(b.mat)]))
+            return bsdf_sample_surface(b, u0, u1, u2, dir_gen, pdf, cos_gen, sc, event, sin_theta);
+        // camera path at a guidable surface: the learned product with the cosine lobe with
+        // probability alpha_s (variate u2, rescaled for the BSDF's component choice otherwise)
+        const V n = b.local_dir_fix.z < 0.f ? -b.frame.z : b.frame.z;
+        const GuideLobe* rec = guide_record(sc.guide, b.pos);
+        if (u2 < a) {
+            dir_gen = guide_prod_sample(rec, cosine_fit(n), u2 / a, u0, u1);
+            event = BSDF::DIFFUSE;
+            sin_theta = 0.f;
+            pdf = 0.f;
+            return bsdf_evaluate(b, dir_gen, sc, cos_gen, &pdf, nullptr, nullptr);
+        }
+        const V f = bsdf_sample_surface(b, u0, u1, (u2 - a) / (1.f - a), dir_gen, pdf, cos_gen, sc, event, sin_theta);
+        if (black_or_neg(f)) return f;
+        pdf = a * guide_prod_pdf(rec, cosine_fit(n), dir_gen) + (1.f - a) * pdf;
+        return f;
     }
     cos_gen = 1.f;
     event = BSDF::SCATTER;

@@ -214,7 +214,7 @@ struct GuideLobe {
     float kappa, w, c; // concentration, normalised weight, w kappa / (2 pi (1 - exp(-2 kappa)))
 };
 struct Guide {
-    float alpha = 0.f;
+    float alpha = 0.f, alpha_s = 0.f; // media, non-specular surfaces
     int res = 0;
     V lo{0, 0, 0}, inv_cell{0, 0, 0};
     std::vector<GuideLobe> lobes; // 8 per record, record (z * res + y) * res + x
@@ -243,11 +243,14 @@ void prepare_scene(const upbp_scene* in, uint32_t res_x, uint32_t res_y, Scene& 
 void init_guide(Guide& g, const float* lobes, int res, const float* lo, const float* hi, float alpha);
 extern Guide g_oracle_guide;
 const GuideLobe* guide_record(const Guide& g, V p);
-struct PhaseFit { // h(w) = sum_j a_j exp(k_j (mu . w - 1))
+struct PhaseFit { // h(w) = sum_{j < n} a_j exp(k_j (mu . w - 1))
     V mu;
+    int n;
     float k[3], a[3];
 };
 PhaseFit phase_fit(V arrive, float mean_cosine);
+PhaseFit cosine_fit(V normal);
+bool surface_guidable(const Material& m);
 float guide_prod_pdf(const GuideLobe* rec, const PhaseFit& h, V w);
 V guide_prod_sample(const GuideLobe* rec, const PhaseFit& h, float us, float u0, float u1);
 float guide_pdf(const Guide& g, float mean_cosine, V p, V arrive, V gen);

@@ -35,7 +35,7 @@ def lib():
         h.oracle_sphere_hits.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp]
         h.oracle_env_samples.argtypes = [vp, vp, C.c_uint32, vp]
         h.oracle_set_guide_lobes.argtypes = [vp, C.c_int32, vp, vp, C.c_float]
-        h.oracle_guide_samples.argtypes = [vp, vp, C.c_float, vp, C.c_uint32, vp]
+        h.oracle_guide_samples.argtypes = [vp, vp, C.c_float, C.c_int32, vp, C.c_uint32, vp]
         h.oracle_mlp_param_count.restype = C.c_uint32
         h.oracle_mlp_encode.argtypes = [vp, vp, vp]
         h.oracle_mlp_forward.argtypes = [vp, vp, C.c_uint32, vp]
@@ -114,14 +114,15 @@ def sphere_hits(spheres, rays):
     return out
 
 
-def guide_samples(lobes, u, arrive=(0.0, 0.0, 1.0), g=0.0):
-    """One learned-proposal record (8, 5) times the phase lobe of a path arriving along `arrive`
-    in a medium of mean cosine g, sampled at u (n, 3): (n, 4) direction and product pdf."""
+def guide_samples(lobes, u, arrive=(0.0, 0.0, 1.0), g=0.0, surface=False):
+    """One learned-proposal record (8, 5) times the phase lobes of a path arriving along `arrive`
+    in a medium of mean cosine g (or, with surface, the cosine lobe about the normal `arrive`),
+    sampled at u (n, 3): (n, 4) direction and product pdf."""
     lobes = np.ascontiguousarray(lobes, np.float32)
     arrive = np.ascontiguousarray(arrive, np.float32)
     u = np.ascontiguousarray(u, np.float32)
     out = np.zeros((u.shape[0], 4), np.float32)
-    lib().oracle_guide_samples(_p(lobes), _p(arrive), g, _p(u), u.shape[0], _p(out))
+    lib().oracle_guide_samples(_p(lobes), _p(arrive), g, int(surface), _p(u), u.shape[0], _p(out))
     return out
 
 

@@ -2165,14 +2165,15 @@ extern "C" int oracle_sphere_hits(const float* spheres, uint32_t n_sph, const fl
     return UPBP_OK;
 }
 
-extern "C" int oracle_guide_samples(const float* lobes, const float* arrive, float g, const float* u, uint32_t n,
-                                    float* out)
+extern "C" int oracle_guide_samples(const float* lobes, const float* arrive, float g, int32_t surface, const float* u,
+                                    uint32_t n, float* out)
 {
     Guide gd;
     const float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
     init_guide(gd, lobes, 1, lo, hi, 1.f);
     const GuideLobe* r = gd.lobes.data();
-    const PhaseFit h = phase_fit({arrive[0], arrive[1], arrive[2]}, g);
+    const V ar{arrive[0], arrive[1], arrive[2]};
+    const PhaseFit h = surface ? cosine_fit(ar) : phase_fit(ar, g);
     for (uint32_t i = 0; i < n; i++) {
         const V d = guide_prod_sample(r, h, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
         out[4 * i] = d.x;

@@ -36,6 +36,20 @@ def test_product_sampler_matches_oracle(mean_cosine):
         assert np.mean(rp < 1e-3) > 0.999, np.mean(rp < 1e-3)
 
 
+def test_surface_product_sampler_matches_oracle():
+    rng = np.random.default_rng(3)
+    for lobes in _lobes(rng, 3):
+        u = rng.uniform(0, 1, (20000, 3)).astype(np.float32)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        g = gpu.guide_samples(lobes, u, n, surface=True)
+        o = oracle_py.guide_samples(lobes, u, n, surface=True)
+        dd = np.max(np.abs(g[:, :3] - o[:, :3]), axis=1)
+        assert np.mean(dd < 2e-5) > 0.999 and dd.max() < 1e-2, (np.mean(dd < 2e-5), dd.max())
+        rp = np.abs(g[:, 3] - o[:, 3]) / o[:, 3]
+        assert np.mean(rp < 1e-3) > 0.999, np.mean(rp < 1e-3)
+
+
 def _render(obj, args, lobes, res, lo, hi, alpha):
     rc, cfg = capi.parse_args(["-s", "-1", obj] + args)
     assert rc == 0

@@ -10,9 +10,9 @@ mathematics fixes.
   the mixture times the phase lobes, normalised (quadrature over the sphere).
 * The fitted phase lobes follow the Henyey-Greenstein function (chi-square ratio near 1).
 * alpha = 0 is the plain renderer bit for bit.
-* With alpha > 0 the renderer stays unbiased: its converged image matches the plain one. A
-  proposal pdf placed in the wrong recursion (say the light technique's side) breaks the MIS
-  partition of unity and shows up here as a biased image.
+* With alpha > 0 (in media and at surfaces) the renderer stays unbiased: its converged image
+  matches the plain one. A proposal pdf placed in the wrong recursion (say the light
+  technique's side) breaks the MIS partition of unity and shows up here as a biased image.
 """
 import numpy as np
 import pytest
@@ -127,6 +127,40 @@ def test_product_sampler_follows_its_pdf(seed, g):
         assert abs(np.mean(1 / pdf) - 4 * np.pi) / (4 * np.pi) < 0.03
 
 
+def cosine_kappa():
+    import os
+    import re
+    path = os.path.join(os.path.dirname(__file__), "..", "gpuupbp_amd", "csrc", "device", "phase_lobes.h")
+    return float(re.search(r"kCosineKappa = ([-0-9.e+]+)f", open(path).read()).group(1))
+
+
+def test_cosine_lobe_follows_the_clamped_cosine():
+    # the fitted lobe against max(cos, 0) / pi: same mean cosine to a few percent
+    k = cosine_kappa()
+    assert abs(1 / np.tanh(k) - 1 / k - 2 / 3) < 0.03
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_surface_product_sampler_follows_its_pdf(seed):
+    rng = np.random.default_rng(seed)
+    lobes = random_lobes(rng, kappa_hi=100.0)
+    n = unit(rng.normal(size=3))
+    out = oracle_py.guide_samples(lobes, strat(100 ** 3, rng), n, 0.0, surface=True).astype(np.float64)
+    d, pdf = out[:, :3], out[:, 3]
+    quad = fibonacci_sphere(400000)
+    k = cosine_kappa()
+    wts = lobes[:, 4] / lobes[:, 4].sum()
+    mus = lobes[:, :3] / np.linalg.norm(lobes[:, :3], axis=1, keepdims=True)
+    q = lambda x: sum(wts[i] * vmf(x, mus[i], float(lobes[i, 3])) for i in range(8))
+    ref = lambda x: q(x) * vmf(x, n, k)
+    z = np.mean(ref(quad)) * 4 * np.pi
+    rel = np.abs(pdf - ref(d) / z) / (ref(d) / z)
+    assert np.max(rel) < 3e-3, np.max(rel)
+    pq = ref(quad) / z
+    for phi in (lambda x: x[:, 0], lambda x: x[:, 1], lambda x: x[:, 2], lambda x: (x @ n > 0).astype(np.float64)):
+        assert abs(np.mean(phi(d)) - np.mean(phi(quad) * pq) * 4 * np.pi) < 3e-3
+
+
 def test_product_of_one_lobe_is_three_lobes():
     rng = np.random.default_rng(4)
     arrive = unit([0.3, -0.5, 0.8])
@@ -193,22 +227,23 @@ def test_alpha_zero_is_the_plain_renderer(boxfog_obj):
 
 @pytest.mark.parametrize("alg", ["upbp_bpt", "upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d"])
 def test_mixture_keeps_the_estimator_unbiased(boxfog_obj, alg):
-    # A deliberately poor proposal (random sharp lobes, alpha 0.9) must converge to the plain
-    # image. Same iterations, so the same light paths and primary rays: the two estimates differ
-    # only where guided camera paths scatter elsewhere, which is zero-mean noise, and the image
-    # means agree to ~0.1% (the iteration-to-iteration noise is ~4%). Evaluating the camera
-    # technique at light vertices with the phase function instead of the mixture moves the mean
-    # by 0.6-0.8%.
+    # A deliberately poor proposal (random sharp lobes, alpha 0.9 in the medium and at the
+    # diffuse and glossy walls) must converge to the plain image. Same iterations, so the same
+    # light paths and primary rays: the two estimates differ only where guided camera paths
+    # scatter elsewhere, which is zero-mean noise; over 256 iterations the image means agree to
+    # ~0.15% (another seed moves the plain mean by ~0.1%). Evaluating the camera technique at
+    # surface light vertices with the BSDF instead of the proposal moves the mean by ~1%, at
+    # medium light vertices with the phase function by 0.6-0.8%.
     args = box_args("16x16", alg=alg, extra=("-r_initial", "0.05", "-l", "6"))
-    plain, _, sh = _oracle(boxfog_obj, args, 64)
+    plain, _, sh = _oracle(boxfog_obj, args, 256)
     lobes, lo, hi = guide_for(sh, 3, np.random.default_rng(7), kappa_hi=200.0)
     oracle_py.set_guide(lobes, 3, lo, hi, 0.9)
     try:
-        guided, _, _ = _oracle(boxfog_obj, args, 64)
+        guided, _, _ = _oracle(boxfog_obj, args, 256)
     finally:
         oracle_py.set_guide(None, 0, None, None, 0.0)
     assert np.isfinite(guided).all() and (guided >= 0).all()
-    assert abs(guided.mean() - plain.mean()) / plain.mean() < 0.003, (guided.mean(), plain.mean())
+    assert abs(guided.mean() - plain.mean()) / plain.mean() < 0.004, (guided.mean(), plain.mean())
     gb = guided.reshape(4, 4, 4, 4, 3).mean(axis=(1, 3))
     pb = plain.reshape(4, 4, 4, 4, 3).mean(axis=(1, 3))
     assert np.max(np.abs(gb - pb) / (pb + 1e-2)) < 0.05
