@@ -139,6 +139,58 @@ def measured_traffic(kernel):
     return t["traffic_bytes_per_launch"]
 
 
+def batch_jobs(workdir, n_jobs=64, iters=4, threads=8):
+    """Batched jobs: n_jobs independent renderers drawn with seed 2 -- scene box / boxfog /
+    slab room (small and 195k-triangle), resolution in {64, 128, 256, 512}^2, max length in
+    [4, 16], radius 0.01 -- each rendering `iters` iterations, one after another and then all at
+    once through upbp_run_batch. Returns the measurement."""
+    import numpy as np
+    from gpuupbp_amd import capi
+    from gpuupbp_amd.renderer import Renderer, run_batch
+    rng = np.random.default_rng(2)
+    kinds = ["box", "boxfog", "slabroom_small", "slabroom"]
+    objs = {}
+    jobs = []
+    for j in range(n_jobs):
+        kind = kinds[int(rng.integers(len(kinds)))]
+        seed = int(rng.integers(4)) if kind != "slabroom" else 3
+        key = (kind, seed)
+        if key not in objs:
+            d = os.path.join(workdir, "batch_%s_%d" % key)
+            os.makedirs(d, exist_ok=True)
+            objs[key] = capi.generate_scene(kind, seed, os.path.join(d, kind + ".obj"))
+        res = int(rng.choice([64, 128, 256, 512]))
+        length = int(rng.integers(4, 17))
+        rc, cfg = capi.parse_args(["-s", "-1", objs[key], "-r", "%dx%d" % (res, res), "-l", str(length),
+                                   "-r_initial", "0.01", "-a", "upbp_all"])
+        assert rc == 0
+        sh = capi.SceneHandle(objs[key], cfg.res_x, cfg.res_y)
+        jobs.append((sh, Renderer(sh.ptr, cfg), res * res, kind, length))
+    rs = [r for _, r, _, _, _ in jobs]
+    pixels = sum(px for _, _, px, _, _ in jobs)
+    run_batch(rs, [0] * len(rs), threads)  # warm-up: buffers grow to their sizes
+    for r in rs:
+        r.sync()
+    t0 = time.perf_counter()
+    for i in range(1, iters + 1):
+        for r in rs:
+            r.run_iteration(i, want_stats=False)
+            r.sync()
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(iters + 1, 2 * iters + 1):
+        run_batch(rs, [i] * len(rs), threads)
+    t_batch = time.perf_counter() - t0
+    for sh, r, _, _, _ in jobs:
+        r.close()
+        sh.close()
+    mix = {k: sum(1 for _, _, _, kk, _ in jobs if kk == k) for k in kinds}
+    return {"jobs": n_jobs, "iterations_per_job": iters, "threads": threads, "job_mix": mix,
+            "pixels_per_round": pixels, "seconds_sequential": round(t_seq, 3), "seconds_batched": round(t_batch, 3),
+            "samples_per_s_sequential": pixels * iters / t_seq, "samples_per_s_batched": pixels * iters / t_batch,
+            "speedup": t_seq / t_batch}
+
+
 def learned_vs_plain():
     """The committed equal-time comparison of the learned proposal with the plain renderer on the
     slab room (bench.py --learned; a ten-minute run, so the default run reports the committed
@@ -170,6 +222,9 @@ def main():
                     help="run the library's autotuner (upbp_autotune) before the warm-up and use its choice")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel run once for bandwidth and counter calibration (0: skip)")
+    ap.add_argument("--batch", type=int, default=None, metavar="JOBS",
+                    help="batched-jobs measurement over JOBS independent renderers (upbp_run_batch)")
+    ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--learned", type=float, default=None,
                     help="seconds per renderer of the learned-vs-plain equal-time comparison (slab room)")
     ap.add_argument("--ref-seconds", type=float, default=None, help="reference render time (default 5 x --learned)")
@@ -191,6 +246,10 @@ def main():
 
     workdir = a.workdir or tempfile.mkdtemp(prefix="upbp_bench_")
     os.makedirs(workdir, exist_ok=True)
+    if a.batch is not None:
+        res = batch_jobs(workdir, n_jobs=a.batch, threads=a.batch_threads)
+        print(json.dumps({"metric": "batched jobs, samples/s, 1 MI355X", **res}))
+        return
     if a.learned is not None:
         w, h = (int(v) for v in a.learned_res.split("x"))
         lists = {k: ([conv(v) for v in str(val).split(",")] if val is not None else [None])

@@ -151,6 +151,7 @@ _SIGS = [
     ("upbp_tuning_defaults", None, [C.c_void_p]),
     ("upbp_set_tuning", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("upbp_run_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_test_env_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

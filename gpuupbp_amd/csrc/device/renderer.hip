@@ -13,11 +13,13 @@
 #include "../../../include/upbp_capi.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace upbp;
@@ -1086,6 +1088,36 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
     if (e) return e;
     if (chosen) *chosen = best;
     if (ms_per_iteration) *ms_per_iteration = float(best_ms);
+    return UPBP_OK;
+}
+
+// One iteration of each of n independent renderers at once. Each renderer owns its stream, so
+// host threads that drive different renderers put their kernels on the GPU side by side:
+// small jobs, which fill only part of the chip on their own, overlap. Threads take the next
+// job from a shared counter; the first error (lowest job index) is returned.
+extern "C" int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iterations, int32_t n, int32_t threads)
+{
+    if (n < 0 || (n > 0 && (!rs || !iterations)) || threads < 1) return fail(nullptr, "invalid batch", UPBP_E_ARG);
+    for (int32_t i = 0; i < n; i++)
+        if (!rs[i]) return fail(nullptr, "null renderer in batch", UPBP_E_ARG);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return fail(nullptr, "no HIP device", UPBP_E_HIP);
+    std::atomic<int32_t> next{0};
+    std::vector<int> rc(size_t(n), UPBP_OK);
+    auto work = [&]() {
+        if (hipSetDevice(device) != hipSuccess) return;
+        for (int32_t i = next++; i < n; i = next++) {
+            rc[size_t(i)] = upbp_run_iteration(rs[i], iterations[i], nullptr);
+            if (!rc[size_t(i)] && hipStreamSynchronize(rs[i]->stream) != hipSuccess) rc[size_t(i)] = UPBP_E_HIP;
+        }
+    };
+    const int32_t t = std::min(threads, std::max<int32_t>(n, 1));
+    std::vector<std::thread> pool;
+    for (int32_t k = 1; k < t; k++) pool.emplace_back(work);
+    work();
+    for (std::thread& th : pool) th.join();
+    for (int32_t i = 0; i < n; i++)
+        if (rc[size_t(i)]) return rc[size_t(i)];
     return UPBP_OK;
 }
 

@@ -218,6 +218,19 @@ def guide_samples(lobes, u, arrive=(0.0, 0.0, 1.0), g=0.0, surface=False):
     return out
 
 
+def run_batch(renderers, iterations, threads=8):
+    """upbp_run_batch: one iteration of each renderer (iterations[i] for renderers[i]) with up to
+    `threads` host threads driving their streams side by side."""
+    n = len(renderers)
+    hs = (C.c_void_p * n)(*[r.h for r in renderers])
+    its = np.ascontiguousarray(iterations, np.int32)
+    rc = capi.lib().upbp_run_batch(hs, _p(its), n, threads)
+    if rc != 0:
+        raise RuntimeError("upbp_run_batch failed: %d" % rc)
+    for r, it in zip(renderers, iterations):
+        r.iterations = max(r.iterations, int(it) + 1)
+
+
 def closest_hit(verts, rays):
     verts = np.ascontiguousarray(verts, np.float32)
     rays = np.ascontiguousarray(rays, np.float32)

@@ -279,6 +279,12 @@ typedef struct upbp_tuning {
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
+/* Batched jobs (no reference counterpart): one iteration of each of n independent renderers
+ * (iterations[i] for renderer i), driven by up to `threads` host threads at once. Each renderer
+ * owns a HIP stream, so the jobs' kernels overlap on the GPU; a renderer must not appear twice
+ * in one batch. Returns the first job's error, if any. */
+int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iterations, int32_t n, int32_t threads);
+
 /* Autotuner (no reference counterpart): a coordinate search over subbeam_factor {8, 12, 16,
  * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2} and piece_order {0, 2},
  * from the current tuning, timing `repeats` (1..64) renders of `iteration` per candidate after
