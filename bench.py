@@ -191,6 +191,48 @@ def batch_jobs(workdir, n_jobs=64, iters=4, threads=8):
             "speedup": t_seq / t_batch}
 
 
+def stress(workdir, steps=3, warmup=1):
+    """Combined stress workload: 3840x2160, 2.05M-triangle blob grid with eight overlapping media
+    (g from -0.3 to 0.95), l = 16, 8M light paths per iteration, all estimators; the plain
+    renderer, then the learned proposal (library defaults, trained online) on the same scene."""
+    import ctypes as C
+    from gpuupbp_amd import capi
+    from gpuupbp_amd.renderer import Renderer
+    obj = os.path.join(workdir, "stress.obj")
+    if not os.path.exists(obj + ".aux"):
+        capi.generate_scene("stress", 1, obj)
+    rc, cfg = capi.parse_args(["-s", "-1", obj, "-r", "3840x2160", "-l", "16", "-pcpi", "8000000",
+                               "-r_initial", "0.005", "-a", "upbp_all"])
+    assert rc == 0
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+    out = {"workload": "stress: 10x10 noise-displaced icospheres (%d tris), 8 media spheres (g -0.3..0.95), "
+                       "3840x2160, l=16, 8M light paths/iter, r=0.005, upbp_all" % sh.scene.real.count}
+    for label, guide in (("plain", None), ("learned", {})):
+        r = Renderer(sh.ptr, cfg)
+        if guide is not None:
+            r.set_guide(**guide)
+        for i in range(warmup):
+            r.run_iteration(i, want_stats=False)
+        r.sync()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            r.run_iteration(warmup + i, want_stats=False)
+        r.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        st = r.run_iteration(warmup + steps, want_stats=True)
+        free, total = C.c_uint64(), C.c_uint64()
+        capi.lib().upbp_memory_info(C.byref(free), C.byref(total))
+        r.close()
+        out[label] = {"ms_per_iteration": round(ms, 1), "samples_per_s": 3840 * 2160 / (ms * 1e-3),
+                      "surface_points": st.surface_vertices, "medium_points": st.medium_vertices,
+                      "beams": st.beams, "subbeams": st.subbeams, "bb1d_accepts": st.bb1d_accepts,
+                      "path_errors": st.path_errors, "vertex_drops": st.vertex_drops, "beam_drops": st.beam_drops,
+                      "device_memory_used_gib": round((total.value - free.value) / 2 ** 30, 1)}
+        print("bench: stress %s %.1f ms per iteration" % (label, ms), file=sys.stderr, flush=True)
+    sh.close()
+    return out
+
+
 def learned_vs_plain():
     """The committed equal-time comparison of the learned proposal with the plain renderer on the
     slab room (bench.py --learned; a ten-minute run, so the default run reports the committed
@@ -222,6 +264,7 @@ def main():
                     help="run the library's autotuner (upbp_autotune) before the warm-up and use its choice")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain copy kernel run once for bandwidth and counter calibration (0: skip)")
+    ap.add_argument("--stress", action="store_true", help="combined stress workload (3840x2160, 2M tris, 8 media)")
     ap.add_argument("--batch", type=int, default=None, metavar="JOBS",
                     help="batched-jobs measurement over JOBS independent renderers (upbp_run_batch)")
     ap.add_argument("--batch-threads", type=int, default=8)
@@ -246,6 +289,9 @@ def main():
 
     workdir = a.workdir or tempfile.mkdtemp(prefix="upbp_bench_")
     os.makedirs(workdir, exist_ok=True)
+    if a.stress:
+        print(json.dumps({"metric": "combined stress, samples/s, 1 MI355X", **stress(workdir)}))
+        return
     if a.batch is not None:
         res = batch_jobs(workdir, n_jobs=a.batch, threads=a.batch_threads)
         print(json.dumps({"metric": "batched jobs, samples/s, 1 MI355X", **res}))

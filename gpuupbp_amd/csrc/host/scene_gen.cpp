@@ -8,6 +8,8 @@
 //   "blobs" : 8x8 grid of noise-displaced icospheres (subdivision 5, 20480 triangles
 //             each, 1.31M in total) on a floor, three overlapping imaginary media
 //             spheres with priorities 1..3, two quad lights and a point light.
+//   "stress": the same construction on a 10x10 grid (2.05M triangles) with eight
+//             overlapping media spheres (priorities 1..8, g from -0.3 to 0.95).
 //   "furnace": closed box with albedo-1 diffuse walls lit by a quad light (tests only).
 //   "slabroom": room of ~196k triangles with a dense forward-scattering medium slab, lit
 //             only through a 5% aperture in one wall ("slabroom_small": coarse walls).
@@ -250,7 +252,7 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace, bool fog =
     return UPBP_OK;
 }
 
-int gen_blobs(const std::string& obj_path, uint32_t seed)
+int gen_blobs(const std::string& obj_path, uint32_t seed, int grid_n = 8, int n_media = 3)
 {
     const std::string mtl_path = strip_ext(obj_path) + ".mtl";
     FILE* obj = fopen(obj_path.c_str(), "w");
@@ -272,7 +274,7 @@ int gen_blobs(const std::string& obj_path, uint32_t seed)
     fprintf(mtl, "newmtl floor\nNs 0\nKd 0.6 0.6 0.6\nKs 0 0 0\nKe 0 0 0\n\n");
     fprintf(mtl, "newmtl light_a\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 14 12 10\n\n");
     fprintf(mtl, "newmtl light_b\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 6 8 12\n\n");
-    for (int i = 0; i < 3; i++) fprintf(mtl, "newmtl fog%d\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 0 0 0\n\n", i);
+    for (int i = 0; i < n_media; i++) fprintf(mtl, "newmtl fog%d\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 0 0 0\n\n", i);
 
     Writer w;
     w.obj = obj;
@@ -285,8 +287,9 @@ int gen_blobs(const std::string& obj_path, uint32_t seed)
     w.quad({1.5, 3.5, 0.5}, {2.5, 3.5, 0.5}, {2.5, 3.5, 1.5}, {1.5, 3.5, 1.5});
 
     const Mesh base = icosphere(5);
-    for (int gy = 0; gy < 8; gy++) {
-        for (int gx = 0; gx < 8; gx++) {
+    const double half = 0.5 * double(grid_n - 1);
+    for (int gy = 0; gy < grid_n; gy++) {
+        for (int gx = 0; gx < grid_n; gx++) {
             Mesh m = base;
             const uint64_t nseed = rng.next();
             const double amp = rng.uniform(0.08, 0.2);
@@ -296,30 +299,39 @@ int gen_blobs(const std::string& obj_path, uint32_t seed)
             char name[16];
             snprintf(name, sizeof(name), "blob%d", static_cast<int>(rng.next() % n_blob_mats));
             w.usemtl(name);
-            const P3 c = {(gx - 3.5) * 1.0, 0.0, (gy - 3.5) * 1.0};
+            const P3 c = {(gx - half) * 1.0, 0.0, (gy - half) * 1.0};
             w.mesh(m, c, 0.38, true);
         }
     }
     const Mesh fog = icosphere(3);
-    const P3 fog_c[3] = {{-1.2, 0.2, -0.8}, {1.0, 0.4, 0.3}, {0.0, 0.0, 1.4}};
-    const double fog_r[3] = {2.2, 1.8, 1.6};
-    for (int i = 0; i < 3; i++) {
+    const P3 fog_c3[3] = {{-1.2, 0.2, -0.8}, {1.0, 0.4, 0.3}, {0.0, 0.0, 1.4}};
+    const double fog_r3[3] = {2.2, 1.8, 1.6};
+    for (int i = 0; i < n_media; i++) {
         char name[16];
         snprintf(name, sizeof(name), "fog%d", i);
         w.usemtl(name);
-        w.mesh(fog, fog_c[i], fog_r[i], false);
+        if (n_media == 3) {
+            w.mesh(fog, fog_c3[i], fog_r3[i], false);
+        } else { // a ring of overlapping spheres over the grid
+            const double a = 2.0 * 3.14159265358979 * double(i) / double(n_media);
+            w.mesh(fog, {2.6 * std::cos(a), 0.1 + 0.1 * (i % 3), 2.6 * std::sin(a)}, 1.5 + 0.1 * (i % 4), false);
+        }
     }
     fclose(obj);
     fclose(mtl);
 
-    camera_aux(aux, {0.0, 4.5, 7.5}, {0.0, -0.3, 0.0}, {0, 1, 0}, 55.0);
-    const double gs[3] = {0.0, 0.5, 0.9};
-    for (int i = 0; i < 3; i++) {
+    if (grid_n == 8)
+        camera_aux(aux, {0.0, 4.5, 7.5}, {0.0, -0.3, 0.0}, {0, 1, 0}, 55.0);
+    else
+        camera_aux(aux, {0.0, 5.5, 9.5}, {0.0, -0.3, 0.0}, {0, 1, 0}, 55.0);
+    const double gs3[3] = {0.0, 0.5, 0.9}, gs8[8] = {-0.3, 0.0, 0.3, 0.5, 0.7, 0.8, 0.9, 0.95};
+    for (int i = 0; i < n_media; i++) {
+        const double g = n_media == 3 ? gs3[i] : gs8[i % 8];
         const double st = rng.uniform(0.2, 2.0);
         const double albedo = rng.uniform(0.6, 0.95);
         fprintf(aux, "medium fogmed%d\nabsorption %.5f %.5f %.5f\nemission 0 0 0\nscattering %.5f %.5f %.5f\ng %.2f\n\n",
                 i, st * (1 - albedo), st * (1 - albedo), st * (1 - albedo), st * albedo, st * albedo * 0.9,
-                st * albedo * 0.8, gs[i]);
+                st * albedo * 0.8, g);
         fprintf(aux, "material fog%d\ngeometryType imaginary\nmediumId fogmed%d\npriority %d\n\n", i, i, i + 1);
     }
     fprintf(aux, "light_point 2.5 2.0 3.0 6.0 6.0 6.0\n");
@@ -417,6 +429,7 @@ extern "C" int upbp_generate_scene(const char* kind, uint32_t seed, const char* 
     if (k == "furnace") return gen_box(obj_path, seed, true);
     if (k == "boxfog") return gen_box(obj_path, seed, false, true);
     if (k == "blobs") return gen_blobs(obj_path, seed);
+    if (k == "stress") return gen_blobs(obj_path, seed, 10, 8);
     if (k == "slabroom") return gen_slabroom(obj_path, seed, 128);
     if (k == "slabroom_small") return gen_slabroom(obj_path, seed, 8); // tests: same room, coarse walls
     return UPBP_E_ARG;
