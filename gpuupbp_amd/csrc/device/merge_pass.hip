@@ -12,8 +12,8 @@
 // by per-wave LDS queues, so that each stage runs with the whole wave converged: the node
 // step (box tests, stack) appends the leaves it hits as candidates; 64 candidates at a time
 // are tested with the exact predicate and piece ownership; 64 accepted (query, photon) pairs
-// at a time are evaluated (attenuation, phase function, both-side MIS updates) and added to
-// the query's camera path with float atomics.
+// at a time are evaluated (attenuation, phase function, both-side MIS updates) and their colour
+// gathered per query in LDS, then added to the query's camera path with float atomics.
 #include "estimators.h"
 #include "kernels.h"
 
@@ -51,6 +51,26 @@ struct PairQueue {
     uint32_t slot[kPairCap]; // light vertex slot (P-B2D) or beam slot (B-B1D)
     float x[kPairCap], y[kPairCap], z[kPairCap], w[kPairCap];
 };
+
+// Colour of accepted pairs, gathered per segment query before it goes to the camera path's
+// merge colour: 64 direct-mapped slots per wave keyed by the query. A query's accepted pairs
+// mostly fall in the few batches while its piece is walked, so most of its float atomics to
+// HBM become LDS adds; a slot taken by another query is written back first.
+struct ColorCache {
+    uint32_t tag[64];   // segment query index + 1 (0: empty)
+    uint32_t pix[64];   // its camera path
+    uint32_t claim[64]; // lane that takes the slot in this batch
+    float r[64], g[64], b[64];
+};
+
+__device__ __forceinline__ uint32_t cache_slot(uint32_t q) { return (q * 2654435761u) >> 26; }
+
+__device__ __forceinline__ void add_color(float* mcolor, uint32_t pix, float r, float g, float b)
+{
+    if (r != 0.f) atomicAdd(&mcolor[3 * pix], r);
+    if (g != 0.f) atomicAdd(&mcolor[3 * pix + 1], g);
+    if (b != 0.f) atomicAdd(&mcolor[3 * pix + 2], b);
+}
 
 // orders this wave's LDS writes before the reads that follow (all lanes of one wave)
 __device__ __forceinline__ void wave_lds_sync()
@@ -138,10 +158,15 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     __shared__ CandQueue cq[kMergeWaves];
     __shared__ PairQueue pq[kMergeWaves];
     __shared__ Steal stl[kMergeWaves];
+    __shared__ ColorCache ccache[kMergeWaves];
     CandQueue& C = cq[threadIdx.x >> 6];
     PairQueue& Q = pq[threadIdx.x >> 6];
     Steal& S = stl[threadIdx.x >> 6];
+    ColorCache& K = ccache[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
+    K.tag[lane] = 0u;
+    K.r[lane] = K.g[lane] = K.b[lane] = 0.f;
+    wave_lds_sync();
     // B-B1D walks query pieces from its XCD's region first (then helps the others);
     // P-B2D (few candidates per query) walks whole segments
     const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
@@ -299,6 +324,9 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                 const uint32_t m = qn < 64u ? qn : 64u;
                 n_pb++;
                 wave_lds_sync();
+                bool has = false;
+                uint32_t key = 0, cslot = 0, cpix = 0;
+                f3 c = mk(0.f);
                 if (lane < m) {
                     const uint32_t e = qn - m + lane;
                     const uint32_t pqi = Q.q[e], pslot = Q.slot[e];
@@ -318,10 +346,32 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                         load_beam(&b.beams[pslot], lb);
                         eval_bb1d(sc, p, sum, q, cw, lb, pz, pw, px, py);
                     }
-                    const f3 c = (tatt * norm) * sum;
-                    if (c.x != 0.f) atomicAdd(&b.mcolor[3 * pix], c.x);
-                    if (c.y != 0.f) atomicAdd(&b.mcolor[3 * pix + 1], c.y);
-                    if (c.z != 0.f) atomicAdd(&b.mcolor[3 * pix + 2], c.z);
+                    c = (tatt * norm) * sum;
+                    has = c.x != 0.f || c.y != 0.f || c.z != 0.f;
+                    key = pqi + 1u;
+                    cslot = cache_slot(pqi);
+                    cpix = pix;
+                }
+                // one lane per contended slot takes it (writing back the query it held); every
+                // lane whose query holds its slot then adds in LDS, the others go to HBM
+                wave_lds_sync();
+                if (has) K.claim[cslot] = lane;
+                wave_lds_sync();
+                if (has && K.claim[cslot] == lane && K.tag[cslot] != key) {
+                    if (K.tag[cslot] != 0u) add_color(b.mcolor, K.pix[cslot], K.r[cslot], K.g[cslot], K.b[cslot]);
+                    K.tag[cslot] = key;
+                    K.pix[cslot] = cpix;
+                    K.r[cslot] = K.g[cslot] = K.b[cslot] = 0.f;
+                }
+                wave_lds_sync();
+                if (has) {
+                    if (K.tag[cslot] == key) {
+                        atomicAdd(&K.r[cslot], c.x);
+                        atomicAdd(&K.g[cslot], c.y);
+                        atomicAdd(&K.b[cslot], c.z);
+                    } else {
+                        add_color(b.mcolor, cpix, c.x, c.y, c.z);
+                    }
                 }
                 wave_lds_sync();
                 qn -= m;
@@ -352,6 +402,9 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
         }
         if (!any_active && exhausted) break;
     }
+    // write back the queries still held in the colour cache
+    wave_lds_sync();
+    if (K.tag[lane] != 0u) add_color(b.mcolor, K.pix[lane], K.r[lane], K.g[lane], K.b[lane]);
     const uint32_t s_nodes = wave_sum(n_nodes), s_cands = wave_sum(n_cands), s_acc = wave_sum(n_acc),
                    s_err = wave_sum(n_err);
     // loop trips and batches are the same in every lane: lane 0 reports them for the wave
