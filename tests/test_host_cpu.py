@@ -262,3 +262,20 @@ def test_cli_help_and_no_gpu_error(box_obj):
         r = subprocess.run([exe, "-s", "-1", box_obj, "-r", "16x16", "-i", "1"], capture_output=True, text=True,
                            timeout=120)
         assert r.returncode == 1 and "no GPU visible" in r.stderr
+
+
+def test_batch_autotune_and_guide_reject_bad_arguments():
+    # argument checks come before any GPU call, so they hold on a machine without one
+    import ctypes as C
+    E_ARG = -1  # UPBP_E_ARG
+    lib = capi.lib()
+    its = (C.c_int32 * 2)(0, 1)
+    assert lib.upbp_run_batch(None, its, 2, 4) == E_ARG
+    assert lib.upbp_run_batch((C.c_void_p * 2)(None, None), its, 2, 4) == E_ARG
+    assert lib.upbp_run_batch(None, None, 0, 0) == E_ARG  # no threads
+    assert lib.upbp_run_batch(None, None, -1, 1) == E_ARG
+    assert lib.upbp_autotune(None, 0, 1, None, None) == E_ARG
+    g = capi.GuideConfig()
+    lib.upbp_guide_defaults(C.byref(g))
+    assert (g.alpha, g.surface_alpha, g.grid_res, g.steps_per_iteration) == (0.0, pytest.approx(0.65), 64, 10)
+    assert lib.upbp_set_guide(None, C.byref(g)) == E_ARG
