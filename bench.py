@@ -233,6 +233,15 @@ def stress(workdir, steps=3, warmup=1):
     return out
 
 
+def committed(name):
+    """A committed measurement under profiles/ (JSON written from a bench.py --batch / --stress
+    run), or None."""
+    try:
+        return json.load(open(os.path.join(ROOT, "profiles", name)))
+    except (OSError, ValueError):
+        return None
+
+
 def learned_vs_plain():
     """The committed equal-time comparison of the learned proposal with the plain renderer on the
     slab room (bench.py --learned; a ten-minute run, so the default run reports the committed
@@ -398,6 +407,17 @@ def main():
         },
     }
     out["learned_vs_plain"] = learned_vs_plain()
+    batch = committed("batched_jobs.json")
+    if batch:
+        best = max(batch["runs"], key=lambda x: x["speedup"])
+        out["batched_jobs"] = {k: best[k] for k in ("jobs", "threads", "samples_per_s_sequential",
+                                                    "samples_per_s_batched", "speedup")}
+        out["batched_jobs"]["source"] = "profiles/batched_jobs.md (python bench.py --batch 64)"
+    stress_run = committed("stress_4k.json")
+    if stress_run:
+        out["stress"] = {"workload": stress_run["workload"], "ms_per_iteration": stress_run["plain"]["ms_per_iteration"],
+                         "samples_per_s": stress_run["plain"]["samples_per_s"],
+                         "source": "profiles/stress_4k.md (python bench.py --stress)"}
     print("bench: GPU done, %.1f ms per iteration" % ms, file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sh.ptr, cfg)
