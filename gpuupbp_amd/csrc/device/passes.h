@@ -104,6 +104,16 @@ UPBP_HD void pack_weights(PathState& s, const StateWeights& sw)
 struct BStack {
     int n;
     int mat[kMaxStack], med[kMaxStack], pr[kMaxStack];
+    // the live entries only: the arrays sit in scratch memory and paths hold one or two entries
+    UPBP_HD void copy_from(const BStack& o)
+    {
+        n = o.n;
+        for (int i = 0; i < o.n; i++) {
+            mat[i] = o.mat[i];
+            med[i] = o.med[i];
+            pr[i] = o.pr[i];
+        }
+    }
     UPBP_HD bool push(int m, int d, int p)
     {
         if (n == kMaxStack) return false;
@@ -580,7 +590,8 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
         t_max = kInf;
         miss_dist = kInf;
     }
-    BStack cp = bstack;
+    BStack cp;
+    cp.copy_from(bstack);
     float t_min = first_eps;
     bool not_yet_real;
     bool ok;
@@ -723,7 +734,8 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
 __device__ __forceinline__ int occluded(const DScene& sc, bool& hit, f3 p, f3 d, float tmax, const BStack& bs,
                                         unsigned flags, VolSeg* segs, int& nseg)
 {
-    BStack cp = bs;
+    BStack cp;
+    cp.copy_from(bs);
     Hit tmp;
     hit_reset(tmp, tmax);
     LiteSeg lite[kMaxSegments];
