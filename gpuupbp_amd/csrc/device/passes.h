@@ -103,7 +103,10 @@ UPBP_HD void pack_weights(PathState& s, const StateWeights& sw)
 // ---- medium priority stack (kernel/pstack.cuh:33-137) ------------------------------
 struct BStack {
     int n;
-    int mat[kMaxStack], med[kMaxStack], pr[kMaxStack];
+    // material and medium ids fit 16 bits (the renderer rejects larger scenes): the arrays
+    // live in scratch memory, so narrower entries mean less scratch traffic
+    int16_t mat[kMaxStack], med[kMaxStack];
+    int pr[kMaxStack];
     // the live entries only: the arrays sit in scratch memory and paths hold one or two entries
     UPBP_HD void copy_from(const BStack& o)
     {
@@ -128,8 +131,8 @@ struct BStack {
             med[i] = med[i - 1];
             pr[i] = pr[i - 1];
         }
-        mat[pos] = m;
-        med[pos] = d;
+        mat[pos] = int16_t(m);
+        med[pos] = int16_t(d);
         pr[pos] = p;
         n++;
         return true;
@@ -163,8 +166,8 @@ __device__ __forceinline__ void load_stack(const StackRec* recs, uint32_t p, BSt
     s.n = r->n;
     for (int i = 0; i < s.n; i++) {
         const int4 v = r->e[i];
-        s.mat[i] = v.x;
-        s.med[i] = v.y;
+        s.mat[i] = int16_t(v.x);
+        s.med[i] = int16_t(v.y);
         s.pr[i] = v.z;
     }
 }

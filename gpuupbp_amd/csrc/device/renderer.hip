@@ -431,6 +431,8 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     *out = nullptr;
     if (scene->light_count == 0) return fail(r, "scene has no lights", UPBP_E_ARG);
     if (const char* bad = invalid_scene(*scene)) return fail(r, bad, UPBP_E_ARG);
+    if (scene->mat_count > 32767u || scene->med_count > 32767u)
+        return fail(r, "at most 32767 materials and 32767 media", UPBP_E_ARG);
     if (cfg->path_count_per_iter < 1.f) return fail(r, "light path count must be positive", UPBP_E_ARG);
     if (cfg->max_path_length >= uint32_t(kMaxBounces)) return fail(r, "max path length must be below 64", UPBP_E_ARG);
     int ndev = 0;
