@@ -41,6 +41,7 @@ class Config(C.Structure):
         ("iteration_offset", C.c_uint64), ("do_log", C.c_int32), ("use_shading_normal", C.c_int32),
         ("scene_id", C.c_int32), ("scene_obj_file", C.c_char * 1024), ("output_name", C.c_char * 1024),
         ("additional_args", C.c_char * 512),
+        ("guide_surface_alpha", C.c_float), ("guide_media_alpha", C.c_float),
     ]
 
 

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -114,6 +115,8 @@ void full_help(const char* prog)
     printf("\n    Other options:\n");
     printf("    -continuous_output <iter_count>  -pcpi <path_count>  -sn <0|1>  -assert  -ioff <iter>  -log\n");
     printf("    -r <WIDTHxHEIGHT>  -ioff <iter>\n");
+    printf("    -guide <alpha>  -guide_media <alpha>   learned scattering proposal at surfaces / in media\n");
+    printf("                    (mixture probability in [0, 1]; 0 = off, the default; not in gpuupbp)\n");
 }
 
 } // namespace
@@ -357,6 +360,15 @@ extern "C" int upbp_parse_commandline(int argc, const char** argv, upbp_config* 
         } else if (arg == "-log") {
             c.do_log = 1;
             extra << "_log";
+        } else if (arg == "-guide" || arg == "-guide_media") {
+            const char* v = need_value(arg.c_str(), "-hf");
+            if (!v) return UPBP_E_ARG;
+            char* end = nullptr;
+            const float a = std::strtof(v, &end);
+            if (end == v || *end != '\0' || !(a >= 0.f && a <= 1.f))
+                return fail(("invalid argument of " + arg + " option, please see help (-hf)").c_str());
+            (arg == "-guide" ? c.guide_surface_alpha : c.guide_media_alpha) = a;
+            extra << (arg == "-guide" ? "_guide" : "_guidem") << v;
         } else {
             fprintf(stderr, "invalid option \"%s\", please see help (-hf)", arg.c_str());
             return UPBP_E_ARG;

@@ -121,6 +121,18 @@ int main(int argc, const char* argv[])
         upbp_scene_free(sh);
         return 1;
     }
+    if (cfg.guide_surface_alpha > 0.f || cfg.guide_media_alpha > 0.f) {
+        upbp_guide_config g;
+        upbp_guide_defaults(&g);
+        g.surface_alpha = cfg.guide_surface_alpha;
+        g.alpha = cfg.guide_media_alpha;
+        if (upbp_set_guide(r, &g)) {
+            fprintf(stderr, "Error: %s\n", upbp_last_error(r));
+            upbp_destroy(r);
+            upbp_scene_free(sh);
+            return 1;
+        }
+    }
     const std::string out = cfg.output_name;
     const std::string name = out.substr(0, out.size() >= 4 ? out.size() - 4 : 0);
     const std::string ext = out.size() >= 3 ? out.substr(out.size() - 3) : "";

@@ -74,6 +74,11 @@ typedef struct upbp_config {
     char scene_obj_file[1024];
     char output_name[1024];
     char additional_args[512];
+    /* Learned scattering proposal (no counterpart in the reference; 0 = off, the default):
+     * -guide <alpha> at non-specular surfaces, -guide_media <alpha> in media. The CLI then
+     * calls upbp_set_guide with the library's other defaults. */
+    float guide_surface_alpha;
+    float guide_media_alpha;
 } upbp_config;
 
 /* ---- flattened scene: replaces struct Scene + Record (gpuupbp/shared/scene.h:41-127) */

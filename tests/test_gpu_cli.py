@@ -32,3 +32,19 @@ def test_cli_renders_and_writes_exr(box_obj, tmp_path):
     _, img = _read_exr(out)
     rel = np.abs(img - ref) / np.maximum(np.abs(ref), 1e-2)
     assert np.mean(rel < 2e-3) > 0.97  # HALF storage plus atomic summation order
+
+
+def test_cli_learned_proposal(boxfog_obj, tmp_path):
+    # -guide / -guide_media switch the learned proposal on; the image stays an unbiased
+    # estimate of the same scene (compared by its mean with the plain CLI render)
+    exe = os.path.join(ROOT, "gpuupbp_amd", "gpuupbp")
+    base = ["-s", "-1", boxfog_obj] + box_args("48x48", extra=("-r_initial", "0.03", "-l", "6")) + ["-i", "24"]
+    plain, guided = str(tmp_path / "plain.exr"), str(tmp_path / "guided.exr")
+    r = subprocess.run([exe] + base + ["-o", plain], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe] + base + ["-guide", "0.65", "-guide_media", "0.5", "-o", guided], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    _, a = _read_exr(plain)
+    _, b = _read_exr(guided)
+    assert np.isfinite(b).all() and abs(b.mean() - a.mean()) / a.mean() < 0.03, (a.mean(), b.mean())

@@ -279,3 +279,14 @@ def test_batch_autotune_and_guide_reject_bad_arguments():
     lib.upbp_guide_defaults(C.byref(g))
     assert (g.alpha, g.surface_alpha, g.grid_res, g.steps_per_iteration) == (0.0, pytest.approx(0.65), 64, 10)
     assert lib.upbp_set_guide(None, C.byref(g)) == E_ARG
+
+
+def test_guide_options():
+    rc, cfg = capi.parse_args(["-s", "-1", "x.obj", "-guide", "0.65", "-guide_media", "0.2"])
+    assert rc == 0 and cfg.guide_surface_alpha == pytest.approx(0.65) and cfg.guide_media_alpha == pytest.approx(0.2)
+    assert b"_guide0.65_guidem0.2" in cfg.output_name
+    rc, cfg = capi.parse_args(["-s", "-1", "x.obj"])
+    assert rc == 0 and cfg.guide_surface_alpha == 0.0 and cfg.guide_media_alpha == 0.0
+    for bad in ("2", "-0.1", "abc"):
+        rc, _ = capi.parse_args(["-s", "-1", "x.obj", "-guide", bad])
+        assert rc != 0
