@@ -288,6 +288,7 @@ def main():
     ap.add_argument("--guide-res", default=None, help="record grid cells per axis (comma list allowed)")
     ap.add_argument("--guide-steps", default=None, help="Adam steps per iteration (comma list allowed)")
     ap.add_argument("--guide-lr", default=None, help="Adam step size (comma list allowed)")
+    ap.add_argument("--guide-history", default=None, help="training-sample ring size (comma list allowed)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the iteration-sharded multi-GPU path is not built", file=sys.stderr)
@@ -310,7 +311,7 @@ def main():
         lists = {k: ([conv(v) for v in str(val).split(",")] if val is not None else [None])
                  for k, val, conv in (("alpha", a.alpha, float), ("surface_alpha", a.surface_alpha, float),
                                       ("grid_res", a.guide_res, int), ("steps_per_iteration", a.guide_steps, int),
-                                      ("learning_rate", a.guide_lr, float))}
+                                      ("learning_rate", a.guide_lr, float), ("history", a.guide_history, int))}
         n = max(len(v) for v in lists.values())
         guides = [{k: v[i % len(v)] for k, v in lists.items() if v[i % len(v)] is not None} for i in range(n)]
         res = equal_time(a.learned, a.ref_seconds or 5 * a.learned, workdir, (w, h), guides)

@@ -115,7 +115,8 @@ class IterStats(C.Structure):
 class GuideConfig(C.Structure):
     _fields_ = [("alpha", C.c_float), ("grid_res", C.c_int32), ("train", C.c_int32),
                 ("steps_per_iteration", C.c_int32), ("batch", C.c_int32), ("learning_rate", C.c_float),
-                ("seed", C.c_uint32), ("max_samples", C.c_uint32), ("surface_alpha", C.c_float)]
+                ("seed", C.c_uint32), ("max_samples", C.c_uint32), ("surface_alpha", C.c_float),
+                ("history", C.c_uint32)]
 
 
 _lib = None

@@ -426,7 +426,8 @@ __device__ __forceinline__ void guide_record_sample(const DScene& sc, const Bufs
 }
 
 // pool of raw records -> training samples (10 floats: position, arrival, scattered, weight) with
-// a positive weight, compacted; after the splat, when path and merge colours are final
+// a positive weight, appended to the sample ring (gcount[1] counts every sample ever appended;
+// the ring keeps the last gring); after the splat, when path and merge colours are final
 __global__ void __launch_bounds__(256) k_guide_finalise(Bufs b)
 {
     const uint32_t n = min(b.gcount[0], b.gcap);
@@ -446,7 +447,7 @@ __global__ void __launch_bounds__(256) k_guide_finalise(Bufs b)
         base = __shfl(base, leader);
         const uint32_t at = base + uint32_t(__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane)))));
         if (keep) {
-            float* o = b.gsamp + size_t(at) * 10;
+            float* o = b.gsamp + size_t(at % b.gring) * 10;
             for (int k = 0; k < 9; k++) o[k] = r[k];
             o[9] = w;
         }

@@ -415,7 +415,8 @@ struct Bufs {
     float* graw;
     float* gsamp;
     uint32_t* gcount;
-    uint32_t gcap;
+    uint32_t gcap;  // raw pool capacity (per iteration)
+    uint32_t gring; // training-sample ring capacity (samples written at gcount[1] modulo gring)
     double* accum;
     unsigned long long* stats; // see STAT_*
     PhotonMaps maps;

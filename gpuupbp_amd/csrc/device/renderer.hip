@@ -178,7 +178,7 @@ struct upbp_renderer {
     bool gtrain = false;
     GuideNet* gnet = nullptr;
     DevBuf gpend, graw, gsamp, gcount;
-    uint32_t gcap = 0;
+    uint32_t gcap = 0, gring = 0;
     double gloss = 0.0, gms = 0.0;
     uint64_t gsamples = 0;
     BvhStore bvh_s, bvh_m, bvh_b;
@@ -711,9 +711,9 @@ static int train_guide(upbp_renderer* r)
     hipStream_t s = r->stream;
     uint32_t n = 0;
     HIPCHK(hipEventRecord(r->ev[4], s));
-    HIPCHK(hipMemcpyAsync(&n, r->gcount.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&n, r->gcount.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s)); // samples in the ring
     HIPCHK(hipStreamSynchronize(s));
-    n = std::min(n, r->gcap);
+    n = std::min(n, r->gring);
     r->gsamples = n;
     r->gloss = 0.0;
     r->gms = 0.0;
@@ -785,10 +785,13 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     b.gsamp = collect ? r->gsamp.as<float>() : nullptr;
     b.gcount = collect ? r->gcount.as<uint32_t>() : nullptr;
     b.gcap = collect ? r->gcap : 0u;
+    b.gring = collect ? r->gring : 1u;
     if (collect) {
         hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(r->n_pixels) * kGuidePend)), dim3(256), 0, s,
                            r->gpend.as<uint32_t>(), 0u, size_t(r->n_pixels) * kGuidePend);
-        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->gcount.as<uint32_t>(), 0u, size_t(4));
+        // the raw counter restarts every iteration; the ring's counter only without history
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->gcount.as<uint32_t>(), 0u,
+                           size_t(r->gcfg.history ? 1 : 2));
     }
     if (p.trace_camera) {
         launch_camera_generate(r->sc, p, b, s);
@@ -966,6 +969,7 @@ extern "C" void upbp_guide_defaults(upbp_guide_config* g)
     g->seed = 3;
     g->max_samples = 1u << 24;
     g->surface_alpha = 0.65f;
+    g->history = 1u << 22;
 }
 
 extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
@@ -973,7 +977,8 @@ extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
     if (!r || !g) return UPBP_E_ARG;
     if (!(g->alpha >= 0.f && g->alpha <= 1.f) || !(g->surface_alpha >= 0.f && g->surface_alpha <= 1.f) ||
         g->grid_res < 1 || g->grid_res > 64 || g->steps_per_iteration < 0 ||
-        g->batch < 16 || g->batch > (1 << 22) || !(g->learning_rate > 0.f) || g->max_samples < 16)
+        g->batch < 16 || g->batch > (1 << 22) || !(g->learning_rate > 0.f) || g->max_samples < 16 ||
+        (g->history != 0 && g->history < 16))
         return fail(r, "invalid guide configuration", UPBP_E_ARG);
     r->gcfg = *g;
     delete r->gnet;
@@ -996,9 +1001,11 @@ extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
     if (r->gnet->init(g->seed) || r->gnet->records(res, r->guide.as<float4>(), r->stream))
         return fail(r, "guide network setup failed", UPBP_E_HIP);
     r->gcap = g->max_samples;
+    r->gring = g->history ? g->history : g->max_samples;
     if (r->gpend.ensure(size_t(r->n_pixels) * kGuidePend * 4) || r->graw.ensure(size_t(r->gcap) * kGuideRaw * 4) ||
-        r->gsamp.ensure(size_t(r->gcap) * 10 * 4) || r->gcount.ensure(16))
+        r->gsamp.ensure(size_t(r->gring) * 10 * 4) || r->gcount.ensure(16))
         return fail(r, "guide allocation failed", UPBP_E_HIP);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, r->stream, r->gcount.as<uint32_t>(), 0u, size_t(4));
     r->gtrain = g->train != 0;
     // samples only from the scatterings whose proposal is on
     dg.train_media = g->alpha > 0.f;

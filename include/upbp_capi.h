@@ -336,6 +336,8 @@ typedef struct upbp_guide_config {
     uint32_t seed;               /* He initialisation (default 3) */
     uint32_t max_samples;        /* training samples kept per iteration (default 16M) */
     float surface_alpha;         /* mixture probability at non-specular surfaces (default 0.65) */
+    uint32_t history;            /* training samples kept across iterations in a ring that the
+                                    batches draw from (default 4M); 0: this iteration's only */
 } upbp_guide_config;
 void upbp_guide_defaults(upbp_guide_config* g);
 int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g);
