@@ -7,9 +7,18 @@ length 12, 900,000 light paths per iteration, radius 0.005, upbp_all
 (BPT + surface PM + P-P3D + P-B2D + B-B1D), FP32 transport, FP64 accumulation.
 
 One step = one 1-spp iteration (light pass, photon LBVH builds, camera pass, FP64
-accumulate) with the scene resident in HBM. Prints one JSON line.
+accumulate) with the scene resident in HBM. Prints one JSON line; it also carries the
+committed measurements of the longer runs below (profiles/*.json).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotune]
+
+Other measurements (each prints its own JSON line):
+  --learned SECONDS [--ref-seconds S]  learned proposal vs plain renderer at equal time on the
+                                       slab room (guide settings: --alpha, --surface-alpha,
+                                       --guide-res, --guide-steps, --guide-lr, --guide-history;
+                                       comma lists compare several against one reference)
+  --batch JOBS                         batched independent jobs (upbp_run_batch)
+  --stress                             3840x2160, 2.05M triangles, 8 media, 8M light paths
 """
 import argparse
 import json
