@@ -171,6 +171,8 @@ _SIGS = [
     ("upbp_set_guide", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_set_guide_lobes", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_float]),
     ("upbp_read_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("upbp_set_mis_log", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("upbp_read_mis_log", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
 ]
 
 

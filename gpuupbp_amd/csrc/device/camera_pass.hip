@@ -153,7 +153,9 @@ struct CamCounters {
     uint32_t conn = 0, pq = 0, sq = 0, acc = 0, pieces = 0;
 };
 
-// One segment of kernel/trace_camera.cu:79-382. Returns whether the path continues.
+// One segment of kernel/trace_camera.cu:79-382. Returns whether the path continues. LOG adds
+// the MIS weight log of the sampled camera paths (Bufs::mislog).
+template <bool LOG>
 __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b, uint32_t pix, int bounce, Sub& s,
                             uint32_t& pflags, f3& color, int& ret, CamCounters& cc, float& guide_den)
 {
@@ -262,9 +264,13 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         w.for_ratio = for_ratio;
         w.rev_ratio = rev_ratio;
     }
+    const bool logged = LOG && pix % b.mis_stride == 0;
     if (is.light >= 0) {
         if (s.path_length < p.min_len) return false;
-        color += s.throughput * light_radiance_mis(sc, p, sc.lights[is.light], s, rsr);
+        float wl = -1.f;
+        color += s.throughput * light_radiance_mis(sc, p, sc.lights[is.light], s, rsr, logged ? &wl : nullptr);
+        if (logged && wl >= 0.f && sc.lights[is.light].type == L_AREA)
+            mis_log(b, MT_LIGHT_HIT, s.path_length, 0, pix, uint32_t(is.light), wl);
         return false;
     }
     if (s.path_length >= p.max_len) return false;
@@ -272,8 +278,12 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         (bs.in_medium() || !only_spec_surf)) {
         f3 ct;
         Rng nrng(kDomainCamera, pix, p.iter_key, seg, P_NEE);
-        ret = direct_illumination(sc, p, ct, s, hitp, bs, nrng);
+        float wl = -1.f;
+        int lid = 0;
+        ret = direct_illumination(sc, p, ct, s, hitp, bs, nrng, logged ? &wl : nullptr, &lid);
         if (ret != S_OK) return false;
+        if (logged && wl >= 0.f && sc.lights[lid].type == L_AREA)
+            mis_log(b, MT_NEE, s.path_length + 1, 1, pix, uint32_t(lid), wl);
         color += s.throughput * ct;
     }
     if (p.do_bpt && !bs.delta() && (bs.in_medium() || !only_spec_surf)) {
@@ -297,8 +307,12 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
             if (!(small || diff_media)) {
                 f3 ct;
                 cc.conn++;
-                ret = connect_vertices(sc, p, ct, lv, bs, hitp, s);
+                float wl = -1.f;
+                ret = connect_vertices(sc, p, ct, lv, bs, hitp, s, logged ? &wl : nullptr);
                 if (ret != S_OK) return false;
+                // logged when the light path's vertices before this one were all stored
+                if (logged && wl >= 0.f && lv.path_length == k + 1)
+                    mis_log(b, MT_BPT, lv.path_length + s.path_length + 1, lv.path_length + 1, pix, lp, wl);
                 color += s.throughput * lv.throughput * ct;
             }
         }
@@ -310,9 +324,13 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
             const float4 pt = b.maps.surf_pts[k];
             if (!pred_point_point(hitp, mk(pt.x, pt.y, pt.z), p.r2_surf)) return;
             LVertex lv;
-            load_vertex(&b.verts[__float_as_int(pt.w)], lv);
+            const uint32_t slot = uint32_t(__float_as_int(pt.w));
+            load_vertex(&b.verts[slot], lv);
             cc.acc++;
-            eval_pp(sc, p, acc, s, bs, lv);
+            float wl = -1.f;
+            eval_pp(sc, p, acc, s, bs, lv, logged ? &wl : nullptr);
+            if (logged && wl >= 0.f && slot % p.vstride + 1 == lv.path_length)
+                mis_log(b, MT_SURF, lv.path_length + s.path_length, lv.path_length, pix, slot / p.vstride, wl);
         });
         color += s.throughput * p.norm_surf * acc;
     }
@@ -323,9 +341,13 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
             const float4 pt = b.maps.med_pts[k];
             if (!pred_point_point(hitp, mk(pt.x, pt.y, pt.z), p.r2_pp3d)) return;
             LVertex lv;
-            load_vertex(&b.verts[__float_as_int(pt.w)], lv);
+            const uint32_t slot = uint32_t(__float_as_int(pt.w));
+            load_vertex(&b.verts[slot], lv);
             cc.acc++;
-            eval_pp(sc, p, acc, s, bs, lv);
+            float wl = -1.f;
+            eval_pp(sc, p, acc, s, bs, lv, logged ? &wl : nullptr);
+            if (logged && wl >= 0.f && slot % p.vstride + 1 == lv.path_length)
+                mis_log(b, MT_PP3D, lv.path_length + s.path_length, lv.path_length, pix, slot / p.vstride, wl);
         });
         color += s.throughput * p.norm_pp3d * acc;
     }
@@ -343,7 +365,10 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
             LBeam lb;
             load_beam(&b.beams[__float_as_int(g1.w)], lb);
             cc.acc++;
-            eval_bp2d(sc, p, acc, s, bs, lb, idist, ir2);
+            float wl = -1.f;
+            eval_bp2d(sc, p, acc, s, bs, lb, idist, ir2, logged ? &wl : nullptr);
+            if (logged && wl >= 0.f)
+                mis_log(b, MT_BP2D, lb.path_length + s.path_length, lb.path_length, pix, lb.path_idx, wl);
         });
         color += s.throughput * p.norm_bp2d * acc;
     }
@@ -426,11 +451,14 @@ __device__ __forceinline__ void guide_record_sample(const DScene& sc, const Bufs
 }
 
 // pool of raw records -> training samples (10 floats: position, arrival, scattered, weight) with
-// a positive weight, appended to the sample ring (gcount[1] counts every sample ever appended;
-// the ring keeps the last gring); after the splat, when path and merge colours are final
+// a positive weight, written to the sample ring after the splat, when path and merge colours are
+// final. Counters: gcount[1] samples kept this iteration, gcount[2] the ring's write position
+// (modulo gring), gcount[3] its fill. At most gring samples are written per iteration, so no ring
+// slot is written twice in one launch.
 __global__ void __launch_bounds__(256) k_guide_finalise(Bufs b)
 {
     const uint32_t n = min(b.gcount[0], b.gcap);
+    const uint32_t start = b.gcount[2];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float* r = b.graw + size_t(i) * kGuideRaw;
         const uint32_t pix = __float_as_uint(r[11]);
@@ -446,14 +474,23 @@ __global__ void __launch_bounds__(256) k_guide_finalise(Bufs b)
         if (int(lane) == leader) base = atomicAdd(b.gcount + 1, uint32_t(__popcll(m)));
         base = __shfl(base, leader);
         const uint32_t at = base + uint32_t(__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane)))));
-        if (keep) {
-            float* o = b.gsamp + size_t(at % b.gring) * 10;
+        if (keep && at < b.gring) {
+            float* o = b.gsamp + size_t((uint64_t(start) + at) % b.gring) * 10;
             for (int k = 0; k < 9; k++) o[k] = r[k];
             o[9] = w;
         }
     }
 }
 
+// advances the ring past this iteration's samples (one thread, after k_guide_finalise)
+__global__ void k_guide_ring_advance(Bufs b)
+{
+    const uint32_t kept = min(b.gcount[1], b.gring);
+    b.gcount[2] = uint32_t((uint64_t(b.gcount[2]) + kept) % b.gring);
+    b.gcount[3] = min(b.gcount[3] + kept, b.gring);
+}
+
+template <bool LOG>
 __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams p, Bufs b, int bounce)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,7 +506,7 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams 
     int ret = S_OK;
     CamCounters cc;
     float guide_den = 0.f;
-    bool cont = camera_step(sc, p, b, pix, bounce, s, pflags, color, ret, cc, guide_den);
+    bool cont = camera_step<LOG>(sc, p, b, pix, bounce, s, pflags, color, ret, cc, guide_den);
     if (ret != S_OK) {
         cont = false;
         stat_add(b.stats, STAT_ERRORS, 1);
@@ -519,7 +556,10 @@ void launch_camera_generate(const DScene& sc, const IterParams& p, const Bufs& b
 
 void launch_camera_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_camera_bounce, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+    if (b.mislog)
+        hipLaunchKernelGGL(k_camera_bounce<true>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+    else
+        hipLaunchKernelGGL(k_camera_bounce<false>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
 }
 
 void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float res_y, hipStream_t s)
@@ -527,7 +567,10 @@ void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float
     hipLaunchKernelGGL(k_camera_splat, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, p, b, res_x, res_y);
     const uint32_t n = 3 * p.n_pixels;
     hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, b.fb32, b.accum, n);
-    if (b.gsamp) hipLaunchKernelGGL(k_guide_finalise, dim3(1024), dim3(256), 0, s, b);
+    if (b.gsamp) {
+        hipLaunchKernelGGL(k_guide_finalise, dim3(1024), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_guide_ring_advance, dim3(1), dim3(1), 0, s, b);
+    }
 }
 
 } // namespace upbp

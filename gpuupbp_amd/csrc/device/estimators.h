@@ -10,8 +10,10 @@
 namespace upbp {
 
 // kernel/functs_camera.cuh:107-165
+// Every estimator below takes an optional wout: when given, it receives the MIS weight of the
+// contribution if one is weighted, and is left unchanged otherwise.
 __device__ __forceinline__ f3 light_radiance_mis(const DScene& sc, const IterParams& p, const DLight& l, const Sub& s,
-                                                 float rsr)
+                                                 float rsr, float* wout = nullptr)
 {
     if (s.specular_path && p.ignore_spec) return mk(0.f);
     const float pick = 1.f / float(sc.n_lights);
@@ -25,12 +27,14 @@ __device__ __forceinline__ f3 light_radiance_mis(const DScene& sc, const IterPar
     const Weights& w = s.sw.w;
     const float bpt_t1 = float(!w.prev_specular);
     const float w_cam = dpdf_a * bpt_t1 * w.d_shared + epdf_w * rsr * w.d_bpt;
+    if (wout) *wout = 1.f / (1.f + w_cam);
     return rad * (1.f / (1.f + w_cam));
 }
 
 // kernel/functs_camera.cuh:170-344
 __device__ __forceinline__ int direct_illumination(const DScene& sc, const IterParams& p, f3& contrib, const Sub& s,
-                                                   f3 hitp, const Bsdf& bs, Rng& rng)
+                                                   f3 hitp, const Bsdf& bs, Rng& rng, float* wout = nullptr,
+                                                   int* light_out = nullptr)
 {
     const float pick = 1.f / float(sc.n_lights);
     const int id = int(rng.next() * float(sc.n_lights));
@@ -68,6 +72,10 @@ __device__ __forceinline__ int direct_illumination(const DScene& sc, const IterP
     const float w_cam =
         (epdf_w * cos_to / (dpdf_w * cos_at)) * nrr * (ft + bt1 * w.d_shared + (rpdf / w.rev_inv) * w.d_bpt);
     const float mis = 1.f / (w_light + 1.f + w_cam);
+    if (wout) {
+        *wout = mis;
+        *light_out = id;
+    }
     contrib = ((cos_to / (pick * dpdf_w)) * (rad * natt * f)) * mis;
     if (black_or_neg(contrib)) contrib = mk(0.f);
     return S_OK;
@@ -75,7 +83,7 @@ __device__ __forceinline__ int direct_illumination(const DScene& sc, const IterP
 
 // kernel/functs_camera.cuh:349-527
 __device__ __forceinline__ int connect_vertices(const DScene& sc, const IterParams& p, f3& contrib, const LVertex& lv,
-                                                const Bsdf& bc, f3 chit, const Sub& s)
+                                                const Bsdf& bc, f3 chit, const Sub& s, float* wout = nullptr)
 {
     contrib = mk(0.f);
     f3 dir = lv.hit - chit;
@@ -122,6 +130,7 @@ __device__ __forceinline__ int connect_vertices(const DScene& sc, const IterPara
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = l_pdf_a * rsr * (ft + bt1 * wc.d_shared + (crp / wc.rev_inv) * wc.d_bpt);
     const float mis = 1.f / (w_light + 1.f + w_cam);
+    if (wout) *wout = mis;
     contrib = (g * fc * fl * att) * mis;
     if (black_or_neg(contrib)) contrib = mk(0.f);
     return S_OK;
@@ -129,7 +138,7 @@ __device__ __forceinline__ int connect_vertices(const DScene& sc, const IterPara
 
 // kernel/pde/functs_pp.cuh:44-136 (SURF and P-P3D merge of one photon)
 __device__ __forceinline__ void eval_pp(const DScene& sc, const IterParams& p, f3& acc, const Sub& s, const Bsdf& bc,
-                                        const LVertex& lv)
+                                        const LVertex& lv, float* wout = nullptr)
 {
     const bool medium = lv.in_medium;
     const uint32_t len = lv.path_length + s.path_length;
@@ -152,6 +161,7 @@ __device__ __forceinline__ void eval_pp(const DScene& sc, const IterParams& p, f
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
     const float mis = 1.f / (w_light + w_local + w_cam);
+    if (wout) *wout = mis;
     acc += mis * f * lv.throughput;
 }
 
@@ -166,7 +176,8 @@ struct SegQ {
 
 // kernel/pde/functs_pb2d.cuh:57-201
 __device__ __forceinline__ void eval_pb2d(const DScene& sc, const IterParams& p, f3& acc, const SegQ& q,
-                                          const StateWeights& cw, const LVertex& lv, float idist, float ir2)
+                                          const StateWeights& cw, const LVertex& lv, float idist, float ir2,
+                                          float* wout = nullptr)
 {
     const uint32_t len = lv.path_length + q.cam_len;
     if (len > p.max_len || len < p.min_len) return;
@@ -219,13 +230,14 @@ __device__ __forceinline__ void eval_pb2d(const DScene& sc, const IterParams& p,
     const float w_local = 1.f + val * (p.mf_bb1d * sin_theta * fpf * rpf + p.mf_bp2d * fpf + p.mf_pp3d);
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
+    if (wout) *wout = 1.f / (w_light + w_local + w_cam);
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
 }
 
 // kernel/pde/functs_bb1d.cuh:57-248 (acceptance already decided by the caller)
 __device__ __forceinline__ void eval_bb1d(const DScene& sc, const IterParams& p, f3& acc, const SegQ& q,
                                           const StateWeights& cw, const LBeam& lb, float bbd, float sin_theta,
-                                          float tc, float tp)
+                                          float tc, float tp, float* wout = nullptr)
 {
     const uint32_t len = lb.path_length + q.cam_len;
     if (len > p.max_len || len < p.min_len) return;
@@ -297,12 +309,13 @@ __device__ __forceinline__ void eval_bb1d(const DScene& sc, const IterParams& p,
     const float w_local = 1.f + val * (p.mf_bp2d * fpf + p.mf_pb2d * rpf + p.mf_pp3d);
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
+    if (wout) *wout = 1.f / (w_light + w_local + w_cam);
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
 }
 
 // kernel/pde/functs_bp2d.cuh:51-198
 __device__ __forceinline__ void eval_bp2d(const DScene& sc, const IterParams& p, f3& acc, const Sub& s, const Bsdf& bc,
-                                          const LBeam& lb, float idist, float ir2)
+                                          const LBeam& lb, float idist, float ir2, float* wout = nullptr)
 {
     const uint32_t len = lb.path_length + s.path_length;
     if (len > p.max_len || len < p.min_len) return;
@@ -353,9 +366,9 @@ __device__ __forceinline__ void eval_bp2d(const DScene& sc, const IterParams& p,
     const float w_local = 1.f + val * (p.mf_bb1d * sin_theta * fpf * rpf + p.mf_pb2d * rpf + p.mf_pp3d);
     const float bt1 = float(!wc.prev_specular);
     const float w_cam = val * (wc.d_shared * bt1 * p.n_bpt + rpdf * wc.d_pde / wc.rev_inv);
+    if (wout) *wout = 1.f / (w_light + w_local + w_cam);
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
 }
-
 
 // SegRec <-> query and camera state
 __device__ __forceinline__ void load_segrec(const SegRec* r, SegQ& q, StateWeights& cw, f3& tatt, uint32_t& pix)

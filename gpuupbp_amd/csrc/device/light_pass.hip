@@ -111,7 +111,7 @@ __device__ __forceinline__ void add_beams(const DScene& sc, const IterParams& p,
 
 // kernel/functs_light.cuh:251-388
 __device__ __forceinline__ int connect_to_camera(const DScene& sc, const IterParams& p, const Bufs& b, const Sub& s,
-                                                 f3 hitp, const Bsdf& bs)
+                                                 f3 hitp, const Bsdf& bs, float* wout = nullptr)
 {
     const DCamera& cam = sc.cam;
     f3 to_cam = cam.origin - hitp;
@@ -148,6 +148,7 @@ __device__ __forceinline__ int connect_to_camera(const DScene& sc, const IterPar
     const float bpt_s1 = float(!wl.prev_specular);
     const float w_light = (i2s / float(p.n_light)) * rsr * (f_s + bpt_s1 * wl.d_shared + (rpdf / wl.rev_inv) * wl.d_bpt);
     const float mis = 1.f / (w_light + 1.f);
+    if (wout) *wout = mis;
     const float s2i = 1.f / i2s;
     const f3 contrib = mis * s.throughput * f * att / (float(p.n_light) * s2i);
     if (black_or_neg(contrib)) return S_OK;
@@ -158,7 +159,9 @@ __device__ __forceinline__ int connect_to_camera(const DScene& sc, const IterPar
     return S_OK;
 }
 
-// One segment of kernel/trace_light.cu:84-321. Returns whether the path continues.
+// One segment of kernel/trace_light.cu:84-321. Returns whether the path continues. LOG adds the
+// MIS weight log of light tracing from the first light paths (Bufs::mislog).
+template <bool LOG>
 __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p, const Bufs& b, uint32_t pid, Sub& s,
                                            uint32_t& pflags, int& ret, uint32_t& nvert, uint32_t& nbeam,
                                            uint32_t& bdrops)
@@ -233,8 +236,12 @@ __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p
     }
     if (p.do_bpt && !bs.delta() && (bs.in_medium() || p.connect_from_surf)) {
         if (s.path_length + 1 >= p.min_len) {
-            ret = connect_to_camera(sc, p, b, s, hitp, bs);
+            // logged when every vertex of the path so far was stored
+            const bool logged = LOG && pid < b.mis_lt && nvert == s.path_length;
+            float wl = -1.f;
+            ret = connect_to_camera(sc, p, b, s, hitp, bs, logged ? &wl : nullptr);
             if (ret != S_OK) return false;
+            if (logged && wl >= 0.f) mis_log(b, MT_LT, s.path_length + 1, s.path_length + 1, pid, pid, wl);
         }
     }
     if (s.path_length + 2 > p.max_len) return false;
@@ -245,6 +252,7 @@ __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p
     return cont;
 }
 
+template <bool LOG>
 __global__ void __launch_bounds__(256) k_light_bounce(DScene sc, IterParams p, Bufs b, int bounce)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -259,7 +267,7 @@ __global__ void __launch_bounds__(256) k_light_bounce(DScene sc, IterParams p, B
     const uint32_t nvert0 = nvert, nbeam0 = nbeam;
     uint32_t pflags = st.flags, bdrops = 0;
     int ret = S_OK;
-    bool cont = light_step(sc, p, b, pid, s, pflags, ret, nvert, nbeam, bdrops);
+    bool cont = light_step<LOG>(sc, p, b, pid, s, pflags, ret, nvert, nbeam, bdrops);
     if (ret != S_OK) {
         cont = false;
         stat_add(b.stats, STAT_ERRORS, 1);
@@ -284,7 +292,10 @@ void launch_light_generate(const DScene& sc, const IterParams& p, const Bufs& b,
 
 void launch_light_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_light_bounce, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+    if (b.mislog)
+        hipLaunchKernelGGL(k_light_bounce<true>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+    else
+        hipLaunchKernelGGL(k_light_bounce<false>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
 }
 
 } // namespace upbp

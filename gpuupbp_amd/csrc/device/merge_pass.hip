@@ -152,7 +152,8 @@ __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMap
     }
 }
 
-template <int KIND>
+// LOG adds the MIS weight log of the sampled camera paths (Bufs::mislog).
+template <int KIND, bool LOG>
 __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams p, Bufs b, int bounce)
 {
     __shared__ CandQueue cq[kMergeWaves];
@@ -337,14 +338,20 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                     uint32_t pix;
                     load_segrec(b.segq + pqi, q, cw, tatt, pix);
                     f3 sum = mk(0.f);
+                    const bool logged = LOG && pix % b.mis_stride == 0;
+                    float wl = -1.f;
                     if (KIND == 0) {
                         LVertex lv;
                         load_vertex(&b.verts[pslot], lv);
-                        eval_pb2d(sc, p, sum, q, cw, lv, px, py);
+                        eval_pb2d(sc, p, sum, q, cw, lv, px, py, logged ? &wl : nullptr);
+                        if (logged && wl >= 0.f && pslot % p.vstride + 1 == lv.path_length)
+                            mis_log(b, MT_PB2D, lv.path_length + q.cam_len, lv.path_length, pix, pslot / p.vstride, wl);
                     } else {
                         LBeam lb;
                         load_beam(&b.beams[pslot], lb);
-                        eval_bb1d(sc, p, sum, q, cw, lb, pz, pw, px, py);
+                        eval_bb1d(sc, p, sum, q, cw, lb, pz, pw, px, py, logged ? &wl : nullptr);
+                        if (logged && wl >= 0.f)
+                            mis_log(b, MT_BB1D, lb.path_length + q.cam_len, lb.path_length, pix, lb.path_idx, wl);
                     }
                     c = (tatt * norm) * sum;
                     has = c.x != 0.f || c.y != 0.f || c.z != 0.f;
@@ -515,8 +522,8 @@ int merge_grid_blocks(int bpc)
         int dev = 0, per0 = 0, per1 = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per0, k_seg_merge<0>, kMergeBlock, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per1, k_seg_merge<1>, kMergeBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per0, k_seg_merge<0, false>, kMergeBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per1, k_seg_merge<1, false>, kMergeBlock, 0);
         per = std::max(1, std::min(per0, per1));
         cus = std::max(1, cus);
     }
@@ -581,15 +588,18 @@ void launch_piece_gather(const Bufs& b, uint32_t n, const uint32_t* vals, hipStr
 void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int kind, int bounce, hipStream_t s)
 {
     const int blocks = merge_grid_blocks(p.merge_bpc);
-    if (kind == 0 && p.build_pb2d && b.maps.med.n > 0)
-        hipLaunchKernelGGL(k_seg_merge<0>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+    if (kind == 0 && p.build_pb2d && b.maps.med.n > 0) {
+        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<0, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        else hipLaunchKernelGGL((k_seg_merge<0, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+    }
     if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0) {
         if (p.piece_order == 1) {
             hipLaunchKernelGGL(k_piece_cells, dim3(1024), dim3(256), 0, s, b, bounce);
             hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
             hipLaunchKernelGGL(k_piece_scatter, dim3(1024), dim3(256), 0, s, b, bounce);
         }
-        hipLaunchKernelGGL(k_seg_merge<1>, dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        else hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
     }
 }
 

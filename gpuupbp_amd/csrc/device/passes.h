@@ -420,7 +420,24 @@ struct Bufs {
     double* accum;
     unsigned long long* stats; // see STAT_*
     PhotonMaps maps;
+    // MIS weight log of the MIS oracle test (null when off): one record per weighted contribution
+    // of the camera paths whose index is a multiple of mis_stride and of the light paths below
+    // mis_lt (light tracing), as (technique | k << 8 | s << 16, camera path or light path, light
+    // path or light, weight bits); x_0 .. x_k is the full path, s its light-side vertex count
+    uint4* mislog;
+    uint32_t* miscount;
+    uint32_t miscap, mis_stride, mis_lt;
 };
+
+// techniques of the MIS weight log
+enum MisTech : uint32_t { MT_LIGHT_HIT = 0, MT_NEE, MT_BPT, MT_LT, MT_SURF, MT_PP3D, MT_PB2D, MT_BP2D, MT_BB1D };
+
+__device__ __forceinline__ void mis_log(const Bufs& b, uint32_t tech, uint32_t k, uint32_t s, uint32_t id,
+                                        uint32_t lp, float w)
+{
+    const uint32_t at = atomicAdd(b.miscount, 1u);
+    if (at < b.miscap) b.mislog[at] = make_uint4(tech | (k << 8) | (s << 16), id, lp, __float_as_uint(w));
+}
 
 enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_POINT_Q, STAT_SEG_Q, STAT_ACCEPT,
                   STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,

@@ -173,6 +173,7 @@ struct upbp_renderer {
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
     DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals;
     DevBuf guide; // learned proposal records
+    DevBuf mislog, miscount; // MIS weight log (upbp_set_mis_log)
     // online training of the proposal (upbp_set_guide)
     upbp_guide_config gcfg{};
     bool gtrain = false;
@@ -711,7 +712,7 @@ static int train_guide(upbp_renderer* r)
     hipStream_t s = r->stream;
     uint32_t n = 0;
     HIPCHK(hipEventRecord(r->ev[4], s));
-    HIPCHK(hipMemcpyAsync(&n, r->gcount.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s)); // samples in the ring
+    HIPCHK(hipMemcpyAsync(&n, r->gcount.as<uint32_t>() + 3, 4, hipMemcpyDeviceToHost, s)); // samples in the ring
     HIPCHK(hipStreamSynchronize(s));
     n = std::min(n, r->gring);
     r->gsamples = n;
@@ -762,6 +763,7 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
                        reinterpret_cast<uint32_t*>(b.mcolor), 0u, 3 * size_t(r->n_pixels));
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(b.stats), 0u,
                        size_t(2 * STAT_COUNT));
+    if (b.mislog) hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, b.miscount, 0u, size_t(1));
     if (p.trace_light) {
         launch_light_generate(r->sc, p, b, s);
         for (int k = 0; k < L; k++) launch_light_bounce(r->sc, p, b, k, s);
@@ -789,9 +791,10 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     if (collect) {
         hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(r->n_pixels) * kGuidePend)), dim3(256), 0, s,
                            r->gpend.as<uint32_t>(), 0u, size_t(r->n_pixels) * kGuidePend);
-        // the raw counter restarts every iteration; the ring's counter only without history
+        // the raw and kept counters restart every iteration; the ring's position and fill only
+        // without history
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->gcount.as<uint32_t>(), 0u,
-                           size_t(r->gcfg.history ? 1 : 2));
+                           size_t(r->gcfg.history ? 2 : 4));
     }
     if (p.trace_camera) {
         launch_camera_generate(r->sc, p, b, s);
@@ -1109,6 +1112,12 @@ extern "C" int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iteration
     if (n < 0 || (n > 0 && (!rs || !iterations)) || threads < 1) return fail(nullptr, "invalid batch", UPBP_E_ARG);
     for (int32_t i = 0; i < n; i++)
         if (!rs[i]) return fail(nullptr, "null renderer in batch", UPBP_E_ARG);
+    { // two threads on one renderer would share its buffers and stream: reject duplicates
+        std::vector<upbp_renderer*> sorted(rs, rs + n);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(nullptr, "a renderer appears twice in the batch", UPBP_E_ARG);
+    }
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess) return fail(nullptr, "no HIP device", UPBP_E_HIP);
     std::atomic<int32_t> next{0};
@@ -1127,6 +1136,47 @@ extern "C" int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iteration
     for (std::thread& th : pool) th.join();
     for (int32_t i = 0; i < n; i++)
         if (rc[size_t(i)]) return rc[size_t(i)];
+    return UPBP_OK;
+}
+
+extern "C" int upbp_set_mis_log(upbp_renderer* r, uint32_t capacity, uint32_t pixel_stride, uint32_t light_paths)
+{
+    if (!r) return fail(r, "null renderer", UPBP_E_ARG);
+    Bufs& b = r->bufs;
+    if (capacity == 0) {
+        HIPCHK(hipStreamSynchronize(r->stream));
+        r->mislog.release();
+        r->miscount.release();
+        b.mislog = nullptr;
+        b.miscount = nullptr;
+        b.miscap = 0;
+        return UPBP_OK;
+    }
+    if (pixel_stride == 0 || capacity > (1u << 28)) return fail(r, "invalid MIS log arguments", UPBP_E_ARG);
+    if (r->mislog.ensure(size_t(capacity) * sizeof(uint4)) || r->miscount.ensure(4))
+        return fail(r, "MIS log allocation failed", UPBP_E_HIP);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, r->stream, r->miscount.as<uint32_t>(), 0u, size_t(1));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    b.mislog = r->mislog.as<uint4>();
+    b.miscount = r->miscount.as<uint32_t>();
+    b.miscap = capacity;
+    b.mis_stride = pixel_stride;
+    b.mis_lt = light_paths;
+    return UPBP_OK;
+}
+
+extern "C" int upbp_read_mis_log(upbp_renderer* r, uint32_t* records, uint64_t max_records, uint64_t* n_records)
+{
+    if (!r || !n_records || (max_records && !records)) return fail(r, "null argument", UPBP_E_ARG);
+    const Bufs& b = r->bufs;
+    if (!b.mislog) return fail(r, "the MIS log is off", UPBP_E_ARG);
+    uint32_t n = 0;
+    HIPCHK(hipStreamSynchronize(r->stream));
+    HIPCHK(hipMemcpy(&n, b.miscount, 4, hipMemcpyDeviceToHost));
+    *n_records = n;
+    const uint64_t m = std::min<uint64_t>(std::min<uint64_t>(n, b.miscap), max_records);
+    if (m) HIPCHK(hipMemcpy(records, b.mislog, m * sizeof(uint4), hipMemcpyDeviceToHost));
+    if (n > b.miscap) return fail(r, "MIS log overflow: " + std::to_string(n) + " records", UPBP_E_CAPACITY);
     return UPBP_OK;
 }
 
@@ -1162,7 +1212,8 @@ extern "C" void upbp_destroy(upbp_renderer* r)
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
                       &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
-                      &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount})
+                      &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount,
+                      &r->mislog, &r->miscount})
         d->release();
     delete r->gnet;
     r->gnet = nullptr;

@@ -51,7 +51,8 @@ struct Reader {
     T get()
     {
         T v{};
-        if (p + sizeof(T) > b.size()) {
+        // written as a subtraction so that a huge p (a crafted offset) cannot wrap around
+        if (p > b.size() || sizeof(T) > b.size() - p) {
             ok = false;
             return v;
         }
@@ -127,7 +128,7 @@ std::string upbp_read_exr_rgb(const std::string& path, std::vector<float>& rgb, 
         if (name.empty()) break;
         const std::string type = r.str();
         const int32_t size = r.get<int32_t>();
-        if (!r.ok || size < 0 || r.p + size_t(size) > b.size()) return path + ": truncated header";
+        if (!r.ok || size < 0 || size_t(size) > b.size() - r.p) return path + ": truncated header";
         const size_t end = r.p + size_t(size);
         if (name == "channels" && type == "chlist") {
             while (r.p < end) {
@@ -142,6 +143,7 @@ std::string upbp_read_exr_rgb(const std::string& path, std::vector<float>& rgb, 
                 ch.push_back(c);
             }
         } else if (name == "compression") {
+            if (size < 1) return path + ": empty compression attribute";
             comp = b[r.p];
         } else if (name == "dataWindow" && size == 16) {
             xmin = r.get<int32_t>();
@@ -168,6 +170,8 @@ std::string upbp_read_exr_rgb(const std::string& path, std::vector<float>& rgb, 
     std::vector<uint64_t> offs(chunks);
     for (auto& o : offs) o = r.get<uint64_t>();
     if (!r.ok) return path + ": truncated offset table";
+    for (const uint64_t o : offs) // each chunk starts with its 8-byte (y, size) prefix
+        if (b.size() < 8 || o > uint64_t(b.size() - 8)) return path + ": chunk offset outside the file";
 
     int ci[3] = {-1, -1, -1};
     for (size_t k = 0; k < ch.size(); k++) {
@@ -191,7 +195,7 @@ std::string upbp_read_exr_rgb(const std::string& path, std::vector<float>& rgb, 
         cr.p = size_t(offs[c]);
         const int32_t y = cr.get<int32_t>();
         const int32_t size = cr.get<int32_t>();
-        if (!cr.ok || size < 0 || cr.p + size_t(size) > b.size()) return path + ": truncated chunk";
+        if (!cr.ok || size < 0 || size_t(size) > b.size() - cr.p) return path + ": truncated chunk";
         if (y < ymin || y > ymax) return path + ": chunk outside the data window";
         const uint32_t y0 = uint32_t(y - ymin);
         const uint32_t nl = std::min<uint32_t>(uint32_t(lines), height - y0);

@@ -75,6 +75,34 @@ class Renderer:
             raise RuntimeError("upbp_set_tuning failed: %s" % capi.lib().upbp_last_error(self.h).decode())
         return t
 
+    def set_mis_log(self, capacity, pixel_stride=1, light_paths=0):
+        """upbp_set_mis_log: log every MIS-weighted contribution of the camera paths whose index is
+        a multiple of pixel_stride and of light tracing from the first light_paths light paths
+        (capacity 0: off)."""
+        rc = capi.lib().upbp_set_mis_log(self.h, capacity, pixel_stride, light_paths)
+        if rc != 0:
+            raise RuntimeError("upbp_set_mis_log failed: %d %s" % (rc, capi.lib().upbp_last_error(self.h).decode()))
+
+    def read_mis_log(self, max_records=1 << 22):
+        """The last iteration's MIS weight log as a structured array with fields tech, k, s, id
+        (camera path; light tracing: light path), lpath (light path; light hits and next-event
+        estimation: the light) and w (FP32 weight)."""
+        buf = np.zeros((max_records, 4), np.uint32)
+        n = C.c_uint64()
+        rc = capi.lib().upbp_read_mis_log(self.h, _p(buf), max_records, C.byref(n))
+        if rc != 0:
+            raise RuntimeError("upbp_read_mis_log failed: %d %s" % (rc, capi.lib().upbp_last_error(self.h).decode()))
+        r = buf[: min(n.value, max_records)]
+        out = np.zeros(len(r), dtype=[("tech", "i4"), ("k", "i4"), ("s", "i4"), ("id", "i8"), ("lpath", "i8"),
+                                      ("w", "f8")])
+        out["tech"] = r[:, 0] & 255
+        out["k"] = (r[:, 0] >> 8) & 255
+        out["s"] = r[:, 0] >> 16
+        out["id"] = r[:, 1]
+        out["lpath"] = r[:, 2]
+        out["w"] = r[:, 3].view(np.float32)
+        return out
+
     def sync(self):
         rc = capi.lib().upbp_sync(self.h)
         if rc != 0:

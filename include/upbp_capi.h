@@ -408,6 +408,24 @@ int upbp_test_env_samples(const upbp_env_map* env, const float* u, uint32_t n, f
 int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, int32_t surface, const float* u,
                             uint32_t n, float* out);
 
+/* MIS weight log (the MIS oracle's hook into the GPU renderer; the reference has no counterpart:
+ * its weights are checked only through images). While on, every iteration logs one record per
+ * MIS-weighted contribution of the camera paths whose index is a multiple of pixel_stride and of
+ * light tracing from the light paths below light_paths: 4 x uint32 (technique | k << 8 | s << 16,
+ * camera path (light tracing: light path), light path (next-event estimation and light hits: the
+ * light), weight as FP32 bits). The full path is x_0 (on the light) .. x_k (the camera) and s its
+ * light-side vertex count; techniques 0 light hit, 1 next-event estimation, 2 vertex connection,
+ * 3 light tracing, 4 SURF, 5 P-P3D, 6 P-B2D, 7 B-P2D, 8 B-B1D. The weights are those of
+ * gpuupbp/kernel/functs_camera.cuh:146-162 (light hit), :285-334 (next-event), :469-518
+ * (connection), functs_light.cuh:358-369 (light tracing) and pde/functs_{pp,pb2d,bp2d,bb1d}.cuh.
+ * Light hits and next-event estimation are logged for area lights only, connections and point
+ * merges when the light vertex's predecessors were all stored, light tracing when every vertex of
+ * the light path so far was stored. capacity 0 switches the log off; the log restarts with each
+ * iteration. upbp_read_mis_log copies up to max_records records of the last iteration, sets
+ * *n_records to the count logged and returns UPBP_E_CAPACITY if that exceeded the capacity. */
+int upbp_set_mis_log(upbp_renderer* r, uint32_t capacity, uint32_t pixel_stride, uint32_t light_paths);
+int upbp_read_mis_log(upbp_renderer* r, uint32_t* records, uint64_t max_records, uint64_t* n_records);
+
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM
  * traffic-counter calibration and the measured copy bandwidth. */

@@ -49,6 +49,9 @@ struct MisRec {
     // d_shared at the merge point (the distance along the beam from its segment start, or 1
     // when it skips the factor); 0 when the true edge length is used
     double ref_light_dist = 0;
+    // key of the contribution: the light path of its light-side vertices (light hits and
+    // next-event estimation: the light)
+    int64_t lpath = -1;
 };
 
 struct MisParams {

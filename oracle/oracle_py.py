@@ -214,10 +214,11 @@ MIS_TECH = {0: "light_hit", 1: "nee", 2: "bpt", 3: "light_tracing", 4: "surf", 5
 
 def mis_check(scene_ptr, cfg, iteration, pixels, n_light_tracing_paths, threads=0, max_records=2000000):
     """Records (technique, s, k, constant-time weight, FP64 balance-heuristic weight, the same
-    with the reference's next-event weight convention) of every contribution on the given
-    camera paths and the first light paths of one iteration."""
+    with the reference's next-event weight convention, camera path (pixel; light tracing: light
+    path), light path (next-event estimation and light hits: the light)) of every contribution
+    on the given camera paths and the first light paths of one iteration."""
     pixels = np.ascontiguousarray(pixels, np.uint32)
-    out = np.zeros((max_records, 6), np.float64)
+    out = np.zeros((max_records, 8), np.float64)
     n = C.c_uint32()
     rc = lib().oracle_mis_check(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), iteration,
                                 _p(pixels), len(pixels), n_light_tracing_paths, threads, _p(out), max_records,

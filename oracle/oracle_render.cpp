@@ -716,7 +716,8 @@ static int sample_scattering(const Ctx& c, bool& cont, const BSDF& b, V hit, con
 // edge between the two sides is a connection whose distance pdfs are conn_rsf (toward
 // the camera side) and conn_rsr (toward the light side).
 static void mis_record(const Ctx& c, int tech, float weight, V x0, int light_id, const LightVertex* lv, int nl,
-                       const CamVert* cam, int nc, bool merge, float conn_rsf, float conn_rsr, float ref_light_dist = 0.f)
+                       const CamVert* cam, int nc, bool merge, float conn_rsf, float conn_rsr, float ref_light_dist = 0.f,
+                       int lpath = -1)
 {
     if (!g_mis || !g_mis->out) return;
     const Scene& sc = *c.sc;
@@ -724,6 +725,7 @@ static void mis_record(const Ctx& c, int tech, float weight, V x0, int light_id,
     r.tech = tech;
     r.weight = weight;
     r.ref_light_dist = ref_light_dist;
+    r.lpath = lpath >= 0 ? lpath : light_id;
     const int k = nl + nc + 1 - (merge ? 1 : 0);
     if (k < 2) return;
     r.k = k;
@@ -974,7 +976,7 @@ static int connect_to_camera(const Ctx& c, const SubpathState& s, V hit, const B
         const int nl = int(g_mis->lverts->size() - g_mis->lbegin);
         if (lv[nl - 1].path_length == uint32_t(nl)) // every vertex of the path was stored
             mis_record(c, MT_LT, mis, c.light_origin[lv->path], c.light_pick[lv->path], lv, nl, nullptr, 0, false,
-                       segs.empty() ? 1.f : acc_for(segs), rsr);
+                       segs.empty() ? 1.f : acc_for(segs), rsr, 0.f, int(lv->path));
     }
     const float s2i = 1.f / i2s;
     const V contrib = mis * s.throughput * f * att / (float(c.p.n_light) * s2i);
@@ -1244,7 +1246,7 @@ static int connect_vertices(const Ctx& c, V& contrib, const LightVertex& lv, con
         const int nl = int(lv.path_length);
         if (first + (nl - 1) == &lv)
             mis_record(c, MT_BPT, mis, c.light_origin[lv.path], c.light_pick[lv.path], first, nl, g_mis->cam.data(),
-                       int(g_mis->cam.size()), false, rsr, rsf);
+                       int(g_mis->cam.size()), false, rsr, rsf, 0.f, int(lv.path));
     }
     contrib = (g * fc * fl * att) * mis;
     if (black_or_neg(contrib)) contrib = v3(0.f);
@@ -1281,7 +1283,7 @@ static void eval_pp(const Ctx& c, V& acc, const SubpathState& s, const BSDF& bc,
         const int nl = int(lv.path_length);
         if (first + (nl - 1) == &lv)
             mis_record(c, medium ? MT_PP3D : MT_SURF, mis, c.light_origin[lv.path], c.light_pick[lv.path], first, nl,
-                       g_mis->cam.data(), int(g_mis->cam.size()), true, 1.f, 1.f);
+                       g_mis->cam.data(), int(g_mis->cam.size()), true, 1.f, 1.f, 0.f, int(lv.path));
     }
     acc += mis * f * lv.throughput;
 }
@@ -1358,7 +1360,7 @@ static void eval_pb2d(const Ctx& c, V& acc, const SegQuery& q, const LightVertex
             std::vector<CamVert> cam = g_mis->cam;
             cam.push_back({q.ro + q.rd * idist, medium_bsdf(c, q.rd, q.med_id, false, q.ro + q.rd * idist), rsf, rsr});
             mis_record(c, MT_PB2D, 1.f / (w_light + w_local + w_cam), c.light_origin[lv.path], c.light_pick[lv.path],
-                       lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f, 1.f);
+                       lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f, 1.f, 0.f, int(lv.path));
         }
     }
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
@@ -1452,7 +1454,7 @@ static void eval_bb1d(const Ctx& c, V& acc, const SegQuery& q, const LightBeam& 
             const float ref_d = (lb.path_length == 1 && !lb.infinite) ? 1.f : tp;
             mis_record(c, MT_BB1D, 1.f / (w_light + w_local + w_cam), c.light_origin[lb.path_idx],
                        c.light_pick[lb.path_idx], lp.data(), int(lp.size()), cam.data(), int(cam.size()), true, 1.f,
-                       1.f, ref_d);
+                       1.f, ref_d, int(lb.path_idx));
         }
     }
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
@@ -1519,7 +1521,7 @@ static void eval_bp2d(const Ctx& c, V& acc, const SubpathState& s, const BSDF& b
             lp.push_back(beam_point(c, lb, idist, rsf, rsr));
             mis_record(c, MT_BP2D, 1.f / (w_light + w_local + w_cam), c.light_origin[lb.path_idx],
                        c.light_pick[lb.path_idx], lp.data(), int(lp.size()), g_mis->cam.data(), int(g_mis->cam.size()),
-                       true, 1.f, 1.f, (lb.path_length == 1 && lb.infinite) ? 1.f : idist);
+                       true, 1.f, 1.f, (lb.path_length == 1 && lb.infinite) ? 1.f : idist, int(lb.path_idx));
         }
     }
     acc += (1.f / (w_light + w_local + w_cam)) * unweighted;
@@ -2024,7 +2026,8 @@ extern "C" int oracle_render(const upbp_scene* scene, const upbp_config* cfg, in
 // Renders one iteration with the MIS log on for the given camera paths (pixels) and the
 // first n_lt_paths light paths; for every logged contribution writes (technique, s, k,
 // constant-time weight, FP64 balance-heuristic weight, the same with the reference's
-// next-event weight convention) to out (6 doubles per record).
+// next-event weight convention, camera path (pixel; light tracing: light path), light path
+// (next-event estimation and light hits: the light)) to out (8 doubles per record).
 extern "C" int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg, int iteration, const uint32_t* pixels,
                                 uint32_t n_pixels, uint32_t n_lt_paths, int n_threads, double* out, uint32_t max_records,
                                 uint32_t* n_records)
@@ -2075,13 +2078,15 @@ extern "C" int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg,
         for (auto& v : *set)
             for (const MisRec& r : v) {
                 if (out && n < max_records) {
-                    double* o = out + 6 * size_t(n);
+                    double* o = out + 8 * size_t(n);
                     o[0] = r.tech;
                     o[1] = r.s;
                     o[2] = r.k;
                     o[3] = r.weight;
                     o[4] = mis_explicit_weight(r, mp);
                     o[5] = mis_explicit_weight(r, mq);
+                    o[6] = set == &cam_recs ? double(pixels[&v - cam_recs.data()]) : double(&v - lt_recs.data());
+                    o[7] = r.lpath;
                 }
                 n++;
             }

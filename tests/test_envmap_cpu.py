@@ -136,6 +136,36 @@ def test_exr_reader_reads_the_writer_and_rejects_bad_files(tmp_path):
         capi.read_exr(piz)
 
 
+@pytest.mark.parametrize("offset", [2 ** 64 - 4, 2 ** 63, 10 ** 9, -8])
+def test_exr_reader_rejects_corrupt_offset_tables(tmp_path, offset):
+    # a chunk offset near 2^64 would wrap an unchecked "offset + size" bounds test; every entry
+    # must be rejected before a chunk is read (offset -8: eight bytes before the end of the file)
+    img = _test_image(4, 8, 2)
+    p = str(tmp_path / "o.exr")
+    write_exr(p, img, 0)
+    data = bytearray(open(p, "rb").read())
+    table = len(data) - sum(len(img[0]) * 3 * 4 + 8 for _ in range(4)) - 8 * 4
+    assert struct.unpack("<Q", bytes(data[table:table + 8]))[0] == table + 8 * 4  # first chunk follows the table
+    bad = offset if offset >= 0 else len(data) - 4
+    data[table + 8:table + 16] = struct.pack("<Q", bad)
+    open(p, "wb").write(bytes(data))
+    with pytest.raises(RuntimeError):
+        capi.read_exr(p)
+
+
+def test_exr_reader_rejects_an_empty_compression_attribute(tmp_path):
+    img = _test_image(4, 8, 3)
+    p = str(tmp_path / "c.exr")
+    write_exr(p, img, 0)
+    data = bytearray(open(p, "rb").read())
+    at = data.index(b"compression\0compression\0") + len(b"compression\0compression\0")
+    assert struct.unpack("<i", bytes(data[at:at + 4]))[0] == 1
+    data[at:at + 5] = struct.pack("<i", 0)  # size 0, value byte dropped
+    open(p, "wb").write(bytes(data))
+    with pytest.raises(RuntimeError):
+        capi.read_exr(p)
+
+
 # ---- AUX light_background_em ------------------------------------------------------------------
 def test_aux_environment_map_loads_rotated_and_scaled(box_obj, tmp_path):
     env = sky()

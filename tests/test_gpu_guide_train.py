@@ -13,14 +13,16 @@ from gpuupbp_amd import renderer as gpu
 pytestmark = pytest.mark.gpu
 
 
-def _run(obj, args, iters, guide=None, first=0):
-    rc, cfg = capi.parse_args(["-s", "-1", obj] + args)
+def _run(obj, args, iters, guide=None, ioff=0):
+    # ioff shifts the random streams only; every render runs iterations 0.. so that all of them
+    # follow the same radius schedule
+    rc, cfg = capi.parse_args(["-s", "-1", obj] + args + ["-ioff", str(ioff)])
     assert rc == 0
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
     with gpu.Renderer(sh.ptr, cfg) as r:
         if guide is not None:
             r.set_guide(**guide)
-        stats = [r.run_iteration(first + i) for i in range(iters)]
+        stats = [r.run_iteration(i) for i in range(iters)]
         img = r.read_accum(iters)
     return img, stats
 
@@ -41,7 +43,7 @@ def test_trained_proposal_lowers_the_error_per_iteration(tmp_path):
     # against a g = 0.9 phase function, which the product proposal learns online.
     obj = capi.generate_scene("slabroom_small", 3, str(tmp_path / "slabroom.obj"))
     args = ["-r", "160x90", "-l", "16", "-r_initial", "0.01"]
-    ref, _ = _run(obj, args, 600, first=1 << 20)
+    ref, _ = _run(obj, args, 600, ioff=1 << 20)
     plain, _ = _run(obj, args, 64)
     # the proposal in the slab and at the walls (the small room's 160x90 frame feeds too few
     # samples per cell of the default 64^3 grid)
@@ -50,6 +52,17 @@ def test_trained_proposal_lowers_the_error_per_iteration(tmp_path):
     print("relMSE plain %.5f learned %.5f" % (rel(plain), rel(learned)))
     assert all(s.path_errors == 0 for s in st)
     assert rel(learned) < 0.9 * rel(plain), (rel(plain), rel(learned))
+
+
+def test_small_sample_ring_wraps(tmp_path):
+    # a 1024-sample ring far below one iteration's samples: each iteration writes at most 1024
+    # (no slot twice in one launch), the ring stays full and training stays finite
+    obj = capi.generate_scene("slabroom_small", 3, str(tmp_path / "slabroom.obj"))
+    args = ["-r", "96x54", "-l", "10", "-r_initial", "0.01"]
+    img, stats = _run(obj, args, 6, guide=dict(grid_res=8, steps_per_iteration=4, batch=512, history=1024))
+    assert all(s.path_errors == 0 for s in stats) and np.isfinite(img).all()
+    assert all(s.guide_samples == 1024 for s in stats), [s.guide_samples for s in stats]
+    assert all(np.isfinite(s.guide_loss) and s.guide_loss != 0.0 for s in stats), [s.guide_loss for s in stats]
 
 
 def test_guided_renderer_converges_to_the_plain_one(boxfog_obj):

@@ -274,6 +274,10 @@ def test_batch_autotune_and_guide_reject_bad_arguments():
     assert lib.upbp_run_batch((C.c_void_p * 2)(None, None), its, 2, 4) == E_ARG
     assert lib.upbp_run_batch(None, None, 0, 0) == E_ARG  # no threads
     assert lib.upbp_run_batch(None, None, -1, 1) == E_ARG
+    # the same renderer twice (the handles are never dereferenced: the check comes first)
+    dup = (C.c_void_p * 3)(0x1000, 0x2000, 0x1000)
+    assert lib.upbp_run_batch(dup, (C.c_int32 * 3)(0, 0, 1), 3, 2) == E_ARG
+    assert b"twice" in lib.upbp_last_error(None)
     assert lib.upbp_autotune(None, 0, 1, None, None) == E_ARG
     g = capi.GuideConfig()
     lib.upbp_guide_defaults(C.byref(g))
