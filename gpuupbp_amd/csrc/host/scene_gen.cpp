@@ -10,7 +10,11 @@
 //             spheres with priorities 1..3, two quad lights and a point light.
 //   "stress": the same construction on a 10x10 grid (2.05M triangles) with eight
 //             overlapping media spheres (priorities 1..8, g from -0.3 to 0.95).
-//   "furnace": closed box with albedo-1 diffuse walls lit by a quad light (tests only).
+//   "furnace": closed box whose floor is an emitter (Ke 1, black) and whose other five walls are
+//              white diffuse (albedo 1), camera inside (tests only). The enclosure is isothermal:
+//              every wall and the emitter have radiosity pi, so the radiance everywhere is exactly
+//              1 for any enclosure shape (energy balance); paths of up to l segments miss only the
+//              paths that bounce l - 1 times off white walls without reaching the floor.
 //   "slabroom": room of ~196k triangles with a dense forward-scattering medium slab, lit
 //             only through a 5% aperture in one wall ("slabroom_small": coarse walls).
 #include "../../../include/upbp_capi.h"
@@ -192,6 +196,7 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace, bool fog =
     Rng rng(seed);
     const char* walls[6] = {"wall_floor", "wall_ceiling", "wall_back", "wall_left", "wall_right", "wall_front"};
     for (int i = 0; i < 6; i++) {
+        if (furnace && i == 0) continue; // the floor is the emitter
         double r = 1, g = 1, b = 1;
         if (!furnace) {
             r = rng.uniform(0.5, 0.8);
@@ -204,14 +209,15 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace, bool fog =
         else
             fprintf(mtl, "newmtl %s\nNs 0\nKd %.6f %.6f %.6f\nKs 0 0 0\nKe 0 0 0\n\n", walls[i], r, g, b);
     }
-    fprintf(mtl, "newmtl light\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 10 10 10\n\n");
+    if (furnace) fprintf(mtl, "newmtl light\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 1 1 1\n\n");
+    else fprintf(mtl, "newmtl light\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 10 10 10\n\n");
     if (!furnace) fprintf(mtl, "newmtl sphere_boundary\nNs 0\nKd 0 0 0\nKs 0 0 0\nKe 0 0 0\n\n");
 
     Writer w;
     w.obj = obj;
     fprintf(obj, "mtllib %s\n", base_name(mtl_path).c_str());
     // inward-facing walls of [-1,1]^3
-    w.usemtl(walls[0]);
+    w.usemtl(furnace ? "light" : walls[0]);
     w.quad({-1, -1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, -1, -1}); // floor, normal +y
     w.usemtl(walls[1]);
     w.quad({-1, 1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, 1, 1}); // ceiling, normal -y
@@ -226,9 +232,10 @@ int gen_box(const std::string& obj_path, uint32_t seed, bool furnace, bool fog =
         w.quad({-1, -1, 1}, {-1, 1, 1}, {1, 1, 1}, {1, -1, 1}); // front, normal -z
     }
     // 0.5 x 0.5 light under the ceiling, facing down
-    const double ly = furnace ? 0.99 : 0.999;
-    w.usemtl("light");
-    w.quad({-0.25, ly, -0.25}, {0.25, ly, -0.25}, {0.25, ly, 0.25}, {-0.25, ly, 0.25});
+    if (!furnace) {
+        w.usemtl("light");
+        w.quad({-0.25, 0.999, -0.25}, {0.25, 0.999, -0.25}, {0.25, 0.999, 0.25}, {-0.25, 0.999, 0.25});
+    }
     if (!furnace) {
         Mesh s = icosphere(3);
         w.usemtl("sphere_boundary");

@@ -171,3 +171,20 @@ extern "C" int oracle_mlp_loss_grad(const double* params, const float* samples, 
     if (grads) memcpy(grads, g.data(), L.total * sizeof(double));
     return 0;
 }
+
+// One Adam step (Kingma and Ba 2015, Algorithm 1) in FP64 over n parameters: moments m, v updated
+// in place with beta1 0.9, beta2 0.999, then params -= lr * m_hat / (sqrt(v_hat) + 1e-8) with the
+// bias corrections m_hat = m / (1 - beta1^step), v_hat = v / (1 - beta2^step); step counts from 1.
+extern "C" int oracle_adam_step(double* params, double* m, double* v, const double* grads, uint32_t n, int step,
+                                double lr)
+{
+    if (!params || !m || !v || !grads || step < 1) return -1;
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    const double c1 = 1.0 - std::pow(b1, double(step)), c2 = 1.0 - std::pow(b2, double(step));
+    for (uint32_t i = 0; i < n; i++) {
+        m[i] = b1 * m[i] + (1.0 - b1) * grads[i];
+        v[i] = b2 * v[i] + (1.0 - b2) * grads[i] * grads[i];
+        params[i] -= lr * (m[i] / c1) / (std::sqrt(v[i] / c2) + eps);
+    }
+    return 0;
+}

@@ -50,6 +50,9 @@ def lib():
         h.oracle_philox.argtypes = [vp, vp, vp]
         h.oracle_mis_check.argtypes = [vp, vp, C.c_int, vp, C.c_uint32, C.c_uint32, C.c_int, vp, C.c_uint32,
                                        C.POINTER(C.c_uint32)]
+        h.oracle_adam_step.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_int, C.c_double]
+        h.oracle_bsdf_eval.argtypes = [vp, vp, C.c_float, C.c_int, vp, C.c_uint32, vp]
+        h.oracle_bsdf_sample.argtypes = [vp, vp, C.c_float, C.c_int, vp, C.c_uint32, vp]
         _lib = h
     return _lib
 
@@ -155,6 +158,17 @@ def mlp_loss_grad(params, samples, want_grad=True):
     return loss.value, g
 
 
+def adam_step(params, m, v, grads, step, lr):
+    """FP64 Adam step (beta1 0.9, beta2 0.999, eps 1e-8); returns new (params, m, v)."""
+    p = np.array(params, np.float64)
+    mm = np.array(m, np.float64)
+    vv = np.array(v, np.float64)
+    g = np.ascontiguousarray(grads, np.float64)
+    if lib().oracle_adam_step(_p(p), _p(mm), _p(vv), _p(g), len(p), step, lr) != 0:
+        raise RuntimeError("oracle_adam_step failed")
+    return p, mm, vv
+
+
 def set_guide(lobes, res, lo, hi, alpha):
     """Install the learned proposal for later oracle renders (alpha 0: off)."""
     if alpha == 0:
@@ -226,3 +240,41 @@ def mis_check(scene_ptr, cfg, iteration, pixels, n_light_tracing_paths, threads=
     if rc != 0:
         raise RuntimeError("oracle_mis_check failed: %d" % rc)
     return out[: min(n.value, max_records)]
+
+
+def _material(diffuse=(0, 0, 0), phong=(0, 0, 0), exponent=1.0, mirror=(0, 0, 0), ior=-1.0):
+    from gpuupbp_amd import capi
+    m = capi.Material()
+    m.diffuse[:] = diffuse
+    m.phong[:] = phong
+    m.phong_exponent = exponent
+    m.mirror[:] = mirror
+    m.ior = ior
+    m.priority = -1
+    m.real = 1
+    m.med_idx = -1
+    return m
+
+
+def bsdf_eval(material, ray_dir, dirs, ior=-1.0, from_light=False):
+    """Surface BSDF (normal +z) of a ray travelling along ray_dir, evaluated toward dirs (n x 3):
+    n x 7 (f.xyz, cosine at the generated side, forward pdf, reverse pdf, continuation prob.)."""
+    m = _material(**material)
+    rd = np.ascontiguousarray(ray_dir, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    out = np.zeros((len(d), 7), np.float32)
+    if lib().oracle_bsdf_eval(C.byref(m), _p(rd), ior, int(from_light), _p(d), len(d), _p(out)) != 0:
+        raise RuntimeError("oracle_bsdf_eval failed")
+    return out
+
+
+def bsdf_sample(material, ray_dir, u, ior=-1.0, from_light=False):
+    """Samples of the same BSDF with variates u (n x 3): n x 9 (direction, pdf, cosine, f.xyz,
+    event bits: 1 diffuse, 2 Phong, 4 reflect, 8 refract)."""
+    m = _material(**material)
+    rd = np.ascontiguousarray(ray_dir, np.float32)
+    uu = np.ascontiguousarray(u, np.float32)
+    out = np.zeros((len(uu), 9), np.float32)
+    if lib().oracle_bsdf_sample(C.byref(m), _p(rd), ior, int(from_light), _p(uu), len(uu), _p(out)) != 0:
+        raise RuntimeError("oracle_bsdf_sample failed")
+    return out

@@ -2289,4 +2289,81 @@ extern "C" void oracle_hg_sample(float g, float u0, float u1, float* cos_theta, 
 
 extern "C" float oracle_fresnel(float cos_inc, float ior) { return fresnel_dielectric(cos_inc, ior); }
 
+// One surface BSDF (kernel/bsdf.cuh:179-660) of material `mat` at a surface with normal +z, set
+// up for a ray travelling along ray_dir (3 floats) with relative index of refraction ior, on a
+// camera path (from_light 0) or a light path (1). The learned proposal is off.
+static bool surface_bsdf(const upbp_material* mat, const float* ray_dir, float ior, int from_light, Scene& sc,
+                         BSDF& b)
+{
+    Material m;
+    m.diffuse = {mat->diffuse[0], mat->diffuse[1], mat->diffuse[2]};
+    m.phong = {mat->phong[0], mat->phong[1], mat->phong[2]};
+    m.mirror = {mat->mirror[0], mat->mirror[1], mat->mirror[2]};
+    m.phong_exp = mat->phong_exponent;
+    m.ior = mat->ior;
+    m.priority = mat->priority;
+    m.real = mat->real != 0;
+    m.med_idx = mat->med_idx;
+    sc.mats.assign(1, m);
+    Isect is;
+    is.dist = 1.f;
+    is.mat = 0;
+    is.normal = is.shading = {0.f, 0.f, 1.f};
+    is.enter = ray_dir[2] < 0.f;
+    bsdf_setup(b, V{ray_dir[0], ray_dir[1], ray_dir[2]}, is, sc, from_light != 0, ior);
+    return b.valid();
+}
+
+// Evaluates the BSDF toward each of n directions (3 floats each); out: 7 floats per direction
+// (f.xyz, cosine at the generated side, forward pdf, reverse pdf, continuation probability).
+extern "C" int oracle_bsdf_eval(const upbp_material* mat, const float* ray_dir, float ior, int from_light,
+                                const float* dirs, uint32_t n, float* out)
+{
+    if (!mat || !ray_dir || (n && (!dirs || !out))) return UPBP_E_ARG;
+    Scene sc;
+    BSDF b;
+    if (!surface_bsdf(mat, ray_dir, ior, from_light, sc, b)) return UPBP_E_ARG;
+    for (uint32_t i = 0; i < n; i++) {
+        float cg = 0.f, fp = 0.f, rp = 0.f, st = 0.f;
+        const V f = bsdf_evaluate(b, V{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]}, sc, cg, &fp, &rp, &st);
+        float* o = out + 7 * size_t(i);
+        o[0] = f.x;
+        o[1] = f.y;
+        o[2] = f.z;
+        o[3] = cg;
+        o[4] = fp;
+        o[5] = rp;
+        o[6] = b.cont_prob;
+    }
+    return UPBP_OK;
+}
+
+// Samples the BSDF with n variate triples u (3 floats each); out: 9 floats per sample (direction,
+// pdf, cosine, f.xyz, event bits of BSDF::Event).
+extern "C" int oracle_bsdf_sample(const upbp_material* mat, const float* ray_dir, float ior, int from_light,
+                                  const float* u, uint32_t n, float* out)
+{
+    if (!mat || !ray_dir || (n && (!u || !out))) return UPBP_E_ARG;
+    Scene sc;
+    BSDF b;
+    if (!surface_bsdf(mat, ray_dir, ior, from_light, sc, b)) return UPBP_E_ARG;
+    for (uint32_t i = 0; i < n; i++) {
+        V d{0, 0, 0};
+        float pdf = 0.f, cg = 0.f, st = 0.f;
+        unsigned ev = 0;
+        const V f = bsdf_sample(b, u[3 * i], u[3 * i + 1], u[3 * i + 2], d, pdf, cg, sc, ev, st);
+        float* o = out + 9 * size_t(i);
+        o[0] = d.x;
+        o[1] = d.y;
+        o[2] = d.z;
+        o[3] = pdf;
+        o[4] = cg;
+        o[5] = f.x;
+        o[6] = f.y;
+        o[7] = f.z;
+        o[8] = float(ev);
+    }
+    return UPBP_OK;
+}
+
 extern "C" void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }

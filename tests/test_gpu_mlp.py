@@ -49,6 +49,30 @@ def test_gradient_matches_fp64_oracle():
         assert c > 0.97, (a, b, c)
 
 
+def test_adam_step_matches_fp64_oracle():
+    # the GPU's Adam (FP32 master weights, FP32 moments) against the FP64 restatement fed with
+    # the GPU's own gradients: two steps, so that the moments and both bias corrections count
+    m = gpu.Mlp(seed=7)
+    rng = np.random.default_rng(4)
+    p = m.params().astype(np.float64)
+    mo, vo = np.zeros_like(p), np.zeros_like(p)
+    for step, lr in ((1, 1e-3), (2, 5e-4)):
+        s = _samples(rng, 4096)
+        _, g = m.loss_grad(s)
+        m.train_step(s, lr=lr)
+        got = m.params().astype(np.float64)
+        want, mo, vo = oracle_py.adam_step(p, mo, vo, g.astype(np.float64), step, lr)
+        # the update is of size ~lr; FP32 parameters carry ~6e-8 relative. The weight gradients
+        # are summed with float atomics in no fixed order, so a gradient that nearly cancels can
+        # differ between the two evaluations: a few parameters may step differently
+        err = np.abs(got - want)
+        bad = err > 2e-3 * lr + 2e-7 * np.abs(want)
+        assert bad.mean() < 1e-3 and err.max() <= 2.01 * lr, (step, bad.mean(), err.max())
+        moved = np.abs(got - p)
+        assert np.median(moved) > 0.3 * lr  # the step did happen
+        p = got
+
+
 def _target_batch(rng, n):
     # scattered directions drawn from one vMF lobe (kappa 25) whose mean depends on the position
     x = rng.uniform(0, 1, (n, 3))

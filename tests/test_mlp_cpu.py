@@ -67,3 +67,34 @@ def test_loss_of_a_single_lobe():
     c = k / (2 * np.pi * -np.expm1(-2 * k))
     want = (1.0 * -np.log(c) + 2.0 * -(np.log(c) - k)) / 2
     assert abs(loss - want) < 1e-6 * abs(want)
+
+
+# ---- FP64 Adam (oracle_adam_step) pinned by what Adam's bias correction fixes ---------------
+def test_adam_first_step_moves_every_parameter_by_lr():
+    # step 1: m_hat = g and v_hat = g^2 exactly, so the update is lr * g / (|g| + eps)
+    rng = np.random.default_rng(0)
+    g = rng.normal(size=1000) * np.exp(rng.uniform(-8, 2, 1000))
+    p0 = rng.normal(size=1000)
+    p1, m, v = oracle_py.adam_step(p0, np.zeros(1000), np.zeros(1000), g, 1, 1e-3)
+    assert np.allclose(p0 - p1, 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-9, atol=1e-15)
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+
+
+def test_adam_constant_gradient_keeps_the_step_size():
+    # with a constant gradient the bias-corrected moments stay exactly g and g^2 at every step,
+    # so each step moves by lr * sign(g); without the correction the first steps are 3.16x larger
+    rng = np.random.default_rng(1)
+    g = rng.normal(size=200)
+    p, m, v = rng.normal(size=200), np.zeros(200), np.zeros(200)
+    for t in range(1, 40):
+        q, m, v = oracle_py.adam_step(p, m, v, g, t, 2e-3)
+        assert np.allclose(p - q, 2e-3 * g / (np.abs(g) + 1e-8), rtol=1e-7)
+        p = q
+
+
+def test_adam_sign_flip_damps_the_step():
+    # gradient +g then -g: m_hat(2) = (0.09 g - 0.1 g) / 0.19, v_hat(2) = g^2, a step of lr / 19
+    g = np.array([0.5, -2.0, 3.0])
+    p, m, v = oracle_py.adam_step(np.zeros(3), np.zeros(3), np.zeros(3), g, 1, 1.0)
+    q, m, v = oracle_py.adam_step(p, m, v, -g, 2, 1.0)
+    assert np.allclose(q - p, np.sign(g) / 19.0, rtol=1e-6)

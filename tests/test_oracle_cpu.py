@@ -132,3 +132,24 @@ def test_technique_subsets_agree(box_obj):
     ref = means["upbp_bpt"]
     for alg, m in means.items():
         assert abs(m - ref) / ref < 0.02, (alg, m, ref)
+
+
+@pytest.mark.parametrize("alg,radius", [("upbp_bpt", "0.002"), ("upbp_bpt+surf", "0.05"), ("upbp_surf", "0.05"),
+                                        ("upbp_all", "0.02")])
+def test_white_furnace_radiance_is_one(scene_dir, alg, radius):
+    # Closed box, floor an emitter of radiance 1 (black), the other walls white diffuse: every
+    # surface has radiosity pi, so the radiance is 1 everywhere (energy balance of an isothermal
+    # enclosure, independent of its shape). Paths of up to 63 segments miss at most the paths
+    # bouncing 62 times without reaching the floor (~0.8^62 < 1e-5). Each technique subset and
+    # their MIS combination must converge to 1: a weight that does not sum to one over the
+    # techniques of a path, a wrong merge normalisation or a lost continuation probability moves
+    # the mean.
+    obj = str(scene_dir / "furnace" / "furnace.obj")
+    rc, cfg = capi.parse_args(["-s", "-1", obj, "-r", "32x32", "-l", "63", "-a", alg, "-r_initial", radius,
+                               "-r_alpha", "1"])
+    assert rc == 0
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+    img, _ = oracle_py.render(sh.ptr, cfg, 0, 64)
+    sh.close()
+    se = img.std() / np.sqrt(img.size / 3)
+    assert abs(img.mean() - 1.0) < max(4 * se, 2e-3) and abs(img.mean() - 1.0) < 0.01, (img.mean(), se)
