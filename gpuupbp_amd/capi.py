@@ -92,7 +92,7 @@ class Scene(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("subbeam_factor", C.c_float), ("query_piece_factor", C.c_float),
-                ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32)]
+                ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32), ("packet_bounces", C.c_int32)]
 
 
 class IterStats(C.Structure):
@@ -109,7 +109,8 @@ class IterStats(C.Structure):
                 ("seg_wave_trips", C.c_uint64), ("seg_cand_batches", C.c_uint64), ("seg_pair_batches", C.c_uint64),
                 ("vertex_drops", C.c_uint64), ("beam_drops", C.c_uint64), ("path_errors", C.c_uint64),
                 ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double),
-                ("guide_samples", C.c_uint64), ("guide_loss", C.c_double), ("ms_guide", C.c_double)]
+                ("guide_samples", C.c_uint64), ("guide_loss", C.c_double), ("ms_guide", C.c_double),
+                ("merge_stack_overflows", C.c_uint64)]
 
 
 class GuideConfig(C.Structure):
@@ -171,6 +172,8 @@ _SIGS = [
     ("upbp_set_guide", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_set_guide_lobes", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_float]),
     ("upbp_read_exr", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("upbp_mlp_forward_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("upbp_mlp_noalloc_max", C.c_uint32, []),
     ("upbp_set_mis_log", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("upbp_read_mis_log", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
 ]

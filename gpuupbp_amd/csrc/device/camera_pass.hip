@@ -128,10 +128,18 @@ __device__ __forceinline__ int generate_camera_sample(const DScene& sc, const It
     return init_stack(sc, s.stack, cam.mat_id, cam.med_id);
 }
 
-__global__ void __launch_bounds__(256) k_camera_generate(DScene sc, IterParams p, Bufs b)
+// Camera paths start in 8x8-pixel tiles per wave (16x16 per block): the queue of live paths, and
+// with it the bounce kernels' waves and the segment-query pieces they write, then hold compact
+// pixel tiles, whose rays are more coherent than a 64-pixel row. Only the order changes; every
+// pixel keeps its own random streams.
+__global__ void __launch_bounds__(256) k_camera_generate(DScene sc, IterParams p, Bufs b, uint32_t res_x,
+                                                         uint32_t tiles_x)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= p.n_pixels) return;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t x = (blockIdx.x % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = (blockIdx.x / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t pix = y * res_x + x;
+    if (x >= res_x || pix >= p.n_pixels) return;
     Rng rng(kDomainCamera, pix, p.iter_key, 0, P_JITTER);
     Sub s;
     float sx, sy;
@@ -551,7 +559,10 @@ __global__ void __launch_bounds__(256) k_accumulate(const float* fb32, double* a
 
 void launch_camera_generate(const DScene& sc, const IterParams& p, const Bufs& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_camera_generate, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, p, b);
+    const uint32_t res_x = uint32_t(sc.cam.res_x);
+    const uint32_t res_y = (p.n_pixels + res_x - 1) / res_x;
+    const uint32_t tiles_x = (res_x + 15) / 16, tiles_y = (res_y + 15) / 16;
+    hipLaunchKernelGGL(k_camera_generate, dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, p, b, res_x, tiles_x);
 }
 
 void launch_camera_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace upbp {
@@ -380,20 +381,30 @@ int GuideNet::reserve(uint32_t n)
 int GuideNet::forward(const float* samples, int stride, uint32_t n, hipStream_t s)
 {
     if (reserve(n)) return UPBP_E_HIP;
+    return forward_into(samples, stride, n, x0.as<uint16_t>(), h1.as<uint16_t>(), h2.as<uint16_t>(),
+                        h3.as<uint16_t>(), out.as<float>(), s);
+}
+
+// the forward pass into explicit buffers: the encoded inputs and the three hidden activations
+// (pad16(n) rows of 64 BF16 each; h2 may alias x0 and h3 may alias h1 when only the outputs are
+// wanted) and the raw outputs (rows of 48 floats)
+int GuideNet::forward_into(const float* samples, int stride, uint32_t n, uint16_t* a0, uint16_t* a1, uint16_t* a2,
+                           uint16_t* a3, float* o, hipStream_t s) const
+{
     const uint32_t np = pad16(n);
     const int M = int(np);
-    hipLaunchKernelGGL(k_encode, dim3((np + 255) / 256), dim3(256), 0, s, samples, stride, n, np, x0.as<uint16_t>());
+    hipLaunchKernelGGL(k_encode, dim3((np + 255) / 256), dim3(256), 0, s, samples, stride, n, np, a0);
     const uint16_t* W = wq.as<uint16_t>();
     const float* B = bq.as<float>();
     // Y = X W^T: B(k, j) = W[j][k], the padded rows of 64
-    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, x0.as<uint16_t>(), kP, 1,
-                       W + kQ1, 1, kP, M, 64, kP, B, nullptr, h1.as<uint16_t>(), nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, h1.as<uint16_t>(), kP, 1,
-                       W + kQ2, 1, kP, M, 64, kP, B + 64, nullptr, h2.as<uint16_t>(), nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, h2.as<uint16_t>(), kP, 1,
-                       W + kQ3, 1, kP, M, 64, kP, B + 128, nullptr, h3.as<uint16_t>(), nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_BIAS_F32>, gemm_grid(M, kOutP, 1), dim3(256), 0, s, h3.as<uint16_t>(), kP, 1,
-                       W + kQ4, 1, kP, M, kOutP, kP, B + 192, nullptr, nullptr, out.as<float>(), kOutP);
+    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, a0, kP, 1, W + kQ1, 1, kP, M, 64,
+                       kP, B, nullptr, a1, nullptr, kP);
+    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, a1, kP, 1, W + kQ2, 1, kP, M, 64,
+                       kP, B + 64, nullptr, a2, nullptr, kP);
+    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, a2, kP, 1, W + kQ3, 1, kP, M, 64,
+                       kP, B + 128, nullptr, a3, nullptr, kP);
+    hipLaunchKernelGGL(k_gemm<EPI_BIAS_F32>, gemm_grid(M, kOutP, 1), dim3(256), 0, s, a3, kP, 1, W + kQ4, 1, kP, M,
+                       kOutP, kP, B + 192, nullptr, nullptr, o, kOutP);
     return hipGetLastError() == hipSuccess ? UPBP_OK : UPBP_E_HIP;
 }
 
@@ -553,4 +564,65 @@ extern "C" int upbp_mlp_train_step(upbp_mlp* m, const float* samples, uint32_t n
     if (e) return e;
     if (loss) *loss = l;
     return hipDeviceSynchronize() == hipSuccess ? UPBP_OK : UPBP_E_HIP;
+}
+
+// ---- forward pass that allocates nothing ------------------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc): the forward pass of upbp_mlp_forward with its input staging,
+// activations and outputs in one file-scope device array. Each GPU holds its own copy of the
+// array until the process ends; its address on the current device is resolved once per device
+// with hipGetSymbolAddress and only that address is used from host code. Every call on one GPU
+// shares the array, so calls that overlap on one GPU race. Nothing in it needs clearing: every
+// byte a call reads was written earlier by the same call.
+namespace {
+
+constexpr uint32_t kNoAllocMax = 32768; // samples per call (a multiple of 16)
+constexpr size_t kScrIn = 0;                                               // n x 6 floats
+constexpr size_t kScrA0 = kScrIn + size_t(kNoAllocMax) * 6 * 4;            // pad16(n) x 64 BF16
+constexpr size_t kScrA1 = kScrA0 + size_t(kNoAllocMax) * upbp::kP * 2;           // pad16(n) x 64 BF16
+constexpr size_t kScrOut = kScrA1 + size_t(kNoAllocMax) * upbp::kP * 2;          // pad16(n) x 48 floats
+constexpr size_t kScrBytes = kScrOut + size_t(kNoAllocMax) * upbp::kOutP * 4;    // 14.75 MiB
+static_assert(kScrA0 % 256 == 0 && kScrA1 % 256 == 0 && kScrOut % 256 == 0, "scratch regions stay aligned");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+// the array's address on the current device (resolved on the first call there), or null
+static unsigned char* scratch_on_current_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+
+extern "C" uint32_t upbp_mlp_noalloc_max(void) { return kNoAllocMax; }
+
+extern "C" int upbp_mlp_forward_noalloc(upbp_mlp* m, const float* samples, uint32_t n, float* out)
+{
+    if (!m || (n && (!samples || !out))) return UPBP_E_ARG;
+    if (n > kNoAllocMax) return UPBP_E_CAPACITY;
+    if (n == 0) return UPBP_OK;
+    unsigned char* base = scratch_on_current_device();
+    if (!base) return UPBP_E_HIP;
+    float* in = reinterpret_cast<float*>(base + kScrIn);
+    uint16_t* a0 = reinterpret_cast<uint16_t*>(base + kScrA0);
+    uint16_t* a1 = reinterpret_cast<uint16_t*>(base + kScrA1);
+    float* o = reinterpret_cast<float*>(base + kScrOut);
+    if (hipMemcpy(in, samples, size_t(n) * 24, hipMemcpyHostToDevice) != hipSuccess) return UPBP_E_HIP;
+    const int e = m->net.forward_into(in, 6, n, a0, a1, a0, a1, o, nullptr);
+    if (e) return e;
+    // rows of 48 on the device, the first 40 of each to the host
+    if (hipMemcpy2D(out, 40 * 4, o, upbp::kOutP * 4, 40 * 4, n, hipMemcpyDeviceToHost) != hipSuccess) return UPBP_E_HIP;
+    return UPBP_OK;
 }

@@ -152,6 +152,74 @@ __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMap
     }
 }
 
+// Evaluates the last min(qn, 64) queued pairs with the whole wave: attenuation, phase function,
+// both-side MIS updates (and the MIS weight log with LOG), then gathers their colour per query in
+// the wave's colour cache. One lane per contended cache slot takes it (writing back the query it
+// held); every lane whose query holds its slot then adds in LDS, the others go to HBM.
+template <int KIND, bool LOG>
+__device__ __forceinline__ void eval_pairs(const DScene& sc, const IterParams& p, const Bufs& b, PairQueue& Q,
+                                           ColorCache& K, uint32_t& qn, float norm)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t m = qn < 64u ? qn : 64u;
+    wave_lds_sync();
+    bool has = false;
+    uint32_t key = 0, cslot = 0, cpix = 0;
+    f3 c = mk(0.f);
+    if (lane < m) {
+        const uint32_t e = qn - m + lane;
+        const uint32_t pqi = Q.q[e], pslot = Q.slot[e];
+        const float px = Q.x[e], py = Q.y[e], pz = Q.z[e], pw = Q.w[e];
+        SegQ q;
+        StateWeights cw;
+        f3 tatt;
+        uint32_t pix;
+        load_segrec(b.segq + pqi, q, cw, tatt, pix);
+        f3 sum = mk(0.f);
+        const bool logged = LOG && pix % b.mis_stride == 0;
+        float wl = -1.f;
+        if (KIND == 0) {
+            LVertex lv;
+            load_vertex(&b.verts[pslot], lv);
+            eval_pb2d(sc, p, sum, q, cw, lv, px, py, logged ? &wl : nullptr);
+            if (logged && wl >= 0.f && pslot % p.vstride + 1 == lv.path_length)
+                mis_log(b, MT_PB2D, lv.path_length + q.cam_len, lv.path_length, pix, pslot / p.vstride, wl);
+        } else {
+            LBeam lb;
+            load_beam(&b.beams[pslot], lb);
+            eval_bb1d(sc, p, sum, q, cw, lb, pz, pw, px, py, logged ? &wl : nullptr);
+            if (logged && wl >= 0.f)
+                mis_log(b, MT_BB1D, lb.path_length + q.cam_len, lb.path_length, pix, lb.path_idx, wl);
+        }
+        c = (tatt * norm) * sum;
+        has = c.x != 0.f || c.y != 0.f || c.z != 0.f;
+        key = pqi + 1u;
+        cslot = cache_slot(pqi);
+        cpix = pix;
+    }
+    wave_lds_sync();
+    if (has) K.claim[cslot] = lane;
+    wave_lds_sync();
+    if (has && K.claim[cslot] == lane && K.tag[cslot] != key) {
+        if (K.tag[cslot] != 0u) add_color(b.mcolor, K.pix[cslot], K.r[cslot], K.g[cslot], K.b[cslot]);
+        K.tag[cslot] = key;
+        K.pix[cslot] = cpix;
+        K.r[cslot] = K.g[cslot] = K.b[cslot] = 0.f;
+    }
+    wave_lds_sync();
+    if (has) {
+        if (K.tag[cslot] == key) {
+            atomicAdd(&K.r[cslot], c.x);
+            atomicAdd(&K.g[cslot], c.y);
+            atomicAdd(&K.b[cslot], c.z);
+        } else {
+            add_color(b.mcolor, cpix, c.x, c.y, c.z);
+        }
+    }
+    wave_lds_sync();
+    qn -= m;
+}
+
 // LOG adds the MIS weight log of the sampled camera paths (Bufs::mislog).
 template <int KIND, bool LOG>
 __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams p, Bufs b, int bounce)
@@ -322,66 +390,8 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
             const bool test_c = !eval_p && (cn >= 64u || (!any_active && cn > 0u));
             if (!eval_p && !test_c) break;
             if (eval_p) {
-                const uint32_t m = qn < 64u ? qn : 64u;
                 n_pb++;
-                wave_lds_sync();
-                bool has = false;
-                uint32_t key = 0, cslot = 0, cpix = 0;
-                f3 c = mk(0.f);
-                if (lane < m) {
-                    const uint32_t e = qn - m + lane;
-                    const uint32_t pqi = Q.q[e], pslot = Q.slot[e];
-                    const float px = Q.x[e], py = Q.y[e], pz = Q.z[e], pw = Q.w[e];
-                    SegQ q;
-                    StateWeights cw;
-                    f3 tatt;
-                    uint32_t pix;
-                    load_segrec(b.segq + pqi, q, cw, tatt, pix);
-                    f3 sum = mk(0.f);
-                    const bool logged = LOG && pix % b.mis_stride == 0;
-                    float wl = -1.f;
-                    if (KIND == 0) {
-                        LVertex lv;
-                        load_vertex(&b.verts[pslot], lv);
-                        eval_pb2d(sc, p, sum, q, cw, lv, px, py, logged ? &wl : nullptr);
-                        if (logged && wl >= 0.f && pslot % p.vstride + 1 == lv.path_length)
-                            mis_log(b, MT_PB2D, lv.path_length + q.cam_len, lv.path_length, pix, pslot / p.vstride, wl);
-                    } else {
-                        LBeam lb;
-                        load_beam(&b.beams[pslot], lb);
-                        eval_bb1d(sc, p, sum, q, cw, lb, pz, pw, px, py, logged ? &wl : nullptr);
-                        if (logged && wl >= 0.f)
-                            mis_log(b, MT_BB1D, lb.path_length + q.cam_len, lb.path_length, pix, lb.path_idx, wl);
-                    }
-                    c = (tatt * norm) * sum;
-                    has = c.x != 0.f || c.y != 0.f || c.z != 0.f;
-                    key = pqi + 1u;
-                    cslot = cache_slot(pqi);
-                    cpix = pix;
-                }
-                // one lane per contended slot takes it (writing back the query it held); every
-                // lane whose query holds its slot then adds in LDS, the others go to HBM
-                wave_lds_sync();
-                if (has) K.claim[cslot] = lane;
-                wave_lds_sync();
-                if (has && K.claim[cslot] == lane && K.tag[cslot] != key) {
-                    if (K.tag[cslot] != 0u) add_color(b.mcolor, K.pix[cslot], K.r[cslot], K.g[cslot], K.b[cslot]);
-                    K.tag[cslot] = key;
-                    K.pix[cslot] = cpix;
-                    K.r[cslot] = K.g[cslot] = K.b[cslot] = 0.f;
-                }
-                wave_lds_sync();
-                if (has) {
-                    if (K.tag[cslot] == key) {
-                        atomicAdd(&K.r[cslot], c.x);
-                        atomicAdd(&K.g[cslot], c.y);
-                        atomicAdd(&K.b[cslot], c.z);
-                    } else {
-                        add_color(b.mcolor, cpix, c.x, c.y, c.z);
-                    }
-                }
-                wave_lds_sync();
-                qn -= m;
+                eval_pairs<KIND, LOG>(sc, p, b, Q, K, qn, norm);
             } else {
                 const uint32_t n = cn < 64u ? cn : 64u;
                 n_cb++;
@@ -421,8 +431,145 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
         stat_add(b.stats, STAT_ACCEPT, s_acc);
         stat_add(b.stats, KIND == 0 ? STAT_ACC_PB2D : STAT_ACC_BB1D, s_acc);
         stat_add(b.stats, STAT_ERRORS, s_err);
+        stat_add(b.stats, STAT_MERGE_STACK, s_err);
         stat_add(b.stats, STAT_SEG_TRIPS, n_trips);
         stat_add(b.stats, STAT_CAND_BATCHES, n_cb);
+        stat_add(b.stats, STAT_PAIR_BATCHES, n_pb);
+    }
+}
+
+// ---- B-B1D packet traversal -------------------------------------------------------------
+// One wave walks the 4-wide beam tree for 64 query pieces at once: the node index is the same in
+// every lane, so each node is fetched once per wave (a scalar load) instead of once per lane, and
+// each lane tests its own piece against the node's child boxes. A child that any lane hits is
+// visited by the whole wave; at a leaf, the lanes whose box test passed run the exact predicate.
+// The traversal stack is per wave in LDS. This pays when the wave's pieces are close together --
+// the same stretch of 64 neighbouring camera rays, which is how the camera pass writes them -- and
+// is used for the bounces up to IterParams::packet_bounces.
+constexpr int kPacketStack = 192;
+
+template <bool LOG>
+__global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, IterParams p, Bufs b, int bounce)
+{
+    __shared__ int stk_all[kMergeWaves][kPacketStack];
+    __shared__ PairQueue pq[kMergeWaves];
+    __shared__ ColorCache ccache[kMergeWaves];
+    int* stk = stk_all[threadIdx.x >> 6];
+    PairQueue& Q = pq[threadIdx.x >> 6];
+    ColorCache& K = ccache[threadIdx.x >> 6];
+    const uint32_t lane = __lane_id();
+    K.tag[lane] = 0u;
+    K.r[lane] = K.g[lane] = K.b[lane] = 0.f;
+    wave_lds_sync();
+    const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
+    const uint32_t total = min(b.qcount[320 + bounce], b.piece_cap);
+    uint32_t* head = &b.qcount[192 + 2 * bounce + 1];
+    const WNode* nodes = b.maps.wbeam.nodes;
+    uint32_t qn = 0; // pairs queued (same value in every lane)
+    uint32_t n_nodes = 0, n_cands = 0, n_acc = 0, n_err = 0, n_pb = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(head, 64u);
+        base = __shfl(base, 0);
+        if (base >= total) break;
+        // this lane's piece: its ray, segment, owned camera range and box-test range
+        const uint32_t wi = base + lane;
+        const uint2 pc = wi < total ? (p.piece_order == 2 ? b.pieces_x : b.pieces)[wi] : make_uint2(~0u, 0u);
+        const bool act = pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16) && (pc.y >> 16) <= kPiecesPerRay;
+        f3 ro = mk(0.f), rd = mk(1.f);
+        float dmin = 0.f, dmax = 0.f, lo = 0.f, hi = 0.f;
+        int med = -1;
+        RayBox rb = make_raybox(ro, rd, 1.f, 0.f);
+        if (act) {
+            const SegRec* r = b.segq + pc.x;
+            const float4 a = r->a, bb = r->b, d = r->d;
+            ro = mk(a.x, a.y, a.z);
+            rd = mk(bb.x, bb.y, bb.z);
+            dmin = a.w;
+            dmax = bb.w;
+            med = __float_as_int(d.z);
+            float ta, tb;
+            piece_range(dmin, dmax, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
+            rb = make_raybox(ro, rd, ta, tb);
+        }
+        if (!__ballot(act)) continue;
+        int sp = 0; // stack depth (same in every lane)
+        int node = 0;
+        bool overflow = false;
+        for (;;) {
+            const int nu = __builtin_amdgcn_readfirstlane(node);
+            const WNode nd = nodes[nu];
+            n_nodes++;
+            const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
+            const int nc = int(nd.meta >> 24);
+            uint32_t hits = 0;
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float e;
+                    if (k < nc && slab(rb, wnode_decode(nd.ox, nd.q[0], k, sx), wnode_decode(nd.oy, nd.q[1], k, sy),
+                                       wnode_decode(nd.oz, nd.q[2], k, sz), wnode_decode(nd.ox, nd.q[3], k, sx),
+                                       wnode_decode(nd.oy, nd.q[4], k, sy), wnode_decode(nd.oz, nd.q[5], k, sz),
+                                       rb.tmax, e))
+                        hits |= 1u << k;
+                }
+            }
+            int next = -1;
+            for (int k = 0; k < nc; k++) {
+                const bool h = (hits >> k) & 1u;
+                if (!__ballot(h)) continue;
+                const int ch = nd.child[k];
+                if (ch < 0) { // leaf: the exact predicate in the lanes that hit its box
+                    uint32_t slot = 0;
+                    float x = 0.f, y = 0.f, z = 0.f, w = 0.f;
+                    bool acc = false;
+                    if (h) {
+                        n_cands++;
+                        acc = leaf_accept<1>(p, b.maps, uint32_t(~ch), ro, rd, dmin, dmax, lo, hi, med, slot, x, y,
+                                             z, w);
+                    }
+                    n_acc += acc ? 1u : 0u;
+                    pq_push(Q, qn, acc, pc.x, slot, x, y, z, w);
+                    if (qn >= 64u) {
+                        n_pb++;
+                        eval_pairs<1, LOG>(sc, p, b, Q, K, qn, p.norm_bb1d);
+                    }
+                } else if (next < 0) {
+                    next = ch;
+                } else if (sp < kPacketStack) {
+                    if (lane == 0) stk[sp] = ch;
+                    sp++;
+                } else {
+                    overflow = true;
+                }
+            }
+            if (overflow) break;
+            if (next < 0) {
+                if (sp == 0) break;
+                wave_lds_sync();
+                next = stk[--sp];
+                wave_lds_sync();
+            }
+            node = next;
+        }
+        // a full stack abandons the wave's pieces; they are counted (zero on every workload here)
+        if (overflow) n_err += act ? 1u : 0u;
+    }
+    while (qn > 0u) {
+        n_pb++;
+        eval_pairs<1, LOG>(sc, p, b, Q, K, qn, p.norm_bb1d);
+    }
+    wave_lds_sync();
+    if (K.tag[lane] != 0u) add_color(b.mcolor, K.pix[lane], K.r[lane], K.g[lane], K.b[lane]);
+    const uint32_t s_cands = wave_sum(n_cands), s_acc = wave_sum(n_acc), s_err = wave_sum(n_err);
+    if (lane == 0) {
+        stat_add(b.stats, STAT_SEG_NODES, n_nodes);
+        stat_add(b.stats, STAT_SEG_CANDS, s_cands);
+        stat_add(b.stats, STAT_ACCEPT, s_acc);
+        stat_add(b.stats, STAT_ACC_BB1D, s_acc);
+        stat_add(b.stats, STAT_ERRORS, s_err);
+        stat_add(b.stats, STAT_MERGE_STACK, s_err);
+        stat_add(b.stats, STAT_SEG_TRIPS, n_nodes);
         stat_add(b.stats, STAT_PAIR_BATCHES, n_pb);
     }
 }
@@ -519,12 +666,13 @@ int merge_grid_blocks(int bpc)
 {
     static int cus = 0, per = 0;
     if (cus == 0) {
-        int dev = 0, per0 = 0, per1 = 0;
+        int dev = 0, per0 = 0, per1 = 0, per2 = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per0, k_seg_merge<0, false>, kMergeBlock, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per1, k_seg_merge<1, false>, kMergeBlock, 0);
-        per = std::max(1, std::min(per0, per1));
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_seg_merge_packet<false>, kMergeBlock, 0);
+        per = std::max(1, std::min(std::min(per0, per1), per2));
         cus = std::max(1, cus);
     }
     return cus * (bpc > 0 ? std::min(bpc, per) : per);
@@ -598,8 +746,16 @@ void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int 
             hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
             hipLaunchKernelGGL(k_piece_scatter, dim3(1024), dim3(256), 0, s, b, bounce);
         }
-        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
-        else hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        if (bounce < p.packet_bounces) {
+            if (b.mislog)
+                hipLaunchKernelGGL((k_seg_merge_packet<true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+            else
+                hipLaunchKernelGGL((k_seg_merge_packet<false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        } else if (b.mislog) {
+            hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        } else {
+            hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+        }
     }
 }
 

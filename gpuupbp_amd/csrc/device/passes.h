@@ -32,6 +32,7 @@ struct IterParams {
     float box_slack; // extra beam-box inflation covering FP32 error of the beam predicates
     int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
     int piece_order; // B-B1D work order: 0 as written, 1 XCD regions, 2 Morton-sorted (pieces_x)
+    int packet_bounces; // B-B1D: camera bounces below this walk the beam tree as 64-piece packets
     uint64_t iter_key;
 };
 
@@ -441,7 +442,8 @@ __device__ __forceinline__ void mis_log(const Bufs& b, uint32_t tech, uint32_t k
 
 enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_POINT_Q, STAT_SEG_Q, STAT_ACCEPT,
                   STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,
-                  STAT_SEG_PIECES, STAT_SEG_TRIPS, STAT_CAND_BATCHES, STAT_PAIR_BATCHES, STAT_COUNT };
+                  STAT_SEG_PIECES, STAT_SEG_TRIPS, STAT_CAND_BATCHES, STAT_PAIR_BATCHES, STAT_MERGE_STACK,
+                  STAT_COUNT };
 constexpr int kQcountWords = 512;
 // XCD-aware work split of the B-B1D merge: 4x4x4 spatial cells, 8 XCDs owning one octant
 // (2x2x2 cells) each. Per bounce: cell counts [0,64), cell offsets [64,128), fill cursors

@@ -378,6 +378,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.box_slack = 2e-4f * r->scene_mag;
     p.merge_bpc = r->tune.merge_blocks_per_cu;
     p.piece_order = r->tune.piece_order;
+    p.packet_bounces = r->tune.packet_bounces;
     {
         float rq = 0.f;
         if (p.do_pb2d) rq = fmaxf(rq, r_pb2d);
@@ -878,6 +879,7 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         stats->vertex_drops = st[STAT_VDROP];
         stats->beam_drops = st[STAT_BDROP];
         stats->path_errors = st[STAT_ERRORS];
+        stats->merge_stack_overflows = st[STAT_MERGE_STACK];
         // Compulsory HBM bytes. Camera bounce kernels: 128-byte state read and write plus 64
         // bytes of stack per live path-step, 112 bytes per light vertex read by a connection or
         // an accepted point merge, 112 bytes per segment query written. Segment merge kernels:
@@ -930,6 +932,7 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     t->subbeam_factor = 12.f;
     t->query_piece_factor = 64.f;
     t->merge_blocks_per_cu = 0;
+    t->packet_bounces = 1;
 }
 
 extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
@@ -1027,7 +1030,7 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
     if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
-        t->piece_order < 0 || t->piece_order > 2)
+        t->piece_order < 0 || t->piece_order > 2 || t->packet_bounces < 0)
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     return UPBP_OK;
@@ -1073,16 +1076,17 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
     double best_ms = 0.0;
     int e = timed(best, best_ms);
     const float subbeam[] = {8.f, 12.f, 16.f, 24.f}, piece[] = {32.f, 64.f, 128.f};
-    const int blocks[] = {0, 1, 2}, order[] = {0, 2};
-    for (int knob = 0; knob < 4 && !e; knob++) {
-        const int n = knob == 0 ? 4 : (knob == 1 ? 3 : (knob == 2 ? 3 : 2));
+    const int blocks[] = {0, 1, 2}, order[] = {0, 2}, packets[] = {0, 1, 2, 64};
+    for (int knob = 0; knob < 5 && !e; knob++) {
+        const int n = knob == 0 ? 4 : (knob == 1 ? 3 : (knob == 2 ? 3 : (knob == 3 ? 2 : 4)));
         const upbp_tuning base = best;
         for (int v = 0; v < n && !e; v++) {
             upbp_tuning t = base;
             if (knob == 0) t.subbeam_factor = subbeam[v];
             else if (knob == 1) t.query_piece_factor = piece[v];
             else if (knob == 2) t.merge_blocks_per_cu = blocks[v];
-            else t.piece_order = order[v];
+            else if (knob == 3) t.piece_order = order[v];
+            else t.packet_bounces = packets[v];
             if (memcmp(&t, &base, sizeof(t)) == 0) continue;
             double ms = 0.0;
             e = timed(t, ms);

@@ -15,6 +15,7 @@
 //              every wall and the emitter have radiosity pi, so the radiance everywhere is exactly
 //              1 for any enclosure shape (energy balance); paths of up to l segments miss only the
 //              paths that bounce l - 1 times off white walls without reaching the floor.
+//   "blobs_small": the blob scene with a 1x1 to 3x3 grid and 1 to 4 media (by the seed).
 //   "slabroom": room of ~196k triangles with a dense forward-scattering medium slab, lit
 //             only through a 5% aperture in one wall ("slabroom_small": coarse walls).
 #include "../../../include/upbp_capi.h"
@@ -437,6 +438,8 @@ extern "C" int upbp_generate_scene(const char* kind, uint32_t seed, const char* 
     if (k == "boxfog") return gen_box(obj_path, seed, false, true);
     if (k == "blobs") return gen_blobs(obj_path, seed);
     if (k == "stress") return gen_blobs(obj_path, seed, 10, 8);
+    // batched-jobs mix: 1 to 9 blobs (20k to 184k triangles) and 1 to 4 media, set by the seed
+    if (k == "blobs_small") return gen_blobs(obj_path, seed, 1 + int(seed % 3), 1 + int((seed / 3) % 4));
     if (k == "slabroom") return gen_slabroom(obj_path, seed, 128);
     if (k == "slabroom_small") return gen_slabroom(obj_path, seed, 8); // tests: same room, coarse walls
     return UPBP_E_ARG;

@@ -186,6 +186,8 @@ typedef struct upbp_iter_stats {
     uint64_t guide_samples;        /* learned proposal: training samples collected */
     double guide_loss;             /* mean training loss of the iteration's Adam steps */
     double ms_guide;               /* training and record refresh time */
+    uint64_t merge_stack_overflows; /* segment-merge query pieces abandoned on a full traversal
+                                       stack (also counted in path_errors) */
 } upbp_iter_stats;
 
 typedef struct upbp_scene_handle upbp_scene_handle;
@@ -281,6 +283,9 @@ typedef struct upbp_tuning {
                                     camera pass wrote them (default); 1 bucketed by region, each
                                     XCD starting on its own; 2 sorted by the Morton code of the
                                     piece midpoint (one host synchronisation per bounce) */
+    int32_t packet_bounces;      /* camera bounces below this walk the B-B1D beam tree as packets
+                                    of 64 query pieces per wave (one node fetch per wave); later
+                                    bounces, whose rays are less coherent, walk it per piece */
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
@@ -291,7 +296,8 @@ int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
 int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iterations, int32_t n, int32_t threads);
 
 /* Autotuner (no reference counterpart): a coordinate search over subbeam_factor {8, 12, 16,
- * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2} and piece_order {0, 2},
+ * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2}, piece_order {0, 2} and
+ * packet_bounces {0, 1, 2, 64},
  * from the current tuning, timing `repeats` (1..64) renders of `iteration` per candidate after
  * one untimed render. The fastest tuning stays set and is returned in *chosen with its time in
  * *ms_per_iteration (either may be NULL). The accumulated image is restored afterwards and
@@ -358,6 +364,14 @@ int upbp_mlp_get_params(upbp_mlp* m, float* params);
 int upbp_mlp_forward(upbp_mlp* m, const float* samples, uint32_t n, float* out); /* n x 40 raw outputs */
 int upbp_mlp_loss_grad(upbp_mlp* m, const float* samples, uint32_t n, float* loss, float* grads);
 int upbp_mlp_train_step(upbp_mlp* m, const float* samples, uint32_t n, float lr, float* loss); /* one Adam step */
+/* upbp_mlp_forward without any allocation (no counterpart in the reference): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc. Same arguments and outputs as upbp_mlp_forward; its input staging,
+ * activations and outputs live in a fixed device array of 14.75 MiB per GPU (held until the
+ * process ends), so n is at most upbp_mlp_noalloc_max() (32768) and a larger n returns
+ * UPBP_E_CAPACITY. Calls on one GPU share that array: calls that overlap on one GPU race. */
+int upbp_mlp_forward_noalloc(upbp_mlp* m, const float* samples, uint32_t n, float* out);
+uint32_t upbp_mlp_noalloc_max(void);
 
 /* ---- test hooks (parity tests and autotuning only) ----------------------
  * Range queries over a device LBVH, returning the accepted index set of every

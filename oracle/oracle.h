@@ -39,9 +39,9 @@ int oracle_render(const upbp_scene* scene, const upbp_config* cfg, int first_ite
 
 /* Renders light paths [0, n_light_paths) and camera paths [0, n_camera_paths) of one
  * iteration (bounded CPU-baseline sample). times[0..2] receive the wall seconds of the light
- * pass, the photon-map build and the camera pass. */
+ * pass, the photon-map build and the camera pass; stats (optional) the light pass's counts. */
 int oracle_render_sample(const upbp_scene* scene, const upbp_config* cfg, int iteration, uint32_t n_camera_paths,
-                         uint32_t n_light_paths, int n_threads, double* times);
+                         uint32_t n_light_paths, int n_threads, double* times, oracle_stats* stats);
 
 /* Brute-force range query (same kinds and layout as upbp_test_range_query). */
 int oracle_range_query(int kind, const float* prims, uint32_t n_prims, const float* queries, uint32_t n_queries,

@@ -28,7 +28,7 @@ def lib():
         h = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         h.oracle_render.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(OracleStats)]
-        h.oracle_render_sample.argtypes = [vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_int, vp]
+        h.oracle_render_sample.argtypes = [vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_int, vp, vp]
         h.oracle_range_query.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, C.c_float, vp, vp, C.c_uint64,
                                          C.POINTER(C.c_uint64)]
         h.oracle_closest_hit.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
@@ -73,14 +73,16 @@ def render(scene_ptr, cfg, first_iteration=0, n_iterations=1, threads=0, want_fb
     return (out, fb, st) if want_fb else (out, st)
 
 
-def render_sample_times(scene_ptr, cfg, n_camera_paths, n_light_paths, threads=0, iteration=0):
-    """Wall seconds (light pass, photon-map build, camera pass) of a bounded sample iteration."""
+def render_sample_times(scene_ptr, cfg, n_camera_paths, n_light_paths, threads=0, iteration=0, want_stats=False):
+    """Wall seconds (light pass, photon-map build, camera pass) of a bounded sample iteration
+    (and the light pass's counts with want_stats)."""
     t = np.zeros(3, np.float64)
+    st = OracleStats()
     rc = lib().oracle_render_sample(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), iteration,
-                                    n_camera_paths, n_light_paths, threads, _p(t))
+                                    n_camera_paths, n_light_paths, threads, _p(t), C.cast(C.pointer(st), C.c_void_p))
     if rc != 0:
         raise RuntimeError("oracle_render_sample failed: %d" % rc)
-    return t
+    return (t, st) if want_stats else t
 
 
 def range_query(kind, prims, queries, radius, max_indices=None):

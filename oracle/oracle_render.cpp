@@ -2095,7 +2095,8 @@ extern "C" int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg,
 }
 
 extern "C" int oracle_render_sample(const upbp_scene* scene, const upbp_config* cfg, int iteration,
-                                    uint32_t n_camera_paths, uint32_t n_light_paths, int n_threads, double* times)
+                                    uint32_t n_camera_paths, uint32_t n_light_paths, int n_threads, double* times,
+                                    oracle_stats* stats)
 {
     if (!scene || !cfg || !times) return UPBP_E_ARG;
     Scene sc;
@@ -2110,7 +2111,7 @@ extern "C" int oracle_render_sample(const upbp_scene* scene, const upbp_config* 
     const int threads = n_threads > 0 ? n_threads : int(std::max(1u, std::thread::hardware_concurrency()));
     std::vector<float> fb(3 * size_t(cfg->res_x) * cfg->res_y, 0.f);
     // times[0]: light pass, times[1]: photon-map build, times[2]: camera pass (wall seconds)
-    render_iteration(c, *cfg, iteration, threads, fb, n_camera_paths, n_light_paths, nullptr, times);
+    render_iteration(c, *cfg, iteration, threads, fb, n_camera_paths, n_light_paths, stats, times);
     return UPBP_OK;
 }
 

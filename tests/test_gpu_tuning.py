@@ -1,5 +1,5 @@
 """Tuning knobs change speed only (GPU): every setting of sub-beam length, query piece length,
-merge blocks per CU and piece order accepts the same photon and beam pairs and renders the same
+merge blocks per CU, piece order and packet traversal accepts the same photon and beam pairs and renders the same
 image as the defaults."""
 import numpy as np
 import pytest
@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 
 SETTINGS = [dict(subbeam_factor=8.0, query_piece_factor=32.0), dict(subbeam_factor=24.0, query_piece_factor=128.0),
             dict(subbeam_factor=0.0, query_piece_factor=0.0), dict(merge_blocks_per_cu=1), dict(piece_order=1),
-            dict(piece_order=2)]
+            dict(piece_order=2), dict(packet_bounces=0), dict(packet_bounces=64),
+            dict(packet_bounces=64, piece_order=2), dict(packet_bounces=64, subbeam_factor=0.0, query_piece_factor=0.0)]
 
 
 def _render(obj, tuning):

@@ -1,0 +1,116 @@
+"""The reference's validation matrix and camera-path modes on the GPU, against the oracle (GPU).
+
+The reference validates its GPU renderer against its CPU one by rendering 37 predefined scenes
+with six technique sets each (gpuupbp/tools/validate/a_params.txt:3, a_03_generate_gpu.bat:9-25:
+-s SCENE -a upbp_{all, bb1d, bpt, pb2d, pp3d, surf}). The same matrix runs here at 32x32, one
+iteration, GPU against the oracle on the same random streams. The camera-path modes -previous,
+-compatible and -speconly (gpuupbp/kernel/trace_camera.cu:76,363-381) and -ignorespec
+(functs_camera.cuh:136-138) run on scenes with mirror and glass spheres, path for path and
+converged. -log must write the reference logger's CSV columns (gpuupbp/misc/logger.cpp:31-86).
+"""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpuupbp_amd import capi
+from gpuupbp_amd import renderer as gpu
+from oracle import oracle_py
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SCENES = [0, 1, 2, 5, 6, 7, 8, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32,
+          33, 34, 35, 36, 37, 38, 39, 40]
+TECHS = ["all", "bb1d", "bpt", "pb2d", "pp3d", "surf"]
+
+
+def relmse(a, b):
+    return float(np.mean((a - b) ** 2 / (b ** 2 + 1e-2)))
+
+
+def _pair(sid, args, iters=1):
+    rc, cfg = capi.parse_args(["-s", str(sid)] + args)
+    assert rc == 0
+    sh = capi.SceneHandle(sid, cfg.res_x, cfg.res_y)
+    with gpu.Renderer(sh.ptr, cfg) as r:
+        stats = [r.run_iteration(i) for i in range(iters)]
+        g = r.read_accum(iters)
+    o, ost = oracle_py.render(sh.ptr, cfg, 0, iters)
+    cfg.iteration_offset = 1 << 16
+    o2, _ = oracle_py.render(sh.ptr, cfg, 0, iters)
+    sh.close()
+    return g, o, o2, stats, ost
+
+
+@pytest.mark.parametrize("sid", SCENES)
+def test_validation_matrix(sid):
+    # one scene, its six technique sets
+    for tech in TECHS:
+        g, o, o2, st, ost = _pair(sid, ["-r", "32x32", "-a", "upbp_" + tech])
+        s = st[0]
+        assert np.isfinite(g).all(), (sid, tech)
+        assert s.path_errors == ost.path_errors and s.beam_drops == 0 and s.vertex_drops == 0, (sid, tech)
+        assert s.surface_vertices + s.medium_vertices == ost.light_vertices, (sid, tech)
+        if o.max() == 0:
+            assert g.max() == 0, (sid, tech)
+            continue
+        rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
+        same = float(np.mean(rel < 1e-3))
+        r_same, r_ind = relmse(g, o), relmse(o2, o)
+        assert same > 0.9, (sid, tech, same)
+        assert r_same <= 0.2 * r_ind + 1e-9, (sid, tech, r_same, r_ind)
+
+
+# mirror and glass spheres (5, 6), a glass sphere holding a dense medium (39), water / ice / glass
+# interfaces inside an absorbing sphere (22)
+MODES = [("-previous",), ("-compatible",), ("-speconly",), ("-ignorespec", "1")]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("sid", [5, 6, 22, 39])
+def test_camera_path_modes(sid, mode):
+    args = ["-r", "48x48", "-l", "8", "-a", "upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d", "-r_initial", "0.02",
+            "-r_alpha", "1"] + list(mode)
+    g, o, o2, st, ost = _pair(sid, args)
+    assert st[0].path_errors == ost.path_errors
+    if o.max() == 0:
+        assert g.max() == 0
+        return
+    rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
+    assert np.mean(rel < 1e-3) > 0.93, (sid, mode, np.mean(rel < 1e-3))
+    assert relmse(g, o) < 0.2 * relmse(o2, o) + 1e-9, (sid, mode)
+    assert abs(g.mean() - o.mean()) <= 0.02 * o.mean() + 1e-9, (sid, mode, g.mean(), o.mean())
+
+
+def test_modes_change_the_image():
+    # each mode must reach the kernels: -speconly keeps only purely specular camera paths, which
+    # are darker than the full estimate on a scene with diffuse walls
+    base = ["-r", "32x32", "-l", "8", "-r_initial", "0.02", "-r_alpha", "1"]
+    full, _, _, _, _ = _pair(5, base, 2)
+    for mode in MODES[:3]:
+        img, _, _, _, _ = _pair(5, base + list(mode), 2)
+        assert img.mean() < full.mean(), mode
+
+
+def test_log_csv_columns(tmp_path):
+    exe = os.path.join(ROOT, "gpuupbp_amd", "gpuupbp")
+    r = subprocess.run([exe, "-s", "21", "-r", "32x24", "-l", "6", "-i", "3", "-log", "-o", str(tmp_path / "a.bmp")],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    logs = glob.glob(str(tmp_path / "*_log.txt"))
+    assert len(logs) == 1
+    lines = open(logs[0]).read().splitlines()
+    head = [c.strip() for c in lines[0].split(",")]
+    assert head == ["scene", "iter", "lighttime", "buildtime", "cameratime", "res_x", "res_y", "mem_used",
+                    "mem_free", "mem_total", "light_util", "camera_util"]
+    assert len(lines) == 4
+    for k, line in enumerate(lines[1:]):
+        c = [v.strip() for v in line.split(",")]
+        assert len(c) == 12 and int(c[1]) == k and (int(c[5]), int(c[6])) == (32, 24)
+        lt, bt, ct = float(c[2]), float(c[3]), float(c[4])
+        assert lt > 0 and bt >= 0 and ct > 0 and lt + bt + ct < 10
+        used, free, total = float(c[7]), float(c[8]), float(c[9])
+        assert total > 0 and abs(used + free - total) <= 1e-3 * total
