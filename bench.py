@@ -7,18 +7,21 @@ length 12, 900,000 light paths per iteration, radius 0.005, upbp_all
 (BPT + surface PM + P-P3D + P-B2D + B-B1D), FP32 transport, FP64 accumulation.
 
 One step = one 1-spp iteration (light pass, photon LBVH builds, camera pass, FP64
-accumulate) with the scene resident in HBM. Prints one JSON line; it also carries the
-committed measurements of the longer runs below (profiles/*.json).
+accumulate) with the scene resident in HBM. The default run then measures the learned
+proposal against the plain renderer at equal wall-clock time (60 s each) on the slab room,
+against the committed plain reference render (gpuupbp_amd/data/). Prints one JSON line; it
+also carries the committed measurements of the longer runs below (profiles/*.json).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--autotune]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned] [--autotune]
 
 Other measurements (each prints its own JSON line):
-  --learned SECONDS [--ref-seconds S]  learned proposal vs plain renderer at equal time on the
-                                       slab room (guide settings: --alpha, --surface-alpha,
-                                       --guide-res, --guide-steps, --guide-lr, --guide-history;
-                                       comma lists compare several against one reference)
+  --learned SECONDS                    only the learned-vs-plain comparison (guide settings:
+                                       --alpha, --surface-alpha, --guide-res, --guide-steps,
+                                       --guide-lr, --guide-history; comma lists compare several)
   --batch JOBS                         batched independent jobs (upbp_run_batch)
   --stress                             3840x2160, 2.05M triangles, 8 media, 8M light paths
+  --reference-chunk FIRST:LAST         render part of the slab-room reference (GPU), then
+  --reference-merge CHUNK...           join the parts into gpuupbp_amd/data/ (no GPU)
 """
 import argparse
 import json
@@ -64,77 +67,6 @@ def cpu_baseline(scene_ptr, cfg):
     }
 
 
-LEARNED_ARGS = ["-l", "16", "-r_initial", "0.01", "-a", "upbp_all"]
-
-
-def equal_time(seconds, ref_seconds, workdir, res, guides):
-    """Learned proposal vs plain renderer at equal wall-clock time on the slab-room workload
-    (196k triangles, dense g=0.9 slab lit through a 5% aperture): relMSE of each against a
-    plain reference render of ref_seconds (two halves on other random streams). Each learned
-    run's time includes its online training. The reference's own error, estimated from its
-    halves as relMSE(half_a, half_b) / 4, is reported and also taken off every figure.
-    guides: learned-proposal settings to compare (dicts of upbp_guide_config fields), each
-    against the same reference and plain run. Returns the result dictionary (the first setting
-    at the top level, all of them under "learned")."""
-    import numpy as np
-    from gpuupbp_amd import capi
-    from gpuupbp_amd.renderer import Renderer
-    obj = os.path.join(workdir, "slabroom.obj")
-    if not os.path.exists(obj + ".aux"):
-        capi.generate_scene("slabroom", 3, obj)
-    rc, cfg = capi.parse_args(["-s", "-1", obj, "-r", "%dx%d" % res] + LEARNED_ARGS)
-    assert rc == 0
-    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
-
-    def run(sec, g, first, label):
-        r = Renderer(sh.ptr, cfg)
-        if g is not None:
-            r.set_guide(**g)
-        r.sync()
-        t0 = time.perf_counter()
-        n, last = 0, t0
-        while True:
-            r.run_iteration(first + n, want_stats=False)
-            r.sync()
-            n += 1
-            now = time.perf_counter()
-            if now - last > 20:
-                print("bench: %s %d iterations, %.0f s" % (label, n, now - t0), file=sys.stderr, flush=True)
-                last = now
-            if now - t0 >= sec:
-                break
-        img = r.read_accum(n)
-        r.close()
-        return img, n, time.perf_counter() - t0
-
-    ref_a, n_a, _ = run(ref_seconds / 2, None, 1 << 20, "reference a")
-    ref_b, n_b, _ = run(ref_seconds / 2, None, 1 << 21, "reference b")
-    ref = (ref_a * n_a + ref_b * n_b) / (n_a + n_b)
-    rel = lambda im, r: float(np.mean((im - r) ** 2 / (r ** 2 + 1e-2)))
-    ref_err = rel(ref_a, ref_b) / 4
-    plain, n_plain, t_plain = run(seconds, None, 0, "plain")
-    rp = rel(plain, ref)
-    learned = []
-    for g in guides:
-        img, n, t = run(seconds, g, 0, "learned %s" % g)
-        rl = rel(img, ref)
-        learned.append({"guide": g, "iterations": n, "wall_s": round(t, 2), "relmse": rl,
-                        "relmse_less_reference": rl - ref_err, "ratio_plain_over_learned": rp / rl,
-                        "ratio_less_reference": (rp - ref_err) / (rl - ref_err)})
-    first = learned[0]
-    return {"workload": "slabroom: 194,970 tris, slab sigma_t 7.23 (U[2,10], seed 3), albedo 0.95, g 0.9, "
-                        "5%% aperture, %dx%d, l=16, r=0.01, upbp_all" % res,
-            "seconds": seconds, "reference_seconds": ref_seconds, "reference_iterations": n_a + n_b,
-            "iterations_plain": n_plain, "iterations_learned": first["iterations"],
-            "wall_plain_s": round(t_plain, 2), "wall_learned_s": first["wall_s"],
-            "relmse_plain": rp, "relmse_learned": first["relmse"],
-            "relmse_ratio_plain_over_learned": first["ratio_plain_over_learned"],
-            "reference_relmse_estimate": ref_err,
-            "relmse_plain_less_reference": rp - ref_err, "relmse_learned_less_reference": first["relmse_less_reference"],
-            "relmse_ratio_less_reference": first["ratio_less_reference"],
-            "guide": first["guide"], "learned": learned}
-
-
 def measured_traffic(kernel):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (scaled by the copy-kernel calibration), or None."""
@@ -149,32 +81,19 @@ def measured_traffic(kernel):
 
 
 def batch_jobs(workdir, n_jobs=64, iters=4, threads=8):
-    """Batched jobs: n_jobs independent renderers drawn with seed 2 -- scene box / boxfog /
-    slab room (small and 195k-triangle), resolution in {64, 128, 256, 512}^2, max length in
-    [4, 16], radius 0.01 -- each rendering `iters` iterations, one after another and then all at
-    once through upbp_run_batch. Returns the measurement."""
-    import numpy as np
+    """Batched jobs: the n_jobs independent renderers of the seed-2 mix (gpuupbp_amd/workloads.py:
+    box, fog box, blob grids and slab rooms of 10 to 195k triangles and 1 to 4 media, resolution
+    in {64, 128, 256, 512}^2, max length in [4, 16]), each rendering `iters` iterations, one after
+    another and then all at once through upbp_run_batch. Returns the measurement."""
     from gpuupbp_amd import capi
+    from gpuupbp_amd import workloads as wl
     from gpuupbp_amd.renderer import Renderer, run_batch
-    rng = np.random.default_rng(2)
-    kinds = ["box", "boxfog", "slabroom_small", "slabroom"]
-    objs = {}
     jobs = []
-    for j in range(n_jobs):
-        kind = kinds[int(rng.integers(len(kinds)))]
-        seed = int(rng.integers(4)) if kind != "slabroom" else 3
-        key = (kind, seed)
-        if key not in objs:
-            d = os.path.join(workdir, "batch_%s_%d" % key)
-            os.makedirs(d, exist_ok=True)
-            objs[key] = capi.generate_scene(kind, seed, os.path.join(d, kind + ".obj"))
-        res = int(rng.choice([64, 128, 256, 512]))
-        length = int(rng.integers(4, 17))
-        rc, cfg = capi.parse_args(["-s", "-1", objs[key], "-r", "%dx%d" % (res, res), "-l", str(length),
-                                   "-r_initial", "0.01", "-a", "upbp_all"])
+    for obj, argv, px, kind, length in wl.batch_mix(workdir, n_jobs, 2):
+        rc, cfg = capi.parse_args(argv)
         assert rc == 0
-        sh = capi.SceneHandle(objs[key], cfg.res_x, cfg.res_y)
-        jobs.append((sh, Renderer(sh.ptr, cfg), res * res, kind, length))
+        sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+        jobs.append((sh, Renderer(sh.ptr, cfg), px, kind, length))
     rs = [r for _, r, _, _, _ in jobs]
     pixels = sum(px for _, _, px, _, _ in jobs)
     run_batch(rs, [0] * len(rs), threads)  # warm-up: buffers grow to their sizes
@@ -193,7 +112,7 @@ def batch_jobs(workdir, n_jobs=64, iters=4, threads=8):
     for sh, r, _, _, _ in jobs:
         r.close()
         sh.close()
-    mix = {k: sum(1 for _, _, _, kk, _ in jobs if kk == k) for k in kinds}
+    mix = {k: sum(1 for _, _, _, kk, _ in jobs if kk == k) for k in wl.BATCH_KINDS}
     return {"jobs": n_jobs, "iterations_per_job": iters, "threads": threads, "job_mix": mix,
             "pixels_per_round": pixels, "seconds_sequential": round(t_seq, 3), "seconds_batched": round(t_batch, 3),
             "samples_per_s_sequential": pixels * iters / t_seq, "samples_per_s_batched": pixels * iters / t_batch,
@@ -251,22 +170,134 @@ def committed(name):
         return None
 
 
-def learned_vs_plain():
-    """The committed equal-time comparison of the learned proposal with the plain renderer on the
-    slab room (bench.py --learned; a ten-minute run, so the default run reports the committed
-    measurement rather than repeating it), or None."""
-    path = os.path.join(ROOT, "profiles", "learned_vs_plain_slabroom.json")
-    try:
-        d = json.load(open(path))
-    except (OSError, ValueError):
+REFERENCE = os.path.join(ROOT, "gpuupbp_amd", "data", "slabroom_1280x720_reference.npz")
+# the reference render's random streams: shifted by -ioff so that they share nothing with the
+# equal-time renders (iterations 0.. on unshifted streams), while every render follows the same
+# radius schedule from iteration 0
+REFERENCE_IOFF = 1 << 24
+
+
+def slabroom(workdir, res=(1280, 720), ioff=0):
+    from gpuupbp_amd import capi
+    from gpuupbp_amd import workloads as wl
+    obj = wl.scene(workdir, "slabroom", 3)
+    args = list(wl.SLABROOM_ARGS)
+    args[args.index("-r") + 1] = "%dx%d" % res
+    rc, cfg = capi.parse_args(["-s", "-1", obj] + args + ["-ioff", str(ioff)])
+    assert rc == 0
+    return obj, cfg
+
+
+def reference_chunk(workdir, first, last, seconds, out_dir):
+    """Renders iterations [first, last) of the plain slab-room reference (or as many as fit in
+    `seconds`) and writes their FP64 sum with the range it covers to out_dir."""
+    import numpy as np
+    from gpuupbp_amd import capi
+    from gpuupbp_amd.renderer import Renderer
+    obj, cfg = slabroom(workdir, ioff=REFERENCE_IOFF)
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+    r = Renderer(sh.ptr, cfg)
+    t0 = last_print = time.perf_counter()
+    it = first
+    while it < last:
+        r.run_iteration(it, want_stats=False)
+        it += 1
+        if it % 16 == 0:
+            r.sync()
+            now = time.perf_counter()
+            if now - last_print > 20:
+                print("bench: reference iterations %d..%d, %.0f s" % (first, it, now - t0), file=sys.stderr, flush=True)
+                last_print = now
+            if now - t0 > seconds:
+                break
+    img = r.read_accum(1)  # the sum over the iterations rendered
+    r.close()
+    sh.close()
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, "ref_%06d_%06d.npy" % (first, it))
+    np.save(path, img)
+    return {"first": first, "end": it, "seconds": round(time.perf_counter() - t0, 1), "file": path}
+
+
+def reference_merge(paths):
+    """Joins reference chunks (contiguous iteration ranges from 0) into the committed reference
+    image: the FP64 mean, stored as FP16 (relative precision 5e-4, far below the error measured)."""
+    import numpy as np
+    chunks = sorted((tuple(int(v) for v in os.path.basename(p)[4:-4].split("_")), p) for p in paths)
+    at, total = 0, None
+    for (a, b), p in chunks:
+        if a != at:
+            raise SystemExit("reference chunks are not contiguous at iteration %d" % at)
+        img = np.load(p)
+        total = img if total is None else total + img
+        at = b
+    os.makedirs(os.path.dirname(REFERENCE), exist_ok=True)
+    np.savez_compressed(REFERENCE, image=(total / at).astype(np.float16), iterations=np.int64(at),
+                        ioff=np.int64(REFERENCE_IOFF))
+    return {"iterations": at, "file": REFERENCE}
+
+
+def learned_vs_plain(workdir, seconds=60.0, guides=({},)):
+    """The learned proposal against the plain renderer at equal wall-clock time on the slab room
+    (194,970 triangles, dense g = 0.9 slab lit through a 5% aperture, 1280x720, l = 16): relMSE of
+    each against the committed plain reference, rendered from iteration 0 on other random streams
+    for at least 64 times as many iterations as the plain renderer manages here. The learned run's
+    time includes its online training. guides: learned-proposal settings to compare (dicts of
+    upbp_guide_config fields), each against the same reference and plain run. None if the
+    reference is missing."""
+    import numpy as np
+    from gpuupbp_amd import capi
+    from gpuupbp_amd.renderer import Renderer
+    if not os.path.exists(REFERENCE):
         return None
-    # the run with the library's default guide settings (lr 8e-3 is the default)
-    best = next((x for x in d["learned"] if x["guide"] in ({}, {"learning_rate": 0.008})), d["learned"][0])
-    return {"workload": d["workload"], "seconds": d["seconds"], "reference_seconds": d["reference_seconds"],
-            "relmse_plain": d["relmse_plain"], "relmse_learned": best["relmse"], "guide": best["guide"],
-            "ratio_plain_over_learned": best["ratio_plain_over_learned"],
-            "ratio_less_reference_error": best["ratio_less_reference"],
-            "source": "profiles/learned_vs_plain_slabroom.md (python bench.py --learned 60 --ref-seconds 480)"}
+    ref_npz = np.load(REFERENCE)
+    ref = ref_npz["image"].astype(np.float64)
+    ref_iters = int(ref_npz["iterations"])
+    obj, cfg = slabroom(workdir)
+    sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
+
+    def run(g, label):
+        r = Renderer(sh.ptr, cfg)
+        if g is not None:
+            r.set_guide(**g)
+        r.sync()
+        t0 = last = time.perf_counter()
+        n = 0
+        while True:
+            r.run_iteration(n, want_stats=False)
+            r.sync()
+            n += 1
+            now = time.perf_counter()
+            if now - last > 20:
+                print("bench: %s %d iterations, %.0f s" % (label, n, now - t0), file=sys.stderr, flush=True)
+                last = now
+            if now - t0 >= seconds:
+                break
+        img = r.read_accum(n)
+        r.close()
+        return img, n, time.perf_counter() - t0
+
+    rel = lambda im: float(np.mean((im - ref) ** 2 / (ref ** 2 + 1e-2)))
+    plain, n_plain, t_plain = run(None, "plain")
+    rp = rel(plain)
+    learned = []
+    for g in guides:
+        img, n, t = run(g, "learned %s" % g)
+        rl = rel(img)
+        learned.append({"guide": g, "iterations": n, "wall_s": round(t, 2), "relmse": rl,
+                        "ratio_plain_over_learned": rp / rl})
+    sh.close()
+    first = learned[0]
+    return {"workload": "slabroom: 194,970 tris, slab sigma_t 7.23 (U[2,10], seed 3), albedo 0.95, g 0.9, "
+                        "5% aperture, 1280x720, l=16, r=0.01, upbp_all",
+            "seconds": seconds, "reference_iterations": ref_iters,
+            "reference_spp_over_plain": ref_iters / n_plain,
+            "iterations_plain": n_plain, "iterations_learned": first["iterations"],
+            "wall_plain_s": round(t_plain, 2), "wall_learned_s": first["wall_s"],
+            "relmse_plain": rp, "relmse_learned": first["relmse"],
+            "ratio_plain_over_learned": first["ratio_plain_over_learned"], "target": 1.5,
+            "guide": first["guide"], "learned": learned,
+            "measured": "in this run (raw relMSE, nothing subtracted)"}
 
 
 def main():
@@ -287,9 +318,18 @@ def main():
                     help="batched-jobs measurement over JOBS independent renderers (upbp_run_batch)")
     ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--learned", type=float, default=None,
-                    help="seconds per renderer of the learned-vs-plain equal-time comparison (slab room)")
-    ap.add_argument("--ref-seconds", type=float, default=None, help="reference render time (default 5 x --learned)")
-    ap.add_argument("--learned-res", default="1280x720")
+                    help="only the learned-vs-plain equal-time comparison (slab room), SECONDS per renderer; "
+                         "guide settings below (comma lists compare several against one plain run)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain comparison of the default run")
+    ap.add_argument("--learned-seconds", type=float, default=60.0,
+                    help="seconds per renderer of the default run's learned-vs-plain comparison")
+    ap.add_argument("--reference-chunk", default=None, metavar="FIRST:LAST",
+                    help="render iterations [FIRST, LAST) of the plain slab-room reference (at most "
+                         "--reference-seconds) into gpurun_out/reference/")
+    ap.add_argument("--reference-seconds", type=float, default=900.0)
+    ap.add_argument("--reference-merge", nargs="+", default=None, metavar="CHUNK",
+                    help="join reference chunks into the committed reference image (no GPU)")
     ap.add_argument("--alpha", default=None,
                     help="learned-proposal mixture weight in media (library default); a comma list compares several")
     ap.add_argument("--surface-alpha", default=None,
@@ -299,6 +339,9 @@ def main():
     ap.add_argument("--guide-lr", default=None, help="Adam step size (comma list allowed)")
     ap.add_argument("--guide-history", default=None, help="training-sample ring size (comma list allowed)")
     a = ap.parse_args()
+    if a.reference_merge:
+        print(json.dumps(reference_merge(a.reference_merge)))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: the iteration-sharded multi-GPU path is not built", file=sys.stderr)
         sys.exit(2)
@@ -315,15 +358,21 @@ def main():
         res = batch_jobs(workdir, n_jobs=a.batch, threads=a.batch_threads)
         print(json.dumps({"metric": "batched jobs, samples/s, 1 MI355X", **res}))
         return
+    if a.reference_chunk:
+        first, last = (int(v) for v in a.reference_chunk.split(":"))
+        print(json.dumps(reference_chunk(workdir, first, last, a.reference_seconds,
+                                         os.path.join(ROOT, "gpurun_out", "reference"))))
+        return
     if a.learned is not None:
-        w, h = (int(v) for v in a.learned_res.split("x"))
         lists = {k: ([conv(v) for v in str(val).split(",")] if val is not None else [None])
                  for k, val, conv in (("alpha", a.alpha, float), ("surface_alpha", a.surface_alpha, float),
                                       ("grid_res", a.guide_res, int), ("steps_per_iteration", a.guide_steps, int),
                                       ("learning_rate", a.guide_lr, float), ("history", a.guide_history, int))}
         n = max(len(v) for v in lists.values())
         guides = [{k: v[i % len(v)] for k, v in lists.items() if v[i % len(v)] is not None} for i in range(n)]
-        res = equal_time(a.learned, a.ref_seconds or 5 * a.learned, workdir, (w, h), guides)
+        res = learned_vs_plain(workdir, a.learned, guides)
+        if res is None:
+            raise SystemExit("the slab-room reference image is missing: " + REFERENCE)
         print(json.dumps({"metric": "equal-time relMSE, learned vs plain, 1 MI355X", **res}))
         return
     obj = make_scene(workdir)
@@ -416,7 +465,27 @@ def main():
             "measured_copy": copy,
         },
     }
-    out["learned_vs_plain"] = learned_vs_plain()
+    # the iteration as a whole: compulsory bytes of every pass (path state read and written per
+    # extend step, vertex / beam records written and read, LBVH builds, framebuffers) over the
+    # iteration's wall time
+    t_iter = st.ms_total * 1e-3
+    out["iteration_roofline"] = {"algorithmic_bytes": st.algorithmic_bytes,
+                                 "achieved": st.algorithmic_bytes / t_iter / 1e9 if t_iter > 0 else 0.0,
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": st.algorithmic_bytes / t_iter / 1e9 / HBM_PEAK_GBS if t_iter > 0 else 0.0}
+    # segment queries with their counted traversal: 44 bytes per query (record and result), 64
+    # per node fetched, 16 per candidate tested, the record of every accepted photon (112) or beam
+    # (128), over the two merge kernels' time
+    t_q = (st.ms_merge_pb2d + st.ms_merge_bb1d) * 1e-3
+    b_q = (44.0 * st.segment_queries + 64.0 * st.seg_nodes + 16.0 * st.seg_candidates + 112.0 * st.pb2d_accepts +
+           128.0 * st.bb1d_accepts)
+    out["query_roofline"] = {"bytes": b_q, "queries": st.segment_queries,
+                             "achieved": b_q / t_q / 1e9 if t_q > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": b_q / t_q / 1e9 / HBM_PEAK_GBS if t_q > 0 else 0.0}
+    out["paths_per_s_frac_of_roofline"] = out["iteration_roofline"]["frac"]
+    print("bench: GPU done, %.1f ms per iteration" % ms, file=sys.stderr, flush=True)
+    if not a.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain(workdir, a.learned_seconds)
     batch = committed("batched_jobs.json")
     if batch:
         best = max(batch["runs"], key=lambda x: x["speedup"])
@@ -428,7 +497,6 @@ def main():
         out["stress"] = {"workload": stress_run["workload"], "ms_per_iteration": stress_run["plain"]["ms_per_iteration"],
                          "samples_per_s": stress_run["plain"]["samples_per_s"],
                          "source": "profiles/stress_4k.md (python bench.py --stress)"}
-    print("bench: GPU done, %.1f ms per iteration" % ms, file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sh.ptr, cfg)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
