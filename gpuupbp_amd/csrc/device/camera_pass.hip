@@ -178,7 +178,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     int nseg, nlite;
     bool hit;
     const unsigned rf = origin_in_medium ? kOriginInMedium : 0u;
-    ret = intersect(sc, hit, s.origin, s.dir, is, s.stack, false, rf, segs, nseg, lite, nlite, &mrng);
+    ret = intersect(sc, hit, s.origin, s.dir, is, s.stack, rf, segs, nseg, lite, nlite, &mrng);
     if (ret != S_OK) return false;
     const bool use_full = is.on_surface() || p.qbt == T_BEAM_SHORT || !hit;
     bool ok = true;

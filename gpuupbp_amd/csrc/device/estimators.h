@@ -52,19 +52,14 @@ __device__ __forceinline__ int direct_illumination(const DScene& sc, const IterP
     const float cp = bs.cont_prob;
     fpdf *= light_delta(l) ? 0.f : cp;
     rpdf *= cp;
-    VolSeg segs[kMaxSegments];
-    int nseg;
+    SegAcc sa;
     bool occ;
-    const int ret = occluded(sc, occ, hitp, dir, dist, s.stack, bs.in_medium() ? kOriginInMedium : 0u, segs, nseg);
+    const int ret = occluded(sc, occ, hitp, dir, dist, s.stack, bs.in_medium() ? kOriginInMedium : 0u, sa);
     if (ret != S_OK) return ret;
     if (occ) return S_OK;
-    float nrr = 1.f;
-    f3 natt = mk(1.f);
-    if (nseg) {
-        nrr = acc_rev(segs, nseg);
-        natt = acc_att(segs, nseg);
-        if (!positive(natt)) return S_OK;
-    }
+    const float nrr = sa.pr;
+    const f3 natt = sa.att;
+    if (!positive(natt)) return S_OK;
     const float w_light = fpdf / (pick * dpdf_w);
     const Weights& w = s.sw.w;
     const float ft = pde_factor(p, bs.in_medium(), false, 1.f / nrr, w.rev_ratio, w.for_inv, w.for_ratio, sin_theta);
@@ -107,20 +102,14 @@ __device__ __forceinline__ int connect_vertices(const DScene& sc, const IterPara
     unsigned rf = 0;
     if (bc.in_medium()) rf |= kOriginInMedium;
     if (lv.in_medium) rf |= kEndInMedium;
-    VolSeg segs[kMaxSegments];
-    int nseg;
+    SegAcc sa;
     bool occ;
-    const int ret = occluded(sc, occ, chit, dir, dist, s.stack, rf, segs, nseg);
+    const int ret = occluded(sc, occ, chit, dir, dist, s.stack, rf, sa);
     if (ret != S_OK) return ret;
     if (occ) return S_OK;
-    float rsf = 1.f, rsr = 1.f;
-    f3 att = mk(1.f);
-    if (nseg) {
-        rsf = acc_for(segs, nseg);
-        rsr = acc_rev(segs, nseg);
-        att = acc_att(segs, nseg);
-        if (!positive(att)) return S_OK;
-    }
+    const float rsf = sa.pf, rsr = sa.pr;
+    const f3 att = sa.att;
+    if (!positive(att)) return S_OK;
     const Weights& wl = lv.w;
     const Weights& wc = s.sw.w;
     const float fs = pde_factor(p, lv.bsdf.in_medium(), false, wl.for_inv, wl.for_ratio, 1.f / rsf, wl.rev_ratio, sin_l);

@@ -130,19 +130,14 @@ __device__ __forceinline__ int connect_to_camera(const DScene& sc, const IterPar
     const float ip_dist = cam.ipd / cos_at_cam;
     const float i2sa = ip_dist * ip_dist / cos_at_cam;
     const float i2s = i2sa * fabsf(cos_to_cam) / (dist * dist);
-    VolSeg segs[kMaxSegments];
-    int nseg;
+    SegAcc sa;
     bool occ;
-    const int ret = occluded(sc, occ, hitp, to_cam, dist, s.stack, bs.in_medium() ? kOriginInMedium : 0u, segs, nseg);
+    const int ret = occluded(sc, occ, hitp, to_cam, dist, s.stack, bs.in_medium() ? kOriginInMedium : 0u, sa);
     if (ret != S_OK) return ret;
     if (occ) return S_OK;
-    float rsr = 1.f;
-    f3 att = mk(1.f);
-    if (nseg) {
-        rsr = acc_rev(segs, nseg);
-        att = acc_att(segs, nseg);
-        if (!positive(att)) return S_OK;
-    }
+    const float rsr = sa.pr;
+    const f3 att = sa.att;
+    if (!positive(att)) return S_OK;
     const Weights& wl = s.sw.w;
     const float f_s = pde_factor(p, bs.in_medium(), false, wl.for_inv, wl.for_ratio, 1.f / rsr, wl.rev_ratio, sin_theta);
     const float bpt_s1 = float(!wl.prev_specular);
@@ -177,7 +172,7 @@ __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p
     int nseg, nlite;
     bool hit;
     const unsigned rf = origin_in_medium ? kOriginInMedium : 0u;
-    ret = intersect(sc, hit, s.origin, s.dir, is, s.stack, false, rf, segs, nseg, lite, nlite, &mrng);
+    ret = intersect(sc, hit, s.origin, s.dir, is, s.stack, rf, segs, nseg, lite, nlite, &mrng);
     if (ret != S_OK) return false;
     if (store_beams) add_beams(sc, p, b, pid, s, rf, segs, nseg, lite, nlite, nbeam, bdrops);
     if (!hit) return false;

@@ -591,14 +591,26 @@ __device__ __forceinline__ bool trace_imag(const DScene& sc, f3 o, f3 d, float t
     return !overflow;
 }
 
-// kernel/intersector.cuh:45-393. rng == nullptr selects the occlusion variant.
-__device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd, Hit& result, BStack& bstack, bool occlusion,
-                         unsigned flags, VolSeg* segs, int& nseg, LiteSeg* lite, int& nlite, Rng* rng)
+// Products over the medium segments of an occlusion ray, in segment order: the forward and
+// reverse ray-sampling pdfs and the attenuation (what acc_for / acc_rev / acc_att give over the
+// segment list of a traced ray).
+struct SegAcc {
+    float pf, pr;
+    f3 att;
+};
+
+// kernel/intersector.cuh:45-393. OCC selects the occlusion variant (intersector.cuh:396-418):
+// no distance sampling, the segments only accumulated into *acc (no segment lists are stored),
+// and the caller's stack left unchanged. The traced variant stores the segments in segs and the
+// light segments in lite, samples a scattering distance with rng and moves bstack along the ray.
+template <bool OCC>
+__device__ __noinline__ int intersect_impl(const DScene& sc, bool& hit, f3 ro, f3 rd, Hit& result, BStack& bstack,
+                                           unsigned flags, VolSeg* segs, int& nseg, LiteSeg* lite, int& nlite,
+                                           Rng* rng, SegAcc* acc)
 {
     hit = false;
     nseg = 0;
     nlite = 0;
-    const bool sample_media = !occlusion;
     Hit tmp = result;
     float dist = 0.f;
     LiteHit hwm[kMaxRealHits];
@@ -606,7 +618,7 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
     const int top_priority = bstack.pr[bstack.n - 1];
     const float first_eps = (flags & kOriginInMedium) ? 0.f : kEpsRay;
     float t_max, miss_dist;
-    if (occlusion) {
+    if (OCC) {
         t_max = tmp.dist - kEpsRay;
         miss_dist = tmp.dist;
     } else {
@@ -642,7 +654,7 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
                 t_min = dist + kEpsRay;
                 // the remaining occlusion range is shortened by the hit distance again, as in
                 // intersector.cuh:175-177
-                t_max = occlusion ? miss_dist - dist - kEpsRay : kInf;
+                t_max = OCC ? miss_dist - dist - kEpsRay : kInf;
             }
         }
     } while (not_yet_real && t_max > t_min);
@@ -653,6 +665,28 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
     result = tmp;
     result.dist = dist;
 
+    const bool origin_in_medium = flags & kOriginInMedium;
+    const bool end_in_medium = flags & kEndInMedium;
+    if (OCC) {
+        acc->pf = acc->pr = 1.f;
+        acc->att = mk(1.f);
+    }
+    // The occlusion variant consumes each light segment as it is found: the pdfs and attenuation
+    // of its medium, multiplied in segment order. A segment without a medium adds nothing.
+    auto occ_segment = [&](float dmin, float dmax, int med) {
+        if (med < 0) return;
+        const DMedium& m = medium_at(sc, med);
+        unsigned f = 0;
+        if (origin_in_medium && dmin == 0) f |= kOriginInMedium;
+        if (end_in_medium && dmax == dist) f |= kEndInMedium;
+        float fp = 1.f, rp = 1.f;
+        if (m.has_scattering) fp = ray_sample_pdf(m, dmin, dmax, f, &rp);
+        acc->att *= eval_attenuation(m, dmax - dmin);
+        acc->pf *= fp;
+        acc->pr *= rp;
+        nseg++;
+    };
+
     bool scattered = false;
     LiteHit imag[kMaxImaginary];
     int nimag = 0;
@@ -661,7 +695,7 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
         have_imag = true;
         if (!trace_imag(sc, ro, rd, first_eps, dist - kEpsRay, imag, nimag, ok)) return S_ERR_IMAGINARY;
         if (!ok) return S_ERR_STACK;
-        cp = bstack;
+        cp.copy_from(bstack);
         float dmin = 0.f;
         for (int i = 0; i < nimag; i++) {
             const LiteHit& is = imag[i];
@@ -669,14 +703,18 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
             if (is.enter) {
                 if (priority >= cp.pr[cp.n - 1]) {
                     if (nlite == kMaxSegments) return S_ERR_SEGMENTS;
-                    lite[nlite++] = {dmin, is.dist, cp.med[cp.n - 1]};
+                    if (OCC) occ_segment(dmin, is.dist, cp.med[cp.n - 1]);
+                    else lite[nlite] = {dmin, is.dist, cp.med[cp.n - 1]};
+                    nlite++;
                     dmin = is.dist;
                 }
                 if (!cp.push(is.mat, is.med, priority)) return S_ERR_STACK;
             } else {
                 if (priority == cp.pr[cp.n - 1] && is.mat == cp.mat[cp.n - 1] && is.med == cp.med[cp.n - 1]) {
                     if (nlite == kMaxSegments) return S_ERR_SEGMENTS;
-                    lite[nlite++] = {dmin, is.dist, cp.med[cp.n - 1]};
+                    if (OCC) occ_segment(dmin, is.dist, cp.med[cp.n - 1]);
+                    else lite[nlite] = {dmin, is.dist, cp.med[cp.n - 1]};
+                    nlite++;
                     dmin = is.dist;
                 }
                 cp.pop(is.mat, is.med, priority);
@@ -684,14 +722,17 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
         }
         if (dmin < dist) {
             if (nlite == kMaxSegments) return S_ERR_SEGMENTS;
-            lite[nlite++] = {dmin, dist, cp.med[cp.n - 1]};
+            if (OCC) occ_segment(dmin, dist, cp.med[cp.n - 1]);
+            else lite[nlite] = {dmin, dist, cp.med[cp.n - 1]};
+            nlite++;
         }
     } else {
-        lite[nlite++] = {0.f, dist, bstack.med[bstack.n - 1]};
+        if (OCC) occ_segment(0.f, dist, bstack.med[bstack.n - 1]);
+        else lite[nlite] = {0.f, dist, bstack.med[bstack.n - 1]};
+        nlite++;
     }
+    if (OCC) return S_OK;
 
-    const bool origin_in_medium = flags & kOriginInMedium;
-    const bool end_in_medium = flags & kEndInMedium;
     int popped = 0;
     for (int i = 0; i < nlite; i++) {
         const LiteSeg sg = lite[i];
@@ -703,26 +744,22 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
         float fp = 1.f, rp = 1.f;
         float dmax = sg.dmax;
         if (m.has_scattering) {
-            if (sample_media) {
-                float r;
-                do {
-                    r = rng->next();
-                } while (r == 0.f);
-                const float to_next = sg.dmax - sg.dmin;
-                const float dm = sample_ray(m, to_next, r, &fp, f, &rp);
-                if (dm < to_next) {
-                    result.dist = sg.dmin + dm;
-                    result.mat = -1;
-                    result.med = sg.med;
-                    result.light = -1;
-                    result.normal = mk(0.f);
-                    result.enter = false;
-                    hit = true;
-                    scattered = true;
-                    dmax = result.dist;
-                }
-            } else {
-                fp = ray_sample_pdf(m, sg.dmin, sg.dmax, f, &rp);
+            float r;
+            do {
+                r = rng->next();
+            } while (r == 0.f);
+            const float to_next = sg.dmax - sg.dmin;
+            const float dm = sample_ray(m, to_next, r, &fp, f, &rp);
+            if (dm < to_next) {
+                result.dist = sg.dmin + dm;
+                result.mat = -1;
+                result.med = sg.med;
+                result.light = -1;
+                result.normal = mk(0.f);
+                result.enter = false;
+                hit = true;
+                scattered = true;
+                dmax = result.dist;
             }
         }
         if (nseg == kMaxSegments) return S_ERR_SEGMENTS;
@@ -753,17 +790,23 @@ __device__ __noinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd,
     return S_OK;
 }
 
-// kernel/intersector.cuh:396-418
-__device__ __forceinline__ int occluded(const DScene& sc, bool& hit, f3 p, f3 d, float tmax, const BStack& bs,
-                                        unsigned flags, VolSeg* segs, int& nseg)
+// a traced ray (kernel/intersector.cuh:45-393): distance sampling with rng
+__device__ __forceinline__ int intersect(const DScene& sc, bool& hit, f3 ro, f3 rd, Hit& result, BStack& bstack,
+                                         unsigned flags, VolSeg* segs, int& nseg, LiteSeg* lite, int& nlite, Rng* rng)
 {
-    BStack cp;
-    cp.copy_from(bs);
+    return intersect_impl<false>(sc, hit, ro, rd, result, bstack, flags, segs, nseg, lite, nlite, rng, nullptr);
+}
+
+// kernel/intersector.cuh:396-418: whether the segment [0, tmax) of the ray is blocked, and the
+// segment products along it (all 1 when the ray crosses no medium)
+__device__ __forceinline__ int occluded(const DScene& sc, bool& hit, f3 p, f3 d, float tmax, const BStack& bs,
+                                        unsigned flags, SegAcc& acc)
+{
     Hit tmp;
     hit_reset(tmp, tmax);
-    LiteSeg lite[kMaxSegments];
-    int nlite;
-    return intersect(sc, hit, p, d, tmp, cp, true, flags, segs, nseg, lite, nlite, nullptr);
+    int nseg, nlite;
+    return intersect_impl<true>(sc, hit, p, d, tmp, const_cast<BStack&>(bs), flags, nullptr, nseg, nullptr, nlite,
+                                nullptr, &acc);
 }
 
 UPBP_HD float acc_for(const VolSeg* s, int n)
