@@ -92,7 +92,8 @@ class Scene(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("subbeam_factor", C.c_float), ("query_piece_factor", C.c_float),
-                ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32), ("packet_bounces", C.c_int32)]
+                ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32), ("packet_bounces", C.c_int32),
+                ("graph", C.c_int32)]
 
 
 class IterStats(C.Structure):
@@ -110,7 +111,8 @@ class IterStats(C.Structure):
                 ("vertex_drops", C.c_uint64), ("beam_drops", C.c_uint64), ("path_errors", C.c_uint64),
                 ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double),
                 ("guide_samples", C.c_uint64), ("guide_loss", C.c_double), ("ms_guide", C.c_double),
-                ("merge_stack_overflows", C.c_uint64)]
+                ("merge_stack_overflows", C.c_uint64), ("photon_list_resizes", C.c_uint64),
+                ("graph_launches", C.c_uint64)]
 
 
 class GuideConfig(C.Structure):

@@ -132,9 +132,9 @@ __device__ __forceinline__ int generate_camera_sample(const DScene& sc, const It
 // with it the bounce kernels' waves and the segment-query pieces they write, then hold compact
 // pixel tiles, whose rays are more coherent than a 64-pixel row. Only the order changes; every
 // pixel keeps its own random streams.
-__global__ void __launch_bounds__(256) k_camera_generate(DScene sc, IterParams p, Bufs b, uint32_t res_x,
-                                                         uint32_t tiles_x)
+__global__ void __launch_bounds__(256) k_camera_generate(DScene sc, Bufs b, uint32_t res_x, uint32_t tiles_x)
 {
+    const IterParams& p = *b.params;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t x = (blockIdx.x % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t y = (blockIdx.x / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
@@ -182,7 +182,7 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
     if (ret != S_OK) return false;
     const bool use_full = is.on_surface() || p.qbt == T_BEAM_SHORT || !hit;
     bool ok = true;
-    if ((p.build_pb2d && b.maps.med.n > 0) || (p.build_bb1d && b.maps.beam.n > 0)) {
+    if ((p.build_pb2d && b.maps->med.n > 0) || (p.build_bb1d && b.maps->beam.n > 0)) {
         const VolSeg* sg = use_full ? segs : nullptr;
         const LiteSeg* lt = use_full ? nullptr : lite;
         const int ns = use_full ? nseg : nlite;
@@ -325,11 +325,11 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
             }
         }
     }
-    if (p.build_surf && b.maps.surf.n > 0 && bs.on_surface() && !bs.delta() && !only_spec_surf) {
+    if (p.build_surf && b.maps->surf.n > 0 && bs.on_surface() && !bs.delta() && !only_spec_surf) {
         f3 acc = mk(0.f);
         cc.pq++;
-        ok = bvh_point_query(b.maps.surf, hitp, [&](uint32_t k) {
-            const float4 pt = b.maps.surf_pts[k];
+        ok = bvh_point_query(b.maps->surf, hitp, [&](uint32_t k) {
+            const float4 pt = b.maps->surf_pts[k];
             if (!pred_point_point(hitp, mk(pt.x, pt.y, pt.z), p.r2_surf)) return;
             LVertex lv;
             const uint32_t slot = uint32_t(__float_as_int(pt.w));
@@ -342,11 +342,11 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         });
         color += s.throughput * p.norm_surf * acc;
     }
-    if (p.build_pp3d && b.maps.med.n > 0 && bs.in_medium() && !bs.delta()) {
+    if (p.build_pp3d && b.maps->med.n > 0 && bs.in_medium() && !bs.delta()) {
         f3 acc = mk(0.f);
         cc.pq++;
-        ok &= bvh_point_query(b.maps.med, hitp, [&](uint32_t k) {
-            const float4 pt = b.maps.med_pts[k];
+        ok &= bvh_point_query(b.maps->med, hitp, [&](uint32_t k) {
+            const float4 pt = b.maps->med_pts[k];
             if (!pred_point_point(hitp, mk(pt.x, pt.y, pt.z), p.r2_pp3d)) return;
             LVertex lv;
             const uint32_t slot = uint32_t(__float_as_int(pt.w));
@@ -359,16 +359,16 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         });
         color += s.throughput * p.norm_pp3d * acc;
     }
-    if (p.build_bp2d && b.maps.beam.n > 0 && bs.in_medium()) {
+    if (p.build_bp2d && b.maps->beam.n > 0 && bs.in_medium()) {
         f3 acc = mk(0.f);
         cc.pq++;
-        ok &= bvh_point_query(b.maps.beam, hitp, [&](uint32_t k) {
-            const float4 g0 = b.maps.beam_geo[3 * k], g1 = b.maps.beam_geo[3 * k + 1];
+        ok &= bvh_point_query(b.maps->beam, hitp, [&](uint32_t k) {
+            const float4 g0 = b.maps->beam_geo[3 * k], g1 = b.maps->beam_geo[3 * k + 1];
             const f3 bo = mk(g0.x, g0.y, g0.z), bd = mk(g1.x, g1.y, g1.z);
             if (!pred_point_beam(bo, bd, g0.w, hitp, p.r2_bp2d)) return; // program_bp2d.cu:31-45
             float idist, ir2;
             if (!pred_bre(idist, ir2, bo, bd, 0.f, g0.w, hitp, p.r2_bp2d)) return;
-            const float4 g2 = b.maps.beam_geo[3 * k + 2];
+            const float4 g2 = b.maps->beam_geo[3 * k + 2];
             if (!(idist >= g2.x && idist < g2.y)) return; // another sub-beam of this beam owns the point
             LBeam lb;
             load_beam(&b.beams[__float_as_int(g1.w)], lb);
@@ -499,8 +499,9 @@ __global__ void k_guide_ring_advance(Bufs b)
 }
 
 template <bool LOG>
-__global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams p, Bufs b, int bounce)
+__global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, Bufs b, int bounce)
 {
+    const IterParams& p = *b.params;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.qcount[64 + bounce];
     if (i >= n) return;
@@ -537,8 +538,9 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, IterParams 
 }
 
 // adds every camera path's colour at its jittered raster position (frame_buffer.cuh:35-44)
-__global__ void __launch_bounds__(256) k_camera_splat(IterParams p, Bufs b, float res_x, float res_y)
+__global__ void __launch_bounds__(256) k_camera_splat(Bufs b, float res_x, float res_y)
 {
+    const IterParams& p = *b.params;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= p.n_pixels) return;
     const PathState& st = b.cstate[pix];
@@ -550,11 +552,12 @@ __global__ void __launch_bounds__(256) k_camera_splat(IterParams p, Bufs b, floa
     atomicAdd(&b.fb32[3 * tgt + 2], c.z);
 }
 
-// folds the FP32 iteration buffer into the FP64 cross-iteration accumulator
-__global__ void __launch_bounds__(256) k_accumulate(const float* fb32, double* accum, uint32_t n)
+// folds the FP32 iteration buffer into the FP64 cross-iteration accumulator, unless the iteration
+// overflowed a photon list (then it is discarded and rendered again)
+__global__ void __launch_bounds__(256) k_accumulate(const float* fb32, double* accum, uint32_t n, const uint32_t* ovf)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) accum[i] += double(fb32[i]);
+    if (i < n && *ovf == 0u) accum[i] += double(fb32[i]);
 }
 
 void launch_camera_generate(const DScene& sc, const IterParams& p, const Bufs& b, hipStream_t s)
@@ -562,22 +565,23 @@ void launch_camera_generate(const DScene& sc, const IterParams& p, const Bufs& b
     const uint32_t res_x = uint32_t(sc.cam.res_x);
     const uint32_t res_y = (p.n_pixels + res_x - 1) / res_x;
     const uint32_t tiles_x = (res_x + 15) / 16, tiles_y = (res_y + 15) / 16;
-    hipLaunchKernelGGL(k_camera_generate, dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, p, b, res_x, tiles_x);
+    hipLaunchKernelGGL(k_camera_generate, dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, b, res_x, tiles_x);
 }
 
 void launch_camera_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)
 {
     if (b.mislog)
-        hipLaunchKernelGGL(k_camera_bounce<true>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+        hipLaunchKernelGGL(k_camera_bounce<true>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, b, bounce);
     else
-        hipLaunchKernelGGL(k_camera_bounce<false>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+        hipLaunchKernelGGL(k_camera_bounce<false>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, b, bounce);
 }
 
-void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float res_y, hipStream_t s)
+void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float res_y, const uint32_t* ovf,
+                          hipStream_t s)
 {
-    hipLaunchKernelGGL(k_camera_splat, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, p, b, res_x, res_y);
+    hipLaunchKernelGGL(k_camera_splat, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, b, res_x, res_y);
     const uint32_t n = 3 * p.n_pixels;
-    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, b.fb32, b.accum, n);
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, b.fb32, b.accum, n, ovf);
     if (b.gsamp) {
         hipLaunchKernelGGL(k_guide_finalise, dim3(1024), dim3(256), 0, s, b);
         hipLaunchKernelGGL(k_guide_ring_advance, dim3(1), dim3(1), 0, s, b);

@@ -15,6 +15,8 @@ int merge_grid_blocks(int bpc);
 // pieces into pieces_x in sorted order
 void launch_piece_keys(const Bufs& b, int bounce, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
 void launch_piece_gather(const Bufs& b, uint32_t n, const uint32_t* vals, hipStream_t s);
-void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float res_y, hipStream_t s);
+// splat and FP64 accumulation; an iteration whose ovf[0] is set is not accumulated
+void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float res_y, const uint32_t* ovf,
+                          hipStream_t s);
 
 } // namespace upbp

@@ -281,16 +281,21 @@ struct WBvhStore {
 };
 
 // Primitive boxes are given as lo/hi float4 arrays (w unused), already inflated.
-// Builds into st on stream s; returns 0 or a hipError_t value.
-int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s);
+// Builds into st on stream s; returns 0 or a hipError_t value. With n_dev, the primitive count
+// is read from device memory when the kernels run (at most n, which then sizes the buffers and
+// launches): no host synchronisation, and zero or one primitive are handled on the device.
+int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s,
+               const uint32_t* n_dev = nullptr);
 void lbvh_free(BvhStore& st);
 // Collapses the refitted binary LBVH `bin` (n leaves) into a 4-wide one; same leaf order.
-int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s);
+int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s,
+               const uint32_t* n_dev = nullptr);
 void wbvh_free(WBvhStore& w);
 void lbvh_scratch_free(BvhScratch& sc);
 
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits; keys/vals updated in place.
-int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhScratch& sc, hipStream_t s);
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhScratch& sc, hipStream_t s,
+                     const uint32_t* n_dev = nullptr);
 // Exclusive prefix sum of n u32 values; in and out may alias. Returns the total through *total_dev
 // (device pointer, may be null).
 int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev, BvhScratch& sc,

@@ -36,9 +36,14 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n)
 }
 
 // centroid bounds: bounds[0..2] = ordered min, bounds[3..5] = ordered max
-__global__ void k_centroid_bounds(const float4* lo, const float4* hi, uint32_t n, uint32_t* bounds)
+// n_dev (when given): the count in device memory, at most n
+__device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* n_dev) { return n_dev ? min(*n_dev, n) : n; }
+
+__global__ void k_centroid_bounds(const float4* lo, const float4* hi, uint32_t cap, const uint32_t* n_dev,
+                                  uint32_t* bounds)
 {
     __shared__ float smin[3][256], smax[3][256];
+    const uint32_t n = dev_count(cap, n_dev);
     const uint32_t t = threadIdx.x;
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x) {
@@ -78,11 +83,11 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v)
     return v;
 }
 
-__global__ void k_morton(const float4* lo, const float4* hi, uint32_t n, const uint32_t* bounds, uint32_t* keys,
-                         uint32_t* vals)
+__global__ void k_morton(const float4* lo, const float4* hi, uint32_t cap, const uint32_t* n_dev, const uint32_t* bounds,
+                         uint32_t* keys, uint32_t* vals)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= dev_count(cap, n_dev)) return;
     const float4 a = lo[i], b = hi[i];
     const float c[3] = {(a.x + b.x) * .5f, (a.y + b.y) * .5f, (a.z + b.z) * .5f};
     uint32_t q[3];
@@ -97,10 +102,12 @@ __global__ void k_morton(const float4* lo, const float4* hi, uint32_t n, const u
 }
 
 // ---- radix sort ----------------------------------------------------------------
-__global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
-                                                             uint32_t* hist, uint32_t nblocks)
+__global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* keys, uint32_t cap,
+                                                             const uint32_t* n_dev, int shift, uint32_t* hist,
+                                                             uint32_t nblocks)
 {
     __shared__ uint32_t h[256];
+    const uint32_t n = dev_count(cap, n_dev);
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
@@ -114,10 +121,12 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* key
 
 // Stable scatter: element order within a tile is preserved (round r, then thread id).
 __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* keys_in, const uint32_t* vals_in,
-                                                                uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
-                                                                int shift, const uint32_t* offsets, uint32_t nblocks)
+                                                                uint32_t* keys_out, uint32_t* vals_out, uint32_t cap,
+                                                                const uint32_t* n_dev, int shift,
+                                                                const uint32_t* offsets, uint32_t nblocks)
 {
     constexpr int kWaves = kSortThreads / 64;
+    const uint32_t n = dev_count(cap, n_dev);
     __shared__ uint32_t running[256];
     __shared__ uint32_t wave_hist[kWaves][256];
     const uint32_t t = threadIdx.x;
@@ -217,8 +226,9 @@ __device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j)
     return __clz(a ^ b);
 }
 
-__global__ void k_karras(const uint32_t* keys, int n, BNode* nodes)
+__global__ void k_karras(const uint32_t* keys, uint32_t cap, const uint32_t* n_dev, BNode* nodes)
 {
+    const int n = int(dev_count(cap, n_dev));
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -272,12 +282,15 @@ __device__ __forceinline__ void write_child_box(BNode* nd, int slot, float4 a, f
 // codes with index tie-breaks is at most 30 + 32 levels deep.
 constexpr int kMaxLevels = 64;
 
-__global__ void k_level_init(uint32_t* meta, uint32_t* lvl)
+// the root level holds the root when the tree has an inner node to refit (n >= 2); a tree of
+// one primitive has its root written whole by k_single
+__global__ void k_level_init(uint32_t* meta, uint32_t* lvl, uint32_t cap, const uint32_t* n_dev)
 {
     const int t = threadIdx.x;
+    const uint32_t n = dev_count(cap, n_dev);
     if (t <= kMaxLevels) {
         meta[t] = 0;
-        meta[kMaxLevels + 1 + t] = t == 0 ? 1u : 0u;
+        meta[kMaxLevels + 1 + t] = t == 0 && n >= 2 ? 1u : 0u;
     }
     if (t == 0) lvl[0] = 0;
 }
@@ -383,12 +396,13 @@ __device__ __forceinline__ uint32_t quant_hi(float o, float step, float v)
     return q;
 }
 
-__global__ void k_wide_init(uint32_t* meta, uint32_t* bin)
+__global__ void k_wide_init(uint32_t* meta, uint32_t* bin, uint32_t cap, const uint32_t* n_dev)
 {
     const int t = threadIdx.x;
+    const uint32_t n = dev_count(cap, n_dev);
     if (t <= kMaxLevels) {
         meta[t] = 0;
-        meta[kMaxLevels + 1 + t] = t == 0 ? 1u : 0u;
+        meta[kMaxLevels + 1 + t] = t == 0 && n >= 2 ? 1u : 0u;
     }
     if (t == 0) bin[0] = 0;
 }
@@ -485,9 +499,21 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
     }
 }
 
-// one primitive: a wide root with a single leaf child
-__global__ void k_wide_single(const BNode* bn, WNode* wn)
+// one primitive: a wide root with a single leaf child; none: a wide root without children (with
+// n_dev, the kernel runs every build and acts only for those two counts)
+__global__ void k_wide_single(const BNode* bn, WNode* wn, const uint32_t* n_dev)
 {
+    if (n_dev && *n_dev >= 2) return;
+    if (n_dev && *n_dev == 0) {
+        WNode w;
+        w.ox = w.oy = w.oz = 0.f;
+        w.meta = (127u) | (127u << 8) | (127u << 16);
+        for (int k = 0; k < 4; k++) w.child[k] = 0;
+        for (int k = 0; k < 6; k++) w.q[k] = 0;
+        w.pad0 = w.pad1 = 0;
+        wn[0] = w;
+        return;
+    }
     const BNode& b = bn[0];
     WNode w;
     w.ox = b.lx0;
@@ -505,8 +531,9 @@ __global__ void k_wide_single(const BNode* bn, WNode* wn)
     wn[0] = w;
 }
 
-__global__ void k_single(const float4* lo, const float4* hi, BNode* nodes, uint32_t* order)
+__global__ void k_single(const float4* lo, const float4* hi, BNode* nodes, uint32_t* order, const uint32_t* n_dev)
 {
+    if (n_dev && *n_dev != 1) return; // with n_dev the kernel runs every build and acts for one primitive
     BNode nd;
     nd.lx0 = lo[0].x;
     nd.ly0 = lo[0].y;
@@ -560,7 +587,8 @@ int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tota
     return scan_rec(in, out, n, total_dev, sc, 0, s);
 }
 
-int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhScratch& sc, hipStream_t s)
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhScratch& sc, hipStream_t s,
+                     const uint32_t* n_dev)
 {
     if (n <= 1) return 0;
     const uint32_t nb = blocks_for(n, kSortTile);
@@ -570,10 +598,11 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhSc
     uint32_t *ka = keys, *va = vals, *kb = sc.keys2.as<uint32_t>(), *vb = sc.vals2.as<uint32_t>();
     int passes = 0;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ka, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ka, n, n_dev, shift, hist, nb);
         const int e = scan_exclusive(hist, hist, 256 * nb, nullptr, sc, s);
         if (e) return e;
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, n_dev, shift, hist,
+                           nb);
         std::swap(ka, kb);
         std::swap(va, vb);
         passes++;
@@ -585,7 +614,8 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, BvhSc
     return int(hipGetLastError());
 }
 
-int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s)
+int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s,
+               const uint32_t* n_dev)
 {
     st.n = n;
     if (n == 0) return 0;
@@ -598,7 +628,7 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
         st.cap = c;
     }
     if (n == 1) {
-        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, lo, hi, st.nodes, st.order);
+        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, lo, hi, st.nodes, st.order, n_dev);
         return int(hipGetLastError());
     }
     if (sc.keys.ensure(size_t(n) * 4) || sc.lvl.ensure(size_t(n) * 4) || sc.meta.ensure(2 * (kMaxLevels + 1) * 4) ||
@@ -610,11 +640,13 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(3), 0, s, bounds, 0xffffffffu, 3u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(3), 0, s, bounds + 3, 0u, 3u);
     hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<uint32_t>(blocks_for(n, tpb), 1024)), dim3(tpb), 0, s, lo, hi,
-                       n, bounds);
-    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, tpb)), dim3(tpb), 0, s, lo, hi, n, bounds, keys, st.order);
-    const int e = radix_sort_pairs(keys, st.order, n, 30, sc, s);
+                       n, n_dev, bounds);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, tpb)), dim3(tpb), 0, s, lo, hi, n, n_dev, bounds, keys, st.order);
+    const int e = radix_sort_pairs(keys, st.order, n, 30, sc, s, n_dev);
     if (e) return e;
-    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, keys, int(n), st.nodes);
+    // with a count in device memory the tree may hold a single primitive (written whole here) or none
+    if (n_dev) hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, lo, hi, st.nodes, st.order, n_dev);
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, tpb)), dim3(tpb), 0, s, keys, n, n_dev, st.nodes);
     uint32_t* lvl = sc.lvl.as<uint32_t>();
     uint32_t* meta = sc.meta.as<uint32_t>();
     // levels deeper than a tree over n leaves can reach are skipped on the host
@@ -622,7 +654,7 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
     while (levels < kMaxLevels && (uint64_t(1) << std::min(levels - 1, 62)) < n) levels++;
     levels = std::min(kMaxLevels, levels + 31);
     const uint32_t grid = std::min<uint32_t>(blocks_for(n, tpb), 2048);
-    hipLaunchKernelGGL(k_level_init, dim3(1), dim3(kMaxLevels + 1), 0, s, meta, lvl);
+    hipLaunchKernelGGL(k_level_init, dim3(1), dim3(kMaxLevels + 1), 0, s, meta, lvl, n, n_dev);
     for (int l = 0; l + 1 < levels; l++)
         hipLaunchKernelGGL(k_bfs_level, dim3(grid), dim3(tpb), 0, s, st.nodes, lvl, meta, l);
     for (int l = levels - 1; l >= 0; l--)
@@ -630,7 +662,7 @@ int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi,
     return int(hipGetLastError());
 }
 
-int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s)
+int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s, const uint32_t* n_dev)
 {
     w.n = n;
     if (n == 0) return 0;
@@ -642,17 +674,15 @@ int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hi
         if (hipMalloc(&w.bin, size_t(c) * sizeof(uint32_t)) != hipSuccess) return 1;
         w.cap = c;
     }
-    if (n == 1) {
-        hipLaunchKernelGGL(k_wide_single, dim3(1), dim3(1), 0, s, bin.nodes, w.nodes);
-        return int(hipGetLastError());
-    }
+    if (n == 1 || n_dev) hipLaunchKernelGGL(k_wide_single, dim3(1), dim3(1), 0, s, bin.nodes, w.nodes, n_dev);
+    if (n == 1) return int(hipGetLastError());
     if (sc.meta.ensure(2 * (kMaxLevels + 1) * 4)) return 1;
     uint32_t* meta = sc.meta.as<uint32_t>();
     int levels = 1;
     while (levels < kMaxLevels && (uint64_t(1) << std::min(levels - 1, 62)) < n) levels++;
     levels = std::min(kMaxLevels, levels + 31);
     const uint32_t grid = std::min<uint32_t>(blocks_for(n, 256), 2048);
-    hipLaunchKernelGGL(k_wide_init, dim3(1), dim3(kMaxLevels + 1), 0, s, meta, w.bin);
+    hipLaunchKernelGGL(k_wide_init, dim3(1), dim3(kMaxLevels + 1), 0, s, meta, w.bin, n, n_dev);
     for (int l = 0; l + 1 < levels; l++)
         hipLaunchKernelGGL(k_wide_level, dim3(grid), dim3(256), 0, s, bin.nodes, w.nodes, w.bin, meta, l);
     return int(hipGetLastError());

@@ -34,8 +34,9 @@ __device__ __forceinline__ int generate_light_sample(const DScene& sc, const Ite
     return init_stack(sc, s.stack, l.mat_id, l.med_id);
 }
 
-__global__ void __launch_bounds__(256) k_light_generate(DScene sc, IterParams p, Bufs b)
+__global__ void __launch_bounds__(256) k_light_generate(DScene sc, Bufs b)
 {
+    const IterParams& p = *b.params;
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= p.n_light) return;
     b.nvert[pid] = 0;
@@ -248,8 +249,9 @@ __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p
 }
 
 template <bool LOG>
-__global__ void __launch_bounds__(256) k_light_bounce(DScene sc, IterParams p, Bufs b, int bounce)
+__global__ void __launch_bounds__(256) k_light_bounce(DScene sc, Bufs b, int bounce)
 {
+    const IterParams& p = *b.params;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.qcount[bounce];
     if (i >= n) return;
@@ -282,15 +284,15 @@ __global__ void __launch_bounds__(256) k_light_bounce(DScene sc, IterParams p, B
 
 void launch_light_generate(const DScene& sc, const IterParams& p, const Bufs& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_light_generate, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, p, b);
+    hipLaunchKernelGGL(k_light_generate, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, b);
 }
 
 void launch_light_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)
 {
     if (b.mislog)
-        hipLaunchKernelGGL(k_light_bounce<true>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+        hipLaunchKernelGGL(k_light_bounce<true>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, b, bounce);
     else
-        hipLaunchKernelGGL(k_light_bounce<false>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, p, b, bounce);
+        hipLaunchKernelGGL(k_light_bounce<false>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, b, bounce);
 }
 
 } // namespace upbp

@@ -222,8 +222,9 @@ __device__ __forceinline__ void eval_pairs(const DScene& sc, const IterParams& p
 
 // LOG adds the MIS weight log of the sampled camera paths (Bufs::mislog).
 template <int KIND, bool LOG>
-__global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams p, Bufs b, int bounce)
+__global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, int bounce)
 {
+    const IterParams& p = *b.params;
     __shared__ CandQueue cq[kMergeWaves];
     __shared__ PairQueue pq[kMergeWaves];
     __shared__ Steal stl[kMergeWaves];
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
     const bool split = KIND == 1 && p.piece_order == 1;
     const uint2* plist = KIND == 1 && p.piece_order != 0 ? b.pieces_x : b.pieces;
     uint32_t src = blockIdx.x % kXcds, tried = 0; // work range being consumed (same in every lane)
-    const WBvh& bvh = KIND == 0 ? b.maps.wmed : b.maps.wbeam;
+    const WBvh& bvh = KIND == 0 ? b.maps->wmed : b.maps->wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
 
     uint32_t cn = 0, qn = 0; // candidates / pairs queued (same value in every lane)
@@ -407,7 +408,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
                     const float4 a = r->a, bb = r->b, d = r->d;
                     float clo, chi, cta, ctb;
                     piece_range(a.w, bb.w, ckn & 63u, ckn >> 6, clo, chi, cta, ctb);
-                    acc = leaf_accept<KIND>(p, b.maps, leaf, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), a.w, bb.w, clo,
+                    acc = leaf_accept<KIND>(p, *b.maps, leaf, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), a.w, bb.w, clo,
                                             chi, __float_as_int(d.z), slot, x, y, z, w);
                 }
                 wave_lds_sync();
@@ -449,8 +450,9 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, IterParams
 constexpr int kPacketStack = 192;
 
 template <bool LOG>
-__global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, IterParams p, Bufs b, int bounce)
+__global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, Bufs b, int bounce)
 {
+    const IterParams& p = *b.params;
     __shared__ int stk_all[kMergeWaves][kPacketStack];
     __shared__ PairQueue pq[kMergeWaves];
     __shared__ ColorCache ccache[kMergeWaves];
@@ -464,7 +466,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, Ite
     const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
     const uint32_t total = min(b.qcount[320 + bounce], b.piece_cap);
     uint32_t* head = &b.qcount[192 + 2 * bounce + 1];
-    const WNode* nodes = b.maps.wbeam.nodes;
+    const WNode* nodes = b.maps->wbeam.nodes;
     uint32_t qn = 0; // pairs queued (same value in every lane)
     uint32_t n_nodes = 0, n_cands = 0, n_acc = 0, n_err = 0, n_pb = 0;
     for (;;) {
@@ -525,7 +527,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, Ite
                     bool acc = false;
                     if (h) {
                         n_cands++;
-                        acc = leaf_accept<1>(p, b.maps, uint32_t(~ch), ro, rd, dmin, dmax, lo, hi, med, slot, x, y,
+                        acc = leaf_accept<1>(p, *b.maps, uint32_t(~ch), ro, rd, dmin, dmax, lo, hi, med, slot, x, y,
                                              z, w);
                     }
                     n_acc += acc ? 1u : 0u;
@@ -609,7 +611,7 @@ __global__ void __launch_bounds__(256) k_piece_cells(Bufs b, int bounce)
     __syncthreads();
     const uint32_t n = min(b.qcount[320 + bounce], b.piece_cap);
     uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
-    const WNode root = b.maps.wbeam.nodes[0];
+    const WNode root = b.maps->wbeam.nodes[0];
     const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint2 pc = b.pieces[i];
@@ -697,7 +699,7 @@ __global__ void __launch_bounds__(256) k_piece_keys(Bufs b, int bounce, uint32_t
     const uint2 pc = b.pieces[i];
     uint32_t key = 0;
     if (pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16)) {
-        const WNode root = b.maps.wbeam.nodes[0];
+        const WNode root = b.maps->wbeam.nodes[0];
         const SegRec* r = b.segq + pc.x;
         const float4 a = r->a, bb = r->b;
         float lo, hi, ta, tb;
@@ -736,11 +738,11 @@ void launch_piece_gather(const Bufs& b, uint32_t n, const uint32_t* vals, hipStr
 void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int kind, int bounce, hipStream_t s)
 {
     const int blocks = merge_grid_blocks(p.merge_bpc);
-    if (kind == 0 && p.build_pb2d && b.maps.med.n > 0) {
-        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<0, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
-        else hipLaunchKernelGGL((k_seg_merge<0, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+    if (kind == 0 && p.build_pb2d && p.cap_med > 0) {
+        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<0, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
+        else hipLaunchKernelGGL((k_seg_merge<0, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
     }
-    if (kind == 1 && p.build_bb1d && b.maps.beam.n > 0) {
+    if (kind == 1 && p.build_bb1d && p.cap_beam > 0) {
         if (p.piece_order == 1) {
             hipLaunchKernelGGL(k_piece_cells, dim3(1024), dim3(256), 0, s, b, bounce);
             hipLaunchKernelGGL(k_piece_offsets, dim3(1), dim3(64), 0, s, b, bounce);
@@ -748,13 +750,13 @@ void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int 
         }
         if (bounce < p.packet_bounces) {
             if (b.mislog)
-                hipLaunchKernelGGL((k_seg_merge_packet<true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+                hipLaunchKernelGGL((k_seg_merge_packet<true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
             else
-                hipLaunchKernelGGL((k_seg_merge_packet<false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+                hipLaunchKernelGGL((k_seg_merge_packet<false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
         } else if (b.mislog) {
-            hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+            hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
         } else {
-            hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, p, b, bounce);
+            hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
         }
     }
 }

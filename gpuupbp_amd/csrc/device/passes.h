@@ -33,6 +33,8 @@ struct IterParams {
     int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
     int piece_order; // B-B1D work order: 0 as written, 1 XCD regions, 2 Morton-sorted (pieces_x)
     int packet_bounces; // B-B1D: camera bounces below this walk the beam tree as 64-piece packets
+    uint32_t cap_med, cap_beam; // capacities of the medium-point and sub-beam lists (host launch
+                                // decisions; the counts themselves are only on the device)
     uint64_t iter_key;
 };
 
@@ -420,7 +422,11 @@ struct Bufs {
     uint32_t gring; // training-sample ring capacity (samples written at gcount[1] modulo gring)
     double* accum;
     unsigned long long* stats; // see STAT_*
-    PhotonMaps maps;
+    // this iteration's parameters and photon maps, in device memory: kernel arguments stay the
+    // same from one iteration to the next (a captured iteration can be replayed), and the photon
+    // counts are written by the device (no host synchronisation before the camera pass)
+    const IterParams* params;
+    const PhotonMaps* maps;
     // MIS weight log of the MIS oracle test (null when off): one record per weighted contribution
     // of the camera paths whose index is a multiple of mis_stride and of the light paths below
     // mis_lt (light tracing), as (technique | k << 8 | s << 16, camera path or light path, light

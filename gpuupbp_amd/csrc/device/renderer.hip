@@ -36,9 +36,9 @@ __global__ void k_fill(uint32_t* p, uint32_t v, size_t n)
 }
 
 // per light path: how many surface points, medium points and beams it contributes
-__global__ void k_count_photons(IterParams p, Bufs b, uint32_t* cs, uint32_t* cm, uint32_t* cb, int want_pts,
-                                int want_beams)
+__global__ void k_count_photons(Bufs b, uint32_t* cs, uint32_t* cm, uint32_t* cb, int want_pts, int want_beams)
 {
+    const IterParams& p = *b.params;
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= p.n_light) return;
     uint32_t ns = 0, nm = 0;
@@ -60,11 +60,14 @@ __global__ void k_count_photons(IterParams p, Bufs b, uint32_t* cs, uint32_t* cm
     cb[pid] = nb;
 }
 
-// writes the photon lists in light-path order together with inflated primitive boxes
-__global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const uint32_t* om, const uint32_t* ob,
-                               int want_surf, int want_med, int want_beams, uint32_t* ls, float4* slo, float4* shi,
-                               uint32_t* lm, float4* mlo, float4* mhi, float4* geo, float4* blo, float4* bhi)
+// writes the photon lists in light-path order together with inflated primitive boxes; entries at
+// or beyond a list's capacity (cap: surface, medium, sub-beams) are not written (the iteration is
+// then discarded, see k_map_counts)
+__global__ void k_emit_photons(Bufs b, const uint32_t* os, const uint32_t* om, const uint32_t* ob, int want_surf,
+                               int want_med, int want_beams, uint32_t* ls, float4* slo, float4* shi, uint32_t* lm,
+                               float4* mlo, float4* mhi, float4* geo, float4* blo, float4* bhi, uint3 cap)
 {
+    const IterParams& p = *b.params;
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= p.n_light) return;
     uint32_t js = os[pid], jm = om[pid];
@@ -78,7 +81,7 @@ __global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const u
             const float mag = fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
             if (fl & VF_IN_MEDIUM) {
                 const float r = p.rad_point_med * (1.f + 1e-5f) + 1e-6f * mag;
-                if (want_med) {
+                if (want_med && jm < cap.y) {
                     lm[jm] = slot;
                     mlo[jm] = make_float4(a.x - r, a.y - r, a.z - r, 0.f);
                     mhi[jm] = make_float4(a.x + r, a.y + r, a.z + r, 0.f);
@@ -86,7 +89,7 @@ __global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const u
                 jm++;
             } else {
                 const float r = p.r_surf * (1.f + 1e-5f) + 1e-6f * mag;
-                if (want_surf) {
+                if (want_surf && js < cap.x) {
                     ls[js] = slot;
                     slo[js] = make_float4(a.x - r, a.y - r, a.z - r, 0.f);
                     shi[js] = make_float4(a.x + r, a.y + r, a.z + r, 0.f);
@@ -105,7 +108,7 @@ __global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const u
             const int flags = uint32_t(__float_as_int(dd.z)) < p.bb1d_used ? SB_BB1D : 0;
             const f3 o = mk(a.x, a.y, a.z), dir = mk(d.x, d.y, d.z);
             const uint32_t n = subbeam_count(a.w, p.sub_len);
-            for (uint32_t j = 0; j < n; j++, jb++) {
+            for (uint32_t j = 0; j < n && jb < cap.z; j++, jb++) {
                 float lo, hi, ta, tb;
                 subbeam_range(a.w, j, n, lo, hi, ta, tb);
                 subbeam_box(o, dir, a.w, ta, tb, p.rad_beam, p.box_slack, blo[jb], bhi[jb]);
@@ -117,20 +120,21 @@ __global__ void k_emit_photons(IterParams p, Bufs b, const uint32_t* os, const u
     }
 }
 
-__global__ void k_gather_points(const uint32_t* order, const uint32_t* list, const VertexRec* verts, uint32_t n,
-                                float4* out)
+__global__ void k_gather_points(const uint32_t* order, const uint32_t* list, const VertexRec* verts, uint32_t cap,
+                                const uint32_t* n_dev, float4* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= min(cap, *n_dev)) return;
     const uint32_t slot = list[order[i]];
     const float4 a = verts[slot].a;
     out[i] = make_float4(a.x, a.y, a.z, __int_as_float(int(slot)));
 }
 
-__global__ void k_gather_beams(const uint32_t* order, const float4* geo_u, uint32_t n, float4* out)
+__global__ void k_gather_beams(const uint32_t* order, const float4* geo_u, uint32_t cap, const uint32_t* n_dev,
+                               float4* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= min(cap, *n_dev)) return;
     const size_t j = order[i];
     out[3 * size_t(i)] = geo_u[3 * j];
     out[3 * size_t(i) + 1] = geo_u[3 * j + 1];
@@ -147,6 +151,28 @@ __global__ void k_gather_tris(const uint32_t* order, const float4* tri, const fl
     rec_s[i] = rec[j];
 }
 
+// the photon counts of this iteration (totals of the photon-list scans, tot[0..2]), each clamped
+// to its list's capacity, into n[0..2] for the builds; a total beyond a capacity raises ovf[0],
+// which makes k_accumulate discard the iteration (the host then grows the lists and renders it
+// again)
+__global__ void k_map_counts(const uint32_t* tot, uint3 cap, uint3 want, uint32_t* n, uint32_t* ovf)
+{
+    if (threadIdx.x != 0) return;
+    n[0] = want.x ? min(tot[0], cap.x) : 0u;
+    n[1] = want.y ? min(tot[1], cap.y) : 0u;
+    n[2] = want.z ? min(tot[2], cap.z) : 0u;
+    if ((want.x && tot[0] > cap.x) || (want.y && tot[1] > cap.y) || (want.z && tot[2] > cap.z)) ovf[0] = 1u;
+}
+
+// the counts into the map descriptor (whose pointers the host has just written)
+__global__ void k_map_n(const uint32_t* n, PhotonMaps* m)
+{
+    if (threadIdx.x != 0) return;
+    m->surf.n = n[0];
+    m->med.n = m->wmed.n = n[1];
+    m->beam.n = m->wbeam.n = n[2];
+}
+
 inline uint32_t nblk(size_t n, uint32_t t = 256) { return uint32_t((n + t - 1) / t); }
 
 struct TriStore {
@@ -156,6 +182,14 @@ struct TriStore {
 };
 
 } // namespace
+
+// per-iteration values the device reads, staged in pinned host memory: two slots used in turn,
+// each reused only after the iteration that last used it has consumed it (its event)
+struct Staging {
+    IterParams p;
+    PhotonMaps m;
+    uint32_t ovf; // the iteration's photon-list overflow flag, copied back
+};
 
 struct upbp_renderer {
     std::string err;
@@ -174,6 +208,25 @@ struct upbp_renderer {
     DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals;
     DevBuf guide; // learned proposal records
     DevBuf mislog, miscount; // MIS weight log (upbp_set_mis_log)
+    // device-resident iteration: parameters and photon-map descriptor in device memory, photon
+    // counts (clamped) and the overflow flag; grow-only photon-list capacities
+    DevBuf dparams, dmaps, mapn, ovf;
+    Staging* stage = nullptr;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int pending[2] = {-1, -1}; // iteration whose overflow flag a staging slot holds (-1: none)
+    int slot = 0;
+    bool need_sizing = true;
+    uint32_t cap_s = 0, cap_m = 0, cap_b = 0;
+    PhotonMaps hmaps{};
+    uint64_t gen = 0; // bumped when buffers or settings baked into kernel arguments change
+    // captured iterations (upbp_tuning.graph): one per staging slot, with the generation it was
+    // captured at; eager_gen: the generation of the last iteration run without capture
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    uint64_t gexec_gen[2] = {~0ull, ~0ull};
+    uint64_t eager_gen = ~0ull;
+    uint64_t graph_launches = 0;
+    uint64_t resizes = 0; // photon-list sizing passes
     // online training of the proposal (upbp_set_guide)
     upbp_guide_config gcfg{};
     bool gtrain = false;
@@ -379,6 +432,8 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.merge_bpc = r->tune.merge_blocks_per_cu;
     p.piece_order = r->tune.piece_order;
     p.packet_bounces = r->tune.packet_bounces;
+    p.cap_med = r->cap_m;
+    p.cap_beam = r->cap_b;
     {
         float rq = 0.f;
         if (p.do_pb2d) rq = fmaxf(rq, r_pb2d);
@@ -604,10 +659,20 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
         upbp_destroy(r);
         return fail(nullptr, "device allocation failed", UPBP_E_HIP);
     }
+    if (r->dparams.ensure(sizeof(IterParams)) || r->dmaps.ensure(sizeof(PhotonMaps)) || r->mapn.ensure(16) ||
+        r->ovf.ensure(16) || hipHostMalloc(reinterpret_cast<void**>(&r->stage), 2 * sizeof(Staging)) != hipSuccess) {
+        upbp_destroy(r);
+        return fail(nullptr, "device allocation failed", UPBP_E_HIP);
+    }
+    memset(r->stage, 0, 2 * sizeof(Staging));
+    for (auto& e : r->stage_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipMemsetAsync(r->accum.p, 0, NP * 24, r->stream));
+    HIPCHK(hipMemsetAsync(r->dmaps.p, 0, sizeof(PhotonMaps), r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));
     Bufs& b = r->bufs;
     memset(&b, 0, sizeof(b));
+    b.params = r->dparams.as<IterParams>();
+    b.maps = r->dmaps.as<PhotonMaps>();
     b.lstate = r->lstate.as<PathState>();
     b.lstack = r->lstack.as<StackRec>();
     b.cstate = r->cstate.as<PathState>();
@@ -636,8 +701,12 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
 
 namespace {
 
-// photon lists (path order) -> boxes -> three LBVHs -> sorted payloads
-int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
+// Photon lists (path order) -> boxes -> three LBVHs -> sorted payloads. The lists live at
+// grow-only capacities; their counts stay on the device (k_map_counts), so the build, like the
+// rest of the iteration, needs no host synchronisation. A sizing pass (the first iteration, and
+// the one after a capacity was exceeded) reads the totals back once and sets the capacities with
+// a quarter to spare.
+int build_photon_maps(upbp_renderer* r, const IterParams& p, Staging& st)
 {
     Bufs& b = r->bufs;
     hipStream_t s = r->stream;
@@ -648,52 +717,70 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, uint64_t counts[3])
     uint32_t* cnt = r->cnt.as<uint32_t>();
     uint32_t* off = r->off.as<uint32_t>();
     uint32_t* tot = r->tot.as<uint32_t>();
-    hipLaunchKernelGGL(k_count_photons, dim3(nblk(NL)), dim3(256), 0, s, p, b, cnt, cnt + NL, cnt + 2 * NL, want_pts,
+    hipLaunchKernelGGL(k_count_photons, dim3(nblk(NL)), dim3(256), 0, s, b, cnt, cnt + NL, cnt + 2 * NL, want_pts,
                        want_beams);
     for (int k = 0; k < 3; k++)
         if (scan_exclusive(cnt + size_t(k) * NL, off + size_t(k) * NL, NL, tot + k, r->scratch, s))
             return fail(r, "photon list scan failed", UPBP_E_HIP);
-    uint32_t host_tot[3];
-    HIPCHK(hipMemcpyAsync(host_tot, tot, 12, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const uint32_t ns = want_surf ? host_tot[0] : 0, nm = want_med ? host_tot[1] : 0, nb = want_beams ? host_tot[2] : 0;
-    counts[0] = ns;
-    counts[1] = nm;
-    counts[2] = nb;
-    bool ok = !r->list_s.ensure(std::max<size_t>(ns, 1) * 4) && !r->lo_s.ensure(std::max<size_t>(ns, 1) * 16) &&
-              !r->hi_s.ensure(std::max<size_t>(ns, 1) * 16) && !r->pts_s.ensure(std::max<size_t>(ns, 1) * 16) &&
-              !r->list_m.ensure(std::max<size_t>(nm, 1) * 4) && !r->lo_m.ensure(std::max<size_t>(nm, 1) * 16) &&
-              !r->hi_m.ensure(std::max<size_t>(nm, 1) * 16) && !r->pts_m.ensure(std::max<size_t>(nm, 1) * 16) &&
-              !r->geo_u.ensure(std::max<size_t>(nb, 1) * 48) && !r->lo_b.ensure(std::max<size_t>(nb, 1) * 16) &&
-              !r->hi_b.ensure(std::max<size_t>(nb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(nb, 1) * 48);
+    if (r->need_sizing) {
+        uint32_t host_tot[3];
+        HIPCHK(hipMemcpyAsync(host_tot, tot, 12, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        auto grow = [](uint32_t cap, uint32_t need) {
+            const uint64_t c = std::max<uint64_t>(uint64_t(need) + need / 4, 1024);
+            return uint32_t(std::min<uint64_t>(std::max<uint64_t>(cap, c), 0xffffffffu));
+        };
+        r->cap_s = want_surf ? grow(r->cap_s, host_tot[0]) : r->cap_s;
+        r->cap_m = want_med ? grow(r->cap_m, host_tot[1]) : r->cap_m;
+        r->cap_b = want_beams ? grow(r->cap_b, host_tot[2]) : r->cap_b;
+        r->need_sizing = false;
+        r->resizes++;
+        r->gen++;
+    }
+    const uint32_t cs = want_surf ? r->cap_s : 0, cm = want_med ? r->cap_m : 0, cb = want_beams ? r->cap_b : 0;
+    bool ok = !r->list_s.ensure(std::max<size_t>(cs, 1) * 4) && !r->lo_s.ensure(std::max<size_t>(cs, 1) * 16) &&
+              !r->hi_s.ensure(std::max<size_t>(cs, 1) * 16) && !r->pts_s.ensure(std::max<size_t>(cs, 1) * 16) &&
+              !r->list_m.ensure(std::max<size_t>(cm, 1) * 4) && !r->lo_m.ensure(std::max<size_t>(cm, 1) * 16) &&
+              !r->hi_m.ensure(std::max<size_t>(cm, 1) * 16) && !r->pts_m.ensure(std::max<size_t>(cm, 1) * 16) &&
+              !r->geo_u.ensure(std::max<size_t>(cb, 1) * 48) && !r->lo_b.ensure(std::max<size_t>(cb, 1) * 16) &&
+              !r->hi_b.ensure(std::max<size_t>(cb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(cb, 1) * 48);
     if (!ok) return fail(r, "photon list allocation failed", UPBP_E_HIP);
-    hipLaunchKernelGGL(k_emit_photons, dim3(nblk(NL)), dim3(256), 0, s, p, b, off, off + NL, off + 2 * NL, want_surf,
+    const uint3 cap = make_uint3(cs, cm, cb);
+    hipLaunchKernelGGL(k_emit_photons, dim3(nblk(NL)), dim3(256), 0, s, b, off, off + NL, off + 2 * NL, want_surf,
                        want_med, want_beams, r->list_s.as<uint32_t>(), r->lo_s.as<float4>(), r->hi_s.as<float4>(),
                        r->list_m.as<uint32_t>(), r->lo_m.as<float4>(), r->hi_m.as<float4>(), r->geo_u.as<float4>(),
-                       r->lo_b.as<float4>(), r->hi_b.as<float4>());
-    PhotonMaps& M = b.maps;
-    if (lbvh_build(r->bvh_s, r->scratch, r->lo_s.as<float4>(), r->hi_s.as<float4>(), ns, s) ||
-        lbvh_build(r->bvh_m, r->scratch, r->lo_m.as<float4>(), r->hi_m.as<float4>(), nm, s) ||
-        lbvh_build(r->bvh_b, r->scratch, r->lo_b.as<float4>(), r->hi_b.as<float4>(), nb, s) ||
-        wbvh_build(r->wbvh_m, r->bvh_m, r->scratch, nm, s) || wbvh_build(r->wbvh_b, r->bvh_b, r->scratch, nb, s))
+                       r->lo_b.as<float4>(), r->hi_b.as<float4>(), cap);
+    uint32_t* n = r->mapn.as<uint32_t>();
+    hipLaunchKernelGGL(k_map_counts, dim3(1), dim3(64), 0, s, tot, cap, make_uint3(want_surf, want_med, want_beams), n,
+                       r->ovf.as<uint32_t>());
+    if (lbvh_build(r->bvh_s, r->scratch, r->lo_s.as<float4>(), r->hi_s.as<float4>(), cs, s, n) ||
+        lbvh_build(r->bvh_m, r->scratch, r->lo_m.as<float4>(), r->hi_m.as<float4>(), cm, s, n + 1) ||
+        lbvh_build(r->bvh_b, r->scratch, r->lo_b.as<float4>(), r->hi_b.as<float4>(), cb, s, n + 2) ||
+        wbvh_build(r->wbvh_m, r->bvh_m, r->scratch, cm, s, n + 1) ||
+        wbvh_build(r->wbvh_b, r->bvh_b, r->scratch, cb, s, n + 2))
         return fail(r, "photon LBVH build failed", UPBP_E_HIP);
-    if (ns)
-        hipLaunchKernelGGL(k_gather_points, dim3(nblk(ns)), dim3(256), 0, s, r->bvh_s.order, r->list_s.as<uint32_t>(),
-                           b.verts, ns, r->pts_s.as<float4>());
-    if (nm)
-        hipLaunchKernelGGL(k_gather_points, dim3(nblk(nm)), dim3(256), 0, s, r->bvh_m.order, r->list_m.as<uint32_t>(),
-                           b.verts, nm, r->pts_m.as<float4>());
-    if (nb)
-        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(nb)), dim3(256), 0, s, r->bvh_b.order, r->geo_u.as<float4>(), nb,
-                           r->geo_b.as<float4>());
-    M.surf = {r->bvh_s.nodes, r->bvh_s.order, ns};
-    M.med = {r->bvh_m.nodes, r->bvh_m.order, nm};
-    M.beam = {r->bvh_b.nodes, r->bvh_b.order, nb};
-    M.wmed = {r->wbvh_m.nodes, nm};
-    M.wbeam = {r->wbvh_b.nodes, nb};
+    // the descriptor: pointers and capacities from the host, counts from the device
+    PhotonMaps& M = r->hmaps;
+    M.surf = {r->bvh_s.nodes, r->bvh_s.order, cs};
+    M.med = {r->bvh_m.nodes, r->bvh_m.order, cm};
+    M.beam = {r->bvh_b.nodes, r->bvh_b.order, cb};
+    M.wmed = {r->wbvh_m.nodes, cm};
+    M.wbeam = {r->wbvh_b.nodes, cb};
     M.surf_pts = r->pts_s.as<float4>();
     M.med_pts = r->pts_m.as<float4>();
     M.beam_geo = r->geo_b.as<float4>();
+    st.m = M;
+    HIPCHK(hipMemcpyAsync(r->dmaps.p, &st.m, sizeof(PhotonMaps), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_map_n, dim3(1), dim3(64), 0, s, n, r->dmaps.as<PhotonMaps>());
+    if (cs)
+        hipLaunchKernelGGL(k_gather_points, dim3(nblk(cs)), dim3(256), 0, s, r->bvh_s.order, r->list_s.as<uint32_t>(),
+                           b.verts, cs, n, r->pts_s.as<float4>());
+    if (cm)
+        hipLaunchKernelGGL(k_gather_points, dim3(nblk(cm)), dim3(256), 0, s, r->bvh_m.order, r->list_m.as<uint32_t>(),
+                           b.verts, cm, n + 1, r->pts_m.as<float4>());
+    if (cb)
+        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(cb)), dim3(256), 0, s, r->bvh_b.order, r->geo_u.as<float4>(), cb,
+                           n + 2, r->geo_b.as<float4>());
     return 0;
 }
 
@@ -734,6 +821,7 @@ static int train_guide(upbp_renderer* r)
     }
     if (g.steps_per_iteration > 0) {
         if (r->gnet->records(r->sc.guide.res, r->guide.as<float4>(), s)) return fail(r, "guide refresh failed", UPBP_E_HIP);
+        if (r->sc.guide.alpha != g.alpha || r->sc.guide.alpha_s != g.surface_alpha) r->gen++;
         r->sc.guide.alpha = g.alpha;
         r->sc.guide.alpha_s = g.surface_alpha;
         r->gloss = loss / double(g.steps_per_iteration);
@@ -744,17 +832,22 @@ static int train_guide(upbp_renderer* r)
     return UPBP_OK;
 }
 
-extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
+namespace {
+
+int settle(upbp_renderer* r, bool wait);
+
+// Enqueues one iteration on r's stream. Nothing in it waits for the device, except a sizing
+// pass of the photon lists and the statistics (stats != null), which synchronise.
+// The work of one iteration on r's stream, reading its parameters from the staging slot st.
+// p.cap_* are set to the photon-list capacities the launches below depend on.
+int enqueue_body(upbp_renderer* r, IterParams& p, Staging& st, bool timed)
 {
-    if (!r) return fail(r, "null renderer", UPBP_E_ARG);
-    IterParams p;
-    setup_params(r, iteration, p);
     hipStream_t s = r->stream;
     Bufs& b = r->bufs;
     const int L = int(r->cfg.max_path_length);
-    const bool timed = stats != nullptr; // per-kernel events only when statistics are asked for
     hipEvent_t* ev = r->ev;
-    HIPCHK(hipEventRecord(ev[0], s));
+    if (timed) HIPCHK(hipEventRecord(ev[0], s));
+    HIPCHK(hipMemcpyAsync(r->dparams.p, &st.p, sizeof(IterParams), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill, dim3(nblk(kQcountWords)), dim3(256), 0, s, b.qcount, 0u, size_t(kQcountWords));
     hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(kMaxBounces) * kXWork)), dim3(256), 0, s, b.xwork, 0u,
                        size_t(kMaxBounces) * kXWork);
@@ -764,6 +857,8 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
                        reinterpret_cast<uint32_t*>(b.mcolor), 0u, 3 * size_t(r->n_pixels));
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(b.stats), 0u,
                        size_t(2 * STAT_COUNT));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->ovf.as<uint32_t>(), 0u, size_t(1));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->mapn.as<uint32_t>(), 0u, size_t(3));
     if (b.mislog) hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, b.miscount, 0u, size_t(1));
     if (p.trace_light) {
         launch_light_generate(r->sc, p, b, s);
@@ -772,16 +867,19 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         hipLaunchKernelGGL(k_fill, dim3(nblk(r->n_light)), dim3(256), 0, s, b.nvert, 0u, size_t(r->n_light));
         hipLaunchKernelGGL(k_fill, dim3(nblk(r->n_light)), dim3(256), 0, s, b.nbeam, 0u, size_t(r->n_light));
     }
-    HIPCHK(hipEventRecord(ev[1], s));
-    uint64_t counts[3] = {0, 0, 0};
+    if (timed) HIPCHK(hipEventRecord(ev[1], s));
     if (p.trace_light) {
-        const int e = build_photon_maps(r, p, counts);
+        const int e = build_photon_maps(r, p, st);
         if (e) return e;
+        // launch decisions below use the capacities a sizing pass may just have set
+        p.cap_med = (p.build_pp3d || p.build_pb2d) ? r->cap_m : 0u;
+        p.cap_beam = (p.build_bb1d || p.build_bp2d) && r->want_beams ? r->cap_b : 0u;
     } else {
-        b.maps = PhotonMaps();
-        memset(&b.maps, 0, sizeof(b.maps));
+        p.cap_med = p.cap_beam = 0u;
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->dmaps.as<uint32_t>(), 0u,
+                           sizeof(PhotonMaps) / sizeof(uint32_t));
     }
-    HIPCHK(hipEventRecord(ev[2], s));
+    if (timed) HIPCHK(hipEventRecord(ev[2], s));
     const bool collect = r->gtrain && p.trace_camera;
     b.gpend = collect ? r->gpend.as<float>() : nullptr;
     b.graw = collect ? r->graw.as<float>() : nullptr;
@@ -806,7 +904,7 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
             if (timed) HIPCHK(hipEventRecord(e[1], s));
             launch_seg_merge(r->sc, p, b, 0, k, s);
             if (timed) HIPCHK(hipEventRecord(e[2], s));
-            if (p.piece_order == 2 && p.build_bb1d && b.maps.beam.n > 0) {
+            if (p.piece_order == 2 && p.build_bb1d && p.cap_beam > 0) {
                 // Morton order of the pieces: count to the host, sort (key, index), gather
                 uint32_t cnt = 0;
                 HIPCHK(hipMemcpyAsync(&cnt, b.qcount + 320 + k, 4, hipMemcpyDeviceToHost, s));
@@ -823,15 +921,80 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
             if (timed) HIPCHK(hipEventRecord(e[3], s));
         }
     }
-    launch_camera_finish(p, b, float(r->cfg.res_x), float(r->cfg.res_y), s);
-    HIPCHK(hipEventRecord(ev[3], s));
+    launch_camera_finish(p, b, float(r->cfg.res_x), float(r->cfg.res_y), r->ovf.as<uint32_t>(), s);
+    if (timed) HIPCHK(hipEventRecord(ev[3], s));
     HIPCHK(hipGetLastError());
     if (collect) { // train the proposal on this iteration's samples, then refresh its records
         const int e = train_guide(r);
         if (e) return e;
     }
+    // the overflow flag comes back with the slot; settle() checks it once the slot's event passed
+    HIPCHK(hipMemcpyAsync(&st.ovf, r->ovf.p, 4, hipMemcpyDeviceToHost, s));
+    return UPBP_OK;
+}
+
+// Enqueues one iteration on r's stream. Nothing in it waits for the device, except a sizing
+// pass of the photon lists, online training of the proposal and the statistics (stats != null).
+int run_one(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
+{
+    IterParams p;
+    setup_params(r, iteration, p);
+    hipStream_t s = r->stream;
+    const int L = int(r->cfg.max_path_length);
+    const bool timed = stats != nullptr; // per-kernel events only when statistics are asked for
+    hipEvent_t* ev = r->ev;
+    // a staging slot is reused only once the iteration that used it last has read it
+    const int sl = r->slot;
+    r->slot ^= 1;
+    if (r->pending[sl] >= 0) {
+        const int e = settle(r, true);
+        if (e) return e;
+    }
+    Staging& st = r->stage[sl];
+    st.p = p;
+    st.ovf = 0;
+    // Nothing in the body reads back to the host when no statistics are asked for, the photon
+    // lists are sized, the proposal is not training and pieces are not Morton-sorted. Then the
+    // body is captured once -- after one eager iteration with the same settings, which allocated
+    // every buffer -- and replayed; the replay reads this iteration's parameters from the slot.
+    const bool capturable = r->use_graph && !timed && !r->need_sizing && !r->gtrain && p.piece_order != 2 &&
+                            p.trace_light && p.trace_camera;
+    if (capturable && r->eager_gen == r->gen) {
+        if (!r->gexec[sl] || r->gexec_gen[sl] != r->gen) {
+            if (r->gexec[sl]) (void)hipGraphExecDestroy(r->gexec[sl]);
+            r->gexec[sl] = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int e = enqueue_body(r, p, st, false);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(s, &g);
+            if (e || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return e ? e : fail(r, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce), UPBP_E_HIP);
+            }
+            const hipError_t ie = hipGraphInstantiate(&r->gexec[sl], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail(r, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie), UPBP_E_HIP);
+            r->gexec_gen[sl] = r->gen;
+        }
+        HIPCHK(hipGraphLaunch(r->gexec[sl], s));
+        r->graph_launches++;
+    } else {
+        const int e = enqueue_body(r, p, st, timed);
+        if (e) return e;
+        r->eager_gen = r->gen;
+    }
+    HIPCHK(hipEventRecord(r->stage_ev[sl], s));
+    r->pending[sl] = iteration;
     if (stats) {
+        const Bufs& b = r->bufs;
         HIPCHK(hipEventSynchronize(ev[3]));
+        uint32_t overflowed = 0;
+        HIPCHK(hipMemcpy(&overflowed, r->ovf.p, 4, hipMemcpyDeviceToHost));
+        if (overflowed) { // not accumulated: resize the photon lists and render it again
+            r->pending[sl] = -1;
+            r->need_sizing = true;
+            return run_one(r, iteration, stats);
+        }
         memset(stats, 0, sizeof(*stats));
         uint32_t q[kQcountWords];
         unsigned long long st[STAT_COUNT];
@@ -852,9 +1015,11 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
                 stats->ms_merge_pb2d += elapsed(e[1], e[2]);
                 stats->ms_merge_bb1d += elapsed(e[2], e[3]);
             }
+        uint32_t counts[3] = {0, 0, 0};
+        if (p.trace_light) HIPCHK(hipMemcpy(counts, r->mapn.p, 12, hipMemcpyDeviceToHost));
         stats->surface_vertices = counts[0];
         stats->medium_vertices = counts[1];
-        stats->light_vertices = counts[0] + counts[1];
+        stats->light_vertices = uint64_t(counts[0]) + counts[1];
         stats->subbeams = counts[2];
         stats->beams = 0;
         {
@@ -880,6 +1045,8 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
         stats->beam_drops = st[STAT_BDROP];
         stats->path_errors = st[STAT_ERRORS];
         stats->merge_stack_overflows = st[STAT_MERGE_STACK];
+        stats->photon_list_resizes = r->resizes;
+        stats->graph_launches = r->graph_launches;
         // Compulsory HBM bytes. Camera bounce kernels: 128-byte state read and write plus 64
         // bytes of stack per live path-step, 112 bytes per light vertex read by a connection or
         // an accepted point merge, 112 bytes per segment query written. Segment merge kernels:
@@ -914,9 +1081,50 @@ extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_sta
     return UPBP_OK;
 }
 
+// Checks the iterations whose overflow flags are back (all of them, waiting, when `wait`): one
+// that overflowed a photon list was not accumulated, so the lists are resized from its totals and
+// it is rendered again.
+int settle(upbp_renderer* r, bool wait)
+{
+    for (int k = 0; k < 2; k++) {
+        const int it = r->pending[k];
+        if (it < 0) continue;
+        if (wait) HIPCHK(hipEventSynchronize(r->stage_ev[k]));
+        else if (hipEventQuery(r->stage_ev[k]) != hipSuccess) continue;
+        r->pending[k] = -1;
+        if (r->stage[k].ovf) {
+            r->need_sizing = true;
+            const int e = run_one(r, it, nullptr);
+            if (e) return e;
+        }
+    }
+    return UPBP_OK;
+}
+
+} // namespace
+
+extern "C" int upbp_run_iteration(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
+{
+    if (!r) return fail(r, "null renderer", UPBP_E_ARG);
+    const int e = settle(r, false);
+    return e ? e : run_one(r, iteration, stats);
+}
+
+// waits for every iteration enqueued so far and renders again any that overflowed a photon list
+static int settle_all(upbp_renderer* r)
+{
+    while (r->pending[0] >= 0 || r->pending[1] >= 0) {
+        const int e = settle(r, true);
+        if (e) return e;
+    }
+    return UPBP_OK;
+}
+
 extern "C" int upbp_read_accum(upbp_renderer* r, double* rgb, int iters_done)
 {
     if (!r || !rgb || iters_done < 1) return fail(r, "invalid argument", UPBP_E_ARG);
+    const int e = settle_all(r);
+    if (e) return e;
     HIPCHK(hipStreamSynchronize(r->stream));
     const size_t n = 3 * size_t(r->n_pixels);
     HIPCHK(hipMemcpy(rgb, r->bufs.accum, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -933,6 +1141,7 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     t->query_piece_factor = 64.f;
     t->merge_blocks_per_cu = 0;
     t->packet_bounces = 1;
+    t->graph = 0;
 }
 
 extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
@@ -940,6 +1149,7 @@ extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_
 {
     if (!r) return UPBP_E_ARG;
     if (!(alpha >= 0.f && alpha <= 1.f)) return fail(r, "guide alpha must lie in [0, 1]", UPBP_E_ARG);
+    r->gen++;
     if (alpha == 0.f) {
         r->sc.guide.alpha = r->sc.guide.alpha_s = 0.f;
         return UPBP_OK;
@@ -959,6 +1169,7 @@ extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_
     g.lo = F3(lo);
     g.inv_cell = mk(float(res) / (hi[0] - lo[0]), float(res) / (hi[1] - lo[1]), float(res) / (hi[2] - lo[2]));
     g.alpha = g.alpha_s = alpha;
+    r->gen++;
     return UPBP_OK;
 }
 
@@ -987,6 +1198,7 @@ extern "C" int upbp_set_guide(upbp_renderer* r, const upbp_guide_config* g)
         (g->history != 0 && g->history < 16))
         return fail(r, "invalid guide configuration", UPBP_E_ARG);
     r->gcfg = *g;
+    r->gen++;
     delete r->gnet;
     r->gnet = nullptr;
     r->gtrain = false;
@@ -1033,6 +1245,8 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
         t->piece_order < 0 || t->piece_order > 2 || t->packet_bounces < 0)
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
+    r->use_graph = t->graph != 0;
+    r->gen++;
     return UPBP_OK;
 }
 
@@ -1054,6 +1268,7 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
         ~Release() { b.release(); }
     } release_saved{saved};
     if (saved.ensure(acc_bytes)) return fail(r, "autotune allocation failed", UPBP_E_HIP);
+    if (const int es = settle_all(r)) return es;
     HIPCHK(hipMemcpyAsync(saved.p, r->accum.p, acc_bytes, hipMemcpyDeviceToDevice, r->stream));
     const bool train = r->gtrain;
     r->gtrain = false;
@@ -1096,6 +1311,7 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
             }
         }
     }
+    if (!e) e = settle_all(r);
     r->tune = e ? start : best;
     r->gtrain = train;
     if (hipMemcpyAsync(r->accum.p, saved.p, acc_bytes, hipMemcpyDeviceToDevice, r->stream) != hipSuccess ||
@@ -1130,7 +1346,7 @@ extern "C" int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iteration
         if (hipSetDevice(device) != hipSuccess) return;
         for (int32_t i = next++; i < n; i = next++) {
             rc[size_t(i)] = upbp_run_iteration(rs[i], iterations[i], nullptr);
-            if (!rc[size_t(i)] && hipStreamSynchronize(rs[i]->stream) != hipSuccess) rc[size_t(i)] = UPBP_E_HIP;
+            if (!rc[size_t(i)]) rc[size_t(i)] = upbp_sync(rs[i]);
         }
     };
     const int32_t t = std::min(threads, std::max<int32_t>(n, 1));
@@ -1154,6 +1370,7 @@ extern "C" int upbp_set_mis_log(upbp_renderer* r, uint32_t capacity, uint32_t pi
         b.mislog = nullptr;
         b.miscount = nullptr;
         b.miscap = 0;
+        r->gen++;
         return UPBP_OK;
     }
     if (pixel_stride == 0 || capacity > (1u << 28)) return fail(r, "invalid MIS log arguments", UPBP_E_ARG);
@@ -1166,6 +1383,7 @@ extern "C" int upbp_set_mis_log(upbp_renderer* r, uint32_t capacity, uint32_t pi
     b.miscap = capacity;
     b.mis_stride = pixel_stride;
     b.mis_lt = light_paths;
+    r->gen++;
     return UPBP_OK;
 }
 
@@ -1197,6 +1415,8 @@ extern "C" int upbp_memory_info(uint64_t* free_bytes, uint64_t* total_bytes)
 extern "C" int upbp_sync(upbp_renderer* r)
 {
     if (!r) return fail(r, "null renderer", UPBP_E_ARG);
+    const int e = settle_all(r);
+    if (e) return e;
     HIPCHK(hipStreamSynchronize(r->stream));
     return UPBP_OK;
 }
@@ -1206,6 +1426,8 @@ extern "C" void upbp_destroy(upbp_renderer* r)
     if (!r) return;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (DevBuf& d : r->owned) d.release();
+    for (auto& g : r->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     for (TriStore* t : {&r->real, &r->imag}) {
         if (t->tri) (void)hipFree(t->tri);
         if (t->nrm) (void)hipFree(t->nrm);
@@ -1217,8 +1439,11 @@ extern "C" void upbp_destroy(upbp_renderer* r)
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
                       &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
                       &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount,
-                      &r->mislog, &r->miscount})
+                      &r->mislog, &r->miscount, &r->dparams, &r->dmaps, &r->mapn, &r->ovf})
         d->release();
+    if (r->stage) (void)hipHostFree(r->stage);
+    for (auto& e : r->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     delete r->gnet;
     r->gnet = nullptr;
     lbvh_free(r->bvh_s);

@@ -188,6 +188,10 @@ typedef struct upbp_iter_stats {
     double ms_guide;               /* training and record refresh time */
     uint64_t merge_stack_overflows; /* segment-merge query pieces abandoned on a full traversal
                                        stack (also counted in path_errors) */
+    uint64_t photon_list_resizes;   /* sizing passes of the photon lists so far (the first
+                                       iteration, then one per iteration that outgrew them and
+                                       was rendered again) */
+    uint64_t graph_launches;        /* iterations replayed from a captured graph so far */
 } upbp_iter_stats;
 
 typedef struct upbp_scene_handle upbp_scene_handle;
@@ -286,6 +290,9 @@ typedef struct upbp_tuning {
     int32_t packet_bounces;      /* camera bounces below this walk the B-B1D beam tree as packets
                                     of 64 query pieces per wave (one node fetch per wave); later
                                     bounces, whose rays are less coherent, walk it per piece */
+    int32_t graph;               /* 1: capture the iteration's work in a HIP graph once and replay it
+                                    (iterations without statistics, online training or Morton piece
+                                    order, which read back to the host); 0: launch kernel by kernel */
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
