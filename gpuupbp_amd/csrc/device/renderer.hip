@@ -1140,7 +1140,7 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     t->subbeam_factor = 12.f;
     t->query_piece_factor = 64.f;
     t->merge_blocks_per_cu = 0;
-    t->packet_bounces = 1;
+    t->packet_bounces = 0;
     t->graph = 0;
 }
 

@@ -289,7 +289,8 @@ typedef struct upbp_tuning {
                                     piece midpoint (one host synchronisation per bounce) */
     int32_t packet_bounces;      /* camera bounces below this walk the B-B1D beam tree as packets
                                     of 64 query pieces per wave (one node fetch per wave); later
-                                    bounces, whose rays are less coherent, walk it per piece */
+                                    bounces walk it per piece (default 0: per piece everywhere,
+                                    the faster of the two on the blob grid) */
     int32_t graph;               /* 1: capture the iteration's work in a HIP graph once and replay it
                                     (iterations without statistics, online training or Morton piece
                                     order, which read back to the host); 0: launch kernel by kernel */

@@ -4,7 +4,7 @@
   oracle at full size (path for path on the first iteration, converged relMSE over all four).
 * Blob grid (1,310,720 triangles, three media, 900k light paths at 1280x720): the full scene at
   1/8 the resolution per axis (1/64 of the light paths) path for path against the oracle; at full
-  size, the light pass's photon and beam counts equal the oracle's, and no path error, drop or
+  size, the light pass's photon and beam counts agree with the oracle's, and no path error, drop or
   abandoned merge piece occurs.
 * Slab room (194,970 triangles, dense g = 0.9 slab) with the learned proposal on: path for path
   against the oracle with fixed proposal lobes at reduced resolution; at full size with online
@@ -18,7 +18,11 @@
 "Path for path" means the same random streams on both sides: all but a few percent of the pixels
 agree to FP32 rounding (paths whose transcendental results differ in the last bit may take another
 turn), and the remaining pixels are bounded through the relMSE between the two images, which must
-stay below a tenth of the relMSE between two independent renders of the same iteration count.
+stay below a tenth of the relMSE between two independent renders of the same iteration count. In
+the blob and stress scenes one light path that takes another turn changes every pixel its beams
+cross in the large fog spheres, so there up to 15% of the pixels may differ beyond 1e-3 while the
+image's relMSE must stay below a thousandth of the independent renders'. Photon counts of a light
+pass agree to 1e-4 (a few diverged paths in a million).
 """
 import os
 
@@ -65,7 +69,7 @@ def _oracle(sh, cfg, iters=1, guide_lobes=None, ioff=0):
     return img, st
 
 
-def _path_parity(g, o, o_indep, frac=0.95):
+def _path_parity(g, o, o_indep, frac=0.95, ratio=0.1):
     """Same-stream images agree pixel for pixel except for a few diverged paths, whose effect on
     the whole image stays far below that of independent sampling."""
     rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
@@ -73,7 +77,14 @@ def _path_parity(g, o, o_indep, frac=0.95):
     r_same, r_indep = relmse(g, o), relmse(o_indep, o)
     print("pixels within 1e-3: %.4f, relMSE same streams %.3g, independent %.3g" % (same, r_same, r_indep))
     assert same > frac, same
-    assert r_same < 0.1 * r_indep, (r_same, r_indep)
+    assert r_same < ratio * r_indep, (r_same, r_indep)
+
+
+def _same_counts(gpu_n, oracle_n):
+    """Photon counts of one light pass on the same random streams. FP32 transcendental results
+    that differ in the last bit between the GPU and the host let a handful of the light paths
+    take another turn (a few in a million at these sizes), so the counts agree to 1e-4."""
+    assert abs(int(gpu_n) - int(oracle_n)) <= max(2, 1e-4 * oracle_n), (gpu_n, oracle_n)
 
 
 def _no_errors(stats):
@@ -119,9 +130,13 @@ def test_blobs_full_scene_reduced_resolution_path_parity(workdir):
     o, ost = _oracle(sh, cfg)
     o2, _ = _oracle(sh, cfg, ioff=1 << 16)
     _no_errors(st)
-    assert st[0].surface_vertices + st[0].medium_vertices == ost.light_vertices and st[0].beams == ost.beams
+    _same_counts(st[0].surface_vertices + st[0].medium_vertices, ost.light_vertices)
+    _same_counts(st[0].beams, ost.beams)
     assert st[0].bb1d_accepts > 0 and st[0].pb2d_accepts > 0
-    _path_parity(g, o, o2)
+    # one light path that takes another turn moves every pixel its beams cross in the fog
+    # spheres: fewer pixels agree to 1e-3 than in small scenes, but the image as a whole stays
+    # within a thousandth of the independent-sampling error
+    _path_parity(g, o, o2, 0.85, 1e-3)
     sh.close()
 
 
@@ -134,8 +149,8 @@ def test_blobs_full_size_counts_and_errors(workdir):
     _, ost = oracle_py.render_sample_times(sh.ptr, cfg, 0, int(cfg.path_count_per_iter), want_stats=True)
     print("GPU points %d beams %d; oracle %d %d" % (st[0].surface_vertices + st[0].medium_vertices, st[0].beams,
                                                    ost.light_vertices, ost.beams))
-    assert st[0].surface_vertices + st[0].medium_vertices == ost.light_vertices
-    assert st[0].beams == ost.beams
+    _same_counts(st[0].surface_vertices + st[0].medium_vertices, ost.light_vertices)
+    _same_counts(st[0].beams, ost.beams)
     sh.close()
 
 
@@ -173,9 +188,10 @@ def test_stress_full_scene_reduced_resolution_path_parity(workdir):
     g, st, sh, cfg = _gpu(obj, args)
     o, ost = _oracle(sh, cfg)
     o2, _ = _oracle(sh, cfg, ioff=1 << 16)
-    assert st[0].surface_vertices + st[0].medium_vertices == ost.light_vertices and st[0].beams == ost.beams
-    assert st[0].path_errors == ost.path_errors
-    _path_parity(g, o, o2, 0.93)
+    _same_counts(st[0].surface_vertices + st[0].medium_vertices, ost.light_vertices)
+    _same_counts(st[0].beams, ost.beams)
+    assert abs(int(st[0].path_errors) - int(ost.path_errors)) <= max(2, 0.01 * ost.path_errors)
+    _path_parity(g, o, o2, 0.85, 1e-3)
     sh.close()
 
 
@@ -189,7 +205,8 @@ def test_stress_full_size_bounds(workdir):
     print("stress: %d path errors of %d paths" % (s.path_errors, paths))
     assert s.path_errors < 2e-3 * paths
     _, ost = oracle_py.render_sample_times(sh.ptr, cfg, 0, int(cfg.path_count_per_iter), want_stats=True)
-    assert s.surface_vertices + s.medium_vertices == ost.light_vertices and s.beams == ost.beams
+    _same_counts(s.surface_vertices + s.medium_vertices, ost.light_vertices)
+    _same_counts(s.beams, ost.beams)
     sh.close()
 
 

@@ -53,13 +53,15 @@ def test_graph_replays_match_eager_iterations(scene_dir, kind):
 
 @pytest.mark.parametrize("graph", [0, 1])
 def test_photon_lists_outgrow_their_capacity(scene_dir, graph):
-    # The B-B1D radius shrinks with the iteration (-r_alpha_bb1d 0.5), so beams are cut into more
-    # sub-beams every iteration and outgrow the capacity the first iteration sized: those
-    # iterations are discarded on the device, the lists resized and the iterations rendered again.
-    # The same iterations in reverse order size for the largest count first and never overflow;
-    # both renderers must end with the same image.
+    # The B-B1D radius shrinks with the iteration (-r_alpha_bb1d 0.5: r0 / sqrt(1 + i)), and
+    # without B-P2D (whose radius would otherwise set the sub-beam length) beams are cut into more
+    # sub-beams every iteration and outgrow the capacity the first iteration sized (a quarter to
+    # spare): those iterations are discarded on the device, the lists resized and the iterations
+    # rendered again. The same iterations in reverse order size for the largest count first and
+    # never overflow; both renderers must end with the same image.
     obj = str(scene_dir / "boxfog" / "boxfog.obj")
-    args = box_args("48x48", extra=("-r_initial", "0.03", "-r_alpha_bb1d", "0.5"))
+    args = box_args("48x48", alg="upbp_bpt+surf+pp3d+pb2d+bb1d",
+                    extra=("-r_initial", "0.03", "-r_alpha_bb1d", "0.5"))
     rc, cfg = capi.parse_args(["-s", "-1", obj] + args)
     assert rc == 0
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
