@@ -9,16 +9,18 @@ length 12, 900,000 light paths per iteration, radius 0.005, upbp_all
 One step = one 1-spp iteration (light pass, photon LBVH builds, camera pass, FP64
 accumulate) with the scene resident in HBM. The default run then measures the learned
 proposal against the plain renderer at equal wall-clock time (60 s each) on the slab room,
-against the committed plain reference render (gpuupbp_amd/data/). Prints one JSON line; it
-also carries the committed measurements of the longer runs below (profiles/*.json).
+against the committed plain reference render (gpuupbp_amd/data/), and the 64 batched jobs.
+Prints one JSON line; it also carries the committed stress measurement (profiles/stress_4k.json).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned] [--autotune]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-learned] [--no-batch]
+                       [--autotune]
 
 Other measurements (each prints its own JSON line):
   --learned SECONDS                    only the learned-vs-plain comparison (guide settings:
                                        --alpha, --surface-alpha, --guide-res, --guide-steps,
                                        --guide-lr, --guide-history; comma lists compare several)
-  --batch JOBS                         batched independent jobs (upbp_run_batch)
+  --batch JOBS                         batched independent jobs: one after another, from host
+                                       threads (upbp_run_batch), one launch per stage (upbp_batch_*)
   --stress                             3840x2160, 2.05M triangles, 8 media, 8M light paths
   --reference-chunk FIRST:LAST         render part of the slab-room reference (GPU), then
   --reference-merge CHUNK...           join the parts into gpuupbp_amd/data/ (no GPU)
@@ -83,11 +85,13 @@ def measured_traffic(kernel):
 def batch_jobs(workdir, n_jobs=64, iters=4, threads=8):
     """Batched jobs: the n_jobs independent renderers of the seed-2 mix (gpuupbp_amd/workloads.py:
     box, fog box, blob grids and slab rooms of 10 to 195k triangles and 1 to 4 media, resolution
-    in {64, 128, 256, 512}^2, max length in [4, 16]), each rendering `iters` iterations, one after
-    another and then all at once through upbp_run_batch. Returns the measurement."""
+    in {64, 128, 256, 512}^2, max length in [4, 16]). Each renders `iters` iterations per mode:
+    one job after another, all at once from host threads (upbp_run_batch), and in rounds of one
+    launch per stage over all jobs (upbp_batch_run), eagerly and replayed from the captured HIP
+    graph. Returns the measurement."""
     from gpuupbp_amd import capi
     from gpuupbp_amd import workloads as wl
-    from gpuupbp_amd.renderer import Renderer, run_batch
+    from gpuupbp_amd.renderer import Batch, Renderer, run_batch
     jobs = []
     for obj, argv, px, kind, length in wl.batch_mix(workdir, n_jobs, 2):
         rc, cfg = capi.parse_args(argv)
@@ -99,24 +103,44 @@ def batch_jobs(workdir, n_jobs=64, iters=4, threads=8):
     run_batch(rs, [0] * len(rs), threads)  # warm-up: buffers grow to their sizes
     for r in rs:
         r.sync()
+    it = 1
     t0 = time.perf_counter()
-    for i in range(1, iters + 1):
+    for _ in range(iters):
         for r in rs:
-            r.run_iteration(i, want_stats=False)
+            r.run_iteration(it, want_stats=False)
             r.sync()
+        it += 1
     t_seq = time.perf_counter() - t0
     t0 = time.perf_counter()
-    for i in range(iters + 1, 2 * iters + 1):
-        run_batch(rs, [i] * len(rs), threads)
-    t_batch = time.perf_counter() - t0
+    for _ in range(iters):
+        run_batch(rs, [it] * len(rs), threads)
+        it += 1
+    t_thr = time.perf_counter() - t0
+    res = {}
+    for label, graph in (("eager", 0), ("graph", 1)):
+        with Batch(rs, use_graph=graph) as bt:
+            for _ in range(2):  # warm-up rounds: an eager one, then the capture
+                bt.run([it] * len(rs))
+                it += 1
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                bt.run([it] * len(rs))
+                it += 1
+            res[label] = (time.perf_counter() - t0, bt.counts())
     for sh, r, _, _, _ in jobs:
         r.close()
         sh.close()
     mix = {k: sum(1 for _, _, _, kk, _ in jobs if kk == k) for k in wl.BATCH_KINDS}
+    t_eager, c_eager = res["eager"]
+    t_graph, c_graph = res["graph"]
     return {"jobs": n_jobs, "iterations_per_job": iters, "threads": threads, "job_mix": mix,
-            "pixels_per_round": pixels, "seconds_sequential": round(t_seq, 3), "seconds_batched": round(t_batch, 3),
-            "samples_per_s_sequential": pixels * iters / t_seq, "samples_per_s_batched": pixels * iters / t_batch,
-            "speedup": t_seq / t_batch}
+            "pixels_per_round": pixels, "seconds_sequential": round(t_seq, 3), "seconds_threads": round(t_thr, 3),
+            "seconds_stage_launches": round(t_eager, 3), "seconds_stage_launches_graph": round(t_graph, 3),
+            "samples_per_s_sequential": pixels * iters / t_seq, "samples_per_s_threads": pixels * iters / t_thr,
+            "samples_per_s_stage_launches": pixels * iters / t_eager,
+            "samples_per_s_stage_launches_graph": pixels * iters / t_graph,
+            "speedup_threads": t_seq / t_thr, "speedup_stage_launches_graph": t_seq / t_graph,
+            "rounds_graph_replays_eager_alone": {"eager": list(c_eager), "graph": list(c_graph)}}
 
 
 def stress(workdir, steps=3, warmup=1):
@@ -315,8 +339,10 @@ def main():
                     help="size of the plain copy kernel run once for bandwidth and counter calibration (0: skip)")
     ap.add_argument("--stress", action="store_true", help="combined stress workload (3840x2160, 2M tris, 8 media)")
     ap.add_argument("--batch", type=int, default=None, metavar="JOBS",
-                    help="batched-jobs measurement over JOBS independent renderers (upbp_run_batch)")
+                    help="batched-jobs measurement over JOBS independent renderers")
     ap.add_argument("--batch-threads", type=int, default=8)
+    ap.add_argument("--no-batch", dest="batch_in_default", action="store_false",
+                    help="skip the batched-jobs measurement of the default run")
     ap.add_argument("--learned", type=float, default=None,
                     help="only the learned-vs-plain equal-time comparison (slab room), SECONDS per renderer; "
                          "guide settings below (comma lists compare several against one plain run)")
@@ -486,12 +512,10 @@ def main():
     print("bench: GPU done, %.1f ms per iteration" % ms, file=sys.stderr, flush=True)
     if not a.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(workdir, a.learned_seconds)
-    batch = committed("batched_jobs.json")
-    if batch:
-        best = max(batch["runs"], key=lambda x: x["speedup"])
-        out["batched_jobs"] = {k: best[k] for k in ("jobs", "threads", "samples_per_s_sequential",
-                                                    "samples_per_s_batched", "speedup")}
-        out["batched_jobs"]["source"] = "profiles/batched_jobs.md (python bench.py --batch 64)"
+    if a.batch_in_default:
+        print("bench: batched jobs", file=sys.stderr, flush=True)
+        out["batched_jobs"] = batch_jobs(workdir, n_jobs=64, iters=3, threads=8)
+        out["batched_jobs"]["measured"] = "in this run"
     stress_run = committed("stress_4k.json")
     if stress_run:
         out["stress"] = {"workload": stress_run["workload"], "ms_per_iteration": stress_run["plain"]["ms_per_iteration"],

@@ -93,7 +93,7 @@ class Scene(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("subbeam_factor", C.c_float), ("query_piece_factor", C.c_float),
                 ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32), ("packet_bounces", C.c_int32),
-                ("graph", C.c_int32)]
+                ("beam_leaf", C.c_int32), ("graph", C.c_int32)]
 
 
 class IterStats(C.Structure):
@@ -157,6 +157,12 @@ _SIGS = [
     ("upbp_set_tuning", C.c_int, [C.c_void_p, C.c_void_p]),
     ("upbp_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     ("upbp_run_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
+    ("upbp_batch_create", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("upbp_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    ("upbp_batch_counts", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("upbp_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("upbp_batch_destroy", None, [C.c_void_p]),
     ("upbp_test_closest_hit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("upbp_test_sphere_hits", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("upbp_test_env_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

@@ -132,12 +132,14 @@ __device__ __forceinline__ int generate_camera_sample(const DScene& sc, const It
 // with it the bounce kernels' waves and the segment-query pieces they write, then hold compact
 // pixel tiles, whose rays are more coherent than a 64-pixel row. Only the order changes; every
 // pixel keeps its own random streams.
-__global__ void __launch_bounds__(256) k_camera_generate(DScene sc, Bufs b, uint32_t res_x, uint32_t tiles_x)
+// vb: the block's index within its job (blockIdx.x for a single job)
+__device__ __forceinline__ void camera_generate_body(const DScene& sc, const Bufs& b, uint32_t res_x, uint32_t tiles_x,
+                                                     uint32_t vb)
 {
     const IterParams& p = *b.params;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t x = (blockIdx.x % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t y = (blockIdx.x / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t x = (vb % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t y = (vb / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
     const uint32_t pix = y * res_x + x;
     if (x >= res_x || pix >= p.n_pixels) return;
     Rng rng(kDomainCamera, pix, p.iter_key, 0, P_JITTER);
@@ -155,6 +157,18 @@ __global__ void __launch_bounds__(256) k_camera_generate(DScene sc, Bufs b, uint
     if (alive) store_stack(b.cstack, pix, s.stack);
     else stat_add(b.stats, STAT_ERRORS, 1);
     queue_append(alive, pix, b.queue[0], &b.qcount[64]);
+}
+
+__global__ void __launch_bounds__(256) k_camera_generate(DScene sc, Bufs b, uint32_t res_x, uint32_t tiles_x)
+{
+    camera_generate_body(sc, b, res_x, tiles_x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_camera_generate_batch(const BatchJob* jobs, const uint2* bmap)
+{
+    const uint2 m = bmap[blockIdx.x];
+    const BatchJob& J = jobs[m.x];
+    camera_generate_body(J.sc, J.b, J.res_x, J.tiles_x, m.y);
 }
 
 struct CamCounters {
@@ -499,10 +513,10 @@ __global__ void k_guide_ring_advance(Bufs b)
 }
 
 template <bool LOG>
-__global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, Bufs b, int bounce)
+__device__ __forceinline__ void camera_bounce_body(const DScene& sc, const Bufs& b, int bounce, uint32_t vb)
 {
     const IterParams& p = *b.params;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = vb * 256u + threadIdx.x;
     const uint32_t n = b.qcount[64 + bounce];
     if (i >= n) return;
     const uint32_t pix = b.queue[bounce & 1][i];
@@ -537,11 +551,23 @@ __global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, Bufs b, int
     queue_append(cont, pix, b.queue[(bounce + 1) & 1], &b.qcount[64 + bounce + 1]);
 }
 
+template <bool LOG>
+__global__ void __launch_bounds__(256, 2) k_camera_bounce(DScene sc, Bufs b, int bounce)
+{
+    camera_bounce_body<LOG>(sc, b, bounce, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256, 2) k_camera_bounce_batch(const BatchJob* jobs, const uint2* bmap, int bounce)
+{
+    const uint2 m = bmap[blockIdx.x];
+    camera_bounce_body<false>(jobs[m.x].sc, jobs[m.x].b, bounce, m.y);
+}
+
 // adds every camera path's colour at its jittered raster position (frame_buffer.cuh:35-44)
-__global__ void __launch_bounds__(256) k_camera_splat(Bufs b, float res_x, float res_y)
+__device__ __forceinline__ void camera_splat_body(const Bufs& b, float res_x, float res_y, uint32_t vb)
 {
     const IterParams& p = *b.params;
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pix = vb * 256u + threadIdx.x;
     if (pix >= p.n_pixels) return;
     const PathState& st = b.cstate[pix];
     if (!(st.sx >= 0.f && st.sx < res_x && st.sy >= 0.f && st.sy < res_y)) return;
@@ -552,6 +578,11 @@ __global__ void __launch_bounds__(256) k_camera_splat(Bufs b, float res_x, float
     atomicAdd(&b.fb32[3 * tgt + 2], c.z);
 }
 
+__global__ void __launch_bounds__(256) k_camera_splat(Bufs b, float res_x, float res_y)
+{
+    camera_splat_body(b, res_x, res_y, blockIdx.x);
+}
+
 // folds the FP32 iteration buffer into the FP64 cross-iteration accumulator, unless the iteration
 // overflowed a photon list (then it is discarded and rendered again)
 __global__ void __launch_bounds__(256) k_accumulate(const float* fb32, double* accum, uint32_t n, const uint32_t* ovf)
@@ -560,12 +591,45 @@ __global__ void __launch_bounds__(256) k_accumulate(const float* fb32, double* a
     if (i < n && *ovf == 0u) accum[i] += double(fb32[i]);
 }
 
+// splat (phase 0, a block per 256 pixels of each job) and accumulate (phase 1, a block per 256
+// values of each job) of every job; bmap gives each block its job and its block within the job.
+// A job's accumulate must see all of its splat, so the phases are two launches.
+__global__ void __launch_bounds__(256) k_finish_batch(const BatchJob* jobs, const uint2* bmap, int phase)
+{
+    const uint2 m = bmap[blockIdx.x];
+    const BatchJob& J = jobs[m.x];
+    if (phase == 0) {
+        camera_splat_body(J.b, J.fres_x, J.fres_y, m.y);
+    } else {
+        const uint32_t i = m.y * 256u + threadIdx.x;
+        const uint32_t n = 3u * J.b.params->n_pixels;
+        if (i < n && *J.ovf == 0u) J.b.accum[i] += double(J.b.fb32[i]);
+    }
+}
+
 void launch_camera_generate(const DScene& sc, const IterParams& p, const Bufs& b, hipStream_t s)
 {
     const uint32_t res_x = uint32_t(sc.cam.res_x);
     const uint32_t res_y = (p.n_pixels + res_x - 1) / res_x;
     const uint32_t tiles_x = (res_x + 15) / 16, tiles_y = (res_y + 15) / 16;
     hipLaunchKernelGGL(k_camera_generate, dim3(tiles_x * tiles_y), dim3(256), 0, s, sc, b, res_x, tiles_x);
+}
+
+void launch_camera_generate_batch(const BatchJob* jobs, const uint2* bmap, uint32_t blocks, hipStream_t s)
+{
+    if (blocks) hipLaunchKernelGGL(k_camera_generate_batch, dim3(blocks), dim3(256), 0, s, jobs, bmap);
+}
+
+void launch_camera_bounce_batch(const BatchJob* jobs, const uint2* bmap, uint32_t blocks, int bounce, hipStream_t s)
+{
+    if (blocks) hipLaunchKernelGGL(k_camera_bounce_batch, dim3(blocks), dim3(256), 0, s, jobs, bmap, bounce);
+}
+
+void launch_camera_finish_batch(const BatchJob* jobs, const uint2* bmap_pix, uint32_t blocks_pix, const uint2* bmap_acc,
+                                uint32_t blocks_acc, hipStream_t s)
+{
+    if (blocks_pix) hipLaunchKernelGGL(k_finish_batch, dim3(blocks_pix), dim3(256), 0, s, jobs, bmap_pix, 0);
+    if (blocks_acc) hipLaunchKernelGGL(k_finish_batch, dim3(blocks_acc), dim3(256), 0, s, jobs, bmap_acc, 1);
 }
 
 void launch_camera_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)

@@ -34,10 +34,11 @@ __device__ __forceinline__ int generate_light_sample(const DScene& sc, const Ite
     return init_stack(sc, s.stack, l.mat_id, l.med_id);
 }
 
-__global__ void __launch_bounds__(256) k_light_generate(DScene sc, Bufs b)
+// vb: the block's index within its job (blockIdx.x for a single job)
+__device__ __forceinline__ void light_generate_body(const DScene& sc, const Bufs& b, uint32_t vb)
 {
     const IterParams& p = *b.params;
-    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pid = vb * 256u + threadIdx.x;
     if (pid >= p.n_light) return;
     b.nvert[pid] = 0;
     b.nbeam[pid] = 0;
@@ -56,6 +57,14 @@ __global__ void __launch_bounds__(256) k_light_generate(DScene sc, Bufs b)
         store_stack(b.lstack, pid, s.stack);
     }
     queue_append(alive, pid, b.queue[0], &b.qcount[0]);
+}
+
+__global__ void __launch_bounds__(256) k_light_generate(DScene sc, Bufs b) { light_generate_body(sc, b, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) k_light_generate_batch(const BatchJob* jobs, const uint2* bmap)
+{
+    const uint2 m = bmap[blockIdx.x];
+    light_generate_body(jobs[m.x].sc, jobs[m.x].b, m.y);
 }
 
 // kernel/functs_light.cuh:159-247
@@ -249,10 +258,10 @@ __device__ __forceinline__ bool light_step(const DScene& sc, const IterParams& p
 }
 
 template <bool LOG>
-__global__ void __launch_bounds__(256) k_light_bounce(DScene sc, Bufs b, int bounce)
+__device__ __forceinline__ void light_bounce_body(const DScene& sc, const Bufs& b, int bounce, uint32_t vb)
 {
     const IterParams& p = *b.params;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = vb * 256u + threadIdx.x;
     const uint32_t n = b.qcount[bounce];
     if (i >= n) return;
     const uint32_t pid = b.queue[bounce & 1][i];
@@ -282,9 +291,31 @@ __global__ void __launch_bounds__(256) k_light_bounce(DScene sc, Bufs b, int bou
     queue_append(cont, pid, b.queue[(bounce + 1) & 1], &b.qcount[bounce + 1]);
 }
 
+template <bool LOG>
+__global__ void __launch_bounds__(256) k_light_bounce(DScene sc, Bufs b, int bounce)
+{
+    light_bounce_body<LOG>(sc, b, bounce, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_light_bounce_batch(const BatchJob* jobs, const uint2* bmap, int bounce)
+{
+    const uint2 m = bmap[blockIdx.x];
+    light_bounce_body<false>(jobs[m.x].sc, jobs[m.x].b, bounce, m.y);
+}
+
 void launch_light_generate(const DScene& sc, const IterParams& p, const Bufs& b, hipStream_t s)
 {
     hipLaunchKernelGGL(k_light_generate, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, b);
+}
+
+void launch_light_generate_batch(const BatchJob* jobs, const uint2* bmap, uint32_t blocks, hipStream_t s)
+{
+    if (blocks) hipLaunchKernelGGL(k_light_generate_batch, dim3(blocks), dim3(256), 0, s, jobs, bmap);
+}
+
+void launch_light_bounce_batch(const BatchJob* jobs, const uint2* bmap, uint32_t blocks, int bounce, hipStream_t s)
+{
+    if (blocks) hipLaunchKernelGGL(k_light_bounce_batch, dim3(blocks), dim3(256), 0, s, jobs, bmap, bounce);
 }
 
 void launch_light_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)

@@ -124,9 +124,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 // accepted and its parameters: P-B2D (idist, ir2); B-B1D (t camera, t beam, distance, sin).
 // Only the query piece owning the pair's camera parameter [lo, hi) accepts it.
 template <int KIND>
-__device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMaps& M, uint32_t k, f3 ro, f3 rd,
-                                            float dmin, float dmax, float lo, float hi, int med, uint32_t& slot,
-                                            float& x, float& y, float& z, float& w)
+__device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMaps& M, const BeamRec* beams, uint32_t k,
+                                            f3 ro, f3 rd, float dmin, float dmax, float lo, float hi, int med,
+                                            uint32_t& slot, float& x, float& y, float& z, float& w)
 {
     if (KIND == 0) { // program_pb2d.cu:31-46
         const float4 pt = M.med_pts[k];
@@ -136,6 +136,20 @@ __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMap
         if (!(x >= lo && x < hi)) return false;
         slot = uint32_t(__float_as_int(pt.w));
         z = w = 0.f;
+        return true;
+    } else if (p.beam_leaf) { // the same test from the sub-beam's reference and its beam's record
+        const uint2 rf = M.beam_ref[k];
+        const BeamRec& br = beams[rf.x];
+        const float4 a = br.a, d = br.b;
+        if (!pred_beam_beam(ro, rd, dmin, dmax, mk(a.x, a.y, a.z), mk(d.x, d.y, d.z), 0.f, a.w, p.r2_bb1d, z, w, x,
+                            y))
+            return false;
+        if (!(x >= lo && x < hi)) return false;
+        float blo, bhi, ta, tb;
+        subbeam_range(a.w, rf.y & 0xffffu, rf.y >> 16, blo, bhi, ta, tb);
+        if (!(y >= blo && y < bhi)) return false;
+        if (!(uint32_t(__float_as_int(br.d.z)) < p.bb1d_used) || __float_as_int(d.w) != med) return false;
+        slot = rf.x;
         return true;
     } else { // program_bb1d.cu:31-48 on the whole beam, then sub-beam ownership of t beam
         const float4 g0 = M.beam_geo[3 * k], g1 = M.beam_geo[3 * k + 1];
@@ -220,19 +234,15 @@ __device__ __forceinline__ void eval_pairs(const DScene& sc, const IterParams& p
     qn -= m;
 }
 
-// LOG adds the MIS weight log of the sampled camera paths (Bufs::mislog).
+// One job's segment merges of one bounce, by the wave the queues belong to. bid: the block's
+// index (its XCD is bid % 8); pass_order: take the pieces as the camera pass wrote them whatever
+// the job's piece order (batched launches, which sort nothing). LOG adds the MIS weight log of the
+// sampled camera paths (Bufs::mislog).
 template <int KIND, bool LOG>
-__global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, int bounce)
+__device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, int bounce, uint32_t bid,
+                                               bool pass_order, CandQueue& C, PairQueue& Q, Steal& S, ColorCache& K)
 {
     const IterParams& p = *b.params;
-    __shared__ CandQueue cq[kMergeWaves];
-    __shared__ PairQueue pq[kMergeWaves];
-    __shared__ Steal stl[kMergeWaves];
-    __shared__ ColorCache ccache[kMergeWaves];
-    CandQueue& C = cq[threadIdx.x >> 6];
-    PairQueue& Q = pq[threadIdx.x >> 6];
-    Steal& S = stl[threadIdx.x >> 6];
-    ColorCache& K = ccache[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
     K.tag[lane] = 0u;
     K.r[lane] = K.g[lane] = K.b[lane] = 0.f;
@@ -243,9 +253,10 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, in
     const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : n_recs;
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
     uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
-    const bool split = KIND == 1 && p.piece_order == 1;
-    const uint2* plist = KIND == 1 && p.piece_order != 0 ? b.pieces_x : b.pieces;
-    uint32_t src = blockIdx.x % kXcds, tried = 0; // work range being consumed (same in every lane)
+    const int order = pass_order ? 0 : p.piece_order;
+    const bool split = KIND == 1 && (order == 1 || order >= 3);
+    const uint2* plist = KIND == 1 && order != 0 ? b.pieces_x : b.pieces;
+    uint32_t src = bid % kXcds, tried = 0; // work range being consumed (same in every lane)
     const WBvh& bvh = KIND == 0 ? b.maps->wmed : b.maps->wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
 
@@ -408,8 +419,8 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, in
                     const float4 a = r->a, bb = r->b, d = r->d;
                     float clo, chi, cta, ctb;
                     piece_range(a.w, bb.w, ckn & 63u, ckn >> 6, clo, chi, cta, ctb);
-                    acc = leaf_accept<KIND>(p, *b.maps, leaf, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), a.w, bb.w, clo,
-                                            chi, __float_as_int(d.z), slot, x, y, z, w);
+                    acc = leaf_accept<KIND>(p, *b.maps, b.beams, leaf, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), a.w,
+                                            bb.w, clo, chi, __float_as_int(d.z), slot, x, y, z, w);
                 }
                 wave_lds_sync();
                 cn -= n;
@@ -436,6 +447,38 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, in
         stat_add(b.stats, STAT_SEG_TRIPS, n_trips);
         stat_add(b.stats, STAT_CAND_BATCHES, n_cb);
         stat_add(b.stats, STAT_PAIR_BATCHES, n_pb);
+    }
+}
+
+template <int KIND, bool LOG>
+__global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, int bounce)
+{
+    __shared__ CandQueue cq[kMergeWaves];
+    __shared__ PairQueue pq[kMergeWaves];
+    __shared__ Steal stl[kMergeWaves];
+    __shared__ ColorCache ccache[kMergeWaves];
+    const uint32_t w = threadIdx.x >> 6;
+    seg_merge_body<KIND, LOG>(sc, b, bounce, blockIdx.x, false, cq[w], pq[w], stl[w], ccache[w]);
+}
+
+// The segment merges of one bounce for every job of a batch in one persistent launch: each block
+// walks the jobs in turn (starting at its own, so that the blocks spread over the jobs), and its
+// waves take pieces of a job until that job has none left, so that all resident waves help with
+// whatever work is left, whatever the jobs' sizes.
+template <int KIND>
+__global__ void __launch_bounds__(kMergeBlock) k_seg_merge_batch(const BatchJob* jobs, uint32_t njobs, int bounce)
+{
+    __shared__ CandQueue cq[kMergeWaves];
+    __shared__ PairQueue pq[kMergeWaves];
+    __shared__ Steal stl[kMergeWaves];
+    __shared__ ColorCache ccache[kMergeWaves];
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint32_t t = 0; t < njobs; t++) {
+        const BatchJob& J = jobs[(blockIdx.x + t) % njobs];
+        const IterParams& p = *J.b.params;
+        const bool on = KIND == 0 ? (p.build_pb2d && J.b.maps->wmed.n > 0) : (p.build_bb1d && J.b.maps->wbeam.n > 0);
+        if (!on || bounce >= int(p.max_len)) continue;
+        seg_merge_body<KIND, false>(J.sc, J.b, bounce, blockIdx.x, true, cq[w], pq[w], stl[w], ccache[w]);
     }
 }
 
@@ -527,8 +570,8 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, Buf
                     bool acc = false;
                     if (h) {
                         n_cands++;
-                        acc = leaf_accept<1>(p, *b.maps, uint32_t(~ch), ro, rd, dmin, dmax, lo, hi, med, slot, x, y,
-                                             z, w);
+                        acc = leaf_accept<1>(p, *b.maps, b.beams, uint32_t(~ch), ro, rd, dmin, dmax, lo, hi, med, slot,
+                                             x, y, z, w);
                     }
                     n_acc += acc ? 1u : 0u;
                     pq_push(Q, qn, acc, pc.x, slot, x, y, z, w);
@@ -691,32 +734,74 @@ __device__ __forceinline__ uint32_t spread_bits10(uint32_t v)
     return v;
 }
 
+// 30-bit Morton code of a piece's midpoint in the beam map's root box (0 for an unwritten piece)
+__device__ __forceinline__ uint32_t piece_morton(const Bufs& b, uint32_t n_recs, uint2 pc)
+{
+    if (!(pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16))) return 0u;
+    const WNode root = b.maps->wbeam.nodes[0];
+    const SegRec* r = b.segq + pc.x;
+    const float4 a = r->a, bb = r->b;
+    float lo, hi, ta, tb;
+    piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
+    const float t = 0.5f * (ta + tb);
+    const float m[3] = {a.x + bb.x * t, a.y + bb.y * t, a.z + bb.z * t};
+    const float o[3] = {root.ox, root.oy, root.oz};
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        const float ext = 256.f * wnode_step(root.meta, k);
+        const float u = ext > 0.f ? (m[k] - o[k]) / ext : 0.f;
+        q[k] = uint32_t(fminf(fmaxf(u * 1024.f, 0.f), 1023.f));
+    }
+    return (spread_bits10(q[0]) << 2) | (spread_bits10(q[1]) << 1) | spread_bits10(q[2]);
+}
+
 __global__ void __launch_bounds__(256) k_piece_keys(Bufs b, int bounce, uint32_t n, uint32_t* keys, uint32_t* vals)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
-    const uint2 pc = b.pieces[i];
-    uint32_t key = 0;
-    if (pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16)) {
-        const WNode root = b.maps->wbeam.nodes[0];
-        const SegRec* r = b.segq + pc.x;
-        const float4 a = r->a, bb = r->b;
-        float lo, hi, ta, tb;
-        piece_range(a.w, bb.w, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
-        const float t = 0.5f * (ta + tb);
-        const float m[3] = {a.x + bb.x * t, a.y + bb.y * t, a.z + bb.z * t};
-        const float o[3] = {root.ox, root.oy, root.oz};
-        uint32_t q[3];
-        for (int k = 0; k < 3; k++) {
-            const float ext = 256.f * wnode_step(root.meta, k);
-            const float u = ext > 0.f ? (m[k] - o[k]) / ext : 0.f;
-            q[k] = uint32_t(fminf(fmaxf(u * 1024.f, 0.f), 1023.f));
-        }
-        key = (spread_bits10(q[0]) << 2) | (spread_bits10(q[1]) << 1) | spread_bits10(q[2]);
-    }
-    keys[i] = key;
+    keys[i] = piece_morton(b, n_recs, b.pieces[i]);
     vals[i] = i;
+}
+
+// ---- spatial order of whole units of 64 pieces ------------------------------------------
+// The camera pass writes a wave's pieces as one run: 64 consecutive pieces are neighbouring
+// rays' stretches of similar depth, which is what keeps one merge wave's node and leaf fetches
+// shared. Units of 64 consecutive pieces are kept whole and ordered by the Morton code of their
+// middle piece, so that the waves in flight at one time work on one region of the scene and the
+// caches hold the part of the beam tree they walk. The sorted units are then cut into eight
+// contiguous ranges, one per XCD (workgroups are dispatched to the XCDs round-robin, so block i
+// starts on range i % 8 and each XCD's L2 holds one region's subtrees), or left as one range.
+__global__ void __launch_bounds__(256) k_unit_keys(Bufs b, int bounce, uint32_t cap_units, uint32_t* keys,
+                                                    uint32_t* vals, uint32_t* nu_dev)
+{
+    const uint32_t n = min(b.qcount[320 + bounce], b.piece_cap);
+    const uint32_t nu = (n + 63u) / 64u;
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0) *nu_dev = nu;
+    if (u >= nu || u >= cap_units) return;
+    const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
+    keys[u] = piece_morton(b, n_recs, b.pieces[min(u * 64u + 32u, n - 1u)]);
+    vals[u] = u;
+}
+
+__global__ void __launch_bounds__(256) k_unit_gather(Bufs b, int bounce, uint32_t cap_units, const uint32_t* vals,
+                                                      int per_xcd)
+{
+    const uint32_t n = min(b.qcount[320 + bounce], b.piece_cap);
+    const uint32_t nu = min((n + 63u) / 64u, cap_units);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
+        for (uint32_t x = 0; x < uint32_t(kXcds); x++) {
+            xw[192 + x] = per_xcd ? (x * nu / kXcds) * 64u : 0u;
+            xw[200 + x] = per_xcd ? ((x + 1) * nu / kXcds) * 64u : (x == 0 ? nu * 64u : 0u);
+            xw[208 + x] = 0u;
+        }
+    }
+    if (i >= nu * 64u) return;
+    const uint32_t src = vals[i >> 6] * 64u + (i & 63u);
+    b.pieces_x[i] = src < n ? b.pieces[src] : make_uint2(~0u, 0u);
 }
 
 __global__ void __launch_bounds__(256) k_piece_gather(Bufs b, uint32_t n, const uint32_t* vals)
@@ -733,6 +818,28 @@ void launch_piece_keys(const Bufs& b, int bounce, uint32_t n, uint32_t* keys, ui
 void launch_piece_gather(const Bufs& b, uint32_t n, const uint32_t* vals, hipStream_t s)
 {
     if (n) hipLaunchKernelGGL(k_piece_gather, dim3((n + 255) / 256), dim3(256), 0, s, b, n, vals);
+}
+
+void launch_unit_keys(const Bufs& b, int bounce, uint32_t cap_units, uint32_t* keys, uint32_t* vals, uint32_t* nu_dev,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_unit_keys, dim3((cap_units + 255) / 256), dim3(256), 0, s, b, bounce, cap_units, keys, vals,
+                       nu_dev);
+}
+
+void launch_unit_gather(const Bufs& b, int bounce, uint32_t cap_units, const uint32_t* vals, int per_xcd,
+                        hipStream_t s)
+{
+    const size_t n = size_t(cap_units) * 64u;
+    hipLaunchKernelGGL(k_unit_gather, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, b, bounce, cap_units, vals,
+                       per_xcd);
+}
+
+void launch_seg_merge_batch(const BatchJob* jobs, uint32_t njobs, int kind, int bounce, hipStream_t s)
+{
+    const int blocks = merge_grid_blocks(0);
+    if (kind == 0) hipLaunchKernelGGL(k_seg_merge_batch<0>, dim3(blocks), dim3(kMergeBlock), 0, s, jobs, njobs, bounce);
+    else hipLaunchKernelGGL(k_seg_merge_batch<1>, dim3(blocks), dim3(kMergeBlock), 0, s, jobs, njobs, bounce);
 }
 
 void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int kind, int bounce, hipStream_t s)

@@ -33,6 +33,8 @@ struct IterParams {
     int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
     int piece_order; // B-B1D work order: 0 as written, 1 XCD regions, 2 Morton-sorted (pieces_x)
     int packet_bounces; // B-B1D: camera bounces below this walk the beam tree as 64-piece packets
+    int beam_leaf;      // B-B1D leaf test reads 0: the sorted sub-beam records (beam_geo); 1: the
+                        // sorted (beam slot, sub-beam j | n << 16) references and the beam record
     uint32_t cap_med, cap_beam; // capacities of the medium-point and sub-beam lists (host launch
                                 // decisions; the counts themselves are only on the device)
     uint64_t iter_key;
@@ -352,6 +354,7 @@ struct PhotonMaps {
     const float4* med_pts;
     const float4* beam_geo; // sorted: 3 per sub-beam: (origin.xyz, length), (dir.xyz, beam slot),
                             // (lo, hi, medium, SB_* flags)
+    const uint2* beam_ref;  // sorted: per sub-beam (beam slot, j | n << 16), sub-beam j of n
 };
 
 // A camera-segment query is walked in pieces, each an independent work item of the merge
@@ -451,6 +454,17 @@ enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_P
                   STAT_SEG_PIECES, STAT_SEG_TRIPS, STAT_CAND_BATCHES, STAT_PAIR_BATCHES, STAT_MERGE_STACK,
                   STAT_COUNT };
 constexpr int kQcountWords = 512;
+// One render job of a batched launch (device memory): its scene, buffers and the few launch
+// values the single-job launches pass as arguments. A batched stage kernel runs the blocks of
+// every job in one grid; a block map (job, block of that job) per stage says which block is whose.
+struct BatchJob {
+    DScene sc;
+    Bufs b;
+    const uint32_t* ovf; // the job's photon-list overflow flag (its iteration is then not accumulated)
+    uint32_t res_x, tiles_x;
+    float fres_x, fres_y;
+};
+
 // XCD-aware work split of the B-B1D merge: 4x4x4 spatial cells, 8 XCDs owning one octant
 // (2x2x2 cells) each. Per bounce: cell counts [0,64), cell offsets [64,128), fill cursors
 // [128,192), XCD range begin [192,200), end [200,208), heads [208,216).

@@ -65,7 +65,7 @@ __global__ void k_count_photons(Bufs b, uint32_t* cs, uint32_t* cm, uint32_t* cb
 // then discarded, see k_map_counts)
 __global__ void k_emit_photons(Bufs b, const uint32_t* os, const uint32_t* om, const uint32_t* ob, int want_surf,
                                int want_med, int want_beams, uint32_t* ls, float4* slo, float4* shi, uint32_t* lm,
-                               float4* mlo, float4* mhi, float4* geo, float4* blo, float4* bhi, uint3 cap)
+                               float4* mlo, float4* mhi, float4* geo, uint2* ref, float4* blo, float4* bhi, uint3 cap)
 {
     const IterParams& p = *b.params;
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,6 +115,7 @@ __global__ void k_emit_photons(Bufs b, const uint32_t* os, const uint32_t* om, c
                 geo[3 * size_t(jb)] = a;
                 geo[3 * size_t(jb) + 1] = make_float4(d.x, d.y, d.z, __int_as_float(int(slot)));
                 geo[3 * size_t(jb) + 2] = make_float4(lo, hi, d.w, __int_as_float(flags));
+                ref[jb] = make_uint2(slot, j | (n << 16));
             }
         }
     }
@@ -130,8 +131,8 @@ __global__ void k_gather_points(const uint32_t* order, const uint32_t* list, con
     out[i] = make_float4(a.x, a.y, a.z, __int_as_float(int(slot)));
 }
 
-__global__ void k_gather_beams(const uint32_t* order, const float4* geo_u, uint32_t cap, const uint32_t* n_dev,
-                               float4* out)
+__global__ void k_gather_beams(const uint32_t* order, const float4* geo_u, const uint2* ref_u, uint32_t cap,
+                               const uint32_t* n_dev, float4* out, uint2* ref_out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= min(cap, *n_dev)) return;
@@ -139,6 +140,7 @@ __global__ void k_gather_beams(const uint32_t* order, const float4* geo_u, uint3
     out[3 * size_t(i)] = geo_u[3 * j];
     out[3 * size_t(i) + 1] = geo_u[3 * j + 1];
     out[3 * size_t(i) + 2] = geo_u[3 * j + 2];
+    ref_out[i] = ref_u[j];
 }
 __global__ void k_gather_tris(const uint32_t* order, const float4* tri, const float4* nrm, const int2* rec, uint32_t n,
                               float4* tri_s, float4* nrm_s, int2* rec_s)
@@ -204,8 +206,8 @@ struct upbp_renderer {
     bool want_beams;
     Bufs bufs;
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
-    DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b;
-    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals;
+    DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b, ref_u, ref_b;
+    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals, unitn;
     DevBuf guide; // learned proposal records
     DevBuf mislog, miscount; // MIS weight log (upbp_set_mis_log)
     // device-resident iteration: parameters and photon-map descriptor in device memory, photon
@@ -432,6 +434,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.merge_bpc = r->tune.merge_blocks_per_cu;
     p.piece_order = r->tune.piece_order;
     p.packet_bounces = r->tune.packet_bounces;
+    p.beam_leaf = r->tune.beam_leaf;
     p.cap_med = r->cap_m;
     p.cap_beam = r->cap_b;
     {
@@ -653,8 +656,10 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     const size_t piece_cap = seg_merges ? NP * (kPiecesPerRay + kMaxSegments + 1) : 1;
     if (ok)
         ok = !r->segq.ensure(seg_cap * sizeof(SegRec)) && !r->pieces.ensure(piece_cap * sizeof(uint2)) &&
-             !r->pieces_x.ensure(piece_cap * sizeof(uint2)) && !r->piece_cell.ensure(piece_cap) &&
-             !r->xwork.ensure(size_t(kMaxBounces) * kXWork * 4);
+             !r->pieces_x.ensure((piece_cap + 63) / 64 * 64 * sizeof(uint2)) && !r->piece_cell.ensure(piece_cap) &&
+             !r->xwork.ensure(size_t(kMaxBounces) * kXWork * 4) &&
+             !r->pkeys.ensure((piece_cap + 63) / 64 * 4) && !r->pvals.ensure((piece_cap + 63) / 64 * 4) &&
+             !r->unitn.ensure(16);
     if (!ok) {
         upbp_destroy(r);
         return fail(nullptr, "device allocation failed", UPBP_E_HIP);
@@ -743,12 +748,14 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, Staging& st)
               !r->list_m.ensure(std::max<size_t>(cm, 1) * 4) && !r->lo_m.ensure(std::max<size_t>(cm, 1) * 16) &&
               !r->hi_m.ensure(std::max<size_t>(cm, 1) * 16) && !r->pts_m.ensure(std::max<size_t>(cm, 1) * 16) &&
               !r->geo_u.ensure(std::max<size_t>(cb, 1) * 48) && !r->lo_b.ensure(std::max<size_t>(cb, 1) * 16) &&
-              !r->hi_b.ensure(std::max<size_t>(cb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(cb, 1) * 48);
+              !r->hi_b.ensure(std::max<size_t>(cb, 1) * 16) && !r->geo_b.ensure(std::max<size_t>(cb, 1) * 48) &&
+              !r->ref_u.ensure(std::max<size_t>(cb, 1) * 8) && !r->ref_b.ensure(std::max<size_t>(cb, 1) * 8);
     if (!ok) return fail(r, "photon list allocation failed", UPBP_E_HIP);
     const uint3 cap = make_uint3(cs, cm, cb);
     hipLaunchKernelGGL(k_emit_photons, dim3(nblk(NL)), dim3(256), 0, s, b, off, off + NL, off + 2 * NL, want_surf,
                        want_med, want_beams, r->list_s.as<uint32_t>(), r->lo_s.as<float4>(), r->hi_s.as<float4>(),
                        r->list_m.as<uint32_t>(), r->lo_m.as<float4>(), r->hi_m.as<float4>(), r->geo_u.as<float4>(),
+                       r->ref_u.as<uint2>(),
                        r->lo_b.as<float4>(), r->hi_b.as<float4>(), cap);
     uint32_t* n = r->mapn.as<uint32_t>();
     hipLaunchKernelGGL(k_map_counts, dim3(1), dim3(64), 0, s, tot, cap, make_uint3(want_surf, want_med, want_beams), n,
@@ -769,6 +776,7 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, Staging& st)
     M.surf_pts = r->pts_s.as<float4>();
     M.med_pts = r->pts_m.as<float4>();
     M.beam_geo = r->geo_b.as<float4>();
+    M.beam_ref = r->ref_b.as<uint2>();
     st.m = M;
     HIPCHK(hipMemcpyAsync(r->dmaps.p, &st.m, sizeof(PhotonMaps), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_map_n, dim3(1), dim3(64), 0, s, n, r->dmaps.as<PhotonMaps>());
@@ -779,8 +787,8 @@ int build_photon_maps(upbp_renderer* r, const IterParams& p, Staging& st)
         hipLaunchKernelGGL(k_gather_points, dim3(nblk(cm)), dim3(256), 0, s, r->bvh_m.order, r->list_m.as<uint32_t>(),
                            b.verts, cm, n + 1, r->pts_m.as<float4>());
     if (cb)
-        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(cb)), dim3(256), 0, s, r->bvh_b.order, r->geo_u.as<float4>(), cb,
-                           n + 2, r->geo_b.as<float4>());
+        hipLaunchKernelGGL(k_gather_beams, dim3(nblk(cb)), dim3(256), 0, s, r->bvh_b.order, r->geo_u.as<float4>(),
+                           r->ref_u.as<uint2>(), cb, n + 2, r->geo_b.as<float4>(), r->ref_b.as<uint2>());
     return 0;
 }
 
@@ -840,13 +848,12 @@ int settle(upbp_renderer* r, bool wait);
 // pass of the photon lists and the statistics (stats != null), which synchronise.
 // The work of one iteration on r's stream, reading its parameters from the staging slot st.
 // p.cap_* are set to the photon-list capacities the launches below depend on.
-int enqueue_body(upbp_renderer* r, IterParams& p, Staging& st, bool timed)
+// The iteration's start on r's stream: its parameters from the staging slot to the device and
+// the per-iteration counters and buffers cleared.
+int enqueue_head(upbp_renderer* r, Staging& st)
 {
     hipStream_t s = r->stream;
     Bufs& b = r->bufs;
-    const int L = int(r->cfg.max_path_length);
-    hipEvent_t* ev = r->ev;
-    if (timed) HIPCHK(hipEventRecord(ev[0], s));
     HIPCHK(hipMemcpyAsync(r->dparams.p, &st.p, sizeof(IterParams), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill, dim3(nblk(kQcountWords)), dim3(256), 0, s, b.qcount, 0u, size_t(kQcountWords));
     hipLaunchKernelGGL(k_fill, dim3(nblk(size_t(kMaxBounces) * kXWork)), dim3(256), 0, s, b.xwork, 0u,
@@ -860,6 +867,31 @@ int enqueue_body(upbp_renderer* r, IterParams& p, Staging& st, bool timed)
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->ovf.as<uint32_t>(), 0u, size_t(1));
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->mapn.as<uint32_t>(), 0u, size_t(3));
     if (b.mislog) hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, b.miscount, 0u, size_t(1));
+    return UPBP_OK;
+}
+
+// The photon maps of the iteration's light paths on r's stream; p.cap_* are then the photon-list
+// capacities the launches after it depend on.
+int enqueue_maps(upbp_renderer* r, IterParams& p, Staging& st)
+{
+    const int e = build_photon_maps(r, p, st);
+    if (e) return e;
+    p.cap_med = (p.build_pp3d || p.build_pb2d) ? r->cap_m : 0u;
+    p.cap_beam = (p.build_bb1d || p.build_bp2d) && r->want_beams ? r->cap_b : 0u;
+    return UPBP_OK;
+}
+
+int enqueue_body(upbp_renderer* r, IterParams& p, Staging& st, bool timed)
+{
+    hipStream_t s = r->stream;
+    Bufs& b = r->bufs;
+    const int L = int(r->cfg.max_path_length);
+    hipEvent_t* ev = r->ev;
+    if (timed) HIPCHK(hipEventRecord(ev[0], s));
+    {
+        const int e = enqueue_head(r, st);
+        if (e) return e;
+    }
     if (p.trace_light) {
         launch_light_generate(r->sc, p, b, s);
         for (int k = 0; k < L; k++) launch_light_bounce(r->sc, p, b, k, s);
@@ -869,11 +901,9 @@ int enqueue_body(upbp_renderer* r, IterParams& p, Staging& st, bool timed)
     }
     if (timed) HIPCHK(hipEventRecord(ev[1], s));
     if (p.trace_light) {
-        const int e = build_photon_maps(r, p, st);
-        if (e) return e;
         // launch decisions below use the capacities a sizing pass may just have set
-        p.cap_med = (p.build_pp3d || p.build_pb2d) ? r->cap_m : 0u;
-        p.cap_beam = (p.build_bb1d || p.build_bp2d) && r->want_beams ? r->cap_b : 0u;
+        const int e = enqueue_maps(r, p, st);
+        if (e) return e;
     } else {
         p.cap_med = p.cap_beam = 0u;
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, r->dmaps.as<uint32_t>(), 0u,
@@ -916,6 +946,17 @@ int enqueue_body(upbp_renderer* r, IterParams& p, Staging& st, bool timed)
                 if (radix_sort_pairs(r->pkeys.as<uint32_t>(), r->pvals.as<uint32_t>(), n, 30, r->scratch, s))
                     return fail(r, "piece sort failed", UPBP_E_HIP);
                 launch_piece_gather(b, n, r->pvals.as<uint32_t>(), s);
+            }
+            if (p.piece_order >= 3 && p.build_bb1d && p.cap_beam > 0) {
+                // whole units of 64 pieces in spatial order, all on the device
+                const uint32_t cu = (b.piece_cap + 63u) / 64u;
+                uint32_t* keys = r->pkeys.as<uint32_t>();
+                uint32_t* vals = r->pvals.as<uint32_t>();
+                uint32_t* nu = r->unitn.as<uint32_t>();
+                launch_unit_keys(b, k, cu, keys, vals, nu, s);
+                if (radix_sort_pairs(keys, vals, cu, 30, r->scratch, s, nu))
+                    return fail(r, "piece unit sort failed", UPBP_E_HIP);
+                launch_unit_gather(b, k, cu, vals, p.piece_order == 3, s);
             }
             launch_seg_merge(r->sc, p, b, 1, k, s);
             if (timed) HIPCHK(hipEventRecord(e[3], s));
@@ -1242,7 +1283,7 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
     if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
-        t->piece_order < 0 || t->piece_order > 2 || t->packet_bounces < 0)
+        t->piece_order < 0 || t->piece_order > 4 || t->packet_bounces < 0 || t->beam_leaf < 0 || t->beam_leaf > 1)
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     r->use_graph = t->graph != 0;
@@ -1359,6 +1400,289 @@ extern "C" int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iteration
     return UPBP_OK;
 }
 
+// ---- batched jobs in one launch per stage -----------------------------------------------------
+// A round renders one iteration of every job. The wavefront stages -- light generate, each light
+// bounce, camera generate, each camera bounce with its two segment merges, splat and accumulate --
+// run as one launch each over all jobs: the block maps give each block its job and its block in
+// that job's own grid, and the persistent merge kernels walk every job's pieces. The per-job work
+// between them (parameters and counters, photon lists and their LBVH builds) is enqueued on each
+// job's own stream, forked from the batch stream and joined back, so the jobs' builds overlap. The
+// round is captured in a HIP graph once (after one eager round with the same settings) and replayed.
+struct upbp_batch {
+    std::string err;
+    std::vector<upbp_renderer*> rs;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> join; // 2 per job
+    std::vector<BatchJob> hjobs;
+    DevBuf jobs, map_light, map_camgen, map_cam, map_pix, map_acc;
+    // light and camera bounce maps list the jobs by decreasing max path length: bounce k runs the
+    // first light_upto[k] / cam_upto[k] blocks, those of the jobs whose paths reach bounce k
+    std::vector<uint32_t> light_upto, cam_upto;
+    uint32_t nb_light = 0, nb_camgen = 0, nb_pix = 0, nb_acc = 0;
+    int lmax = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint64_t> key_graph, key_eager; // the jobs' settings generations at capture / last eager round
+    uint64_t rounds = 0, graph_rounds = 0, eager_rounds = 0, solo_rounds = 0;
+};
+
+namespace {
+
+int bfail(upbp_batch* bt, const std::string& m, int code)
+{
+    if (bt) bt->err = m;
+    g_err = m;
+    return code;
+}
+
+#define BHIPCHK(expr)                                                                                                   \
+    do {                                                                                                                \
+        hipError_t e_ = (expr);                                                                                         \
+        if (e_ != hipSuccess) return bfail(bt, std::string(#expr) + ": " + hipGetErrorString(e_), UPBP_E_HIP);          \
+    } while (0)
+
+// the work of one round on the batch stream (eager or under capture)
+int batch_enqueue(upbp_batch* bt, std::vector<IterParams>& ps)
+{
+    hipStream_t S = bt->stream;
+    const uint32_t n = uint32_t(bt->rs.size());
+    const BatchJob* jobs = bt->jobs.as<BatchJob>();
+    BHIPCHK(hipEventRecord(bt->fork[0], S));
+    for (uint32_t j = 0; j < n; j++) {
+        upbp_renderer* r = bt->rs[j];
+        BHIPCHK(hipStreamWaitEvent(r->stream, bt->fork[0], 0));
+        if (enqueue_head(r, r->stage[0])) return bfail(bt, "job " + std::to_string(j) + ": " + r->err, UPBP_E_HIP);
+        BHIPCHK(hipEventRecord(bt->join[2 * j], r->stream));
+        BHIPCHK(hipStreamWaitEvent(S, bt->join[2 * j], 0));
+    }
+    launch_light_generate_batch(jobs, bt->map_light.as<uint2>(), bt->nb_light, S);
+    for (int k = 0; k < bt->lmax; k++) launch_light_bounce_batch(jobs, bt->map_light.as<uint2>(), bt->light_upto[k], k, S);
+    BHIPCHK(hipEventRecord(bt->fork[1], S));
+    for (uint32_t j = 0; j < n; j++) {
+        upbp_renderer* r = bt->rs[j];
+        BHIPCHK(hipStreamWaitEvent(r->stream, bt->fork[1], 0));
+        if (enqueue_maps(r, ps[j], r->stage[0])) return bfail(bt, "job " + std::to_string(j) + ": " + r->err, UPBP_E_HIP);
+        BHIPCHK(hipEventRecord(bt->join[2 * j + 1], r->stream));
+        BHIPCHK(hipStreamWaitEvent(S, bt->join[2 * j + 1], 0));
+    }
+    launch_camera_generate_batch(jobs, bt->map_camgen.as<uint2>(), bt->nb_camgen, S);
+    for (int k = 0; k < bt->lmax; k++) {
+        launch_camera_bounce_batch(jobs, bt->map_cam.as<uint2>(), bt->cam_upto[k], k, S);
+        launch_seg_merge_batch(jobs, n, 0, k, S);
+        launch_seg_merge_batch(jobs, n, 1, k, S);
+    }
+    launch_camera_finish_batch(jobs, bt->map_pix.as<uint2>(), bt->nb_pix, bt->map_acc.as<uint2>(), bt->nb_acc, S);
+    for (uint32_t j = 0; j < n; j++) {
+        upbp_renderer* r = bt->rs[j];
+        BHIPCHK(hipMemcpyAsync(&r->stage[0].ovf, r->ovf.p, 4, hipMemcpyDeviceToHost, S));
+    }
+    BHIPCHK(hipGetLastError());
+    return UPBP_OK;
+}
+
+} // namespace
+
+extern "C" int upbp_batch_create(upbp_renderer* const* rs, int32_t n, upbp_batch** out)
+{
+    if (!out || n < 1 || !rs) return bfail(nullptr, "invalid batch", UPBP_E_ARG);
+    *out = nullptr;
+    for (int32_t i = 0; i < n; i++)
+        if (!rs[i]) return bfail(nullptr, "null renderer in batch", UPBP_E_ARG);
+    {
+        std::vector<upbp_renderer*> sorted(rs, rs + n);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return bfail(nullptr, "a renderer appears twice in the batch", UPBP_E_ARG);
+    }
+    auto* bt = new upbp_batch();
+    bt->rs.assign(rs, rs + n);
+    auto cleanup = [&](int code) {
+        upbp_batch_destroy(bt);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&bt->fork[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&bt->fork[1], hipEventDisableTiming) != hipSuccess)
+        return cleanup(bfail(nullptr, "batch stream or event creation failed", UPBP_E_HIP));
+    bt->join.assign(size_t(2 * n), nullptr);
+    for (hipEvent_t& e : bt->join)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return cleanup(bfail(nullptr, "batch event creation failed", UPBP_E_HIP));
+    // block maps; the bounce maps by decreasing max path length
+    std::vector<uint32_t> order(static_cast<size_t>(n));
+    for (int32_t i = 0; i < n; i++) order[size_t(i)] = uint32_t(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return rs[a]->cfg.max_path_length > rs[b]->cfg.max_path_length;
+    });
+    std::vector<uint2> ml, mg, mc, mp, ma;
+    for (uint32_t j : order) {
+        const upbp_renderer* r = rs[j];
+        for (uint32_t k = 0; k < (r->n_light + 255) / 256; k++) ml.push_back(make_uint2(j, k));
+        for (uint32_t k = 0; k < (r->n_pixels + 255) / 256; k++) mc.push_back(make_uint2(j, k));
+        bt->lmax = std::max(bt->lmax, int(r->cfg.max_path_length));
+    }
+    bt->light_upto.assign(size_t(bt->lmax), 0u);
+    bt->cam_upto.assign(size_t(bt->lmax), 0u);
+    {
+        uint32_t al = 0, ac = 0;
+        for (uint32_t j : order) {
+            al += (rs[j]->n_light + 255) / 256;
+            ac += (rs[j]->n_pixels + 255) / 256;
+            for (int k = 0; k < int(rs[j]->cfg.max_path_length); k++) {
+                bt->light_upto[size_t(k)] = al;
+                bt->cam_upto[size_t(k)] = ac;
+            }
+        }
+    }
+    bt->hjobs.resize(size_t(n));
+    for (int32_t j = 0; j < n; j++) {
+        const upbp_renderer* r = rs[j];
+        const uint32_t res_x = uint32_t(r->sc.cam.res_x);
+        const uint32_t res_y = (r->n_pixels + res_x - 1) / res_x;
+        const uint32_t tiles_x = (res_x + 15) / 16, tiles_y = (res_y + 15) / 16;
+        for (uint32_t k = 0; k < tiles_x * tiles_y; k++) mg.push_back(make_uint2(uint32_t(j), k));
+        for (uint32_t k = 0; k < (r->n_pixels + 255) / 256; k++) mp.push_back(make_uint2(uint32_t(j), k));
+        for (uint32_t k = 0; k < (3 * r->n_pixels + 255) / 256; k++) ma.push_back(make_uint2(uint32_t(j), k));
+        BatchJob& J = bt->hjobs[size_t(j)];
+        J.res_x = res_x;
+        J.tiles_x = tiles_x;
+        J.fres_x = float(r->cfg.res_x);
+        J.fres_y = float(r->cfg.res_y);
+    }
+    bt->nb_light = uint32_t(ml.size());
+    bt->nb_camgen = uint32_t(mg.size());
+    bt->nb_pix = uint32_t(mp.size());
+    bt->nb_acc = uint32_t(ma.size());
+    struct M {
+        DevBuf* buf;
+        const std::vector<uint2>* v;
+    };
+    for (const M& m : {M{&bt->map_light, &ml}, M{&bt->map_camgen, &mg}, M{&bt->map_cam, &mc}, M{&bt->map_pix, &mp},
+                       M{&bt->map_acc, &ma}})
+        if (m.buf->ensure(std::max<size_t>(m.v->size(), 1) * sizeof(uint2)) ||
+            (!m.v->empty() &&
+             hipMemcpy(m.buf->p, m.v->data(), m.v->size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess))
+            return cleanup(bfail(nullptr, "batch block maps: device allocation failed", UPBP_E_HIP));
+    if (bt->jobs.ensure(size_t(n) * sizeof(BatchJob)))
+        return cleanup(bfail(nullptr, "batch jobs: device allocation failed", UPBP_E_HIP));
+    *out = bt;
+    return UPBP_OK;
+}
+
+extern "C" int upbp_batch_run(upbp_batch* bt, const int32_t* iterations, int32_t use_graph)
+{
+    if (!bt || !iterations) return bfail(bt, "invalid argument", UPBP_E_ARG);
+    const uint32_t n = uint32_t(bt->rs.size());
+    // iterations still pending on a job's own stream come first (they may render again)
+    for (upbp_renderer* r : bt->rs) {
+        const int e = settle_all(r);
+        if (e) return bfail(bt, r->err, e);
+        BHIPCHK(hipStreamSynchronize(r->stream));
+    }
+    std::vector<IterParams> ps(n);
+    bool batched = true;
+    for (uint32_t j = 0; j < n; j++) {
+        upbp_renderer* r = bt->rs[j];
+        setup_params(r, iterations[j], ps[j]);
+        // a round in one launch per stage needs every job sized, plain (no training, no MIS log)
+        // and tracing both subpaths
+        if (r->need_sizing || r->gtrain || r->bufs.mislog || !ps[j].trace_light || !ps[j].trace_camera)
+            batched = false;
+    }
+    bt->rounds++;
+    if (!batched) { // each job alone on its own stream (a sizing pass, training, a special mode)
+        for (uint32_t j = 0; j < n; j++) {
+            upbp_renderer* r = bt->rs[j];
+            int e = upbp_run_iteration(r, iterations[j], nullptr);
+            if (!e) e = settle_all(r);
+            if (e) return bfail(bt, "job " + std::to_string(j) + ": " + r->err, e);
+        }
+        bt->solo_rounds++;
+        return UPBP_OK;
+    }
+    std::vector<uint64_t> key(n);
+    for (uint32_t j = 0; j < n; j++) {
+        upbp_renderer* r = bt->rs[j];
+        Staging& st = r->stage[0];
+        st.p = ps[j];
+        st.ovf = 0;
+        BatchJob& J = bt->hjobs[j];
+        J.sc = r->sc;
+        J.b = r->bufs;
+        J.b.gpend = J.b.graw = J.b.gsamp = nullptr; // no training samples are collected in a batch
+        J.b.gcount = nullptr;
+        J.b.gcap = 0u;
+        J.b.gring = 1u;
+        J.ovf = r->ovf.as<uint32_t>();
+        key[j] = r->gen;
+    }
+    BHIPCHK(hipMemcpy(bt->jobs.p, bt->hjobs.data(), n * sizeof(BatchJob), hipMemcpyHostToDevice));
+    const bool capture = use_graph && key == bt->key_eager;
+    if (capture && (!bt->gexec || key != bt->key_graph)) {
+        if (bt->gexec) (void)hipGraphExecDestroy(bt->gexec);
+        bt->gexec = nullptr;
+        BHIPCHK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+        const int e = batch_enqueue(bt, ps);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(bt->stream, &g);
+        if (e || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return e ? e : bfail(bt, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce), UPBP_E_HIP);
+        }
+        const hipError_t ie = hipGraphInstantiate(&bt->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return bfail(bt, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie), UPBP_E_HIP);
+        bt->key_graph = key;
+    }
+    if (capture) {
+        BHIPCHK(hipGraphLaunch(bt->gexec, bt->stream));
+        bt->graph_rounds++;
+    } else {
+        const int e = batch_enqueue(bt, ps);
+        if (e) return e;
+        bt->key_eager = key;
+        bt->eager_rounds++;
+    }
+    BHIPCHK(hipStreamSynchronize(bt->stream));
+    // a job whose photon lists overflowed was not accumulated: it sizes them and renders again alone
+    for (uint32_t j = 0; j < n; j++) {
+        upbp_renderer* r = bt->rs[j];
+        if (!r->stage[0].ovf) continue;
+        r->need_sizing = true;
+        int e = run_one(r, iterations[j], nullptr);
+        if (!e) e = settle_all(r);
+        if (e) return bfail(bt, "job " + std::to_string(j) + ": " + r->err, e);
+    }
+    return UPBP_OK;
+}
+
+extern "C" int upbp_batch_counts(const upbp_batch* bt, uint64_t* rounds, uint64_t* graph_rounds,
+                                 uint64_t* eager_rounds, uint64_t* solo_rounds)
+{
+    if (!bt) return bfail(nullptr, "null batch", UPBP_E_ARG);
+    if (rounds) *rounds = bt->rounds;
+    if (graph_rounds) *graph_rounds = bt->graph_rounds;
+    if (eager_rounds) *eager_rounds = bt->eager_rounds;
+    if (solo_rounds) *solo_rounds = bt->solo_rounds;
+    return UPBP_OK;
+}
+
+extern "C" const char* upbp_batch_last_error(const upbp_batch* bt) { return bt ? bt->err.c_str() : g_err.c_str(); }
+
+extern "C" void upbp_batch_destroy(upbp_batch* bt)
+{
+    if (!bt) return;
+    if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+    if (bt->gexec) (void)hipGraphExecDestroy(bt->gexec);
+    for (hipEvent_t e : bt->fork)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : bt->join)
+        if (e) (void)hipEventDestroy(e);
+    for (DevBuf* d : {&bt->jobs, &bt->map_light, &bt->map_camgen, &bt->map_cam, &bt->map_pix, &bt->map_acc}) d->release();
+    if (bt->stream) (void)hipStreamDestroy(bt->stream);
+    delete bt;
+}
+
 extern "C" int upbp_set_mis_log(upbp_renderer* r, uint32_t capacity, uint32_t pixel_stride, uint32_t light_paths)
 {
     if (!r) return fail(r, "null renderer", UPBP_E_ARG);
@@ -1437,8 +1761,8 @@ extern "C" void upbp_destroy(upbp_renderer* r)
     for (DevBuf* d : {&r->lstate, &r->lstack, &r->cstate, &r->cstack, &r->q0, &r->q1, &r->qcount, &r->verts, &r->nvert,
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
-                      &r->pts_m, &r->geo_u, &r->geo_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
-                      &r->xwork, &r->pkeys, &r->pvals, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount,
+                      &r->pts_m, &r->geo_u, &r->geo_b, &r->ref_u, &r->ref_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
+                      &r->xwork, &r->pkeys, &r->pvals, &r->unitn, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount,
                       &r->mislog, &r->miscount, &r->dparams, &r->dmaps, &r->mapn, &r->ovf})
         d->release();
     if (r->stage) (void)hipHostFree(r->stage);

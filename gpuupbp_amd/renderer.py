@@ -259,6 +259,47 @@ def run_batch(renderers, iterations, threads=8):
         r.iterations = max(r.iterations, int(it) + 1)
 
 
+class Batch:
+    """upbp_batch_*: many renderers rendered in rounds of one launch per stage (see the C ABI).
+    run(iterations) renders one iteration of each renderer; counts() returns (rounds, graph
+    replays, eager batched rounds, rounds of jobs alone). Close it before its renderers."""
+
+    def __init__(self, renderers, use_graph=True):
+        self.renderers = list(renderers)
+        self.use_graph = int(bool(use_graph))
+        hs = (C.c_void_p * len(self.renderers))(*[r.h for r in self.renderers])
+        self.h = C.c_void_p()
+        rc = capi.lib().upbp_batch_create(hs, len(self.renderers), C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("upbp_batch_create failed: %d %s" % (rc, capi.lib().upbp_batch_last_error(None).decode()))
+
+    def run(self, iterations):
+        its = np.ascontiguousarray(iterations, np.int32)
+        if len(its) != len(self.renderers):
+            raise ValueError("one iteration per renderer")
+        rc = capi.lib().upbp_batch_run(self.h, _p(its), self.use_graph)
+        if rc != 0:
+            raise RuntimeError("upbp_batch_run failed: %d %s" % (rc, capi.lib().upbp_batch_last_error(self.h).decode()))
+        for r, it in zip(self.renderers, its):
+            r.iterations = max(r.iterations, int(it) + 1)
+
+    def counts(self):
+        v = [C.c_uint64() for _ in range(4)]
+        capi.lib().upbp_batch_counts(self.h, *[C.byref(x) for x in v])
+        return tuple(int(x.value) for x in v)
+
+    def close(self):
+        if self.h:
+            capi.lib().upbp_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def closest_hit(verts, rays):
     verts = np.ascontiguousarray(verts, np.float32)
     rays = np.ascontiguousarray(rays, np.float32)

@@ -286,11 +286,18 @@ typedef struct upbp_tuning {
     int32_t piece_order;         /* order in which the B-B1D merge takes query pieces: 0 as the
                                     camera pass wrote them (default); 1 bucketed by region, each
                                     XCD starting on its own; 2 sorted by the Morton code of the
-                                    piece midpoint (one host synchronisation per bounce) */
+                                    piece midpoint (one host synchronisation per bounce); 3 units
+                                    of 64 consecutive pieces kept whole, sorted by the Morton code
+                                    of their middle piece and cut into one range per XCD; 4 the
+                                    same sorted units as one range */
     int32_t packet_bounces;      /* camera bounces below this walk the B-B1D beam tree as packets
                                     of 64 query pieces per wave (one node fetch per wave); later
                                     bounces walk it per piece (default 0: per piece everywhere,
                                     the faster of the two on the blob grid) */
+    int32_t beam_leaf;           /* B-B1D leaf test: 0 reads the sorted 48-byte sub-beam records
+                                    (default); 1 reads 8-byte sorted references (beam slot,
+                                    sub-beam index) and the beam's own record, which all its
+                                    sub-beams share (a smaller working set) */
     int32_t graph;               /* 1: capture the iteration's work in a HIP graph once and replay it
                                     (iterations without statistics, online training or Morton piece
                                     order, which read back to the host); 0: launch kernel by kernel */
@@ -302,6 +309,29 @@ int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);
  * owns a HIP stream, so the jobs' kernels overlap on the GPU; a renderer must not appear twice
  * in one batch. Returns the first job's error, if any. */
 int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iterations, int32_t n, int32_t threads);
+
+/* Batched jobs in one launch per stage (no reference counterpart; the reference renders one job
+ * per process, renderer.cpp:122-195). A batch groups n independent renderers (each at most once);
+ * upbp_batch_run renders one iteration of each (iterations[i] for renderer i) as a round in which
+ * every wavefront stage -- light generate, each light bounce, camera generate, each camera bounce
+ * and its P-B2D and B-B1D segment merges, splat, accumulate -- is one kernel launch over all jobs,
+ * the persistent merge kernels walking every job's query pieces. Each job's parameters, counters,
+ * photon lists and LBVH builds go on its own stream between those launches, overlapping the other
+ * jobs'. With use_graph, the round is captured in a HIP graph after one eager round with the same
+ * settings and replayed (captured again whenever a job's settings or buffers change). A round
+ * whose jobs cannot all take part -- a job still sizing its photon lists, training its proposal,
+ * logging MIS weights or tracing only one subpath -- renders each job alone instead, as does a job
+ * whose photon lists overflowed in the round (its batched iteration is then not accumulated).
+ * The images equal the renderers' own iterations, up to the order of floating-point atomic sums;
+ * the merges walk the pieces in the camera pass's order whatever the piece_order tuning. */
+typedef struct upbp_batch upbp_batch;
+int upbp_batch_create(upbp_renderer* const* rs, int32_t n, upbp_batch** out);
+int upbp_batch_run(upbp_batch* b, const int32_t* iterations, int32_t use_graph);
+/* rounds run, and of those: replayed graphs, batched rounds launched eagerly, rounds of jobs alone */
+int upbp_batch_counts(const upbp_batch* b, uint64_t* rounds, uint64_t* graph_rounds, uint64_t* eager_rounds,
+                      uint64_t* solo_rounds);
+const char* upbp_batch_last_error(const upbp_batch* b);
+void upbp_batch_destroy(upbp_batch* b); /* the renderers stay; destroy the batch before them */
 
 /* Autotuner (no reference counterpart): a coordinate search over subbeam_factor {8, 12, 16,
  * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2}, piece_order {0, 2} and

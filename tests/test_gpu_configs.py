@@ -13,7 +13,8 @@
   1/20 the resolution per axis path for path; at full size, counts against the oracle's light
   pass and the share of paths ended by the intersector's fixed limits (8 imaginary hits, 10
   segments, as in the reference) below 0.2%.
-* Batched jobs (64 jobs, seed 2): every job's image from the batched path equals its solo render.
+* Batched jobs (64 jobs, seed 2): every job's image from rounds of one launch per stage over all
+  jobs (replayed from a captured HIP graph) equals its own iterations.
 
 "Path for path" means the same random streams on both sides: all but a few percent of the pixels
 agree to FP32 rounding (paths whose transcendental results differ in the last bit may take another
@@ -212,6 +213,9 @@ def test_stress_full_size_bounds(workdir):
 
 # ---- batched jobs -----------------------------------------------------------------------------
 def test_batched_mix_matches_each_job_alone(workdir):
+    # rounds of one launch per stage over all 64 jobs: the first sizes the jobs (alone), the second
+    # runs batched eagerly, the third and fourth replay the captured round
+    n_it = 4
     jobs = wl.batch_mix(workdir, 64, 2)
     alone, handles = [], []
     for obj, argv, _, _, _ in jobs:
@@ -220,16 +224,22 @@ def test_batched_mix_matches_each_job_alone(workdir):
         sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
         handles.append((sh, cfg))
         with gpu.Renderer(sh.ptr, cfg) as r:
-            r.run_iteration(0, want_stats=False)
-            alone.append(r.read_accum(1))
+            for i in range(n_it):
+                r.run_iteration(i, want_stats=False)
+            alone.append(r.read_accum(n_it))
     rs = [gpu.Renderer(sh.ptr, cfg) for sh, cfg in handles]
-    gpu.run_batch(rs, [0] * len(rs), threads=8)
+    with gpu.Batch(rs) as bt:
+        for i in range(n_it):
+            bt.run([i] * len(rs))
+        counts = bt.counts()
+    print("rounds, graph replays, eager batched, alone:", counts)
+    assert counts == (n_it, n_it - 2, 1, 1), counts
     for r, ref, (obj, argv, _, kind, _) in zip(rs, alone, jobs):
-        img = r.read_accum(1)
+        img = r.read_accum(n_it)
         r.close()
         rel = np.abs(img - ref) / np.maximum(np.abs(ref), 1e-3)
         # the same paths; only the order of floating-point atomic sums may differ
-        assert np.mean(rel < 1e-4) > 0.99, (kind, argv)
+        assert np.isfinite(img).all() and np.mean(rel < 1e-4) > 0.99, (kind, argv)
         assert abs(img.mean() - ref.mean()) <= 1e-5 * abs(ref.mean()) + 1e-12, (kind, argv)
     for sh, _ in handles:
         sh.close()

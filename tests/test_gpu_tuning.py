@@ -13,7 +13,9 @@ pytestmark = pytest.mark.gpu
 SETTINGS = [dict(subbeam_factor=8.0, query_piece_factor=32.0), dict(subbeam_factor=24.0, query_piece_factor=128.0),
             dict(subbeam_factor=0.0, query_piece_factor=0.0), dict(merge_blocks_per_cu=1), dict(piece_order=1),
             dict(piece_order=2), dict(packet_bounces=0), dict(packet_bounces=64),
-            dict(packet_bounces=64, piece_order=2), dict(packet_bounces=64, subbeam_factor=0.0, query_piece_factor=0.0)]
+            dict(packet_bounces=64, piece_order=2), dict(packet_bounces=64, subbeam_factor=0.0, query_piece_factor=0.0),
+            dict(piece_order=3), dict(piece_order=4), dict(piece_order=3, graph=1), dict(beam_leaf=1),
+            dict(beam_leaf=1, piece_order=3, packet_bounces=64)]
 
 
 def _render(obj, tuning):
