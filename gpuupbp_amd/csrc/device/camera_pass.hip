@@ -516,7 +516,7 @@ template <bool LOG>
 __device__ __forceinline__ void camera_bounce_body(const DScene& sc, const Bufs& b, int bounce, uint32_t vb)
 {
     const IterParams& p = *b.params;
-    const uint32_t i = vb * 256u + threadIdx.x;
+    const uint32_t i = vb * blockDim.x + threadIdx.x;
     const uint32_t n = b.qcount[64 + bounce];
     if (i >= n) return;
     const uint32_t pix = b.queue[bounce & 1][i];
@@ -635,9 +635,11 @@ void launch_camera_finish_batch(const BatchJob* jobs, const uint2* bmap_pix, uin
 void launch_camera_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)
 {
     if (b.mislog)
-        hipLaunchKernelGGL(k_camera_bounce<true>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, b, bounce);
+        hipLaunchKernelGGL(k_camera_bounce<true>, dim3((p.n_pixels + p.block - 1) / p.block), dim3(p.block), 0, s, sc,
+                           b, bounce);
     else
-        hipLaunchKernelGGL(k_camera_bounce<false>, dim3((p.n_pixels + 255) / 256), dim3(256), 0, s, sc, b, bounce);
+        hipLaunchKernelGGL(k_camera_bounce<false>, dim3((p.n_pixels + p.block - 1) / p.block), dim3(p.block), 0, s, sc,
+                           b, bounce);
 }
 
 void launch_camera_finish(const IterParams& p, const Bufs& b, float res_x, float res_y, const uint32_t* ovf,

@@ -32,10 +32,8 @@ struct GuideNet {
     // raw outputs of n device samples (stride floats each: position in [0,1]^3, arrival
     // direction) into out (rows of 48 floats, the first 40 used)
     int forward(const float* samples, int stride, uint32_t n, hipStream_t s);
-    // the same into caller-provided buffers: a0..a3 pad16(n) x 64 BF16 each (a2 may alias a0 and a3
-    // may alias a1), o pad16(n) x 48 floats
-    int forward_into(const float* samples, int stride, uint32_t n, uint16_t* a0, uint16_t* a1, uint16_t* a2,
-                     uint16_t* a3, float* o, hipStream_t s) const;
+    // the raw outputs alone into o (pad16(n) x 48 floats); nothing else is written
+    int forward_into(const float* samples, int stride, uint32_t n, float* o, hipStream_t s) const;
     // loss sum and flat gradient of n device training samples (10 floats each)
     int gradient(const float* samples, uint32_t n, hipStream_t s);
     // the same for n samples drawn at random (seeded by draw) from a device pool of n_pool

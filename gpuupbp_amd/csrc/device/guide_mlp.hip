@@ -1,6 +1,8 @@
 // Network of the learned scattering proposal: forward, backward and Adam on the GPU, with BF16
-// operands on the MFMA units (v_mfma_f32_16x16x16_bf16) and FP32 accumulation. No counterpart
-// in the reference renderer.
+// operands on the MFMA units (v_mfma_f32_16x16x32_bf16) and FP32 accumulation: one fused kernel
+// for the forward pass (encoding and all four layers per 16-row tile, weights and activations in
+// LDS), one for the backward deltas of all layers, and one per layer for the weight gradients
+// (batch chunks staged in LDS). No counterpart in the reference renderer.
 //
 // Network: 51 inputs (48 Fourier features of the position in [0,1]^3 -- sin and cos of
 // 2^k pi x_j, k = 0..7, feature (j * 8 + k) * 2 + {0: sin, 1: cos} -- and the arrival
@@ -14,7 +16,8 @@
 // Layout: FP32 master parameters in the flat order W1[64][51], b1, W2[64][64], b2, W3[64][64],
 // b3, W4[40][64], b4 (14248 values), Adam moments beside them, and BF16 copies padded for the
 // MFMA tiles: W1 as [64][64] (inputs 51..63 zero), W4 as [48][64] (outputs 40..47 zero).
-// Activations of a training batch (BF16, rows of 64 or 48) stay in HBM between the layer GEMMs.
+// A training batch's activations and deltas (BF16, rows of 64 or 48) go to HBM once each, for the
+// weight gradients.
 #include "guide.h"
 #include "guide_mlp.h"
 
@@ -28,7 +31,7 @@ namespace upbp {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 51, kP = 64, kHid = 64, kOut = 40, kOutP = 48, kLobesN = 8;
@@ -55,74 +58,207 @@ __host__ __device__ inline float bf2f(uint16_t h)
     return f;
 }
 
-// ---- encoding -------------------------------------------------------------------------
-// samples: stride floats per sample, position in [0,1]^3 then the arrival direction
-__global__ void __launch_bounds__(256) k_encode(const float* samples, int stride, uint32_t n, uint32_t n_pad,
-                                                uint16_t* x0)
+// ---- fused forward pass -----------------------------------------------------------------
+// One wave takes 16 rows at a time: it encodes them into its LDS tile (16 rows of 64 BF16), then
+// runs the four layers on v_mfma_f32_16x16x32_bf16, each layer reading its input tile as A
+// fragments and writing its activations (bias, ReLU, BF16) back over the same tile. The block's
+// waves share the weights, copied once into LDS. Rows are padded to 72 BF16 in LDS so that the
+// 16-byte fragment loads of 16 rows at one column spread over the banks.
+// Lane l holds A(l % 16, 8 (l / 16) + e) and B(8 (l / 16) + e, l % 16), e = 0..7 (A and B share the
+// k order, so the product does not depend on how the instruction orders k within a lane), and
+// C(4 (l / 16) + e, l % 16), e = 0..3. With KEEP the encoded inputs and the three hidden layers
+// are also stored (rows of 64 BF16) for the backward pass.
+constexpr int kLdsRow = 72;
+constexpr int kWRows = 3 * kP + kOutP; // the padded BF16 weights: 240 rows of 64
+
+__device__ __forceinline__ void wave_lds_fence()
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_pad) return;
-    uint16_t* row = x0 + size_t(i) * kP;
-    if (i >= n) {
-        for (int k = 0; k < kP; k++) row[k] = 0;
-        return;
-    }
-    const float* s = samples + size_t(i) * stride;
-    for (int j = 0; j < 3; j++)
-        for (int k = 0; k < 8; k++) {
-            const float a = float(1 << k) * kPi * s[j];
-            row[(j * 8 + k) * 2] = f2bf(sinf(a));
-            row[(j * 8 + k) * 2 + 1] = f2bf(cosf(a));
-        }
-    for (int j = 0; j < 3; j++) row[48 + j] = f2bf(s[3 + j]);
-    for (int k = kIn; k < kP; k++) row[k] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// ---- GEMM on the MFMA units ------------------------------------------------------------
-// C(i, j) = sum_k A(i, k) B(k, j) over BF16 operands with strided access,
-// A(i, k) = A[i * sai + k * sak], B(k, j) = B[k * sbk + j * sbj]; one wave per 16x16 tile of C,
-// four waves per block, gridDim.z splits of the k range (M, N, K multiples of 16; K split
-// evenly). Lane l holds A(i0 + l % 16, k0 + 4 (l / 16) + e), B(k0 + 4 (l / 16) + e, j0 + l % 16)
-// and C(i0 + 4 (l / 16) + e, j0 + l % 16), e = 0..3.
-enum Epi : int {
-    EPI_RELU_BF16 = 0,  // out_bf16[i * ldo + j] = relu(c + bias[j]), also the next layer's input
-    EPI_BIAS_F32 = 1,   // out_f32[i * ldo + j] = c + bias[j]
-    EPI_MASK_BF16 = 2,  // out_bf16[i * ldo + j] = c if mask[i * ldo + j] > 0 else 0 (ReLU derivative)
-    EPI_ATOMIC_F32 = 3  // out_f32[i * ldo + j] += c (split-k weight gradients)
-};
-
-template <int EPI>
-__global__ void __launch_bounds__(256) k_gemm(const uint16_t* A, int sai, int sak, const uint16_t* B, int sbk, int sbj,
-                                              int M, int N, int K, const float* bias, const uint16_t* mask,
-                                              uint16_t* out_bf, float* out_f, int ldo)
+template <bool KEEP>
+__global__ void __launch_bounds__(256) k_mlp_forward(const float* samples, int stride, uint32_t n, uint32_t np,
+                                                     const uint16_t* wq, const float* bq, uint16_t* x0, uint16_t* h1,
+                                                     uint16_t* h2, uint16_t* h3, float* out)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tiles_n = N / 16;
-    const int tile = blockIdx.x * 4 + wave;
-    if (tile >= (M / 16) * tiles_n) return;
-    const int i0 = (tile / tiles_n) * 16, j0 = (tile % tiles_n) * 16;
-    const int kc = K / int(gridDim.z);
-    const int kb = int(blockIdx.z) * kc, ke = kb + kc;
-    const int r = lane & 15, q = lane >> 4;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        bf16x4 a, b;
-        for (int e = 0; e < 4; e++) {
-            const int k = k0 + 4 * q + e;
-            a[e] = short(A[size_t(i0 + r) * sai + size_t(k) * sak]);
-            b[e] = short(B[size_t(k) * sbk + size_t(j0 + r) * sbj]);
+    __shared__ __align__(16) uint16_t W[kWRows * kLdsRow];
+    __shared__ float Bs[256];
+    __shared__ __align__(16) uint16_t act[4][16 * kLdsRow];
+    for (int i = threadIdx.x; i < kWRows * 8; i += 256)
+        *reinterpret_cast<uint4*>(&W[(i >> 3) * kLdsRow + (i & 7) * 8]) = reinterpret_cast<const uint4*>(wq)[i];
+    if (threadIdx.x < 3 * kP + kOutP) Bs[threadIdx.x] = bq[threadIdx.x];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t r = lane & 15u, q = lane >> 4;
+    uint16_t* A = act[w];
+    uint16_t* keep[4] = {x0, h1, h2, h3};
+    const uint32_t tiles = np / 16u;
+    for (uint32_t t = blockIdx.x * 4u + w; t < tiles; t += gridDim.x * 4u) {
+        // encoding: lane (r, q) writes columns [16 q, 16 q + 16) of tile row r: for q < 3 the sin and
+        // cos of 2^k pi x_q, k = 0..7 (features (q * 8 + k) * 2 + {0, 1}), for q = 3 the arrival
+        // direction and zeros
+        const uint32_t row = t * 16u + r;
+        uint16_t* dst = A + r * kLdsRow + 16u * q;
+        if (row < n) {
+            const float* smp = samples + size_t(row) * stride;
+            if (q < 3u) {
+                const float x = smp[q];
+                for (int k = 0; k < 8; k++) {
+                    const float a = float(1 << k) * kPi * x;
+                    dst[2 * k] = f2bf(sinf(a));
+                    dst[2 * k + 1] = f2bf(cosf(a));
+                }
+            } else {
+                for (int k = 0; k < 16; k++) dst[k] = k < 3 ? f2bf(smp[3 + k]) : uint16_t(0);
+            }
+        } else {
+            for (int k = 0; k < 16; k++) dst[k] = 0;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        for (int L = 0; L < 4; L++) {
+            wave_lds_fence();
+            if (KEEP) { // the layer's input tile to HBM: 16 rows x 128 bytes, 32 bytes per lane
+                const uint32_t rr = lane >> 2, cc = (lane & 3u) * 16u;
+                const uint4* src = reinterpret_cast<const uint4*>(A + rr * kLdsRow + cc);
+                uint4* to = reinterpret_cast<uint4*>(keep[L] + size_t(t * 16u + rr) * kP + cc);
+                to[0] = src[0];
+                to[1] = src[1];
+            }
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(A + r * kLdsRow + 8u * q);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(A + r * kLdsRow + 32u + 8u * q);
+            wave_lds_fence(); // every lane holds its fragments before the tile is overwritten
+            const uint16_t* Wl = W + L * kP * kLdsRow;
+            const int nt = L < 3 ? kP / 16 : kOutP / 16;
+            for (int j = 0; j < nt; j++) {
+                const uint16_t* wr = Wl + (16 * j + int(r)) * kLdsRow + 8 * int(q);
+                const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(wr);
+                const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(wr + 32);
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
+                const int c = 16 * j + int(r);
+                const float bias = Bs[kP * L + c];
+                for (int e = 0; e < 4; e++) {
+                    const int i = 4 * int(q) + e;
+                    const float v = acc[e] + bias;
+                    if (L < 3) A[i * kLdsRow + c] = f2bf(fmaxf(v, 0.f));
+                    else out[size_t(t * 16u + uint32_t(i)) * kOutP + c] = v;
+                }
+            }
+        }
     }
-    for (int e = 0; e < 4; e++) {
-        const int i = i0 + 4 * q + e, j = j0 + r;
-        const size_t o = size_t(i) * ldo + j;
-        const float c = acc[e];
-        if (EPI == EPI_RELU_BF16) out_bf[o] = f2bf(fmaxf(c + bias[j], 0.f));
-        else if (EPI == EPI_BIAS_F32) out_f[o] = c + bias[j];
-        else if (EPI == EPI_MASK_BF16) out_bf[o] = bf2f(mask[o]) > 0.f ? f2bf(c) : uint16_t(0);
-        else atomicAdd(&out_f[o], c);
+}
+
+// ---- fused backward deltas ---------------------------------------------------------------
+// Per 16-row tile: d3 = (dO W4) o [h3 > 0], d2 = (d3 W3) o [h2 > 0], d1 = (d2 W2) o [h1 > 0] on
+// v_mfma_f32_16x16x32_bf16, the delta tile staying in LDS from layer to layer. B(k, j) = W_l[k][j]
+// is read from the transposed weights, staged once per block in LDS (W4's 48 output rows padded
+// to 64 with zeros, as the delta of the outputs is); the ReLU masks are the forward pass's stored
+// activations, staged per tile. Each delta tile also goes to HBM for the weight gradients.
+__global__ void __launch_bounds__(256) k_mlp_backward(const uint16_t* wq, const uint16_t* dO, const uint16_t* h1,
+                                                      const uint16_t* h2, const uint16_t* h3, uint32_t np,
+                                                      uint16_t* d1, uint16_t* d2, uint16_t* d3)
+{
+    __shared__ __align__(16) uint16_t WT[3][kP * kLdsRow];
+    __shared__ __align__(16) uint16_t D[4][16 * kLdsRow];
+    __shared__ __align__(16) uint16_t H[4][16 * kLdsRow];
+    for (int idx = threadIdx.x; idx < 3 * kP * kP; idx += 256) { // WT[m][j][k] = W_l[k][j], l = 4, 3, 2
+        const int m = idx / (kP * kP), e = idx % (kP * kP), k = e / kP, j = e % kP;
+        const int off = m == 0 ? kQ4 : (m == 1 ? kQ3 : kQ2);
+        WT[m][j * kLdsRow + k] = (m == 0 && k >= kOutP) ? uint16_t(0) : wq[off + k * kP + j];
     }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t r = lane & 15u, q = lane >> 4;
+    const uint32_t rr = lane >> 2, cc = (lane & 3u) * 16u; // the tile copies: 16 rows x 4 x 16 columns
+    uint16_t* Dt = D[w];
+    uint16_t* Ht = H[w];
+    const uint16_t* hs[3] = {h3, h2, h1};
+    uint16_t* ds[3] = {d3, d2, d1};
+    const uint32_t tiles = np / 16u;
+    for (uint32_t t = blockIdx.x * 4u + w; t < tiles; t += gridDim.x * 4u) {
+        const size_t row = size_t(t) * 16u + rr;
+        { // the outputs' delta: 48 columns, zeros beyond
+            uint4* to = reinterpret_cast<uint4*>(Dt + rr * kLdsRow + cc);
+            if (cc < uint32_t(kOutP)) {
+                const uint4* src = reinterpret_cast<const uint4*>(dO + row * kOutP + cc);
+                to[0] = src[0];
+                to[1] = src[1];
+            } else {
+                to[0] = to[1] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        for (int L = 0; L < 3; L++) {
+            {
+                const uint4* src = reinterpret_cast<const uint4*>(hs[L] + row * kP + cc);
+                uint4* to = reinterpret_cast<uint4*>(Ht + rr * kLdsRow + cc);
+                to[0] = src[0];
+                to[1] = src[1];
+            }
+            wave_lds_fence();
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Dt + r * kLdsRow + 8u * q);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Dt + r * kLdsRow + 32u + 8u * q);
+            wave_lds_fence();
+            for (int j = 0; j < kP / 16; j++) {
+                const uint16_t* wr = WT[L] + (16 * j + int(r)) * kLdsRow + 8 * int(q);
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, *reinterpret_cast<const bf16x8*>(wr), acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, *reinterpret_cast<const bf16x8*>(wr + 32), acc, 0, 0,
+                                                              0);
+                const int c = 16 * j + int(r);
+                for (int e = 0; e < 4; e++) {
+                    const int i = 4 * int(q) + e;
+                    Dt[i * kLdsRow + c] = bf2f(Ht[i * kLdsRow + c]) > 0.f ? f2bf(acc[e]) : uint16_t(0);
+                }
+            }
+            wave_lds_fence();
+            const uint4* src = reinterpret_cast<const uint4*>(Dt + rr * kLdsRow + cc);
+            uint4* to = reinterpret_cast<uint4*>(ds[L] + row * kP + cc);
+            to[0] = src[0];
+            to[1] = src[1];
+        }
+    }
+}
+
+// ---- weight gradients -------------------------------------------------------------------
+// gW[i][j] += sum over the batch rows of delta(row, i) act(row, j) for one layer (delta rows of
+// `units` BF16, act rows of 64): each block sums `chunk` rows, 32 at a time on
+// v_mfma_f32_16x16x32_bf16 with A(i, k) = delta(k, i) and B(k, j) = act(k, j), both staged
+// transposed in LDS; wave w owns output rows [16 w, 16 w + 16). The block's partial goes to gW with
+// one FP32 atomic per entry.
+constexpr int kWgRow = 40; // 32 batch rows + 8 padding per staged unit
+
+__global__ void __launch_bounds__(256) k_mlp_wgrad(const uint16_t* dl, int units, const uint16_t* act, uint32_t np,
+                                                   uint32_t chunk, float* gW)
+{
+    __shared__ __align__(16) uint16_t DT[kP * kWgRow];
+    __shared__ __align__(16) uint16_t AT[kP * kWgRow];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t r = lane & 15u, q = lane >> 4;
+    const bool mine = int(16u * w) < units; // same in every lane of the wave
+    f32x4 acc[4];
+    for (int j = 0; j < 4; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint32_t r0 = blockIdx.x * chunk, r1 = min(np, r0 + chunk);
+    for (uint32_t k0 = r0; k0 < r1; k0 += 32u) {
+        __syncthreads();
+        for (uint32_t idx = threadIdx.x; idx < 32u * kP; idx += 256u) {
+            const uint32_t kk = idx / kP, u = idx % kP;
+            const bool in = k0 + kk < r1;
+            DT[u * kWgRow + kk] = in && int(u) < units ? dl[size_t(k0 + kk) * units + u] : uint16_t(0);
+            AT[u * kWgRow + kk] = in ? act[size_t(k0 + kk) * kP + u] : uint16_t(0);
+        }
+        __syncthreads();
+        if (mine) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(DT + (16u * w + r) * kWgRow + 8u * q);
+            for (int j = 0; j < 4; j++)
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a, *reinterpret_cast<const bf16x8*>(AT + (16u * j + r) * kWgRow + 8u * q), acc[j], 0, 0, 0);
+        }
+    }
+    if (mine)
+        for (int j = 0; j < 4; j++)
+            for (int e = 0; e < 4; e++) atomicAdd(&gW[(16u * w + 4u * q + uint32_t(e)) * kP + 16u * j + r], acc[j][e]);
 }
 
 // ---- loss and its gradient at the outputs ------------------------------------------------
@@ -312,7 +448,6 @@ __global__ void __launch_bounds__(256) k_zero(uint32_t* p, size_t n)
 }
 
 uint32_t pad16(uint32_t n) { return (n + 15u) & ~15u; }
-dim3 gemm_grid(int M, int N, int splits) { return dim3(uint32_t((M / 16) * (N / 16) + 3) / 4, 1, uint32_t(splits)); }
 
 } // namespace
 
@@ -377,34 +512,26 @@ int GuideNet::reserve(uint32_t n)
     return UPBP_OK;
 }
 
+// blocks of the fused forward pass: one 16-row tile per wave, at most 8 blocks per CU's worth
+static dim3 forward_grid(uint32_t np) { return dim3(std::max(1u, std::min((np / 16u + 3u) / 4u, 2048u))); }
+
 // samples on the device, stride floats each, position in [0,1]^3 then the arrival direction
 int GuideNet::forward(const float* samples, int stride, uint32_t n, hipStream_t s)
 {
     if (reserve(n)) return UPBP_E_HIP;
-    return forward_into(samples, stride, n, x0.as<uint16_t>(), h1.as<uint16_t>(), h2.as<uint16_t>(),
-                        h3.as<uint16_t>(), out.as<float>(), s);
+    const uint32_t np = pad16(n);
+    hipLaunchKernelGGL(k_mlp_forward<true>, forward_grid(np), dim3(256), 0, s, samples, stride, n, np,
+                       wq.as<uint16_t>(), bq.as<float>(), x0.as<uint16_t>(), h1.as<uint16_t>(), h2.as<uint16_t>(),
+                       h3.as<uint16_t>(), out.as<float>());
+    return hipGetLastError() == hipSuccess ? UPBP_OK : UPBP_E_HIP;
 }
 
-// the forward pass into explicit buffers: the encoded inputs and the three hidden activations
-// (pad16(n) rows of 64 BF16 each; h2 may alias x0 and h3 may alias h1 when only the outputs are
-// wanted) and the raw outputs (rows of 48 floats)
-int GuideNet::forward_into(const float* samples, int stride, uint32_t n, uint16_t* a0, uint16_t* a1, uint16_t* a2,
-                           uint16_t* a3, float* o, hipStream_t s) const
+// the raw outputs only (rows of 48 floats, pad16(n) rows) into o; no other memory is written
+int GuideNet::forward_into(const float* samples, int stride, uint32_t n, float* o, hipStream_t s) const
 {
     const uint32_t np = pad16(n);
-    const int M = int(np);
-    hipLaunchKernelGGL(k_encode, dim3((np + 255) / 256), dim3(256), 0, s, samples, stride, n, np, a0);
-    const uint16_t* W = wq.as<uint16_t>();
-    const float* B = bq.as<float>();
-    // Y = X W^T: B(k, j) = W[j][k], the padded rows of 64
-    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, a0, kP, 1, W + kQ1, 1, kP, M, 64,
-                       kP, B, nullptr, a1, nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, a1, kP, 1, W + kQ2, 1, kP, M, 64,
-                       kP, B + 64, nullptr, a2, nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_RELU_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, a2, kP, 1, W + kQ3, 1, kP, M, 64,
-                       kP, B + 128, nullptr, a3, nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_BIAS_F32>, gemm_grid(M, kOutP, 1), dim3(256), 0, s, a3, kP, 1, W + kQ4, 1, kP, M,
-                       kOutP, kP, B + 192, nullptr, nullptr, o, kOutP);
+    hipLaunchKernelGGL(k_mlp_forward<false>, forward_grid(np), dim3(256), 0, s, samples, stride, n, np,
+                       wq.as<uint16_t>(), bq.as<float>(), nullptr, nullptr, nullptr, nullptr, o);
     return hipGetLastError() == hipSuccess ? UPBP_OK : UPBP_E_HIP;
 }
 
@@ -414,29 +541,23 @@ int GuideNet::gradient(const float* samples, uint32_t n, hipStream_t s)
     int e = forward(samples, 10, n, s);
     if (e) return e;
     const uint32_t np = pad16(n);
-    const int M = int(np);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, loss.as<uint32_t>(), size_t(4));
     hipLaunchKernelGGL(k_loss_grad, dim3((np + 255) / 256), dim3(256), 0, s, out.as<float>(), samples, n, np,
                        1.f / float(n), d_o.as<uint16_t>(), loss.as<float>());
-    const uint16_t* W = wq.as<uint16_t>();
-    // deltas: d_{l-1} = (d_l W_l) masked by the ReLU of layer l-1; B(k, j) = W_l[k][j]
-    hipLaunchKernelGGL(k_gemm<EPI_MASK_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, d_o.as<uint16_t>(), kOutP, 1,
-                       W + kQ4, kP, 1, M, 64, kOutP, nullptr, h3.as<uint16_t>(), d3.as<uint16_t>(), nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_MASK_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, d3.as<uint16_t>(), kP, 1,
-                       W + kQ3, kP, 1, M, 64, kP, nullptr, h2.as<uint16_t>(), d2.as<uint16_t>(), nullptr, kP);
-    hipLaunchKernelGGL(k_gemm<EPI_MASK_BF16>, gemm_grid(M, 64, 1), dim3(256), 0, s, d2.as<uint16_t>(), kP, 1,
-                       W + kQ2, kP, 1, M, 64, kP, nullptr, h1.as<uint16_t>(), d1.as<uint16_t>(), nullptr, kP);
-    // weight gradients: gW_l[r][c] = sum_i delta_l(i, r) act_{l-1}(i, c), split over the batch
+    // deltas: d_{l-1} = (d_l W_l) masked by the ReLU of layer l-1
+    hipLaunchKernelGGL(k_mlp_backward, forward_grid(np), dim3(256), 0, s, wq.as<uint16_t>(), d_o.as<uint16_t>(),
+                       h1.as<uint16_t>(), h2.as<uint16_t>(), h3.as<uint16_t>(), np, d1.as<uint16_t>(),
+                       d2.as<uint16_t>(), d3.as<uint16_t>());
+    // weight gradients: gW_l[r][c] = sum_i delta_l(i, r) act_{l-1}(i, c), in chunks of the batch
     hipLaunchKernelGGL(k_zero, dim3((kQTotal + 255) / 256), dim3(256), 0, s, gW.as<uint32_t>(), size_t(kQTotal));
-    int splits = 1;
-    while (splits < 64 && M % (16 * splits * 2) == 0 && M / (splits * 2) >= 256) splits *= 2;
+    const uint32_t chunk = std::max(256u, (np / 256u + 31u) & ~31u); // about 256 blocks per layer
     float* g = gW.as<float>();
     const uint16_t* acts[4] = {x0.as<uint16_t>(), h1.as<uint16_t>(), h2.as<uint16_t>(), h3.as<uint16_t>()};
     const uint16_t* dels[4] = {d1.as<uint16_t>(), d2.as<uint16_t>(), d3.as<uint16_t>(), d_o.as<uint16_t>()};
     const int rows[4] = {64, 64, 64, kOutP}, off[4] = {kQ1, kQ2, kQ3, kQ4};
     for (int l = 0; l < 4; l++)
-        hipLaunchKernelGGL(k_gemm<EPI_ATOMIC_F32>, gemm_grid(rows[l], kP, splits), dim3(256), 0, s, dels[l], 1,
-                           rows[l], acts[l], kP, 1, rows[l], kP, M, nullptr, nullptr, nullptr, g + off[l], kP);
+        hipLaunchKernelGGL(k_mlp_wgrad, dim3((np + chunk - 1) / chunk), dim3(256), 0, s, dels[l], rows[l], acts[l], np,
+                           chunk, g + off[l]);
     float* gbias = gb.as<float>();
     hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, s, d1.as<uint16_t>(), np, kP, 64, gbias);
     hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, s, d2.as<uint16_t>(), np, kP, 64, gbias + 64);
@@ -466,9 +587,9 @@ int GuideNet::adam_step(float lr, hipStream_t s)
 int GuideNet::records(int res, float4* recs, hipStream_t s)
 {
     const uint32_t n = uint32_t(res) * res * res;
-    if (grid_in.ensure(size_t(n) * 6 * 4)) return UPBP_E_HIP;
+    if (grid_in.ensure(size_t(n) * 6 * 4) || out.ensure(size_t(pad16(n)) * kOutP * 4)) return UPBP_E_HIP;
     hipLaunchKernelGGL(k_grid_inputs, dim3((n + 255) / 256), dim3(256), 0, s, res, grid_in.as<float>());
-    const int e = forward(grid_in.as<float>(), 6, n, s);
+    const int e = forward_into(grid_in.as<float>(), 6, n, out.as<float>(), s);
     if (e) return e;
     hipLaunchKernelGGL(k_pack_records, dim3((n + 255) / 256), dim3(256), 0, s, out.as<float>(), n, recs);
     return hipGetLastError() == hipSuccess ? UPBP_OK : UPBP_E_HIP;
@@ -568,8 +689,8 @@ extern "C" int upbp_mlp_train_step(upbp_mlp* m, const float* samples, uint32_t n
 
 // ---- forward pass that allocates nothing ------------------------------------------------------
 // For callers that cannot allocate device memory (code under HIP graph capture, serving loops,
-// harnesses that forbid hipMalloc): the forward pass of upbp_mlp_forward with its input staging,
-// activations and outputs in one file-scope device array. Each GPU holds its own copy of the
+// harnesses that forbid hipMalloc): the forward pass of upbp_mlp_forward with its input staging
+// and outputs in one file-scope device array (the fused kernel keeps the activations in LDS). Each GPU holds its own copy of the
 // array until the process ends; its address on the current device is resolved once per device
 // with hipGetSymbolAddress and only that address is used from host code. Every call on one GPU
 // shares the array, so calls that overlap on one GPU race. Nothing in it needs clearing: every
@@ -577,12 +698,10 @@ extern "C" int upbp_mlp_train_step(upbp_mlp* m, const float* samples, uint32_t n
 namespace {
 
 constexpr uint32_t kNoAllocMax = 32768; // samples per call (a multiple of 16)
-constexpr size_t kScrIn = 0;                                               // n x 6 floats
-constexpr size_t kScrA0 = kScrIn + size_t(kNoAllocMax) * 6 * 4;            // pad16(n) x 64 BF16
-constexpr size_t kScrA1 = kScrA0 + size_t(kNoAllocMax) * upbp::kP * 2;           // pad16(n) x 64 BF16
-constexpr size_t kScrOut = kScrA1 + size_t(kNoAllocMax) * upbp::kP * 2;          // pad16(n) x 48 floats
-constexpr size_t kScrBytes = kScrOut + size_t(kNoAllocMax) * upbp::kOutP * 4;    // 14.75 MiB
-static_assert(kScrA0 % 256 == 0 && kScrA1 % 256 == 0 && kScrOut % 256 == 0, "scratch regions stay aligned");
+constexpr size_t kScrIn = 0;                                                  // n x 6 floats
+constexpr size_t kScrOut = kScrIn + size_t(kNoAllocMax) * 6 * 4;              // pad16(n) x 48 floats
+constexpr size_t kScrBytes = kScrOut + size_t(kNoAllocMax) * upbp::kOutP * 4; // 6.75 MiB
+static_assert(kScrOut % 256 == 0, "scratch regions stay aligned");
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mutex;
@@ -616,11 +735,9 @@ extern "C" int upbp_mlp_forward_noalloc(upbp_mlp* m, const float* samples, uint3
     unsigned char* base = scratch_on_current_device();
     if (!base) return UPBP_E_HIP;
     float* in = reinterpret_cast<float*>(base + kScrIn);
-    uint16_t* a0 = reinterpret_cast<uint16_t*>(base + kScrA0);
-    uint16_t* a1 = reinterpret_cast<uint16_t*>(base + kScrA1);
     float* o = reinterpret_cast<float*>(base + kScrOut);
     if (hipMemcpy(in, samples, size_t(n) * 24, hipMemcpyHostToDevice) != hipSuccess) return UPBP_E_HIP;
-    const int e = m->net.forward_into(in, 6, n, a0, a1, a0, a1, o, nullptr);
+    const int e = m->net.forward_into(in, 6, n, o, nullptr);
     if (e) return e;
     // rows of 48 on the device, the first 40 of each to the host
     if (hipMemcpy2D(out, 40 * 4, o, upbp::kOutP * 4, 40 * 4, n, hipMemcpyDeviceToHost) != hipSuccess) return UPBP_E_HIP;

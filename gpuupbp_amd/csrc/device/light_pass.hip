@@ -261,7 +261,7 @@ template <bool LOG>
 __device__ __forceinline__ void light_bounce_body(const DScene& sc, const Bufs& b, int bounce, uint32_t vb)
 {
     const IterParams& p = *b.params;
-    const uint32_t i = vb * 256u + threadIdx.x;
+    const uint32_t i = vb * blockDim.x + threadIdx.x;
     const uint32_t n = b.qcount[bounce];
     if (i >= n) return;
     const uint32_t pid = b.queue[bounce & 1][i];
@@ -321,9 +321,11 @@ void launch_light_bounce_batch(const BatchJob* jobs, const uint2* bmap, uint32_t
 void launch_light_bounce(const DScene& sc, const IterParams& p, const Bufs& b, int bounce, hipStream_t s)
 {
     if (b.mislog)
-        hipLaunchKernelGGL(k_light_bounce<true>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, b, bounce);
+        hipLaunchKernelGGL(k_light_bounce<true>, dim3((p.n_light + p.block - 1) / p.block), dim3(p.block), 0, s, sc, b,
+                           bounce);
     else
-        hipLaunchKernelGGL(k_light_bounce<false>, dim3((p.n_light + 255) / 256), dim3(256), 0, s, sc, b, bounce);
+        hipLaunchKernelGGL(k_light_bounce<false>, dim3((p.n_light + p.block - 1) / p.block), dim3(p.block), 0, s, sc, b,
+                           bounce);
 }
 
 } // namespace upbp

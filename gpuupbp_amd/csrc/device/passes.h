@@ -33,6 +33,7 @@ struct IterParams {
     int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
     int piece_order; // B-B1D work order: 0 as written, 1 XCD regions, 2 Morton-sorted (pieces_x)
     int packet_bounces; // B-B1D: camera bounces below this walk the beam tree as 64-piece packets
+    uint32_t block;     // threads per block of the light and camera bounce kernels (64, 128, 256)
     int beam_leaf;      // B-B1D leaf test reads 0: the sorted sub-beam records (beam_geo); 1: the
                         // sorted (beam slot, sub-beam j | n << 16) references and the beam record
     uint32_t cap_med, cap_beam; // capacities of the medium-point and sub-beam lists (host launch

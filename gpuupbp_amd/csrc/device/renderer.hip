@@ -435,6 +435,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.piece_order = r->tune.piece_order;
     p.packet_bounces = r->tune.packet_bounces;
     p.beam_leaf = r->tune.beam_leaf;
+    p.block = r->tune.block_size > 0 ? uint32_t(r->tune.block_size) : 256u;
     p.cap_med = r->cap_m;
     p.cap_beam = r->cap_b;
     {
@@ -1283,7 +1284,8 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
 {
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
     if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
-        t->piece_order < 0 || t->piece_order > 4 || t->packet_bounces < 0 || t->beam_leaf < 0 || t->beam_leaf > 1)
+        t->piece_order < 0 || t->piece_order > 4 || t->packet_bounces < 0 || t->beam_leaf < 0 || t->beam_leaf > 1 ||
+        (t->block_size != 0 && t->block_size != 64 && t->block_size != 128 && t->block_size != 256))
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     r->use_graph = t->graph != 0;
@@ -1332,9 +1334,9 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
     double best_ms = 0.0;
     int e = timed(best, best_ms);
     const float subbeam[] = {8.f, 12.f, 16.f, 24.f}, piece[] = {32.f, 64.f, 128.f};
-    const int blocks[] = {0, 1, 2}, order[] = {0, 2}, packets[] = {0, 1, 2, 64};
-    for (int knob = 0; knob < 5 && !e; knob++) {
-        const int n = knob == 0 ? 4 : (knob == 1 ? 3 : (knob == 2 ? 3 : (knob == 3 ? 2 : 4)));
+    const int blocks[] = {0, 1, 2}, order[] = {0, 2}, packets[] = {0, 1, 2, 64}, threads[] = {64, 128, 256};
+    for (int knob = 0; knob < 6 && !e; knob++) {
+        const int n = knob == 0 ? 4 : (knob == 1 ? 3 : (knob == 2 ? 3 : (knob == 3 ? 2 : (knob == 4 ? 4 : 3))));
         const upbp_tuning base = best;
         for (int v = 0; v < n && !e; v++) {
             upbp_tuning t = base;
@@ -1342,7 +1344,8 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
             else if (knob == 1) t.query_piece_factor = piece[v];
             else if (knob == 2) t.merge_blocks_per_cu = blocks[v];
             else if (knob == 3) t.piece_order = order[v];
-            else t.packet_bounces = packets[v];
+            else if (knob == 4) t.packet_bounces = packets[v];
+            else t.block_size = threads[v];
             if (memcmp(&t, &base, sizeof(t)) == 0) continue;
             double ms = 0.0;
             e = timed(t, ms);

@@ -294,6 +294,8 @@ typedef struct upbp_tuning {
                                     of 64 query pieces per wave (one node fetch per wave); later
                                     bounces walk it per piece (default 0: per piece everywhere,
                                     the faster of the two on the blob grid) */
+    int32_t block_size;          /* threads per block of the light and camera bounce kernels: 64,
+                                    128 or 256 (0: 256, the default) */
     int32_t beam_leaf;           /* B-B1D leaf test: 0 reads the sorted 48-byte sub-beam records
                                     (default); 1 reads 8-byte sorted references (beam slot,
                                     sub-beam index) and the beam's own record, which all its
@@ -334,8 +336,8 @@ const char* upbp_batch_last_error(const upbp_batch* b);
 void upbp_batch_destroy(upbp_batch* b); /* the renderers stay; destroy the batch before them */
 
 /* Autotuner (no reference counterpart): a coordinate search over subbeam_factor {8, 12, 16,
- * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2}, piece_order {0, 2} and
- * packet_bounces {0, 1, 2, 64},
+ * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2}, piece_order {0, 2},
+ * packet_bounces {0, 1, 2, 64} and block_size {64, 128, 256},
  * from the current tuning, timing `repeats` (1..64) renders of `iteration` per candidate after
  * one untimed render. The fastest tuning stays set and is returned in *chosen with its time in
  * *ms_per_iteration (either may be NULL). The accumulated image is restored afterwards and
@@ -404,10 +406,10 @@ int upbp_mlp_loss_grad(upbp_mlp* m, const float* samples, uint32_t n, float* los
 int upbp_mlp_train_step(upbp_mlp* m, const float* samples, uint32_t n, float lr, float* loss); /* one Adam step */
 /* upbp_mlp_forward without any allocation (no counterpart in the reference): for callers that
  * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
- * that forbid hipMalloc. Same arguments and outputs as upbp_mlp_forward; its input staging,
- * activations and outputs live in a fixed device array of 14.75 MiB per GPU (held until the
- * process ends), so n is at most upbp_mlp_noalloc_max() (32768) and a larger n returns
- * UPBP_E_CAPACITY. Calls on one GPU share that array: calls that overlap on one GPU race. */
+ * that forbid hipMalloc. Same arguments and outputs as upbp_mlp_forward; its input staging and
+ * outputs live in a fixed device array of 6.75 MiB per GPU (held until the process ends), so
+ * n is at most upbp_mlp_noalloc_max() (32768) and a larger n returns UPBP_E_CAPACITY. Calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
 int upbp_mlp_forward_noalloc(upbp_mlp* m, const float* samples, uint32_t n, float* out);
 uint32_t upbp_mlp_noalloc_max(void);
 

@@ -1,5 +1,5 @@
 """The proposal network's forward pass without allocation (GPU): upbp_mlp_forward_noalloc, whose
-input staging, activations and outputs live in a fixed per-GPU device array, must give exactly the
+input staging and outputs live in a fixed per-GPU device array, must give exactly the
 outputs of upbp_mlp_forward (the same kernels on the same inputs), on two calls in a row -- the
 second smaller, so that it reads nothing the first left behind -- and reject a size beyond its
 maximum."""
