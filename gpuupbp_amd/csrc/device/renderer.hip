@@ -435,7 +435,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.piece_order = r->tune.piece_order;
     p.packet_bounces = r->tune.packet_bounces;
     p.beam_leaf = r->tune.beam_leaf;
-    p.block = r->tune.block_size > 0 ? uint32_t(r->tune.block_size) : 256u;
+    p.block = r->tune.block_size > 0 ? uint32_t(r->tune.block_size) : 64u;
     p.cap_med = r->cap_m;
     p.cap_beam = r->cap_b;
     {

@@ -295,7 +295,8 @@ typedef struct upbp_tuning {
                                     bounces walk it per piece (default 0: per piece everywhere,
                                     the faster of the two on the blob grid) */
     int32_t block_size;          /* threads per block of the light and camera bounce kernels: 64,
-                                    128 or 256 (0: 256, the default) */
+                                    128 or 256 (0: 64, the default; one wave per block lets the
+                                    sparse last blocks of a bounce finish sooner) */
     int32_t beam_leaf;           /* B-B1D leaf test: 0 reads the sorted 48-byte sub-beam records
                                     (default); 1 reads 8-byte sorted references (beam slot,
                                     sub-beam index) and the beam's own record, which all its

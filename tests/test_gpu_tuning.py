@@ -15,7 +15,7 @@ SETTINGS = [dict(subbeam_factor=8.0, query_piece_factor=32.0), dict(subbeam_fact
             dict(piece_order=2), dict(packet_bounces=0), dict(packet_bounces=64),
             dict(packet_bounces=64, piece_order=2), dict(packet_bounces=64, subbeam_factor=0.0, query_piece_factor=0.0),
             dict(piece_order=3), dict(piece_order=4), dict(piece_order=3, graph=1), dict(beam_leaf=1),
-            dict(beam_leaf=1, piece_order=3, packet_bounces=64), dict(block_size=64), dict(block_size=128)]
+            dict(beam_leaf=1, piece_order=3, packet_bounces=64), dict(block_size=256), dict(block_size=128)]
 
 
 def _render(obj, tuning):

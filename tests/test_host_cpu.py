@@ -278,6 +278,16 @@ def test_batch_autotune_and_guide_reject_bad_arguments():
     dup = (C.c_void_p * 3)(0x1000, 0x2000, 0x1000)
     assert lib.upbp_run_batch(dup, (C.c_int32 * 3)(0, 0, 1), 3, 2) == E_ARG
     assert b"twice" in lib.upbp_last_error(None)
+    # batched rounds: the same checks before any stream or device memory is made
+    out = C.c_void_p()
+    assert lib.upbp_batch_create(None, 2, C.byref(out)) == E_ARG
+    assert lib.upbp_batch_create((C.c_void_p * 2)(None, None), 2, C.byref(out)) == E_ARG
+    assert lib.upbp_batch_create((C.c_void_p * 1)(0x1000), 0, C.byref(out)) == E_ARG
+    assert lib.upbp_batch_create((C.c_void_p * 1)(0x1000), 1, None) == E_ARG
+    assert lib.upbp_batch_create(dup, 3, C.byref(out)) == E_ARG and not out.value
+    assert b"twice" in lib.upbp_batch_last_error(None)
+    assert lib.upbp_batch_run(None, its, 1) == E_ARG
+    lib.upbp_batch_destroy(None)
     assert lib.upbp_autotune(None, 0, 1, None, None) == E_ARG
     g = capi.GuideConfig()
     lib.upbp_guide_defaults(C.byref(g))
