@@ -120,7 +120,7 @@ __device__ __forceinline__ bool bvh_ray_query(const Bvh& b, f3 o, f3 d, float tm
     }
 }
 
-// Visits every leaf of a 4-wide LBVH whose box meets the segment [tmin, tmax] of the ray.
+// Visits every leaf of a wide LBVH whose box meets the segment [tmin, tmax] of the ray.
 template <typename F>
 __device__ __forceinline__ bool wbvh_ray_query(const WBvh& b, f3 o, f3 d, float tmin, float tmax, F&& fn)
 {
@@ -132,14 +132,12 @@ __device__ __forceinline__ bool wbvh_ray_query(const WBvh& b, f3 o, f3 d, float 
     for (;;) {
         const WNode nd = b.nodes[node];
         const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
-        const int nc = int(nd.meta >> 24);
+        const int nc = wnode_count(nd.meta);
         int next = -1;
         for (int k = 0; k < nc; k++) {
-            float e;
-            if (!slab(r, wnode_decode(nd.ox, nd.q[0], k, sx), wnode_decode(nd.oy, nd.q[1], k, sy),
-                      wnode_decode(nd.oz, nd.q[2], k, sz), wnode_decode(nd.ox, nd.q[3], k, sx),
-                      wnode_decode(nd.oy, nd.q[4], k, sy), wnode_decode(nd.oz, nd.q[5], k, sz), tmax, e))
-                continue;
+            float e, lx, ly, lz, ux, uy, uz;
+            wnode_box(nd, k, sx, sy, sz, lx, ly, lz, ux, uy, uz);
+            if (!slab(r, lx, ly, lz, ux, uy, uz, tmax, e)) continue;
             const int ch = nd.child[k];
             if (ch < 0) {
                 fn(uint32_t(~ch));
@@ -272,7 +270,7 @@ struct BvhScratch {
     DevBuf keys, keys2, vals2, hist, scan, lvl, meta, bounds;
 };
 
-// Device storage of a 4-wide LBVH collapsed from a binary one (at most n-1 nodes) and the
+// Device storage of a wide LBVH collapsed from a binary one (at most n-1 nodes) and the
 // wide-node -> binary-node map used while building it.
 struct WBvhStore {
     upbp::WNode* nodes = nullptr;
@@ -287,7 +285,7 @@ struct WBvhStore {
 int lbvh_build(BvhStore& st, BvhScratch& sc, const float4* lo, const float4* hi, uint32_t n, hipStream_t s,
                const uint32_t* n_dev = nullptr);
 void lbvh_free(BvhStore& st);
-// Collapses the refitted binary LBVH `bin` (n leaves) into a 4-wide one; same leaf order.
+// Collapses the refitted binary LBVH `bin` (n leaves) into a kWide-wide one; same leaf order.
 int wbvh_build(WBvhStore& w, const BvhStore& bin, BvhScratch& sc, uint32_t n, hipStream_t s,
                const uint32_t* n_dev = nullptr);
 void wbvh_free(WBvhStore& w);

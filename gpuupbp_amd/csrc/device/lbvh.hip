@@ -357,9 +357,9 @@ __global__ void __launch_bounds__(256) k_refit_level(const float4* lo, const flo
 }
 
 // single primitive: one inner node with leaf 0 on the left and an empty right box
-// ---- 4-wide collapse --------------------------------------------------------------
+// ---- wide collapse ----------------------------------------------------------------
 // Each wide node takes one binary node and opens its largest-area inner child until it
-// has four children (or only leaves). Wide nodes are numbered level by level: level l
+// has kWide children (or only leaves). Wide nodes are numbered level by level: level l
 // holds [meta[l], meta[l] + meta[kMaxLevels + 1 + l]) and bin[] maps them to binary nodes.
 
 __device__ __forceinline__ float box_area(float4 a, float4 b)
@@ -419,8 +419,9 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t j = base + threadIdx.x;
-        int c[4] = {0, 0, 0, 0};
-        float4 lo[4], hi[4];
+        int c[kWide];
+        float4 lo[kWide], hi[kWide];
+        for (int k = 0; k < kWide; k++) c[k] = 0;
         int nc = 0, ninner = 0;
         if (j < n) {
             const BNode& b = bn[bin[s + j]];
@@ -431,7 +432,7 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
             lo[1] = make_float4(b.lx1, b.ly1, b.lz1, 0.f);
             hi[1] = make_float4(b.ux1, b.uy1, b.uz1, 0.f);
             nc = 2;
-            while (nc < 4) {
+            while (nc < kWide) {
                 int best = -1;
                 float ba = -1.f;
                 for (int k = 0; k < nc; k++)
@@ -453,7 +454,7 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
         }
         // wave-aggregated allocation of the inner children at the next level
         uint32_t pre = 0, tot = 0;
-        for (int bit = 0; bit < 3; bit++) {
+        for (int bit = 0; bit < 4; bit++) { // ninner <= kWide = 8: four bits
             const uint64_t m = __ballot((uint32_t(ninner) >> bit) & 1u);
             pre += uint32_t(__popcll(m & lt)) << bit;
             tot += uint32_t(__popcll(m)) << bit;
@@ -476,8 +477,9 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
             w.oy = nlo.y;
             w.oz = nlo.z;
             w.meta = ex | (ey << 8) | (ez << 16) | (uint32_t(nc) << 24);
-            for (int k = 0; k < 6; k++) w.q[k] = 0;
-            for (int k = 0; k < 4; k++) w.child[k] = 0;
+            for (int k = 0; k < 12; k++) w.q[k] = 0;
+            for (int k = 0; k < 8; k++) w.pad[k] = 0;
+            for (int k = 0; k < kWide; k++) w.child[k] = 0;
             for (int k = 0; k < nc; k++) {
                 if (c[k] >= 0) {
                     bin[at] = uint32_t(c[k]);
@@ -485,15 +487,14 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
                 } else {
                     w.child[k] = c[k];
                 }
-                const int sh = 8 * k;
-                w.q[0] |= quant_lo(w.ox, sx, lo[k].x) << sh;
-                w.q[1] |= quant_lo(w.oy, sy, lo[k].y) << sh;
-                w.q[2] |= quant_lo(w.oz, sz, lo[k].z) << sh;
-                w.q[3] |= quant_hi(w.ox, sx, hi[k].x) << sh;
-                w.q[4] |= quant_hi(w.oy, sy, hi[k].y) << sh;
-                w.q[5] |= quant_hi(w.oz, sz, hi[k].z) << sh;
+                const int sh = 8 * (k & 3), wd = k >> 2; // plane p of child k: byte k & 3 of q[2 p + (k >> 2)]
+                w.q[0 + wd] |= quant_lo(w.ox, sx, lo[k].x) << sh;
+                w.q[2 + wd] |= quant_lo(w.oy, sy, lo[k].y) << sh;
+                w.q[4 + wd] |= quant_lo(w.oz, sz, lo[k].z) << sh;
+                w.q[6 + wd] |= quant_hi(w.ox, sx, hi[k].x) << sh;
+                w.q[8 + wd] |= quant_hi(w.oy, sy, hi[k].y) << sh;
+                w.q[10 + wd] |= quant_hi(w.oz, sz, hi[k].z) << sh;
             }
-            w.pad0 = w.pad1 = 0;
             wn[s + j] = w;
         }
     }
@@ -504,30 +505,27 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
 __global__ void k_wide_single(const BNode* bn, WNode* wn, const uint32_t* n_dev)
 {
     if (n_dev && *n_dev >= 2) return;
+    WNode w;
+    for (int k = 0; k < kWide; k++) w.child[k] = 0;
+    for (int k = 0; k < 12; k++) w.q[k] = 0;
+    for (int k = 0; k < 8; k++) w.pad[k] = 0;
     if (n_dev && *n_dev == 0) {
-        WNode w;
         w.ox = w.oy = w.oz = 0.f;
         w.meta = (127u) | (127u << 8) | (127u << 16);
-        for (int k = 0; k < 4; k++) w.child[k] = 0;
-        for (int k = 0; k < 6; k++) w.q[k] = 0;
-        w.pad0 = w.pad1 = 0;
         wn[0] = w;
         return;
     }
     const BNode& b = bn[0];
-    WNode w;
     w.ox = b.lx0;
     w.oy = b.ly0;
     w.oz = b.lz0;
     const uint32_t ex = grid_exponent(b.lx0, b.ux0), ey = grid_exponent(b.ly0, b.uy0), ez = grid_exponent(b.lz0, b.uz0);
     w.meta = ex | (ey << 8) | (ez << 16) | (1u << 24);
     w.child[0] = ~0;
-    w.child[1] = w.child[2] = w.child[3] = 0;
-    w.q[0] = w.q[1] = w.q[2] = 0;
-    w.q[3] = quant_hi(b.lx0, __int_as_float(int(ex << 23)), b.ux0);
-    w.q[4] = quant_hi(b.ly0, __int_as_float(int(ey << 23)), b.uy0);
-    w.q[5] = quant_hi(b.lz0, __int_as_float(int(ez << 23)), b.uz0);
-    w.pad0 = w.pad1 = 0;
+    // child 0: low planes at the origin (byte 0 of q[0], q[2], q[4] stays 0), high planes in q[6..10]
+    w.q[6] = quant_hi(b.lx0, __int_as_float(int(ex << 23)), b.ux0);
+    w.q[8] = quant_hi(b.ly0, __int_as_float(int(ey << 23)), b.uy0);
+    w.q[10] = quant_hi(b.lz0, __int_as_float(int(ez << 23)), b.uz0);
     wn[0] = w;
 }
 

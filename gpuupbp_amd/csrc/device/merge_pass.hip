@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kMergeBlock = 256;
 constexpr int kMergeWaves = kMergeBlock / 64;
-constexpr uint32_t kCandCap = 320; // < 64 pending + up to 4 leaves per lane in one node step
+constexpr uint32_t kCandCap = 64 + 64 * kWide; // < 64 pending + up to kWide leaves per lane in one node step
 constexpr uint32_t kPairCap = 128; // < 64 pending + at most one accept per lane in a test batch
 
 // The piece of a candidate or stolen subtree travels as (k, n) packed in 16 bits (k | n << 6,
@@ -344,23 +344,23 @@ __device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, 
         const bool active = cur >= 0;
         const bool any_active = __ballot(active) != 0;
         n_trips++;
-        // one 4-wide node of this lane's traversal; leaves hit become candidates
-        int lf[4] = {-1, -1, -1, -1};
+        // one wide node of this lane's traversal; leaves hit become candidates
+        int lf[kWide];
+#pragma unroll
+        for (int k = 0; k < kWide; k++) lf[k] = -1;
         uint32_t qnow = 0;
         if (active) {
             const WNode nd = bvh.nodes[node];
             n_nodes++;
             const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
-            const int nc = int(nd.meta >> 24);
+            const int nc = wnode_count(nd.meta);
             int next = -1;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < kWide; k++) {
                 if (k < nc) {
-                    float e;
-                    const bool h = slab(rb, wnode_decode(nd.ox, nd.q[0], k, sx), wnode_decode(nd.oy, nd.q[1], k, sy),
-                                        wnode_decode(nd.oz, nd.q[2], k, sz), wnode_decode(nd.ox, nd.q[3], k, sx),
-                                        wnode_decode(nd.oy, nd.q[4], k, sy), wnode_decode(nd.oz, nd.q[5], k, sz),
-                                        rb.tmax, e);
+                    float e, lx, ly, lz, ux, uy, uz;
+                    wnode_box(nd, k, sx, sy, sz, lx, ly, lz, ux, uy, uz);
+                    const bool h = slab(rb, lx, ly, lz, ux, uy, uz, rb.tmax, e);
                     const int ch = nd.child[k];
                     if (h) {
                         if (ch < 0) {
@@ -378,7 +378,7 @@ __device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, 
                             } else {
                                 n_err++; // traversal stack exhausted: the piece is abandoned and counted
                                 sb = sp = 0;
-                                lf[0] = lf[1] = lf[2] = lf[3] = -1;
+                                for (int t = 0; t < kWide; t++) lf[t] = -1;
                                 next = -1;
                                 break;
                             }
@@ -393,7 +393,7 @@ __device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, 
         }
         // all lanes take part in the queue appends (ballots over the whole wave)
 #pragma unroll
-        for (int k = 0; k < 4; k++) cq_push(C, cn, lf[k] >= 0, qnow, uint32_t(lf[k]), kn);
+        for (int k = 0; k < kWide; k++) cq_push(C, cn, lf[k] >= 0, qnow, uint32_t(lf[k]), kn);
         // Evaluate 64 accepted pairs at a time whenever 64 are queued (this keeps fewer than 64
         // pending before each candidate test), else test 64 candidates at a time. Once no lane
         // has work left, drain both queues: candidates first, then pairs.
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_batch(const BatchJob*
 }
 
 // ---- B-B1D packet traversal -------------------------------------------------------------
-// One wave walks the 4-wide beam tree for 64 query pieces at once: the node index is the same in
+// One wave walks the wide beam tree for 64 query pieces at once: the node index is the same in
 // every lane, so each node is fetched once per wave (a scalar load) instead of once per lane, and
 // each lane tests its own piece against the node's child boxes. A child that any lane hits is
 // visited by the whole wave; at a leaf, the lanes whose box test passed run the exact predicate.
@@ -546,17 +546,15 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_packet(DScene sc, Buf
             const WNode nd = nodes[nu];
             n_nodes++;
             const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
-            const int nc = int(nd.meta >> 24);
+            const int nc = wnode_count(nd.meta);
             uint32_t hits = 0;
             if (act) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    float e;
-                    if (k < nc && slab(rb, wnode_decode(nd.ox, nd.q[0], k, sx), wnode_decode(nd.oy, nd.q[1], k, sy),
-                                       wnode_decode(nd.oz, nd.q[2], k, sz), wnode_decode(nd.ox, nd.q[3], k, sx),
-                                       wnode_decode(nd.oy, nd.q[4], k, sy), wnode_decode(nd.oz, nd.q[5], k, sz),
-                                       rb.tmax, e))
-                        hits |= 1u << k;
+                for (int k = 0; k < kWide; k++) {
+                    float e, lx, ly, lz, ux, uy, uz;
+                    if (k >= nc) continue;
+                    wnode_box(nd, k, sx, sy, sz, lx, ly, lz, ux, uy, uz);
+                    if (slab(rb, lx, ly, lz, ux, uy, uz, rb.tmax, e)) hits |= 1u << k;
                 }
             }
             int next = -1;

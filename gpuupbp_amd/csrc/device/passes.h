@@ -350,7 +350,7 @@ UPBP_HD void subbeam_box(f3 o, f3 d, float len, float ta, float tb, float r, flo
 
 struct PhotonMaps {
     Bvh surf, med, beam;
-    WBvh wmed, wbeam; // 4-wide collapses of med and beam (segment merge kernels)
+    WBvh wmed, wbeam; // wide collapses of med and beam (segment merge kernels)
     const float4* surf_pts; // sorted: (hit.xyz, vertex slot)
     const float4* med_pts;
     const float4* beam_geo; // sorted: 3 per sub-beam: (origin.xyz, length), (dir.xyz, beam slot),

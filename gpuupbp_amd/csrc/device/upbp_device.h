@@ -189,16 +189,17 @@ struct Bvh {
     uint32_t n;            // primitives
 };
 
-// 4-wide node collapsed from the binary LBVH (64 bytes). Child boxes are quantised to 8 bits
-// per plane on a power-of-two grid anchored at the node box's low corner; decoding is
-// origin + q * step in FP32 and the builder picks q so that the decoded box contains the
-// exact child box.
-struct __align__(16) WNode {
+// 8-wide node collapsed from the binary LBVH: 128 bytes, one cache line per node visit. Child
+// boxes are quantised to 8 bits per plane on a power-of-two grid anchored at the node box's low
+// corner; decoding is origin + q * step in FP32 and the builder picks q so that the decoded box
+// contains the exact child box.
+constexpr int kWide = 8;
+struct __align__(128) WNode {
     float ox, oy, oz;
-    uint32_t meta;   // bits 0-7 / 8-15 / 16-23: biased exponents of the x / y / z steps; 24-26: child count
-    int child[4];    // >= 0: wide node, < 0: ~sorted leaf position
-    uint32_t q[6];   // byte k of q[0..2] = child k low x, y, z; of q[3..5] = child k high x, y, z
-    uint32_t pad0, pad1;
+    uint32_t meta;      // bits 0-7 / 8-15 / 16-23: biased exponents of the x / y / z steps; 24-27: child count
+    int child[kWide];   // >= 0: wide node, < 0: ~sorted leaf position
+    uint32_t q[12];     // plane p (low x, y, z, high x, y, z) of child k: byte k & 3 of q[2 p + (k >> 2)]
+    uint32_t pad[8];
 };
 
 struct WBvh {
@@ -211,6 +212,19 @@ UPBP_HD float wnode_decode(float origin, uint32_t word, int k, float step)
 {
 #pragma clang fp contract(off)
     return origin + float((word >> (8 * k)) & 255u) * step;
+}
+UPBP_HD int wnode_count(uint32_t meta) { return int((meta >> 24) & 15u); }
+// child k's box, decoded (steps sx, sy, sz from wnode_step)
+UPBP_HD void wnode_box(const WNode& nd, int k, float sx, float sy, float sz, float& lx, float& ly, float& lz, float& ux,
+                       float& uy, float& uz)
+{
+    const int w = k >> 2, b = k & 3;
+    lx = wnode_decode(nd.ox, nd.q[0 + w], b, sx);
+    ly = wnode_decode(nd.oy, nd.q[2 + w], b, sy);
+    lz = wnode_decode(nd.oz, nd.q[4 + w], b, sz);
+    ux = wnode_decode(nd.ox, nd.q[6 + w], b, sx);
+    uy = wnode_decode(nd.oy, nd.q[8 + w], b, sy);
+    uz = wnode_decode(nd.oz, nd.q[10 + w], b, sz);
 }
 
 struct TriSet {
