@@ -326,7 +326,9 @@ int upbp_run_batch(upbp_renderer* const* rs, const int32_t* iterations, int32_t 
  * logging MIS weights or tracing only one subpath -- renders each job alone instead, as does a job
  * whose photon lists overflowed in the round (its batched iteration is then not accumulated).
  * The images equal the renderers' own iterations, up to the order of floating-point atomic sums;
- * the merges walk the pieces in the camera pass's order whatever the piece_order tuning. */
+ * the merges walk the pieces in the camera pass's order whatever the piece_order tuning. A batch
+ * and its renderers are used from one host thread at a time; upbp_batch_run returns when the
+ * round is complete on the GPU. */
 typedef struct upbp_batch upbp_batch;
 int upbp_batch_create(upbp_renderer* const* rs, int32_t n, upbp_batch** out);
 int upbp_batch_run(upbp_batch* b, const int32_t* iterations, int32_t use_graph);
