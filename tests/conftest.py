@@ -61,3 +61,26 @@ def sky(h=16, w=32):
     img[...] = (0.2 + 0.6 * (1 - v))[:, None, None] * np.array([0.6, 0.8, 1.0], np.float32)
     img[3:5, 20:23] = [40.0, 35.0, 25.0]
     return img
+
+
+def relmse(a, b):
+    return float(np.mean((a - b) ** 2 / (b ** 2 + 1e-2)))
+
+
+def oracle_independent(scene_ptr, cfg, first=0, iters=1):
+    """The oracle's render of the same iterations on other random streams (iteration offset 2^16):
+    the error of independent sampling, against which same-stream differences are judged."""
+    from oracle import oracle_py
+    cfg.iteration_offset = 1 << 16
+    try:
+        img, _ = oracle_py.render(scene_ptr, cfg, first, iters)
+    finally:
+        cfg.iteration_offset = 0
+    return img
+
+
+def assert_whole_image(g, o, o_indep, ratio=0.1):
+    """Same-stream images: beyond the pixels that agree to FP32 rounding, the image as a whole must
+    stay far closer to the oracle's than an independent render of the same iterations is."""
+    r_same, r_ind = relmse(g, o), relmse(o_indep, o)
+    assert r_same < ratio * r_ind, (r_same, r_ind)

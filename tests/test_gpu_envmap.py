@@ -3,7 +3,7 @@ and the box workload lit through its open front by an environment map, path for 
 import numpy as np
 import pytest
 
-from conftest import box_args, env_scene, sky
+from conftest import assert_whole_image, box_args, env_scene, oracle_independent, sky
 from gpuupbp_amd import capi
 from gpuupbp_amd import renderer as gpu
 from oracle import oracle_py
@@ -38,7 +38,9 @@ def test_env_lit_scene_path_parity(box_obj, tmp_path):
         st = r.run_iteration(0)
         g = r.read_accum(1)
     o, ost = oracle_py.render(sh.ptr, cfg, 0, 1)
+    o2 = oracle_independent(sh.ptr, cfg, 0, 1)
     assert st.path_errors == 0 and st.surface_vertices + st.medium_vertices == ost.light_vertices
     rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
     assert np.mean(rel < 1e-3) > 0.95, np.mean(rel < 1e-3)
     assert abs(g.mean() - o.mean()) / o.mean() < 0.01
+    assert_whole_image(g, o, o2)

@@ -3,7 +3,7 @@ and renders with a proposal installed, path for path against the oracle."""
 import numpy as np
 import pytest
 
-from conftest import box_args
+from conftest import assert_whole_image, box_args, oracle_independent
 from gpuupbp_amd import capi
 from gpuupbp_amd import renderer as gpu
 from oracle import oracle_py
@@ -61,9 +61,10 @@ def _render(obj, args, lobes, res, lo, hi, alpha):
     oracle_py.set_guide(lobes, res, lo, hi, alpha)
     try:
         o, ost = oracle_py.render(sh.ptr, cfg, 0, 1)
+        o2 = oracle_independent(sh.ptr, cfg, 0, 1)
     finally:
         oracle_py.set_guide(None, 0, None, None, 0.0)
-    return g, o, st, ost
+    return g, o, st, ost, o2
 
 
 @pytest.mark.parametrize("alpha", [0.0, 0.7])
@@ -72,9 +73,10 @@ def test_guided_render_path_parity(boxfog_obj, alpha):
     sh = capi.SceneHandle(boxfog_obj, 64, 64)
     c, r = np.array(list(sh.scene.scene_center), np.float32), sh.scene.scene_radius
     lobes = _lobes(np.random.default_rng(11), 3 ** 3, kappa_hi=60.0)
-    g, o, st, ost = _render(boxfog_obj, box_args("48x48", extra=("-r_initial", "0.03")), lobes, 3, c - r, c + r,
+    g, o, st, ost, o2 = _render(boxfog_obj, box_args("48x48", extra=("-r_initial", "0.03")), lobes, 3, c - r, c + r,
                             alpha)
     assert st.path_errors == 0 and st.surface_vertices + st.medium_vertices == ost.light_vertices
     rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
     assert np.mean(rel < 1e-3) > 0.93, np.mean(rel < 1e-3)
     assert abs(g.mean() - o.mean()) / o.mean() < 0.01
+    assert_whole_image(g, o, o2)

@@ -8,7 +8,7 @@ transcendental-function differences (tolerances below), and converged images agr
 import numpy as np
 import pytest
 
-from conftest import box_args
+from conftest import assert_whole_image, box_args, oracle_independent
 from gpuupbp_amd import capi
 from gpuupbp_amd import renderer as gpu
 from oracle import oracle_py
@@ -147,7 +147,7 @@ def test_closest_hit_matches_brute_force():
     assert np.mean(gp[hit] == op[hit]) > 0.999
 
 
-def _render_pair(obj, args, iters=1, first=0):
+def _render_pair(obj, args, iters=1, first=0, indep=False):
     rc, cfg = capi.parse_args(["-s", "-1", obj] + args)
     assert rc == 0
     sh = capi.SceneHandle(obj, cfg.res_x, cfg.res_y)
@@ -155,11 +155,13 @@ def _render_pair(obj, args, iters=1, first=0):
         stats = [r.run_iteration(first + i) for i in range(iters)]
         g = r.read_accum(iters)
     o, st = oracle_py.render(sh.ptr, cfg, first, iters)
+    if indep:
+        return g, o, stats, st, oracle_independent(sh.ptr, cfg, first, iters)
     return g, o, stats, st
 
 
 def test_render_path_parity_one_iteration(box_obj):
-    g, o, stats, st = _render_pair(box_obj, box_args("64x64"), iters=1)
+    g, o, stats, st, o2 = _render_pair(box_obj, box_args("64x64"), iters=1, indep=True)
     assert np.isfinite(g).all()
     assert stats[0].path_errors == 0 and stats[0].beam_drops == 0
     assert stats[0].surface_vertices + stats[0].medium_vertices == st.light_vertices
@@ -168,18 +170,21 @@ def test_render_path_parity_one_iteration(box_obj):
     # same random streams: the large majority of pixels agree to FP32 rounding
     assert np.mean(rel < 1e-3) > 0.97, np.mean(rel < 1e-3)
     assert abs(g.mean() - o.mean()) / o.mean() < 0.01
+    assert_whole_image(g, o, o2)
 
 
 @pytest.mark.parametrize("beams", [(), ("-qbt", "S", "-pbt", "S")])
 def test_render_path_parity_dense_medium(boxfog_obj, beams):
     # denser anisotropic fog and a glossy wall: every merge estimator contributes; the GPU's
     # MIS weights must follow the oracle's, which the FP64 MIS oracle pins
-    g, o, stats, _ = _render_pair(boxfog_obj, box_args("48x48", extra=("-r_initial", "0.03") + beams), iters=1)
+    g, o, stats, _, o2 = _render_pair(boxfog_obj, box_args("48x48", extra=("-r_initial", "0.03") + beams), iters=1,
+                                      indep=True)
     assert stats[0].path_errors == 0 and stats[0].beam_drops == 0
     assert stats[0].pb2d_accepts > 0 and stats[0].bb1d_accepts > 0
     rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
     assert np.mean(rel < 1e-3) > 0.95, np.mean(rel < 1e-3)
     assert abs(g.mean() - o.mean()) / o.mean() < 0.01
+    assert_whole_image(g, o, o2)
 
 
 def test_render_converged_relmse(box_obj):

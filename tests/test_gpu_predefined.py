@@ -11,6 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from conftest import assert_whole_image, oracle_independent
 from gpuupbp_amd import capi
 from gpuupbp_amd import renderer as gpu
 from oracle import oracle_py
@@ -41,7 +42,7 @@ def test_sphere_hits_match_oracle_bit_for_bit():
     assert np.array_equal(g[:, :, 1:], want[:, :, 1:])
 
 
-def _render_pair(sid, args, iters=1):
+def _render_pair(sid, args, iters=1, indep=False):
     rc, cfg = capi.parse_args(["-s", str(sid)] + args)
     assert rc == 0
     sh = capi.SceneHandle(sid, cfg.res_x, cfg.res_y)
@@ -49,6 +50,8 @@ def _render_pair(sid, args, iters=1):
         stats = [r.run_iteration(i) for i in range(iters)]
         g = r.read_accum(iters)
     o, st = oracle_py.render(sh.ptr, cfg, 0, iters)
+    if indep:
+        return g, o, stats, st, oracle_independent(sh.ptr, cfg, 0, iters)
     return g, o, stats, st
 
 
@@ -62,7 +65,7 @@ ALL = ["-r", "48x48", "-l", "8", "-a", "upbp_bpt+surf+pp3d+pb2d+bb1d+bp2d", "-r_
 # smallsb (imaginary box), glasssphbck (glass sphere holding a dense anisotropic medium)
 @pytest.mark.parametrize("sid", [0, 1, 2, 5, 6, 7, 16, 17, 22, 29, 35, 39])
 def test_predefined_scene_path_parity(sid):
-    g, o, stats, st = _render_pair(sid, ALL, iters=1)
+    g, o, stats, st, o2 = _render_pair(sid, ALL, iters=1, indep=True)
     assert np.isfinite(g).all()
     assert stats[0].path_errors == 0 and stats[0].beam_drops == 0 and stats[0].vertex_drops == 0
     assert stats[0].surface_vertices + stats[0].medium_vertices == st.light_vertices
@@ -71,6 +74,7 @@ def test_predefined_scene_path_parity(sid):
     rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-3)
     assert np.mean(rel < 1e-3) > 0.95, (sid, np.mean(rel < 1e-3))
     assert abs(g.mean() - o.mean()) / o.mean() < 0.01, (sid, g.mean(), o.mean())
+    assert_whole_image(g, o, o2)
 
 
 def test_predefined_scene_converges_to_oracle():
