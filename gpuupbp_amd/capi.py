@@ -93,7 +93,8 @@ class Scene(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("subbeam_factor", C.c_float), ("query_piece_factor", C.c_float),
                 ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32), ("packet_bounces", C.c_int32),
-                ("block_size", C.c_int32), ("beam_leaf", C.c_int32), ("graph", C.c_int32)]
+                ("block_size", C.c_int32), ("beam_leaf", C.c_int32), ("graph", C.c_int32),
+                ("bundle_bounces", C.c_int32), ("bundle_factor", C.c_float)]
 
 
 class IterStats(C.Structure):
@@ -112,7 +113,8 @@ class IterStats(C.Structure):
                 ("algorithmic_bytes", C.c_double), ("dominant_bytes", C.c_double),
                 ("guide_samples", C.c_uint64), ("guide_loss", C.c_double), ("ms_guide", C.c_double),
                 ("merge_stack_overflows", C.c_uint64), ("photon_list_resizes", C.c_uint64),
-                ("graph_launches", C.c_uint64)]
+                ("graph_launches", C.c_uint64), ("bundles", C.c_uint64), ("bundle_pieces_left", C.c_uint64),
+                ("bundle_nodes", C.c_uint64), ("bundle_leaves", C.c_uint64), ("bundle_tests", C.c_uint64)]
 
 
 class GuideConfig(C.Structure):

@@ -223,10 +223,28 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
             // rays instead of the whole length of one or two rays.
             const uint32_t lane = __lane_id();
             const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            // Bounces walked per bundle also list each j-th run (every lane's j-th piece: 64
+            // neighbouring rays at similar depth) as one bundle. Runs are at most the wave's
+            // pieces, so the bundle list fits the piece capacity.
+            // (Lanes without a query are inactive here: the run count is the largest piece count
+            // of the active lanes, found bit by bit with ballots, and the first active lane
+            // allocates.)
+            uint32_t ubase = 0;
+            const bool bundles = b.units != nullptr && bounce < p.bundle_bounces;
+            const int uleader = __ffsll((unsigned long long)__ballot(1)) - 1;
+            if (bundles) {
+                uint32_t ng = 0;
+                for (int bit = 5; bit >= 0; bit--)
+                    if (__ballot(c1.pieces >= (ng | (1u << bit)))) ng |= 1u << bit;
+                if (int(lane) == uleader && ng) ubase = atomicAdd(&b.qcount[384 + bounce], ng);
+                ubase = __shfl(ubase, uleader);
+            }
             uint32_t start = 0, rec = 0, k = 0;
             for (uint32_t j = 0;; j++) {
                 const uint64_t mj = __ballot(c1.pieces > j);
                 if (!mj) break;
+                if (bundles && int(lane) == uleader && ubase + j < b.piece_cap)
+                    b.units[ubase + j] = make_uint2(pbase + start, uint32_t(__popcll(mj)));
                 if (c1.pieces > j) {
                     b.pieces[pbase + start + uint32_t(__popcll(mj & below))] =
                         make_uint2(at + rec, k | (uint32_t(npl[rec]) << 16));

@@ -120,6 +120,25 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
+// B-B1D test of one query piece against one sub-beam record (g0: beam origin, length; g1:
+// direction, beam slot; g2: owned beam range, medium, flags): program_bb1d.cu:31-48 on the whole
+// beam, then ownership of the camera parameter by the piece and of the beam parameter by the
+// sub-beam.
+__device__ __forceinline__ bool bb1d_accept(const IterParams& p, float4 g0, float4 g1, float4 g2, f3 ro, f3 rd,
+                                            float dmin, float dmax, float lo, float hi, int med, uint32_t& slot,
+                                            float& x, float& y, float& z, float& w)
+{
+    if (!pred_beam_beam(ro, rd, dmin, dmax, mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), 0.f, g0.w, p.r2_bb1d, z, w, x,
+                        y))
+        return false;
+    if (!(x >= lo && x < hi)) return false;
+    if (!(y >= g2.x && y < g2.y)) return false;
+    // B-B1D subset of the light paths and the same-medium test (functs_bb1d.cuh:65)
+    if (!(__float_as_int(g2.w) & SB_BB1D) || __float_as_int(g2.z) != med) return false;
+    slot = uint32_t(__float_as_int(g1.w));
+    return true;
+}
+
 // Leaf test of one query against one photon (sorted leaf k). Returns whether the pair is
 // accepted and its parameters: P-B2D (idist, ir2); B-B1D (t camera, t beam, distance, sin).
 // Only the query piece owning the pair's camera parameter [lo, hi) accepts it.
@@ -151,18 +170,9 @@ __device__ __forceinline__ bool leaf_accept(const IterParams& p, const PhotonMap
         if (!(uint32_t(__float_as_int(br.d.z)) < p.bb1d_used) || __float_as_int(d.w) != med) return false;
         slot = rf.x;
         return true;
-    } else { // program_bb1d.cu:31-48 on the whole beam, then sub-beam ownership of t beam
-        const float4 g0 = M.beam_geo[3 * k], g1 = M.beam_geo[3 * k + 1];
-        if (!pred_beam_beam(ro, rd, dmin, dmax, mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), 0.f, g0.w, p.r2_bb1d, z,
-                            w, x, y))
-            return false;
-        if (!(x >= lo && x < hi)) return false;
-        const float4 g2 = M.beam_geo[3 * k + 2];
-        if (!(y >= g2.x && y < g2.y)) return false;
-        // B-B1D subset of the light paths and the same-medium test (functs_bb1d.cuh:65)
-        if (!(__float_as_int(g2.w) & SB_BB1D) || __float_as_int(g2.z) != med) return false;
-        slot = uint32_t(__float_as_int(g1.w));
-        return true;
+    } else {
+        return bb1d_accept(p, M.beam_geo[3 * k], M.beam_geo[3 * k + 1], M.beam_geo[3 * k + 2], ro, rd, dmin, dmax, lo,
+                           hi, med, slot, x, y, z, w);
     }
 }
 
@@ -236,11 +246,13 @@ __device__ __forceinline__ void eval_pairs(const DScene& sc, const IterParams& p
 
 // One job's segment merges of one bounce, by the wave the queues belong to. bid: the block's
 // index (its XCD is bid % 8); pass_order: take the pieces as the camera pass wrote them whatever
-// the job's piece order (batched launches, which sort nothing). LOG adds the MIS weight log of the
-// sampled camera paths (Bufs::mislog).
+// the job's piece order (batched launches, which sort nothing); leftovers: take the B-B1D pieces
+// the bundle kernel left (pieces_x). LOG adds the MIS weight log of the sampled camera paths
+// (Bufs::mislog).
 template <int KIND, bool LOG>
 __device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, int bounce, uint32_t bid,
-                                               bool pass_order, CandQueue& C, PairQueue& Q, Steal& S, ColorCache& K)
+                                               bool pass_order, bool leftovers, CandQueue& C, PairQueue& Q, Steal& S,
+                                               ColorCache& K)
 {
     const IterParams& p = *b.params;
     const uint32_t lane = __lane_id();
@@ -250,12 +262,12 @@ __device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, 
     // B-B1D walks query pieces from its XCD's region first (then helps the others);
     // P-B2D (few candidates per query) walks whole segments
     const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
-    const uint32_t total = KIND == 1 ? min(b.qcount[320 + bounce], b.piece_cap) : n_recs;
+    const uint32_t total = KIND == 1 ? min(b.qcount[leftovers ? 448 + bounce : 320 + bounce], b.piece_cap) : n_recs;
     uint32_t* head = &b.qcount[192 + 2 * bounce + KIND];
     uint32_t* xw = b.xwork + size_t(bounce) * kXWork;
-    const int order = pass_order ? 0 : p.piece_order;
+    const int order = pass_order || leftovers ? 0 : p.piece_order;
     const bool split = KIND == 1 && (order == 1 || order >= 3);
-    const uint2* plist = KIND == 1 && order != 0 ? b.pieces_x : b.pieces;
+    const uint2* plist = KIND == 1 && (order != 0 || leftovers) ? b.pieces_x : b.pieces;
     uint32_t src = bid % kXcds, tried = 0; // work range being consumed (same in every lane)
     const WBvh& bvh = KIND == 0 ? b.maps->wmed : b.maps->wbeam;
     const float norm = KIND == 0 ? p.norm_pb2d : p.norm_bb1d;
@@ -451,14 +463,14 @@ __device__ __forceinline__ void seg_merge_body(const DScene& sc, const Bufs& b, 
 }
 
 template <int KIND, bool LOG>
-__global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, int bounce)
+__global__ void __launch_bounds__(kMergeBlock) k_seg_merge(DScene sc, Bufs b, int bounce, int leftovers)
 {
     __shared__ CandQueue cq[kMergeWaves];
     __shared__ PairQueue pq[kMergeWaves];
     __shared__ Steal stl[kMergeWaves];
     __shared__ ColorCache ccache[kMergeWaves];
     const uint32_t w = threadIdx.x >> 6;
-    seg_merge_body<KIND, LOG>(sc, b, bounce, blockIdx.x, false, cq[w], pq[w], stl[w], ccache[w]);
+    seg_merge_body<KIND, LOG>(sc, b, bounce, blockIdx.x, false, leftovers != 0, cq[w], pq[w], stl[w], ccache[w]);
 }
 
 // The segment merges of one bounce for every job of a batch in one persistent launch: each block
@@ -478,7 +490,276 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_batch(const BatchJob*
         const IterParams& p = *J.b.params;
         const bool on = KIND == 0 ? (p.build_pb2d && J.b.maps->wmed.n > 0) : (p.build_bb1d && J.b.maps->wbeam.n > 0);
         if (!on || bounce >= int(p.max_len)) continue;
-        seg_merge_body<KIND, false>(J.sc, J.b, bounce, blockIdx.x, true, cq[w], pq[w], stl[w], ccache[w]);
+        // B-B1D on a bundled bounce: the pieces the bundle kernel left
+        const bool left = KIND == 1 && bounce < p.bundle_bounces && J.b.units != nullptr;
+        seg_merge_body<KIND, false>(J.sc, J.b, bounce, blockIdx.x, true, left, cq[w], pq[w], stl[w], ccache[w]);
+    }
+}
+
+// ---- B-B1D bundles -------------------------------------------------------------------------
+// A bundle is one camera wave's run of j-th pieces: up to 64 neighbouring rays' stretches at
+// similar depth (on the first bounce, the primary rays of an 8x8-pixel tile). One wave walks the
+// beam tree once for the whole bundle. Every piece lies in a capsule around one axis segment (the
+// pieces' mean direction through their mean start, over their extent along it; radius: the
+// farthest piece end from that line), and a child is visited when its box, grown by the radius,
+// meets the axis segment. The wave takes up to 64 frontier nodes per step, one per lane, so one
+// fetch of a node serves every ray of the bundle. The leaves the capsule reaches are staged in LDS
+// 64 at a time (the quantised leaf box and the sub-beam record), and every ray tests each of them
+// as the per-piece walk does at a leaf: its own slab test of the leaf box, then the exact
+// predicate with both ownership rules. The capsule meets every box a piece meets, so the bundle
+// lists every leaf the per-piece walk reaches, and the pairs accepted are the same (a leaf box can
+// overhang its parent's by under one quantisation step; a pair the exact predicate places in that
+// sliver is the only possible difference). A bundle wider than p.bundle_radius (rays that diverge, or stretches at
+// different depths) hands its pieces to the per-piece walk (pieces_x, counted in
+// qcount[448 + bounce]), which runs next.
+constexpr uint32_t kFrontCap = 576;                      // frontier size up to which 64 nodes go per step
+constexpr uint32_t kFrontAlloc = kFrontCap + 7 * 64 + 8; // + growth of one-node steps down the deepest tree
+constexpr uint32_t kBundleLeafCap = 64 + 64 * kWide;     // < 64 pending + up to kWide leaves per lane
+
+struct BundleLds {
+    uint32_t front[kFrontAlloc];   // frontier nodes (taken from the top)
+    uint32_t leaf[kBundleLeafCap]; // leaves reached, not yet tested: node << 3 | child
+    float4 g0[64], g1[64], g2[64]; // the sub-beam records of the 64 being tested
+    float lx[64], ly[64], lz[64], ux[64], uy[64], uz[64]; // and their quantised boxes
+};
+
+__device__ __forceinline__ float wave_fsum(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float wave_fmax(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_fmin(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// One job's B-B1D bundles of one bounce, by the wave the LDS belongs to.
+template <bool LOG>
+__device__ __forceinline__ void seg_bundle_body(const DScene& sc, const Bufs& b, int bounce, BundleLds& U,
+                                                PairQueue& Q, ColorCache& K)
+{
+    const IterParams& p = *b.params;
+    const uint32_t lane = __lane_id();
+    K.tag[lane] = 0u;
+    K.r[lane] = K.g[lane] = K.b[lane] = 0.f;
+    wave_lds_sync();
+    const uint32_t n_recs = min(b.qcount[128 + bounce], b.seg_cap);
+    const uint32_t n_units = min(b.qcount[384 + bounce], b.piece_cap);
+    uint32_t* uhead = b.xwork + size_t(bounce) * kXWork + 216;
+    uint32_t* left = &b.qcount[448 + bounce];
+    const WNode* nodes = b.maps->wbeam.nodes;
+    const float4* geo = b.maps->beam_geo;
+    const float norm = p.norm_bb1d;
+    uint32_t qn = 0; // pairs queued (same value in every lane)
+    uint32_t n_bundles = 0, n_left = 0, n_nodes = 0, n_leaves = 0, n_tests = 0, n_acc = 0, n_err = 0;
+
+    for (;;) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(uhead, 1u);
+        u = __shfl(u, 0);
+        if (u >= n_units) break;
+        const uint2 un = b.units[u];
+        const uint32_t cnt = min(un.y, 64u);
+        // this lane's piece (the per-piece walk's validity rule: a piece left unwritten by an
+        // overflowing camera wave is skipped)
+        bool has = false;
+        uint2 pc = make_uint2(~0u, 0u);
+        f3 ro = mk(0.f), rd = mk(0.f);
+        float dmin = 0.f, dmax = 0.f, lo = 0.f, hi = 0.f, ta = 0.f, tb = 0.f;
+        int med = 0;
+        if (lane < cnt && un.x + lane < b.piece_cap) {
+            pc = b.pieces[un.x + lane];
+            if (pc.x < n_recs && (pc.y & 0xffffu) < (pc.y >> 16) && (pc.y >> 16) <= kPiecesPerRay) {
+                const SegRec* r = b.segq + pc.x;
+                const float4 a = r->a, bb = r->b, d = r->d;
+                ro = mk(a.x, a.y, a.z);
+                rd = mk(bb.x, bb.y, bb.z);
+                dmin = a.w;
+                dmax = bb.w;
+                med = __float_as_int(d.z);
+                piece_range(dmin, dmax, pc.y & 0xffffu, pc.y >> 16, lo, hi, ta, tb);
+                has = true;
+            }
+        }
+        const uint64_t hm = __ballot(has);
+        if (!hm) continue;
+        const uint32_t nh = uint32_t(__popcll(hm));
+        // the capsule
+        const f3 e0 = has ? ro + rd * ta : mk(0.f), e1 = has ? ro + rd * tb : mk(0.f);
+        const float inv_n = 1.f / float(nh);
+        const f3 c = mk(wave_fsum(e0.x), wave_fsum(e0.y), wave_fsum(e0.z)) * inv_n;
+        f3 ax = mk(wave_fsum(rd.x), wave_fsum(rd.y), wave_fsum(rd.z));
+        const float al = sqrtf(dot(ax, ax));
+        bool fit = al > 1e-3f * float(nh);
+        ax = fit ? ax / al : mk(1.f, 0.f, 0.f);
+        float smin = kInf, smax = -kInf, r2 = 0.f, mag = 0.f;
+        if (has) {
+            const f3 v0 = e0 - c, v1 = e1 - c;
+            const float s0 = dot(v0, ax), s1 = dot(v1, ax);
+            const f3 q0 = v0 - ax * s0, q1 = v1 - ax * s1;
+            r2 = fmaxf(dot(q0, q0), dot(q1, q1));
+            smin = fminf(s0, s1);
+            smax = fmaxf(s0, s1);
+            mag = fmaxf(fmaxf(fmaxf(fabsf(e0.x), fabsf(e0.y)), fabsf(e0.z)),
+                        fmaxf(fmaxf(fabsf(e1.x), fabsf(e1.y)), fabsf(e1.z)));
+        }
+        smin = wave_fmin(smin);
+        smax = wave_fmax(smax);
+        const float rad = sqrtf(wave_fmax(r2));
+        mag = wave_fmax(mag);
+        fit = fit && rad <= p.bundle_radius;
+        if (!fit) { // to the per-piece walk
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(left, nh);
+            base = __shfl(base, 0);
+            const uint32_t at = base + uint32_t(__popcll(hm & lanes_below()));
+            if (has && at < b.piece_cap) b.pieces_x[at] = pc;
+            n_left += lane == 0 ? nh : 0u;
+            continue;
+        }
+        n_bundles += lane == 0 ? 1u : 0u;
+        // the capsule test only culls: FP32 slack on the radius keeps it conservative
+        const float rg = rad * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f;
+        const f3 a0 = c + ax * smin, a1 = c + ax * smax;
+        const RayBox cap = make_raybox(a0, a1 - a0, 0.f, 1.f);
+        const RayBox prb = make_raybox(ro, rd, ta, tb); // this piece, as the per-piece walk tests it
+        uint32_t fs = 1, ln = 0; // frontier / listed leaves (same value in every lane)
+        if (lane == 0) U.front[0] = 0u;
+        wave_lds_sync();
+        while (fs > 0 || ln > 0) {
+            if (fs > 0) {
+                const uint32_t room = fs < kFrontCap ? (kFrontCap - fs) / 7u : 0u;
+                const uint32_t take = min(min(64u, fs), max(1u, room));
+                if (fs + 7u * take > kFrontAlloc) { // unreachable for trees of at most 64 levels
+                    n_err += lane == 0 ? nh : 0u;
+                    fs = ln = 0;
+                    break;
+                }
+                int node = -1;
+                if (lane < take) node = int(U.front[fs - 1u - lane]);
+                wave_lds_sync();
+                fs -= take;
+                int lf[kWide], in[kWide];
+#pragma unroll
+                for (int k = 0; k < kWide; k++) lf[k] = in[k] = -1;
+                if (node >= 0) {
+                    const WNode nd = nodes[node];
+                    n_nodes++;
+                    const float sx = wnode_step(nd.meta, 0), sy = wnode_step(nd.meta, 1), sz = wnode_step(nd.meta, 2);
+                    const int nc = wnode_count(nd.meta);
+#pragma unroll
+                    for (int k = 0; k < kWide; k++) {
+                        if (k < nc) {
+                            float e, lx, ly, lz, ux, uy, uz;
+                            wnode_box(nd, k, sx, sy, sz, lx, ly, lz, ux, uy, uz);
+                            if (slab(cap, lx - rg, ly - rg, lz - rg, ux + rg, uy + rg, uz + rg, 1.f, e)) {
+                                const int ch = nd.child[k];
+                                if (ch < 0) lf[k] = (node << 3) | k;
+                                else in[k] = ch;
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kWide; k++) {
+                    const uint64_t mi = __ballot(in[k] >= 0), ml = __ballot(lf[k] >= 0);
+                    if (in[k] >= 0) U.front[fs + uint32_t(__popcll(mi & lanes_below()))] = uint32_t(in[k]);
+                    if (lf[k] >= 0) U.leaf[ln + uint32_t(__popcll(ml & lanes_below()))] = uint32_t(lf[k]);
+                    fs += uint32_t(__popcll(mi));
+                    ln += uint32_t(__popcll(ml));
+                }
+                wave_lds_sync();
+            }
+            // test 64 listed leaves at a time, and the rest once the frontier is empty
+            while (ln >= 64u || (fs == 0u && ln > 0u)) {
+                const uint32_t n = min(ln, 64u);
+                if (lane < n) { // the leaf's box from its node (fetched moments ago) and its record
+                    n_leaves++;
+                    const uint32_t e = U.leaf[ln - n + lane], k = e & 7u;
+                    const WNode& nd = nodes[e >> 3];
+                    const uint32_t meta = nd.meta;
+                    float lx, ly, lz, ux, uy, uz;
+                    wnode_box(nd, int(k), wnode_step(meta, 0), wnode_step(meta, 1), wnode_step(meta, 2), lx, ly, lz, ux,
+                              uy, uz);
+                    const uint32_t kl = uint32_t(~nd.child[k]);
+                    U.lx[lane] = lx;
+                    U.ly[lane] = ly;
+                    U.lz[lane] = lz;
+                    U.ux[lane] = ux;
+                    U.uy[lane] = uy;
+                    U.uz[lane] = uz;
+                    U.g0[lane] = geo[3 * kl];
+                    U.g1[lane] = geo[3 * kl + 1];
+                    U.g2[lane] = geo[3 * kl + 2];
+                }
+                wave_lds_sync();
+                ln -= n;
+                for (uint32_t j = 0; j < n; j++) {
+                    bool acc = false;
+                    uint32_t slot = 0;
+                    float x = 0.f, y = 0.f, z = 0.f, w = 0.f, e;
+                    if (has && slab(prb, U.lx[j], U.ly[j], U.lz[j], U.ux[j], U.uy[j], U.uz[j], prb.tmax, e)) {
+                        n_tests++;
+                        acc = bb1d_accept(p, U.g0[j], U.g1[j], U.g2[j], ro, rd, dmin, dmax, lo, hi, med, slot, x, y, z, w);
+                    }
+                    n_acc += acc ? 1u : 0u;
+                    pq_push(Q, qn, acc, pc.x, slot, x, y, z, w);
+                    if (qn >= 64u) eval_pairs<1, LOG>(sc, p, b, Q, K, qn, norm);
+                }
+                wave_lds_sync();
+            }
+        }
+    }
+    while (qn > 0u) eval_pairs<1, LOG>(sc, p, b, Q, K, qn, norm);
+    // write back the queries still held in the colour cache
+    wave_lds_sync();
+    if (K.tag[lane] != 0u) add_color(b.mcolor, K.pix[lane], K.r[lane], K.g[lane], K.b[lane]);
+    const uint32_t s_nodes = wave_sum(n_nodes), s_leaves = wave_sum(n_leaves), s_tests = wave_sum(n_tests),
+                   s_acc = wave_sum(n_acc);
+    if (lane == 0) {
+        stat_add(b.stats, STAT_BUNDLES, n_bundles);
+        stat_add(b.stats, STAT_BUNDLE_LEFT, n_left);
+        stat_add(b.stats, STAT_BUNDLE_NODES, s_nodes);
+        stat_add(b.stats, STAT_BUNDLE_LEAVES, s_leaves);
+        stat_add(b.stats, STAT_BUNDLE_TESTS, s_tests);
+        stat_add(b.stats, STAT_ACCEPT, s_acc);
+        stat_add(b.stats, STAT_ACC_BB1D, s_acc);
+        stat_add(b.stats, STAT_ERRORS, n_err);
+        stat_add(b.stats, STAT_MERGE_STACK, n_err);
+    }
+}
+
+template <bool LOG>
+__global__ void __launch_bounds__(kMergeBlock) k_seg_bundle(DScene sc, Bufs b, int bounce)
+{
+    __shared__ BundleLds blds[kMergeWaves];
+    __shared__ PairQueue pq[kMergeWaves];
+    __shared__ ColorCache ccache[kMergeWaves];
+    const uint32_t w = threadIdx.x >> 6;
+    seg_bundle_body<LOG>(sc, b, bounce, blds[w], pq[w], ccache[w]);
+}
+
+// The bundles of one bounce for every job of a batch that walks bundles on it, in one persistent
+// launch (the jobs walked in turn as in k_seg_merge_batch).
+__global__ void __launch_bounds__(kMergeBlock) k_seg_bundle_batch(const BatchJob* jobs, uint32_t njobs, int bounce)
+{
+    __shared__ BundleLds blds[kMergeWaves];
+    __shared__ PairQueue pq[kMergeWaves];
+    __shared__ ColorCache ccache[kMergeWaves];
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint32_t t = 0; t < njobs; t++) {
+        const BatchJob& J = jobs[(blockIdx.x + t) % njobs];
+        const IterParams& p = *J.b.params;
+        if (!(p.build_bb1d && J.b.maps->wbeam.n > 0) || bounce >= int(p.max_len) || bounce >= p.bundle_bounces ||
+            !J.b.units)
+            continue;
+        seg_bundle_body<false>(J.sc, J.b, bounce, blds[w], pq[w], ccache[w]);
     }
 }
 
@@ -721,6 +1002,23 @@ int merge_grid_blocks(int bpc)
     return cus * (bpc > 0 ? std::min(bpc, per) : per);
 }
 
+// blocks of the persistent bundle grid: CUs x its occupancy limit
+int bundle_grid_blocks()
+{
+    static int blocks = 0;
+    if (blocks == 0) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        int per1 = 0, per2 = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_seg_bundle<false>, kMergeBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per1, k_seg_bundle<true>, kMergeBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_seg_bundle_batch, kMergeBlock, 0);
+        blocks = std::max(1, cus) * std::max(1, std::min(per, std::min(per1, per2)));
+    }
+    return blocks;
+}
+
 // ---- Morton order of the B-B1D query pieces --------------------------------------------
 __device__ __forceinline__ uint32_t spread_bits10(uint32_t v)
 {
@@ -836,16 +1134,21 @@ void launch_unit_gather(const Bufs& b, int bounce, uint32_t cap_units, const uin
 void launch_seg_merge_batch(const BatchJob* jobs, uint32_t njobs, int kind, int bounce, hipStream_t s)
 {
     const int blocks = merge_grid_blocks(0);
-    if (kind == 0) hipLaunchKernelGGL(k_seg_merge_batch<0>, dim3(blocks), dim3(kMergeBlock), 0, s, jobs, njobs, bounce);
-    else hipLaunchKernelGGL(k_seg_merge_batch<1>, dim3(blocks), dim3(kMergeBlock), 0, s, jobs, njobs, bounce);
+    if (kind == 0) {
+        hipLaunchKernelGGL(k_seg_merge_batch<0>, dim3(blocks), dim3(kMergeBlock), 0, s, jobs, njobs, bounce);
+    } else { // the jobs' bundles first (a job that walks none on this bounce is skipped)
+        hipLaunchKernelGGL(k_seg_bundle_batch, dim3(bundle_grid_blocks()), dim3(kMergeBlock), 0, s, jobs, njobs,
+                           bounce);
+        hipLaunchKernelGGL(k_seg_merge_batch<1>, dim3(blocks), dim3(kMergeBlock), 0, s, jobs, njobs, bounce);
+    }
 }
 
 void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int kind, int bounce, hipStream_t s)
 {
     const int blocks = merge_grid_blocks(p.merge_bpc);
     if (kind == 0 && p.build_pb2d && p.cap_med > 0) {
-        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<0, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
-        else hipLaunchKernelGGL((k_seg_merge<0, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
+        if (b.mislog) hipLaunchKernelGGL((k_seg_merge<0, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce, 0);
+        else hipLaunchKernelGGL((k_seg_merge<0, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce, 0);
     }
     if (kind == 1 && p.build_bb1d && p.cap_beam > 0) {
         if (p.piece_order == 1) {
@@ -858,10 +1161,21 @@ void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int 
                 hipLaunchKernelGGL((k_seg_merge_packet<true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
             else
                 hipLaunchKernelGGL((k_seg_merge_packet<false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
-        } else if (b.mislog) {
-            hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
         } else {
-            hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce);
+            // bundles first on the first bundle_bounces bounces; the per-piece walk then takes
+            // the pieces they left
+            const int left = bounce < p.bundle_bounces ? 1 : 0;
+            if (left) {
+                const int bb = bundle_grid_blocks();
+                if (b.mislog)
+                    hipLaunchKernelGGL((k_seg_bundle<true>), dim3(bb), dim3(kMergeBlock), 0, s, sc, b, bounce);
+                else
+                    hipLaunchKernelGGL((k_seg_bundle<false>), dim3(bb), dim3(kMergeBlock), 0, s, sc, b, bounce);
+            }
+            if (b.mislog)
+                hipLaunchKernelGGL((k_seg_merge<1, true>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce, left);
+            else
+                hipLaunchKernelGGL((k_seg_merge<1, false>), dim3(blocks), dim3(kMergeBlock), 0, s, sc, b, bounce, left);
         }
     }
 }

@@ -36,6 +36,8 @@ struct IterParams {
     uint32_t block;     // threads per block of the light and camera bounce kernels (64, 128, 256)
     int beam_leaf;      // B-B1D leaf test reads 0: the sorted sub-beam records (beam_geo); 1: the
                         // sorted (beam slot, sub-beam j | n << 16) references and the beam record
+    int bundle_bounces; // B-B1D: camera bounces below this walk the beam tree per bundle of pieces
+    float bundle_radius; // largest bundle radius walked as one bundle (world units)
     uint32_t cap_med, cap_beam; // capacities of the medium-point and sub-beam lists (host launch
                                 // decisions; the counts themselves are only on the device)
     uint64_t iter_key;
@@ -399,7 +401,8 @@ struct Bufs {
     StackRec* cstack;
     uint32_t* queue[2];
     uint32_t* qcount; // paths alive at bounce k: light [k], camera [64 + k]; segment queries
-                      // [128 + k]; merge work heads [192 + 2k + kind]; query pieces [320 + k]
+                      // [128 + k]; merge work heads [192 + 2k + kind]; query pieces [320 + k];
+                      // piece bundles [384 + k]; pieces left to the per-piece B-B1D walk [448 + k]
     VertexRec* verts;
     uint32_t* nvert; // per light path
     BeamRec* beams;
@@ -408,7 +411,10 @@ struct Bufs {
     float* mcolor;  // per camera path: P-B2D / B-B1D contributions of the merge kernels
     SegRec* segq;   // segment queries of the current camera bounce
     uint2* pieces;  // their pieces: (query, k | n << 16)
-    uint2* pieces_x; // the same, grouped by XCD region for the B-B1D merge kernel
+    uint2* pieces_x; // the same, grouped by XCD region for the B-B1D merge kernel (or the pieces
+                     // the bundle kernel leaves to the per-piece walk)
+    uint2* units;    // piece bundles of a camera wave: (first piece, count <= 64), the wave's
+                     // j-th pieces of its rays (null when no bounce walks bundles)
     uint8_t* piece_cell; // spatial cell of each piece
     uint32_t* xwork; // per bounce: kXWork words of cell counts, offsets and per-XCD work ranges
     uint32_t seg_cap, piece_cap;
@@ -453,6 +459,7 @@ __device__ __forceinline__ void mis_log(const Bufs& b, uint32_t tech, uint32_t k
 enum Stat : int { STAT_LIGHT_STEPS = 0, STAT_CAMERA_STEPS, STAT_BPT_CONN, STAT_POINT_Q, STAT_SEG_Q, STAT_ACCEPT,
                   STAT_VDROP, STAT_BDROP, STAT_ERRORS, STAT_SEG_NODES, STAT_SEG_CANDS, STAT_ACC_PB2D, STAT_ACC_BB1D,
                   STAT_SEG_PIECES, STAT_SEG_TRIPS, STAT_CAND_BATCHES, STAT_PAIR_BATCHES, STAT_MERGE_STACK,
+                  STAT_BUNDLES, STAT_BUNDLE_LEFT, STAT_BUNDLE_NODES, STAT_BUNDLE_LEAVES, STAT_BUNDLE_TESTS,
                   STAT_COUNT };
 constexpr int kQcountWords = 512;
 // One render job of a batched launch (device memory): its scene, buffers and the few launch

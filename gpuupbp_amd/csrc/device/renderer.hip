@@ -207,7 +207,7 @@ struct upbp_renderer {
     Bufs bufs;
     DevBuf lstate, lstack, cstate, cstack, q0, q1, qcount, verts, nvert, beams, nbeam, fb32, accum, stats;
     DevBuf cnt, off, tot, list_s, list_m, lo_s, hi_s, lo_m, hi_m, lo_b, hi_b, pts_s, pts_m, geo_u, geo_b, ref_u, ref_b;
-    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals, unitn;
+    DevBuf mcolor, segq, pieces, pieces_x, piece_cell, xwork, pkeys, pvals, unitn, units;
     DevBuf guide; // learned proposal records
     DevBuf mislog, miscount; // MIS weight log (upbp_set_mis_log)
     // device-resident iteration: parameters and photon-map descriptor in device memory, photon
@@ -435,6 +435,11 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.piece_order = r->tune.piece_order;
     p.packet_bounces = r->tune.packet_bounces;
     p.beam_leaf = r->tune.beam_leaf;
+    // bundles list into pieces_x and read the sub-beam records: not with a sorted piece order,
+    // beam references or packets
+    p.bundle_bounces =
+        r->tune.piece_order == 0 && r->tune.beam_leaf == 0 && r->tune.packet_bounces == 0 ? r->tune.bundle_bounces : 0;
+    p.bundle_radius = r->tune.bundle_factor * p.rad_beam;
     p.block = r->tune.block_size > 0 ? uint32_t(r->tune.block_size) : 64u;
     p.cap_med = r->cap_m;
     p.cap_beam = r->cap_b;
@@ -661,6 +666,8 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
              !r->xwork.ensure(size_t(kMaxBounces) * kXWork * 4) &&
              !r->pkeys.ensure((piece_cap + 63) / 64 * 4) && !r->pvals.ensure((piece_cap + 63) / 64 * 4) &&
              !r->unitn.ensure(16);
+    // piece bundles of the B-B1D merge: at most one per piece
+    if (ok && (c.algorithm_flags & T_BB1D)) ok = !r->units.ensure(piece_cap * sizeof(uint2));
     if (!ok) {
         upbp_destroy(r);
         return fail(nullptr, "device allocation failed", UPBP_E_HIP);
@@ -696,6 +703,7 @@ extern "C" int upbp_create(const upbp_scene* scene, const upbp_config* cfg, upbp
     b.seg_cap = uint32_t(seg_cap);
     b.pieces = r->pieces.as<uint2>();
     b.pieces_x = r->pieces_x.as<uint2>();
+    b.units = (c.algorithm_flags & T_BB1D) ? r->units.as<uint2>() : nullptr;
     b.piece_cell = r->piece_cell.as<uint8_t>();
     b.xwork = r->xwork.as<uint32_t>();
     b.piece_cap = uint32_t(piece_cap);
@@ -1087,6 +1095,11 @@ int run_one(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
         stats->beam_drops = st[STAT_BDROP];
         stats->path_errors = st[STAT_ERRORS];
         stats->merge_stack_overflows = st[STAT_MERGE_STACK];
+        stats->bundles = st[STAT_BUNDLES];
+        stats->bundle_pieces_left = st[STAT_BUNDLE_LEFT];
+        stats->bundle_nodes = st[STAT_BUNDLE_NODES];
+        stats->bundle_leaves = st[STAT_BUNDLE_LEAVES];
+        stats->bundle_tests = st[STAT_BUNDLE_TESTS];
         stats->photon_list_resizes = r->resizes;
         stats->graph_launches = r->graph_launches;
         // Compulsory HBM bytes. Camera bounce kernels: 128-byte state read and write plus 64
@@ -1184,6 +1197,8 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     t->merge_blocks_per_cu = 0;
     t->packet_bounces = 0;
     t->graph = 0;
+    t->bundle_bounces = 1;
+    t->bundle_factor = 32.f;
 }
 
 extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
@@ -1285,7 +1300,8 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
     if (!r || !t) return fail(r, "null argument", UPBP_E_ARG);
     if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
         t->piece_order < 0 || t->piece_order > 4 || t->packet_bounces < 0 || t->beam_leaf < 0 || t->beam_leaf > 1 ||
-        (t->block_size != 0 && t->block_size != 64 && t->block_size != 128 && t->block_size != 256))
+        (t->block_size != 0 && t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
+        t->bundle_bounces < 0 || !(t->bundle_factor >= 0.f))
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     r->use_graph = t->graph != 0;
@@ -1765,7 +1781,7 @@ extern "C" void upbp_destroy(upbp_renderer* r)
                       &r->beams, &r->nbeam, &r->fb32, &r->accum, &r->stats, &r->cnt, &r->off, &r->tot, &r->list_s,
                       &r->list_m, &r->lo_s, &r->hi_s, &r->lo_m, &r->hi_m, &r->lo_b, &r->hi_b, &r->pts_s,
                       &r->pts_m, &r->geo_u, &r->geo_b, &r->ref_u, &r->ref_b, &r->mcolor, &r->segq, &r->pieces, &r->pieces_x, &r->piece_cell,
-                      &r->xwork, &r->pkeys, &r->pvals, &r->unitn, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount,
+                      &r->xwork, &r->pkeys, &r->pvals, &r->unitn, &r->units, &r->guide, &r->gpend, &r->graw, &r->gsamp, &r->gcount,
                       &r->mislog, &r->miscount, &r->dparams, &r->dmaps, &r->mapn, &r->ovf})
         d->release();
     if (r->stage) (void)hipHostFree(r->stage);

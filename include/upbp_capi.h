@@ -192,6 +192,11 @@ typedef struct upbp_iter_stats {
                                        iteration, then one per iteration that outgrew them and
                                        was rendered again) */
     uint64_t graph_launches;        /* iterations replayed from a captured graph so far */
+    uint64_t bundles;               /* B-B1D piece bundles walked as one (upbp_tuning.bundle_bounces) */
+    uint64_t bundle_pieces_left;    /* pieces of wider bundles, walked per piece */
+    uint64_t bundle_nodes;          /* beam-tree nodes fetched by the bundle walks */
+    uint64_t bundle_leaves;         /* sub-beams staged by the bundle walks (once per bundle) */
+    uint64_t bundle_tests;          /* (piece, sub-beam) exact tests of the bundle walks (leaf box hit) */
 } upbp_iter_stats;
 
 typedef struct upbp_scene_handle upbp_scene_handle;
@@ -304,6 +309,15 @@ typedef struct upbp_tuning {
     int32_t graph;               /* 1: capture the iteration's work in a HIP graph once and replay it
                                     (iterations without statistics, online training or Morton piece
                                     order, which read back to the host); 0: launch kernel by kernel */
+    int32_t bundle_bounces;      /* camera bounces below this walk the B-B1D beam tree once per
+                                    bundle: a camera wave's j-th query pieces (up to 64 neighbouring
+                                    rays at similar depth), one node fetch per bundle, the sub-beams
+                                    reached tested by every ray of the bundle from LDS; bundles
+                                    wider than bundle_factor x the beam radius, and later bounces,
+                                    walk per piece (default 1: the first bounce; needs piece_order
+                                    0, beam_leaf 0 and packet_bounces 0, else off) */
+    float bundle_factor;         /* largest bundle radius walked as one bundle, in beam radii
+                                    (default 32) */
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);

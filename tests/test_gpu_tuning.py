@@ -1,6 +1,6 @@
 """Tuning knobs change speed only (GPU): every setting of sub-beam length, query piece length,
-merge blocks per CU, piece order, packet traversal, leaf data and bounce-kernel block size accepts
-the same photon and beam pairs and renders the same image as the defaults."""
+merge blocks per CU, piece order, packet traversal, leaf data, bounce-kernel block size and piece
+bundles accepts the same photon and beam pairs and renders the same image as the defaults."""
 import numpy as np
 import pytest
 
@@ -15,7 +15,8 @@ SETTINGS = [dict(subbeam_factor=8.0, query_piece_factor=32.0), dict(subbeam_fact
             dict(piece_order=2), dict(packet_bounces=0), dict(packet_bounces=64),
             dict(packet_bounces=64, piece_order=2), dict(packet_bounces=64, subbeam_factor=0.0, query_piece_factor=0.0),
             dict(piece_order=3), dict(piece_order=4), dict(piece_order=3, graph=1), dict(beam_leaf=1),
-            dict(beam_leaf=1, piece_order=3, packet_bounces=64), dict(block_size=256), dict(block_size=128)]
+            dict(beam_leaf=1, piece_order=3, packet_bounces=64), dict(block_size=256), dict(block_size=128),
+            dict(bundle_bounces=0), dict(bundle_bounces=64, bundle_factor=1e6), dict(bundle_bounces=3, bundle_factor=2.0)]
 
 
 def _render(obj, tuning):
