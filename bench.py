@@ -475,7 +475,9 @@ def main():
                      "merge_accepts": st.merge_accepts, "pb2d_accepts": st.pb2d_accepts,
                      "bb1d_accepts": st.bb1d_accepts, "seg_nodes": st.seg_nodes,
                      "seg_candidates": st.seg_candidates, "seg_pieces": st.seg_pieces,
-                     "path_errors": st.path_errors,
+                     "bundles": st.bundles, "bundle_pieces_left": st.bundle_pieces_left,
+                     "bundle_nodes": st.bundle_nodes, "bundle_leaves": st.bundle_leaves,
+                     "bundle_tests": st.bundle_tests, "path_errors": st.path_errors,
                      "beam_drops": st.beam_drops},
         "roofline": {
             "kernel": st.dominant_kernel.decode(),
@@ -499,12 +501,13 @@ def main():
                                  "achieved": st.algorithmic_bytes / t_iter / 1e9 if t_iter > 0 else 0.0,
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": st.algorithmic_bytes / t_iter / 1e9 / HBM_PEAK_GBS if t_iter > 0 else 0.0}
-    # segment queries with their counted traversal: 44 bytes per query (record and result), 64
-    # per node fetched, 16 per candidate tested, the record of every accepted photon (112) or beam
-    # (128), over the two merge kernels' time
+    # segment queries with their counted traversal: 44 bytes per query (record and result), 128
+    # per tree node fetched (one 8-wide node), 48 per sub-beam or point record tested by the
+    # per-piece walk, 48 + 32 per sub-beam a bundle stages (record, and its box from its node),
+    # the record of every accepted photon (112) or beam (128), over the two merge stages' time
     t_q = (st.ms_merge_pb2d + st.ms_merge_bb1d) * 1e-3
-    b_q = (44.0 * st.segment_queries + 64.0 * st.seg_nodes + 16.0 * st.seg_candidates + 112.0 * st.pb2d_accepts +
-           128.0 * st.bb1d_accepts)
+    b_q = (44.0 * st.segment_queries + 128.0 * (st.seg_nodes + st.bundle_nodes) + 48.0 * st.seg_candidates +
+           80.0 * st.bundle_leaves + 112.0 * st.pb2d_accepts + 128.0 * st.bb1d_accepts)
     out["query_roofline"] = {"bytes": b_q, "queries": st.segment_queries,
                              "achieved": b_q / t_q / 1e9 if t_q > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": b_q / t_q / 1e9 / HBM_PEAK_GBS if t_q > 0 else 0.0}

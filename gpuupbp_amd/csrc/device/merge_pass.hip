@@ -521,6 +521,7 @@ struct BundleLds {
     uint32_t leaf[kBundleLeafCap]; // leaves reached, not yet tested: node << 3 | child
     float4 g0[64], g1[64], g2[64]; // the sub-beam records of the 64 being tested
     float lx[64], ly[64], lz[64], ux[64], uy[64], uz[64]; // and their quantised boxes
+    uint16_t cand[128];            // (piece lane, staged sub-beam) pairs whose box test passed
 };
 
 __device__ __forceinline__ float wave_fsum(float v)
@@ -700,17 +701,44 @@ __device__ __forceinline__ void seg_bundle_body(const DScene& sc, const Bufs& b,
                 }
                 wave_lds_sync();
                 ln -= n;
-                for (uint32_t j = 0; j < n; j++) {
+                // Each piece's box test of each staged sub-beam; the pairs that pass are queued
+                // and take the exact test 64 at a time, one pair per lane (the piece's values
+                // come from its lane by shuffles), so that the exact tests run on full waves.
+                uint32_t cn = 0;
+                for (uint32_t j = 0; j <= n; j++) {
+                    if (j < n) {
+                        float e;
+                        const bool h =
+                            has && slab(prb, U.lx[j], U.ly[j], U.lz[j], U.ux[j], U.uy[j], U.uz[j], prb.tmax, e);
+                        const uint64_t mh = __ballot(h);
+                        if (h) U.cand[cn + uint32_t(__popcll(mh & lanes_below()))] = uint16_t((j << 6) | lane);
+                        cn += uint32_t(__popcll(mh));
+                        n_tests += h ? 1u : 0u;
+                        if (cn < 64u) continue;
+                    }
+                    if (cn == 0u) continue;
+                    const uint32_t m = min(cn, 64u);
+                    wave_lds_sync();
+                    const uint32_t ce = lane < m ? U.cand[cn - m + lane] : lane;
+                    const uint32_t src = ce & 63u, sj = (ce >> 6) & 63u;
+                    const f3 pro = mk(__shfl(ro.x, src), __shfl(ro.y, src), __shfl(ro.z, src));
+                    const f3 prd = mk(__shfl(rd.x, src), __shfl(rd.y, src), __shfl(rd.z, src));
+                    const float pdmin = __shfl(dmin, src), pdmax = __shfl(dmax, src);
+                    const float plo = __shfl(lo, src), phi = __shfl(hi, src);
+                    const int pmed = __shfl(med, src);
+                    const uint32_t pq_q = uint32_t(__shfl(int(pc.x), src));
                     bool acc = false;
                     uint32_t slot = 0;
-                    float x = 0.f, y = 0.f, z = 0.f, w = 0.f, e;
-                    if (has && slab(prb, U.lx[j], U.ly[j], U.lz[j], U.ux[j], U.uy[j], U.uz[j], prb.tmax, e)) {
-                        n_tests++;
-                        acc = bb1d_accept(p, U.g0[j], U.g1[j], U.g2[j], ro, rd, dmin, dmax, lo, hi, med, slot, x, y, z, w);
-                    }
+                    float x = 0.f, y = 0.f, z = 0.f, w = 0.f;
+                    if (lane < m)
+                        acc = bb1d_accept(p, U.g0[sj], U.g1[sj], U.g2[sj], pro, prd, pdmin, pdmax, plo, phi, pmed, slot,
+                                          x, y, z, w);
+                    wave_lds_sync();
+                    cn -= m;
                     n_acc += acc ? 1u : 0u;
-                    pq_push(Q, qn, acc, pc.x, slot, x, y, z, w);
+                    pq_push(Q, qn, acc, pq_q, slot, x, y, z, w);
                     if (qn >= 64u) eval_pairs<1, LOG>(sc, p, b, Q, K, qn, norm);
+                    if (cn > 0u && j == n) j--; // drain what is left after the last sub-beam
                 }
                 wave_lds_sync();
             }

@@ -1118,7 +1118,7 @@ int run_one(upbp_renderer* r, int iteration, upbp_iter_stats* stats)
             double ms, bytes;
         } g[3] = {{"k_camera_bounce", stats->ms_bounce, b_bounce},
                   {"k_seg_merge<0> (P-B2D)", stats->ms_merge_pb2d, b_pb2d},
-                  {"k_seg_merge<1> (B-B1D)", stats->ms_merge_bb1d, b_bb1d}};
+                  {"B-B1D merge (bundle+piece)", stats->ms_merge_bb1d, b_bb1d}};
         int dom = 0;
         for (int k = 1; k < 3; k++)
             if (g[k].ms > g[dom].ms) dom = k;

@@ -169,8 +169,9 @@ typedef struct upbp_iter_stats {
     double ms_bounce;              /* camera bounce kernels (intersector, BPT, point merges) */
     double ms_merge_pb2d;          /* P-B2D segment merge kernels */
     double ms_merge_bb1d;          /* B-B1D segment merge kernels */
-    double ms_dominant;            /* summed duration of the dominant kernel in this iteration */
-    uint64_t dominant_launches;    /* its launch count */
+    double ms_dominant;            /* summed duration of the dominant merge or bounce stage in this
+                                      iteration (the B-B1D stage: its bundle and per-piece kernels) */
+    uint64_t dominant_launches;    /* its stage count (one per camera bounce) */
     char dominant_kernel[32];      /* its name */
     uint64_t light_vertices, surface_vertices, medium_vertices, beams, subbeams;
     uint64_t light_extend_steps, camera_extend_steps;
