@@ -570,12 +570,12 @@ __device__ __forceinline__ void camera_bounce_body(const DScene& sc, const Bufs&
 }
 
 template <bool LOG>
-__global__ void __launch_bounds__(256, 3) k_camera_bounce(DScene sc, Bufs b, int bounce)
+__global__ void __launch_bounds__(256, 4) k_camera_bounce(DScene sc, Bufs b, int bounce)
 {
     camera_bounce_body<LOG>(sc, b, bounce, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(256, 3) k_camera_bounce_batch(const BatchJob* jobs, const uint2* bmap, int bounce)
+__global__ void __launch_bounds__(256, 4) k_camera_bounce_batch(const BatchJob* jobs, const uint2* bmap, int bounce)
 {
     const uint2 m = bmap[blockIdx.x];
     camera_bounce_body<false>(jobs[m.x].sc, jobs[m.x].b, bounce, m.y);
