@@ -1350,9 +1350,11 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
     double best_ms = 0.0;
     int e = timed(best, best_ms);
     const float subbeam[] = {8.f, 12.f, 16.f, 24.f}, piece[] = {32.f, 64.f, 128.f};
-    const int blocks[] = {0, 1, 2}, order[] = {0, 2}, packets[] = {0, 1, 2, 64}, threads[] = {64, 128, 256};
-    for (int knob = 0; knob < 6 && !e; knob++) {
-        const int n = knob == 0 ? 4 : (knob == 1 ? 3 : (knob == 2 ? 3 : (knob == 3 ? 2 : (knob == 4 ? 4 : 3))));
+    const int blocks[] = {0, 1, 2}, order[] = {0, 2}, packets[] = {0, 1, 2, 64}, threads[] = {64, 128, 256},
+              bundles[] = {0, 1, 2};
+    const int counts[] = {4, 3, 3, 2, 4, 3, 3};
+    for (int knob = 0; knob < 7 && !e; knob++) {
+        const int n = counts[knob];
         const upbp_tuning base = best;
         for (int v = 0; v < n && !e; v++) {
             upbp_tuning t = base;
@@ -1361,7 +1363,8 @@ extern "C" int upbp_autotune(upbp_renderer* r, int32_t iteration, int32_t repeat
             else if (knob == 2) t.merge_blocks_per_cu = blocks[v];
             else if (knob == 3) t.piece_order = order[v];
             else if (knob == 4) t.packet_bounces = packets[v];
-            else t.block_size = threads[v];
+            else if (knob == 5) t.block_size = threads[v];
+            else t.bundle_bounces = bundles[v];
             if (memcmp(&t, &base, sizeof(t)) == 0) continue;
             double ms = 0.0;
             e = timed(t, ms);

@@ -355,7 +355,7 @@ void upbp_batch_destroy(upbp_batch* b); /* the renderers stay; destroy the batch
 
 /* Autotuner (no reference counterpart): a coordinate search over subbeam_factor {8, 12, 16,
  * 24}, query_piece_factor {32, 64, 128}, merge_blocks_per_cu {0, 1, 2}, piece_order {0, 2},
- * packet_bounces {0, 1, 2, 64} and block_size {64, 128, 256},
+ * packet_bounces {0, 1, 2, 64}, block_size {64, 128, 256} and bundle_bounces {0, 1, 2},
  * from the current tuning, timing `repeats` (1..64) renders of `iteration` per candidate after
  * one untimed render. The fastest tuning stays set and is returned in *chosen with its time in
  * *ms_per_iteration (either may be NULL). The accumulated image is restored afterwards and

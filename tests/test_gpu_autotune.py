@@ -25,7 +25,7 @@ def test_autotune_keeps_the_image(boxfog_obj):
         assert np.array_equal(r.read_accum(1), one)  # trial iterations leave no trace
         assert tuning["subbeam_factor"] in (8.0, 12.0, 16.0, 24.0) and tuning["query_piece_factor"] in (32.0, 64.0, 128.0)
         assert tuning["merge_blocks_per_cu"] in (0, 1, 2) and tuning["piece_order"] in (0, 2) and ms > 0
-        assert tuning["packet_bounces"] in (0, 1, 2, 64)
+        assert tuning["packet_bounces"] in (0, 1, 2, 64) and tuning["bundle_bounces"] in (0, 1, 2)
         for i in (1, 2):
             r.run_iteration(i, want_stats=False)
         tuned = r.read_accum(3)
