@@ -503,24 +503,26 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_batch(const BatchJob*
 // pieces' mean direction through their mean start, over their extent along it; radius: the
 // farthest piece end from that line), and a child is visited when its box, grown by the radius,
 // meets the axis segment. The wave takes up to 64 frontier nodes per step, one per lane, so one
-// fetch of a node serves every ray of the bundle. The leaves the capsule reaches are staged in LDS
-// 64 at a time (the quantised leaf box and the sub-beam record), and every ray tests each of them
-// as the per-piece walk does at a leaf: its own slab test of the leaf box, then the exact
+// fetch of a node serves every ray of the bundle. The leaves the capsule reaches are staged in
+// LDS kStage at a time (the quantised leaf box and the sub-beam record), and every ray tests each
+// of them as the per-piece walk does at a leaf: its own slab test of the leaf box, then the exact
 // predicate with both ownership rules. The capsule meets every box a piece meets, so the bundle
-// lists every leaf the per-piece walk reaches, and the pairs accepted are the same (a leaf box can
-// overhang its parent's by under one quantisation step; a pair the exact predicate places in that
-// sliver is the only possible difference). A bundle wider than p.bundle_radius (rays that diverge, or stretches at
-// different depths) hands its pieces to the per-piece walk (pieces_x, counted in
-// qcount[448 + bounce]), which runs next.
-constexpr uint32_t kFrontCap = 576;                      // frontier size up to which 64 nodes go per step
+// lists every leaf the per-piece walk reaches, and the pairs accepted are the same (a leaf box
+// can overhang its parent's by under one quantisation step; a pair the exact predicate places in
+// that sliver is the only possible difference). A bundle wider than p.bundle_radius (rays that
+// diverge, or stretches at different depths) hands its pieces to the per-piece walk (pieces_x,
+// counted in qcount[448 + bounce]), which runs next. Three 4-wave blocks fit a CU (LDS and
+// VGPRs), the per-piece walk's occupancy.
+constexpr uint32_t kFrontCap = 256;                      // frontier size up to which 64 nodes go per step
 constexpr uint32_t kFrontAlloc = kFrontCap + 7 * 64 + 8; // + growth of one-node steps down the deepest tree
-constexpr uint32_t kBundleLeafCap = 64 + 64 * kWide;     // < 64 pending + up to kWide leaves per lane
+constexpr uint32_t kStage = 32;                          // sub-beams staged in LDS at a time
+constexpr uint32_t kBundleLeafCap = kStage + 64 * kWide; // < kStage pending + up to kWide leaves per lane
 
 struct BundleLds {
     uint32_t front[kFrontAlloc];   // frontier nodes (taken from the top)
     uint32_t leaf[kBundleLeafCap]; // leaves reached, not yet tested: node << 3 | child
-    float4 g0[64], g1[64], g2[64]; // the sub-beam records of the 64 being tested
-    float lx[64], ly[64], lz[64], ux[64], uy[64], uz[64]; // and their quantised boxes
+    float4 g0[kStage], g1[kStage], g2[kStage]; // the sub-beam records being tested
+    float lx[kStage], ly[kStage], lz[kStage], ux[kStage], uy[kStage], uz[kStage]; // and their quantised boxes
     uint16_t cand[128];            // (piece lane, staged sub-beam) pairs whose box test passed
 };
 
@@ -677,9 +679,9 @@ __device__ __forceinline__ void seg_bundle_body(const DScene& sc, const Bufs& b,
                 }
                 wave_lds_sync();
             }
-            // test 64 listed leaves at a time, and the rest once the frontier is empty
-            while (ln >= 64u || (fs == 0u && ln > 0u)) {
-                const uint32_t n = min(ln, 64u);
+            // test kStage listed leaves at a time, and the rest once the frontier is empty
+            while (ln >= kStage || (fs == 0u && ln > 0u)) {
+                const uint32_t n = min(ln, kStage);
                 if (lane < n) { // the leaf's box from its node (fetched moments ago) and its record
                     n_leaves++;
                     const uint32_t e = U.leaf[ln - n + lane], k = e & 7u;
@@ -764,7 +766,7 @@ __device__ __forceinline__ void seg_bundle_body(const DScene& sc, const Bufs& b,
 }
 
 template <bool LOG>
-__global__ void __launch_bounds__(kMergeBlock) k_seg_bundle(DScene sc, Bufs b, int bounce)
+__global__ void __launch_bounds__(kMergeBlock, 3) k_seg_bundle(DScene sc, Bufs b, int bounce)
 {
     __shared__ BundleLds blds[kMergeWaves];
     __shared__ PairQueue pq[kMergeWaves];
@@ -775,7 +777,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_bundle(DScene sc, Bufs b, i
 
 // The bundles of one bounce for every job of a batch that walks bundles on it, in one persistent
 // launch (the jobs walked in turn as in k_seg_merge_batch).
-__global__ void __launch_bounds__(kMergeBlock) k_seg_bundle_batch(const BatchJob* jobs, uint32_t njobs, int bounce)
+__global__ void __launch_bounds__(kMergeBlock, 3) k_seg_bundle_batch(const BatchJob* jobs, uint32_t njobs, int bounce)
 {
     __shared__ BundleLds blds[kMergeWaves];
     __shared__ PairQueue pq[kMergeWaves];
