@@ -186,6 +186,7 @@ _SIGS = [
     ("upbp_mlp_noalloc_max", C.c_uint32, []),
     ("upbp_set_mis_log", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("upbp_read_mis_log", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("upbp_read_guide_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
 ]
 
 

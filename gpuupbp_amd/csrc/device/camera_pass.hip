@@ -439,9 +439,9 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
 // colour) and the P-B2D / B-B1D merges along its later segments (the merge colour). With
 // throughput T before the scattering, scattering function f, cosine cos, continuation
 // probability cp and sampling pdf p, the path carries T f cos / (p cp) onward, so
-// lum(collected after) cp / lum(T f cos) estimates L_i / p: the incident
-// radiance along the sampled direction over its pdf, the weight of the KL gradient that fits
-// the proposal's radiance mixture to L_i. The sample is (position, scattered direction) with
+// lum(collected after) cp / lum(T f cos) estimates L_i / p: the incident radiance along the
+// sampled direction (the part the path's later vertices collect under their MIS weights) over
+// its pdf, the weight of the KL gradient that fits the proposal's radiance mixture to L_i. The sample is (position, scattered direction) with
 // the arrival inputs zero: the mixture depends on the position only.
 //
 // The scattering at vertex k waits in the path's pending slot until the next bounce, where the

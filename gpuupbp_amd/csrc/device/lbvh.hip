@@ -295,8 +295,32 @@ __global__ void k_level_init(uint32_t* meta, uint32_t* lvl, uint32_t cap, const 
     if (t == 0) lvl[0] = 0;
 }
 
+// The waves of a block allocate together: one atomic per block on the level's count, the one
+// address every block of the launch hits (per wave, the big levels queued on it). Called by
+// every thread of the block (at most 7 waves); tot: this wave's count; returns its first slot.
+__device__ __forceinline__ uint32_t block_alloc(uint32_t tot, uint32_t* counter, uint32_t* sh)
+{
+    const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (__lane_id() == 0) sh[wave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+            const uint32_t t = sh[w];
+            sh[w] = sum;
+            sum += t;
+        }
+        sh[7] = sum ? atomicAdd(counter, sum) : 0u;
+    }
+    __syncthreads();
+    const uint32_t at = sh[7] + sh[wave];
+    __syncthreads(); // sh is reused by the next call
+    return at;
+}
+
 __global__ void __launch_bounds__(256) k_bfs_level(const BNode* nodes, uint32_t* lvl, uint32_t* meta, int level)
 {
+    __shared__ uint32_t sh[8];
     uint32_t* count = meta + kMaxLevels + 1;
     const uint32_t s = meta[level], n = count[level];
     const uint32_t next = s + n;
@@ -315,9 +339,7 @@ __global__ void __launch_bounds__(256) k_bfs_level(const BNode* nodes, uint32_t*
         const bool k0 = c0 >= 0, k1 = c1 >= 0;
         const uint64_t m0 = __ballot(k0), m1 = __ballot(k1);
         const uint32_t tot = uint32_t(__popcll(m0) + __popcll(m1));
-        uint32_t at = 0;
-        if (lane == 0 && tot) at = atomicAdd(&count[level + 1], tot);
-        at = __shfl(at, 0) + next;
+        const uint32_t at = block_alloc(tot, &count[level + 1], sh) + next;
         const uint32_t p0 = uint32_t(__popcll(m0 & lt)), p1 = uint32_t(__popcll(m0)) + uint32_t(__popcll(m1 & lt));
         if (k0) lvl[at + p0] = uint32_t(c0);
         if (k1) lvl[at + p1] = uint32_t(c1);
@@ -410,6 +432,7 @@ __global__ void k_wide_init(uint32_t* meta, uint32_t* bin, uint32_t cap, const u
 __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, uint32_t* bin, uint32_t* meta,
                                                     int level)
 {
+    __shared__ uint32_t sh[8];
     uint32_t* count = meta + kMaxLevels + 1;
     const uint32_t s = meta[level], n = count[level];
     const uint32_t next = s + n;
@@ -452,16 +475,14 @@ __global__ void __launch_bounds__(256) k_wide_level(const BNode* bn, WNode* wn, 
             }
             for (int k = 0; k < nc; k++) ninner += c[k] >= 0 ? 1 : 0;
         }
-        // wave-aggregated allocation of the inner children at the next level
+        // block-aggregated allocation of the inner children at the next level
         uint32_t pre = 0, tot = 0;
         for (int bit = 0; bit < 4; bit++) { // ninner <= kWide = 8: four bits
             const uint64_t m = __ballot((uint32_t(ninner) >> bit) & 1u);
             pre += uint32_t(__popcll(m & lt)) << bit;
             tot += uint32_t(__popcll(m)) << bit;
         }
-        uint32_t at = 0;
-        if (lane == 0 && tot) at = atomicAdd(&count[level + 1], tot);
-        at = __shfl(at, 0) + next + pre;
+        uint32_t at = block_alloc(tot, &count[level + 1], sh) + next + pre;
         if (j < n) {
             float4 nlo = lo[0], nhi = hi[0];
             for (int k = 1; k < nc; k++) {

@@ -1748,6 +1748,27 @@ extern "C" int upbp_read_mis_log(upbp_renderer* r, uint32_t* records, uint64_t m
     return UPBP_OK;
 }
 
+extern "C" int upbp_read_guide_samples(upbp_renderer* r, float* samples, uint64_t max_samples, uint64_t* n_samples,
+                                       float* grid)
+{
+    if (!r || !n_samples || (max_samples && !samples)) return fail(r, "null argument", UPBP_E_ARG);
+    if (!r->gtrain || !r->gsamp.p) return fail(r, "online training is off", UPBP_E_ARG);
+    uint32_t n = 0;
+    HIPCHK(hipStreamSynchronize(r->stream));
+    HIPCHK(hipMemcpy(&n, r->gcount.as<uint32_t>() + 3, 4, hipMemcpyDeviceToHost)); // the ring's fill
+    *n_samples = n;
+    const uint64_t m = std::min<uint64_t>(std::min<uint64_t>(n, r->gring), max_samples);
+    if (m) HIPCHK(hipMemcpy(samples, r->gsamp.p, m * 10 * sizeof(float), hipMemcpyDeviceToHost));
+    if (grid) {
+        const DGuide& g = r->sc.guide;
+        grid[0] = g.lo.x;
+        grid[1] = g.lo.y;
+        grid[2] = g.lo.z;
+        grid[3] = float(g.res) / g.inv_cell.x;
+    }
+    return UPBP_OK;
+}
+
 extern "C" int upbp_memory_info(uint64_t* free_bytes, uint64_t* total_bytes)
 {
     if (!free_bytes || !total_bytes) return UPBP_E_ARG;

@@ -84,6 +84,18 @@ class Renderer:
         if rc != 0:
             raise RuntimeError("upbp_set_mis_log failed: %d %s" % (rc, capi.lib().upbp_last_error(self.h).decode()))
 
+    def read_guide_samples(self, max_samples=1 << 22):
+        """The training ring's samples ((n, 10): unit-cube position, zero arrival, scattered
+        direction, weight), the record grid's low corner and its edge length."""
+        buf = np.zeros((max_samples, 10), np.float32)
+        grid = np.zeros(4, np.float32)
+        n = C.c_uint64()
+        rc = capi.lib().upbp_read_guide_samples(self.h, _p(buf), max_samples, C.byref(n), _p(grid))
+        if rc != 0:
+            raise RuntimeError("upbp_read_guide_samples failed: %d %s"
+                               % (rc, capi.lib().upbp_last_error(self.h).decode()))
+        return buf[: min(n.value, max_samples)].copy(), grid[:3].copy(), float(grid[3])
+
     def read_mis_log(self, max_records=1 << 22):
         """The last iteration's MIS weight log as a structured array with fields tech, k, s, id
         (camera path; light tracing: light path), lpath (light path; light hits and next-event

@@ -498,6 +498,14 @@ int upbp_test_guide_samples(const float* lobes, const float* arrive, float g, in
 int upbp_set_mis_log(upbp_renderer* r, uint32_t capacity, uint32_t pixel_stride, uint32_t light_paths);
 int upbp_read_mis_log(upbp_renderer* r, uint32_t* records, uint64_t max_records, uint64_t* n_records);
 
+/* Training samples of the learned proposal (its oracle's hook; no reference counterpart): copies
+ * up to max_samples of the samples in the training ring, 10 floats each (position in the record
+ * grid's unit cube, three zero arrival inputs, scattered direction, weight), sets *n_samples to
+ * the ring's fill and, with grid, the grid's low corner (3 floats) and edge length. Read after
+ * the first trained iteration, the ring holds exactly that iteration's samples. Online training
+ * must be on. */
+int upbp_read_guide_samples(upbp_renderer* r, float* samples, uint64_t max_samples, uint64_t* n_samples, float* grid);
+
 /* Streams `bytes` from one device buffer to another `reps` times (after one untimed copy)
  * with a plain copy kernel; *ms_per_copy receives the mean time. Reference kernel for HBM
  * traffic-counter calibration and the measured copy bandwidth. */

@@ -29,6 +29,8 @@ def lib():
         vp = C.c_void_p
         h.oracle_render.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(OracleStats)]
         h.oracle_render_sample.argtypes = [vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_int, vp, vp]
+        h.oracle_training_samples.argtypes = [vp, vp, C.c_int, C.c_int32, C.c_int32, C.c_int, vp, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]
         h.oracle_range_query.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, C.c_float, vp, vp, C.c_uint64,
                                          C.POINTER(C.c_uint64)]
         h.oracle_closest_hit.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
@@ -83,6 +85,22 @@ def render_sample_times(scene_ptr, cfg, n_camera_paths, n_light_paths, threads=0
     if rc != 0:
         raise RuntimeError("oracle_render_sample failed: %d" % rc)
     return (t, st) if want_stats else t
+
+
+def training_samples(scene_ptr, cfg, iteration=0, train_media=True, train_surfaces=True, threads=0):
+    """The learned proposal's training samples of one iteration rendered without the proposal:
+    (n, 8) float32 rows (pixel, world position, scattered direction, weight)."""
+    n = C.c_uint64()
+    rc = lib().oracle_training_samples(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), iteration,
+                                       int(train_media), int(train_surfaces), threads, None, 0, C.byref(n))
+    if rc != 0:
+        raise RuntimeError("oracle_training_samples failed: %d" % rc)
+    out = np.zeros((max(1, n.value), 8), np.float32)
+    rc = lib().oracle_training_samples(C.cast(scene_ptr, C.c_void_p), C.cast(C.pointer(cfg), C.c_void_p), iteration,
+                                       int(train_media), int(train_surfaces), threads, _p(out), n.value, C.byref(n))
+    if rc != 0:
+        raise RuntimeError("oracle_training_samples failed: %d" % rc)
+    return out[: n.value]
 
 
 def range_query(kind, prims, queries, radius, max_indices=None):

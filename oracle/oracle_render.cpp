@@ -5,6 +5,7 @@
 #include "oracle_mis.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -313,6 +314,9 @@ struct Ctx {
     std::vector<std::vector<MisRec>>* mis_cam = nullptr;
     uint32_t mis_lt_paths = 0;
     std::vector<std::vector<MisRec>>* mis_lt = nullptr;
+    // learned-proposal training samples (per pixel) and which scatterings feed them
+    std::vector<std::vector<std::array<float, 7>>>* train = nullptr;
+    bool train_media = false, train_surfaces = false;
 };
 
 // camera-subpath vertex kept for the MIS log: its BSDF (incoming from the next vertex toward
@@ -653,8 +657,11 @@ static int init_stack(const Scene& sc, BStack& st, int mat, int med)
 }
 
 // kernel/functs_shared.cuh:37-154
+// guide_den (camera paths with training samples on): lum(throughput f) cos / cp of a continuing
+// scattering in a medium or at a guidable surface whose training is on, else 0 -- the
+// denominator of the training weight (the GPU's sample_scattering, passes.h)
 static int sample_scattering(const Ctx& c, bool& cont, const BSDF& b, V hit, const Isect& is, SubpathState& s,
-                             Stream& rng)
+                             Stream& rng, float* guide_den = nullptr)
 {
     const Params& p = c.p;
     const float r0 = rng.next(), r1 = rng.next(), r2 = rng.next(), r3 = rng.next();
@@ -676,6 +683,11 @@ static int sample_scattering(const Ctx& c, bool& cont, const BSDF& b, V hit, con
     }
     for_pdf *= cp;
     rev_pdf *= cp;
+    if (guide_den) {
+        const bool trains = b.in_medium() ? c.train_media
+                                          : (c.train_surfaces && surface_guidable(c.sc->mats[size_t(b.mat)]));
+        *guide_den = trains ? luminance(s.throughput * f) * cos_out / cp : 0.f;
+    }
     VertexWeights& w = s.sw.w;
     const float bpt_i1 = float(!w.prev_specular && !specular);
     if (specular) {
@@ -1626,6 +1638,9 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
     bool only_spec_surf = (p.techniques & (UPBP_PREVIOUS | UPBP_COMPATIBLE)) != 0;
     std::vector<VolSeg> segs;
     std::vector<VolLite> lite;
+    // training samples: (position, scattered direction, weight denominator, luminance collected
+    // up to the scattering) of each continuing scattering that trains
+    std::vector<std::array<float, 8>> pend;
     for (; ret == OK; s.path_length++) {
         const uint32_t seg = s.path_length - 1;
         Stream mrng(kDomainCamera, pix, p.iter_key, seg, P_MEDIUM);
@@ -1764,7 +1779,8 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
         }
         bool cont;
         Stream srng(kDomainCamera, pix, p.iter_key, seg, P_SCATTER);
-        ret = sample_scattering(c, cont, b, hitp, is, s, srng);
+        float den = 0.f;
+        ret = sample_scattering(c, cont, b, hitp, is, s, srng, c.train ? &den : nullptr);
         if (ret != OK) break;
         if (!cont) break;
         if (b.on_surface()) {
@@ -1776,9 +1792,18 @@ static V trace_camera_path(const Ctx& c, uint32_t pix, float& sx, float& sy)
                 else break;
             }
         }
+        if (c.train && den > 0.f)
+            pend.push_back({s.origin.x, s.origin.y, s.origin.z, s.dir.x, s.dir.y, s.dir.z, den, luminance(color)});
     }
     g_mis = nullptr;
     if (ret != OK) const_cast<Ctx&>(c).errors++;
+    // a sample's weight: the luminance the path collects after its scattering over the
+    // denominator, kept when positive (the GPU's k_guide_finalise)
+    if (c.train)
+        for (const auto& q : pend) {
+            const float w = (luminance(color) - q[7]) / q[6];
+            if (w > 0.f && std::isfinite(w)) (*c.train)[pix].push_back({q[0], q[1], q[2], q[3], q[4], q[5], w});
+        }
     return color;
 }
 
@@ -2091,6 +2116,48 @@ extern "C" int oracle_mis_check(const upbp_scene* scene, const upbp_config* cfg,
                 n++;
             }
     *n_records = n;
+    return UPBP_OK;
+}
+
+// Renders one iteration with the learned proposal's training samples on (scatterings in media
+// with train_media, at guidable surfaces with train_surfaces): every kept sample as (pixel,
+// world position, scattered direction, weight), 8 floats each, pixel by pixel in the order of
+// the path's scatterings; *n_samples receives the total.
+extern "C" int oracle_training_samples(const upbp_scene* scene, const upbp_config* cfg, int iteration,
+                                       int32_t train_media, int32_t train_surfaces, int n_threads, float* out,
+                                       uint64_t max_samples, uint64_t* n_samples)
+{
+    if (!scene || !cfg || !n_samples) return UPBP_E_ARG;
+    Scene sc;
+    prepare_scene(scene, cfg->res_x, cfg->res_y, sc);
+    Ctx c;
+    c.sc = &sc;
+    {
+        std::vector<Box> rb, ib;
+        for (auto& t : sc.real) rb.push_back(tri_box(t));
+        for (auto& t : sc.imag) ib.push_back(tri_box(t));
+        c.bvh_real.build(rb);
+        c.bvh_imag.build(ib);
+    }
+    const size_t npx = size_t(cfg->res_x) * cfg->res_y;
+    std::vector<std::vector<std::array<float, 7>>> recs(npx);
+    c.train = &recs;
+    c.train_media = train_media != 0;
+    c.train_surfaces = train_surfaces != 0;
+    const int threads = n_threads > 0 ? n_threads : int(std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<float> fb(3 * npx, 0.f);
+    render_iteration(c, *cfg, iteration, threads, fb, UINT32_MAX, UINT32_MAX, nullptr);
+    uint64_t n = 0;
+    for (size_t px = 0; px < npx; px++)
+        for (const auto& r : recs[px]) {
+            if (out && n < max_samples) {
+                float* o = out + 8 * n;
+                o[0] = float(px);
+                for (int k = 0; k < 7; k++) o[1 + k] = r[size_t(k)];
+            }
+            n++;
+        }
+    *n_samples = n;
     return UPBP_OK;
 }
 
