@@ -27,6 +27,9 @@ constexpr int kMergeBlock = 256;
 constexpr int kMergeWaves = kMergeBlock / 64;
 constexpr uint32_t kCandCap = 64 + 64 * kWide; // < 64 pending + up to kWide leaves per lane in one node step
 constexpr uint32_t kPairCap = 128; // < 64 pending + at most one accept per lane in a test batch
+// B-B1D bundles list leaves as node << 3 | child: trees over fewer primitives than this (so fewer
+// wide nodes) walk bundles, larger ones walk per piece
+constexpr uint32_t kBundleMaxPrims = 1u << 28;
 
 // The piece of a candidate or stolen subtree travels as (k, n) packed in 16 bits (k | n << 6,
 // k < n <= 32); the camera range it owns is recomputed from the query record, which the
@@ -491,7 +494,8 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_batch(const BatchJob*
         const bool on = KIND == 0 ? (p.build_pb2d && J.b.maps->wmed.n > 0) : (p.build_bb1d && J.b.maps->wbeam.n > 0);
         if (!on || bounce >= int(p.max_len)) continue;
         // B-B1D on a bundled bounce: the pieces the bundle kernel left
-        const bool left = KIND == 1 && bounce < p.bundle_bounces && J.b.units != nullptr;
+        const bool left = KIND == 1 && bounce < p.bundle_bounces && J.b.units != nullptr &&
+                          J.b.maps->wbeam.n < kBundleMaxPrims;
         seg_merge_body<KIND, false>(J.sc, J.b, bounce, blockIdx.x, true, left, cq[w], pq[w], stl[w], ccache[w]);
     }
 }
@@ -787,7 +791,7 @@ __global__ void __launch_bounds__(kMergeBlock, 3) k_seg_bundle_batch(const Batch
         const BatchJob& J = jobs[(blockIdx.x + t) % njobs];
         const IterParams& p = *J.b.params;
         if (!(p.build_bb1d && J.b.maps->wbeam.n > 0) || bounce >= int(p.max_len) || bounce >= p.bundle_bounces ||
-            !J.b.units)
+            !J.b.units || J.b.maps->wbeam.n >= kBundleMaxPrims)
             continue;
         seg_bundle_body<false>(J.sc, J.b, bounce, blds[w], pq[w], ccache[w]);
     }
@@ -1194,7 +1198,7 @@ void launch_seg_merge(const DScene& sc, const IterParams& p, const Bufs& b, int 
         } else {
             // bundles first on the first bundle_bounces bounces; the per-piece walk then takes
             // the pieces they left
-            const int left = bounce < p.bundle_bounces ? 1 : 0;
+            const int left = bounce < p.bundle_bounces && p.cap_beam < kBundleMaxPrims ? 1 : 0;
             if (left) {
                 const int bb = bundle_grid_blocks();
                 if (b.mislog)
