@@ -513,10 +513,11 @@ __global__ void __launch_bounds__(kMergeBlock) k_seg_merge_batch(const BatchJob*
 // predicate with both ownership rules. The capsule meets every box a piece meets, so the bundle
 // lists every leaf the per-piece walk reaches, and the pairs accepted are the same (a leaf box
 // can overhang its parent's by under one quantisation step; a pair the exact predicate places in
-// that sliver is the only possible difference). A bundle wider than p.bundle_radius (rays that
-// diverge, or stretches at different depths) hands its pieces to the per-piece walk (pieces_x,
-// counted in qcount[448 + bounce]), which runs next. Three 4-wave blocks fit a CU (LDS and
-// VGPRs), the per-piece walk's occupancy.
+// that sliver is the only possible difference). A bundle wider than p.bundle_radius, or whose
+// capsule (grown by a sub-beam box) is larger than its pieces' own grown segments together (rays
+// that diverge, or stretches at different depths), hands its pieces to the per-piece walk
+// (pieces_x, counted in qcount[448 + bounce]), which runs next. Three 4-wave blocks fit a CU
+// (LDS and VGPRs), the per-piece walk's occupancy.
 constexpr uint32_t kFrontCap = 256;                      // frontier size up to which 64 nodes go per step
 constexpr uint32_t kFrontAlloc = kFrontCap + 7 * 64 + 8; // + growth of one-node steps down the deepest tree
 constexpr uint32_t kStage = 32;                          // sub-beams staged in LDS at a time
@@ -620,7 +621,18 @@ __device__ __forceinline__ void seg_bundle_body(const DScene& sc, const Bufs& b,
         smax = wave_fmax(smax);
         const float rad = sqrtf(wave_fmax(r2));
         mag = wave_fmax(mag);
-        fit = fit && rad <= p.bundle_radius;
+        // Cost test: the capsule, grown by a sub-beam box (half size hb), holds no more volume
+        // than the pieces' own grown segments together. Staggered or diverging pieces (a long or
+        // wide capsule) would make every piece test sub-beams far from it.
+        const float hb = p.bundle_box;
+        const float plen = wave_fsum(has ? tb - ta : 0.f);
+        const float cap_vol = (rad + hb) * (rad + hb) * (smax - smin + 2.f * hb);
+        fit = fit && rad <= p.bundle_radius && cap_vol <= hb * hb * (plen + 2.f * hb * float(nh));
+        // the capsule test only culls: FP32 slack on the radius keeps it conservative. A piece
+        // reaching far outside the scene (a huge coordinate) would make the slack, not the pieces,
+        // set the capsule: such a bundle goes to the per-piece walk too.
+        const float rg = rad * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f;
+        fit = fit && rg <= rad + hb;
         if (!fit) { // to the per-piece walk
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(left, nh);
@@ -631,8 +643,6 @@ __device__ __forceinline__ void seg_bundle_body(const DScene& sc, const Bufs& b,
             continue;
         }
         n_bundles += lane == 0 ? 1u : 0u;
-        // the capsule test only culls: FP32 slack on the radius keeps it conservative
-        const float rg = rad * (1.f + 1e-4f) + 1e-5f * mag + 1e-6f;
         const f3 a0 = c + ax * smin, a1 = c + ax * smax;
         const RayBox cap = make_raybox(a0, a1 - a0, 0.f, 1.f);
         const RayBox prb = make_raybox(ro, rd, ta, tb); // this piece, as the per-piece walk tests it

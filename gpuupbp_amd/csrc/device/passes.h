@@ -38,6 +38,7 @@ struct IterParams {
                         // sorted (beam slot, sub-beam j | n << 16) references and the beam record
     int bundle_bounces; // B-B1D: camera bounces below this walk the beam tree per bundle of pieces
     float bundle_radius; // largest bundle radius walked as one bundle (world units)
+    float bundle_box;    // half the size of a sub-beam's box (world units), for the bundle cost test
     uint32_t cap_med, cap_beam; // capacities of the medium-point and sub-beam lists (host launch
                                 // decisions; the counts themselves are only on the device)
     uint64_t iter_key;

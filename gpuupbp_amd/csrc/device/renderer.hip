@@ -440,6 +440,7 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
     p.bundle_bounces =
         r->tune.piece_order == 0 && r->tune.beam_leaf == 0 && r->tune.packet_bounces == 0 ? r->tune.bundle_bounces : 0;
     p.bundle_radius = r->tune.bundle_factor * p.rad_beam;
+    p.bundle_box = p.sub_len > 0.f ? 0.5f * p.sub_len + p.rad_beam : 8.f * p.rad_beam;
     p.block = r->tune.block_size > 0 ? uint32_t(r->tune.block_size) : 64u;
     p.cap_med = r->cap_m;
     p.cap_beam = r->cap_b;
