@@ -94,7 +94,7 @@ class Tuning(C.Structure):
     _fields_ = [("subbeam_factor", C.c_float), ("query_piece_factor", C.c_float),
                 ("merge_blocks_per_cu", C.c_int32), ("piece_order", C.c_int32), ("packet_bounces", C.c_int32),
                 ("block_size", C.c_int32), ("beam_leaf", C.c_int32), ("graph", C.c_int32),
-                ("bundle_bounces", C.c_int32), ("bundle_factor", C.c_float)]
+                ("bundle_bounces", C.c_int32), ("bundle_factor", C.c_float), ("bundle_piece_factor", C.c_float)]
 
 
 class IterStats(C.Structure):

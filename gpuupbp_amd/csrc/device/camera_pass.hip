@@ -203,7 +203,8 @@ __device__ bool camera_step(const DScene& sc, const IterParams& p, const Bufs& b
         const SegCount c0 = segment_queries<0>(sc, sg, lt, ns, rf, p.qbt, 0.f, s, pix, nullptr, nullptr);
         // piece length: the tuned length, stretched so that one ray makes at most kPiecesPerRay
         // pieces plus one per segment
-        const float plen = p.piece_len > 0.f ? fmaxf(p.piece_len, c0.len / float(kPiecesPerRay)) : 0.f;
+        const float pl = bounce < p.bundle_bounces ? p.bundle_piece_len : p.piece_len;
+        const float plen = pl > 0.f ? fmaxf(pl, c0.len / float(kPiecesPerRay)) : 0.f;
         const SegCount c1 = segment_queries<1>(sc, sg, lt, ns, rf, p.qbt, plen, s, pix, nullptr, nullptr);
         const uint32_t at = wave_alloc<4>(c1.recs, &b.qcount[128 + bounce]);
         uint32_t ptot;

@@ -29,6 +29,7 @@ struct IterParams {
     float r2_bb1d, r2_bp2d, r2_pb2d, r2_pp3d, r2_surf;
     float sub_len;   // target sub-beam length of the beam map
     float piece_len; // target length of one camera-segment query piece
+    float bundle_piece_len; // the same on the bounces walked in bundles
     float box_slack; // extra beam-box inflation covering FP32 error of the beam predicates
     int merge_bpc;   // resident blocks per CU of the merge kernels (0: occupancy limit)
     int piece_order; // B-B1D work order: 0 as written, 1 XCD regions, 2 Morton-sorted (pieces_x)

@@ -449,6 +449,9 @@ void setup_params(const upbp_renderer* r, int iteration, IterParams& p)
         if (p.do_pb2d) rq = fmaxf(rq, r_pb2d);
         if (p.do_bb1d) rq = fmaxf(rq, r_bb1d);
         p.piece_len = r->tune.query_piece_factor > 0.f ? r->tune.query_piece_factor * rq : 0.f;
+        p.bundle_piece_len = p.piece_len > 0.f && r->tune.bundle_piece_factor > 0.f
+                                 ? r->tune.bundle_piece_factor * rq
+                                 : p.piece_len;
     }
     p.r2_bb1d = r_bb1d * r_bb1d;
     p.r2_bp2d = r_bp2d * r_bp2d;
@@ -1200,6 +1203,7 @@ extern "C" void upbp_tuning_defaults(upbp_tuning* t)
     t->graph = 0;
     t->bundle_bounces = 1;
     t->bundle_factor = 32.f;
+    t->bundle_piece_factor = 0.f;
 }
 
 extern "C" int upbp_set_guide_lobes(upbp_renderer* r, const float* lobes, int32_t res, const float* lo, const float* hi,
@@ -1302,7 +1306,7 @@ extern "C" int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t)
     if (!(t->subbeam_factor >= 0.f) || !(t->query_piece_factor >= 0.f) || t->merge_blocks_per_cu < 0 ||
         t->piece_order < 0 || t->piece_order > 4 || t->packet_bounces < 0 || t->beam_leaf < 0 || t->beam_leaf > 1 ||
         (t->block_size != 0 && t->block_size != 64 && t->block_size != 128 && t->block_size != 256) ||
-        t->bundle_bounces < 0 || !(t->bundle_factor >= 0.f))
+        t->bundle_bounces < 0 || !(t->bundle_factor >= 0.f) || !(t->bundle_piece_factor >= 0.f))
         return fail(r, "invalid tuning", UPBP_E_ARG);
     r->tune = *t;
     r->use_graph = t->graph != 0;

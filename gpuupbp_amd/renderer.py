@@ -65,7 +65,7 @@ class Renderer:
     def set_tuning(self, **kw):
         """Speed knobs of the C ABI's upbp_tuning (subbeam_factor, query_piece_factor,
         merge_blocks_per_cu, piece_order, packet_bounces, block_size, beam_leaf, graph,
-        bundle_bounces, bundle_factor); unnamed fields keep their defaults."""
+        bundle_bounces, bundle_factor, bundle_piece_factor); unnamed fields keep their defaults."""
         t = capi.Tuning()
         capi.lib().upbp_tuning_defaults(C.byref(t))
         for k, v in kw.items():

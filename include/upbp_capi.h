@@ -319,6 +319,8 @@ typedef struct upbp_tuning {
                                     0, beam_leaf 0 and packet_bounces 0, else off) */
     float bundle_factor;         /* largest bundle radius walked as one bundle, in beam radii
                                     (default 32) */
+    float bundle_piece_factor;   /* query piece length on the bounces walked in bundles, in merge
+                                    radii (default 0: query_piece_factor's) */
 } upbp_tuning;
 void upbp_tuning_defaults(upbp_tuning* t);
 int upbp_set_tuning(upbp_renderer* r, const upbp_tuning* t);

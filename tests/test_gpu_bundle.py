@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 OFF = dict(bundle_bounces=0)
 SETTINGS = [dict(bundle_bounces=1), dict(bundle_bounces=64, bundle_factor=1e6), dict(bundle_bounces=64, bundle_factor=4.0),
-            dict(bundle_bounces=2, bundle_factor=0.0), dict(bundle_bounces=64, bundle_factor=16.0, graph=1)]
+            dict(bundle_bounces=2, bundle_factor=0.0), dict(bundle_bounces=64, bundle_factor=16.0, graph=1),
+            dict(bundle_bounces=1, bundle_piece_factor=128.0)]
 
 
 def _render(obj, args, tuning, iters=2):
