@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -166,6 +167,10 @@ std::string upbp_read_exr_rgb(const std::string& path, std::vector<float>& rgb, 
     size_t pixel_bytes = 0;
     for (const Channel& c : ch) pixel_bytes += c.type == 1 ? 2 : 4;
     const size_t line_bytes = pixel_bytes * width;
+    // deflate expands at most ~1032:1 and RLE 64:1, so a data window beyond that ratio of the file
+    // is corrupt; checked before the pixels are allocated
+    if (double(line_bytes) * height > 1100.0 * double(b.size()) + double(1 << 20))
+        return path + ": data window larger than the file can hold";
     const uint32_t chunks = (height + uint32_t(lines) - 1) / uint32_t(lines);
     std::vector<uint64_t> offs(chunks);
     for (auto& o : offs) o = r.get<uint64_t>();
@@ -249,7 +254,12 @@ extern "C" int upbp_read_exr(const char* path, float* rgb, uint64_t capacity_flo
     if (!path || !width || !height) return UPBP_E_ARG;
     std::vector<float> img;
     uint32_t w = 0, h = 0;
-    const std::string err = upbp_read_exr_rgb(path, img, w, h);
+    std::string err;
+    try { // the C ABI lets no C++ exception through (an image too large for host memory)
+        err = upbp_read_exr_rgb(path, img, w, h);
+    } catch (const std::exception& e) {
+        err = std::string(path) + ": " + e.what();
+    }
     if (!err.empty()) {
         fprintf(stderr, "Error: %s\n", err.c_str());
         return err.compare(0, 14, "could not open") == 0 ? UPBP_E_IO : UPBP_E_PARSE;

@@ -18,7 +18,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <stdexcept>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 namespace {
@@ -112,6 +115,11 @@ struct Tok {
     {
         FILE* f = fopen(path.c_str(), "rb");
         if (!f) return false;
+        struct stat st;
+        if (fstat(fileno(f), &st) != 0 || !S_ISREG(st.st_mode)) { // a directory opens, but is no file
+            fclose(f);
+            return false;
+        }
         fseek(f, 0, SEEK_END);
         long n = ftell(f);
         fseek(f, 0, SEEK_SET);
@@ -459,7 +467,7 @@ void upbp_set_loader_error(const std::string& msg) { g_loader_error = msg; }
 
 extern "C" const char* upbp_global_error(void) { return g_loader_error.c_str(); }
 
-extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_t res_y, upbp_scene_handle** out)
+static int load_obj(const char* obj_path, uint32_t res_x, uint32_t res_y, upbp_scene_handle** out)
 {
     (void)res_x;
     (void)res_y;
@@ -481,7 +489,8 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
         return L.err.compare(0, 14, "could not open") == 0 ? UPBP_E_IO : UPBP_E_PARSE;
     }
 
-    auto* h = new upbp_scene_handle();
+    std::unique_ptr<upbp_scene_handle> own(new upbp_scene_handle());
+    upbp_scene_handle* h = own.get();
     upbp_scene& s = h->scene;
     memset(&s, 0, sizeof(s));
     s.background_light_idx = -1;
@@ -514,12 +523,10 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
                     uint32_t w = 0, ht = 0;
                     const std::string err = upbp_read_exr_rgb(file, img, w, ht);
                     if (!err.empty()) {
-                        delete h;
                         g_loader_error = "environment map loading failed: " + err;
                         return UPBP_E_IO;
                     }
                     if (w < 2 * ht) { // the distribution reads 2 * height columns per row
-                        delete h;
                         g_loader_error = "environment map must be at least twice as wide as high";
                         return UPBP_E_PARSE;
                     }
@@ -551,7 +558,6 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
         V3 p[3];
         for (int k = 0; k < 3; k++) {
             if (t.v[k] >= nv) {
-                delete h;
                 g_loader_error = "face references a missing vertex";
                 return UPBP_E_PARSE;
             }
@@ -565,7 +571,6 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
         if (has_n) {
             for (int k = 0; k < 3; k++) {
                 if (t.n[k] >= nn) {
-                    delete h;
                     g_loader_error = "face references a missing normal";
                     return UPBP_E_PARSE;
                 }
@@ -669,8 +674,20 @@ extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_
     s.scene_radius = std::sqrt(r2) * .5f;
 
     h->bind();
-    *out = h;
+    *out = own.release();
     return UPBP_OK;
+}
+
+// The C ABI lets no C++ exception through: a scene whose sizes exhaust memory is an error code.
+extern "C" int upbp_scene_load_obj(const char* obj_path, uint32_t res_x, uint32_t res_y, upbp_scene_handle** out)
+{
+    try {
+        return load_obj(obj_path, res_x, res_y, out);
+    } catch (const std::exception& e) {
+        g_loader_error = std::string("scene loading failed: ") + e.what();
+        if (out) *out = nullptr;
+        return UPBP_E_PARSE;
+    }
 }
 
 extern "C" const upbp_scene* upbp_scene_get(const upbp_scene_handle* h) { return h ? &h->scene : nullptr; }

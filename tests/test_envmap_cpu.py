@@ -166,6 +166,25 @@ def test_exr_reader_rejects_an_empty_compression_attribute(tmp_path):
         capi.read_exr(p)
 
 
+def test_exr_reader_rejects_a_data_window_the_file_cannot_hold(tmp_path):
+    # a 65536 x 65536 ZIP data window whose 4096 offsets all name the one real chunk: every
+    # offset lies in the file, so only the size bound (deflate expands at most ~1032:1) keeps the
+    # reader from allocating 48 GiB of pixels
+    img = _test_image(4, 8, 4)
+    p = str(tmp_path / "w.exr")
+    write_exr(p, img, 3)
+    data = bytes(open(p, "rb").read())
+    table = next(i for i in range(len(data) - 8) if struct.unpack("<Q", data[i:i + 8])[0] == i + 8)
+    hdr = bytearray(data[:table])
+    dw = hdr.index(b"dataWindow\0box2i\0") + len(b"dataWindow\0box2i\0") + 4
+    hdr[dw:dw + 16] = struct.pack("<iiii", 0, 0, 65535, 65535)
+    chunks = 65536 // 16
+    first = len(hdr) + 8 * chunks
+    open(p, "wb").write(bytes(hdr) + struct.pack("<Q", first) * chunks + data[table + 8:])
+    with pytest.raises(RuntimeError):
+        capi.read_exr(p)
+
+
 # ---- AUX light_background_em ------------------------------------------------------------------
 def test_aux_environment_map_loads_rotated_and_scaled(box_obj, tmp_path):
     env = sky()

@@ -113,6 +113,13 @@ def test_loader_errors(tmp_path):
     (tmp_path / "s.obj.aux").write_text("")
     with pytest.raises(RuntimeError, match="material not found"):
         capi.SceneHandle(str(tmp_path / "s.obj"))
+    # a material library that names a directory (a directory opens for reading on Linux, and
+    # its size reads as 2^63 - 1) is an error code, not an exception through the C ABI
+    (tmp_path / "sub").mkdir()
+    (tmp_path / "d.obj").write_text("mtllib sub\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    (tmp_path / "d.obj.aux").write_text("")
+    with pytest.raises(RuntimeError, match="could not open"):
+        capi.SceneHandle(str(tmp_path / "d.obj"))
 
 
 def test_polygon_fan_and_normals(tmp_path):
