@@ -113,7 +113,38 @@ int main(int argc, char** argv)
         printf("scene %s: %d rounds\n", kind.c_str(), nr);
         fflush(stdout);
     }
-    printf("scene loads: %d ok, %d rejected\n", loads_ok, loads_err);
+    printf("\nscene loads: %d ok, %d rejected\n", loads_ok, loads_err);
+    // a scene lit by an environment map whose EXR file is corrupted
+    {
+        const std::string d = dir + "/env";
+        if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) return 2;
+        const std::string obj = d + "/env.obj";
+        if (upbp_generate_scene("box", 0, obj.c_str()) != 0) return 4;
+        spit(obj + ".aux", slurp(obj + ".aux") + "\nlight_background_em 2 0.25 env.exr\n");
+        const uint32_t W = 32, H = 16;
+        std::vector<float> sky(3 * W * H);
+        for (size_t i = 0; i < sky.size(); i++) sky[i] = 0.5f + float(i % 13) * 0.1f;
+        if (upbp_save_exr((d + "/env.exr").c_str(), sky.data(), W, H) != 0) return 3;
+        const std::string e0 = slurp(d + "/env.exr");
+        int ok = 0, bad = 0;
+        for (int r = 0; r < rounds; r++) {
+            std::string e = e0;
+            const int n = 1 + int(g() % 4);
+            for (int k = 0; k < n && !e.empty(); k++) {
+                const size_t i = g() % e.size();
+                if (g() % 5 == 0) e = e.substr(0, i);
+                else e[i] = char(g() % 256);
+            }
+            spit(d + "/env.exr", r == 0 ? e0 : e);
+            upbp_scene_handle* h = nullptr;
+            if (upbp_scene_load_obj(obj.c_str(), 16, 16, &h) == 0) {
+                ok++;
+                upbp_scene_free(h);
+            } else
+                bad++;
+        }
+        printf("\nenv scene loads: %d ok, %d rejected\n", ok, bad);
+    }
     // predefined scenes
     for (int id = -1; id < 48; id++) {
         upbp_scene_handle* h = nullptr;
@@ -148,7 +179,7 @@ int main(int argc, char** argv)
             if (upbp_read_exr(p.c_str(), out.data(), out.size(), &w, &h) == 0) ok++;
             else bad++;
         }
-        printf("exr: %d ok, %d rejected\n", ok, bad);
+        printf("\nexr: %d ok, %d rejected\n", ok, bad);
     }
     // command lines
     {
@@ -174,8 +205,8 @@ int main(int argc, char** argv)
             if (upbp_parse_commandline(int(av.size()), av.data(), &cfg) == 0) ok++;
             else bad++;
         }
-        printf("argv: %d ok, %d rejected\n", ok, bad);
+        printf("\nargv: %d ok, %d rejected\n", ok, bad);
     }
-    printf("fuzz done\n");
+    printf("\nfuzz done\n");
     return 0;
 }

@@ -1,5 +1,6 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer (no GPU): the scene loader
-(OBJ/MTL/AUX), the OpenEXR reader and the command-line parser are built with g++ and the
+(OBJ/MTL/AUX, and an environment map's EXR through the loader), the OpenEXR reader and the
+command-line parser are built with g++ and the
 sanitizers into a standalone driver (tests/fuzz/host_fuzz.cpp) and fed corrupted inputs. Every
 call must come back with a result or an error code: a sanitizer report, or a C++ exception through
 the C ABI (which would end the caller's process), fails the test."""
@@ -57,7 +58,7 @@ def test_host_parsers_survive_corrupted_inputs(tmp_path):
     assert "AddressSanitizer" not in out and "runtime error" not in out and "LeakSanitizer" not in out, out[-4000:]
     assert r.returncode == 0 and "fuzz done" in r.stdout, out[-4000:]
     # the corruptions reach both outcomes: inputs that still load and inputs that are rejected
-    for what in ("scene loads", "exr", "argv"):
+    for what in ("scene loads", "env scene loads", "exr", "argv"):
         line = next(l for l in r.stdout.splitlines() if l.startswith(what))
         ok, bad = [int(t) for t in line.replace(",", " ").split() if t.isdigit()]
         assert ok > 0 and bad > 0, line
